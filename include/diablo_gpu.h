@@ -1,0 +1,150 @@
+/*
+ * diablo_gpu.h — C ABI of the MI355X (gfx950) RKW3 + Crank–Nicolson channel time step.
+ *
+ * The library is a drop-in for DIABLO's time-advance call
+ *     do rk_step = 1,3 : call rk_chan_1            (for/diablo.f90:68-71)
+ * The reference routine takes no arguments and talks through module globals
+ * (for/flow.f90:12-50, for/parameters.f90:8-65, for/domain.f90:35-85).  Here the
+ * globals that rk_chan_1 reads become one dgpu_config passed at creation, the
+ * state arrays cross the boundary through upload/download calls in the exact
+ * Fortran declared layout, and every call returns an int status (0 = ok) where the
+ * reference would `stop` (e.g. for/les.f90:1198, for/flow.f90:1198).
+ *
+ * Conventions shared by every entry point:
+ *  - Code axes: x (periodic, Fourier), y (wall-normal, finite differences on the
+ *    staggered stretched grid), z (periodic, Fourier).  Ny is the wall-normal count.
+ *  - Fourier arrays cross the boundary as the reference declares them:
+ *        complex(8) cu(0:Nxp, 0:Nz+1, 0:Nyp+1)      (for/fft.f90:71-96)
+ *    column-major, leading dimensions Nxp+1 and Nz+2, passed as interleaved
+ *    (re, im) doubles.  Only the active block i = 0..Nkx, k = 0..2Nkz is read on
+ *    upload; on download the active block is written and the de-aliased columns
+ *    i = Nkx+1..Nxp of rows k = 0..2Nkz are written as zeros; every other entry
+ *    (the physical-alias padding) is left untouched.
+ *  - Physical arrays cross as  real(8) u(0:Nx+1, 0:Nzp+1, 0:Nyp+1)  (same buffer size).
+ *  - The library owns all device memory (allocated in dgpu_create, freed in
+ *    dgpu_destroy) and keeps no host pointer past a call.  Work runs on one HIP
+ *    stream; every call is synchronous on return.  A handle is not thread-safe.
+ *  - Scalars live in arrays indexed n = 0..n_th-1 for the reference's 1..N_th.
+ */
+#ifndef DIABLO_GPU_H
+#define DIABLO_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DGPU_MAX_TH 4
+
+/* Precision of the step.  FP64: every array and operation in double.
+ * MIXED: transforms, nonlinear products and physical-space intermediates in
+ * float with float accumulation; state, RK memory, Fourier-space assembly,
+ * tridiagonal CN solves, the pressure Poisson solve (including the kx=kz=0 mean
+ * mode) and the thermal-wind terms stay double. */
+enum { DGPU_FP64 = 0, DGPU_MIXED = 1 };
+
+/* Boundary-condition codes, as read from input_chan.dat (for/parameters.f90:232-248). */
+enum { DGPU_BC_DIRICHLET = 0, DGPU_BC_NEUMANN = 1 };
+
+/* Everything rk_chan_1 reads from modules parameters/domain, flattened.
+ * Field names follow the reference's variable names (code convention: u = u1,
+ * v = u2 wall-normal, w = u3).  Pointer members are read during dgpu_create only. */
+typedef struct dgpu_config {
+  /* grid_def (for/domain.f90:18-32): one rank, so Nyp = Ny, Nxp = Nx/2, Nzp = Nz */
+  int32_t nx, ny, nz, n_th;
+  int32_t precision;        /* DGPU_FP64 or DGPU_MIXED */
+  int32_t use_les;          /* input.dat USE_LES (for/parameters.f90:98) */
+  int32_t les_model_type;   /* input_chan.dat LES_MODEL_TYPE (for/parameters.f90:217) */
+  int32_t flavor_front;     /* flavor == 'Front' enables thermal-wind terms (for/solver.f90:47) */
+  int32_t f_type;           /* forcing type (for/solver.f90:464-476, 2121) */
+  /* time-stepping row bounds from init_chan_mpi (for/domain.f90:307-374) */
+  int32_t jstart, jend;
+  int32_t jstart_th[DGPU_MAX_TH], jend_th[DGPU_MAX_TH];
+  /* BC codes: u = u1, v = u2 (wall-normal), w = u3 (for/domain.f90:69-84) */
+  int32_t u_bc_ymin, v_bc_ymin, w_bc_ymin;
+  int32_t u_bc_ymax, v_bc_ymax, w_bc_ymax;
+  int32_t th_bc_ymin[DGPU_MAX_TH], th_bc_ymax[DGPU_MAX_TH];
+  int32_t reserved_i[4];
+
+  double lx, ly, lz;        /* domain lengths (for/parameters.f90:101) */
+  double nu, beta, nu_v_scale;
+  double ro_inv, delta;     /* Coriolis f = ro_inv/delta (for/solver.f90:502-509) */
+  double grav_x, grav_y, grav_z;
+  double px0, omega0, amp_omega0, force_start;
+  double u_bc_ymin_c1, v_bc_ymin_c1, w_bc_ymin_c1;
+  double u_bc_ymax_c1, v_bc_ymax_c1, w_bc_ymax_c1;  /* w_bc_ymax_c1 after IC compensation */
+  double th_bc_ymin_c1[DGPU_MAX_TH], th_bc_ymax_c1[DGPU_MAX_TH];
+  double pr[DGPU_MAX_TH], dthdx[DGPU_MAX_TH], dthdz[DGPU_MAX_TH];
+  double reserved_d[4];
+
+  /* grid metrics, each (0:Nyp+1) = ny+2 values (for/domain.f90:35-39, 276-300, 538-564) */
+  const double *gy, *gyf, *dy, *dyf;
+  /* wavenumbers exactly as init_fft computes them (for/fft.f90:209-225):
+   * kx(0:Nxp-1) (nx/2 values), kz(0:2Nkz) (2*(nz/3)+1 values) */
+  const double *kx, *kz;
+} dgpu_config;
+
+typedef struct dgpu_handle dgpu_handle;
+
+/* Build a solver for one grid: validates the config, allocates device memory,
+ * zero-initialises the state, the RK memory and nu_t/kappa_t (for/flow.f90:49-50).
+ * Replaces the allocation half of init_flow (for/flow.f90:100-107). */
+int dgpu_create(const dgpu_config *cfg, dgpu_handle **out);
+
+/* Frees all device memory. Replaces nothing (the reference never frees, for/flow.f90:249-254). */
+int dgpu_destroy(dgpu_handle *h);
+
+/* State upload/download in the Fortran Fourier layout (see header comment).
+ * cth holds n_th consecutive arrays (the reference's cth(...,1:N_th)).
+ * Called by the host before/after save_flow, save_stats, create_flow/read_flow
+ * (for/diablo.f90:87-123, for/flow.f90:118-141). */
+int dgpu_upload_state(dgpu_handle *h, const double *cu1, const double *cu2,
+                      const double *cu3, const double *cp, const double *cth);
+int dgpu_download_state(dgpu_handle *h, double *cu1, double *cu2, double *cu3,
+                        double *cp, double *cth);
+
+/* RK memory cf1..cf3, cfth (for/flow.f90:12-18), carried between substeps
+ * (for/solver.f90:413-429).  Needed only to resume between substeps. */
+int dgpu_upload_rk(dgpu_handle *h, const double *cf1, const double *cf2,
+                   const double *cf3, const double *cfth);
+int dgpu_download_rk(dgpu_handle *h, double *cf1, double *cf2, double *cf3,
+                     double *cfth);
+
+/* One RKW3 substep.  Replaces `call rk_chan_1` (for/solver.f90:366-1273) with the
+ * module values rk_step, delta_t, time, time_step and create_new_flow passed in. */
+int dgpu_rk_substep(dgpu_handle *h, int32_t rk_step, double delta_t, double time,
+                    int32_t time_step, int32_t create_new_flow);
+
+/* One full time step: the three substeps of for/diablo.f90:68-71 queued on the
+ * stream back to back (time is not advanced inside, as in the reference). */
+int dgpu_step(dgpu_handle *h, double delta_t, double time, int32_t time_step,
+              int32_t create_new_flow);
+
+/* Projection of the device velocity onto divergence-free fields: rem_div_chan
+ * (for/solver.f90:1302-1379), called by create_flow_chan (for/flow.f90:487). */
+int dgpu_rem_div(dgpu_handle *h);
+
+/* 2D xz transforms of one field over all ny+2 planes, in the Fortran layouts:
+ * fft_xz_to_fourier (for/fft.f90:242-307) and fft_xz_to_physical
+ * (for/fft.f90:311-374).  Used by the create_th/create_flow hooks and by stats. */
+int dgpu_fft_xz_to_fourier(dgpu_handle *h, const double *phys, double *four);
+int dgpu_fft_xz_to_physical(dgpu_handle *h, const double *four, double *phys);
+
+/* Blocks until all queued work on the handle's stream is done. */
+int dgpu_synchronize(dgpu_handle *h);
+
+/* Device-time of the last dgpu_step/dgpu_rk_substep in milliseconds (HIP events). */
+int dgpu_last_step_ms(dgpu_handle *h, double *ms);
+
+/* Text of the last error on this thread ("" if none). */
+const char *dgpu_last_error(void);
+
+/* Sizes the host needs for Fortran-layout buffers: number of doubles in one
+ * Fourier (= physical) array, i.e. 2*(Nx/2+1)*(Nz+2)*(Ny+2). */
+int64_t dgpu_field_doubles(const dgpu_config *cfg);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DIABLO_GPU_H */

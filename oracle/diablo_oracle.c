@@ -1,0 +1,905 @@
+/*
+ * diablo_oracle.c — CPU FP64 restatement of DIABLO's rk_chan_1 path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the parity checker for the HIP step and the timed
+ * CPU baseline.  Nothing in the product links or loads it.
+ *
+ * Every routine restates the reference routine named beside it (file:line into
+ * wienkers/diablo3).  The structure is the reference's: the state is advanced
+ * with 3D transforms over all Ny+2 planes, the implicit Crank–Nicolson solves are
+ * real Thomas solves per physical (x, z) column over rows 0..Nyp+1, and the
+ * projection is a complex Thomas solve per (kx, kz) column over rows 1..Nyp.
+ *
+ * Differences of form (not of arithmetic):
+ *  - The reference aliases each physical array with its Fourier array (one FFTW
+ *    buffer, for/fft.f90:71-96).  rk_chan_1 never reads a Fourier array after
+ *    transforming it to physical space (or the reverse) before rewriting it, so
+ *    here the two views are separate arrays.  Planes that the implicit solve does
+ *    not write keep the values of the inverse transform and are forward
+ *    transformed again, as in the reference.
+ *  - FFTW (third-party, not vendored, "v3.3+") is replaced by a radix-2
+ *    transform with the FFTW sign and scaling conventions: forward exp(-i..),
+ *    unnormalised; c2r ignores the imaginary part of the DC bin.  Powers of two only.
+ *  - The per-column CN matrices are rebuilt from their initial values for every
+ *    z-plane.  The reference builds them once and overwrites, per plane, every row
+ *    that Thomas modified (rows jstart..jend by the build, the two end rows by the
+ *    BC routines), so the two are the same.
+ * The one rank of every config makes ghost_chan_mpi a no-op (for/flow.f90:885).
+ */
+#include "diablo_oracle.h"
+
+#include <complex.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef double complex zc;
+
+static __thread char g_err[512];
+static void set_err(const char *m) { snprintf(g_err, sizeof g_err, "%s", m); }
+const char *oracle_last_error(void) { return g_err; }
+
+void oracle_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+
+/* ------------------------------------------------------------------------- */
+/* radix-2 complex FFT                                                        */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+  int n;
+  zc *w;   /* exp(-2 pi i k / n), k < n/2, rounded from long double */
+  int *rev;
+} fftplan;
+
+static int is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+static void plan_init(fftplan *p, int n) {
+  p->n = n;
+  p->w = (zc *)malloc(sizeof(zc) * (size_t)(n / 2 > 0 ? n / 2 : 1));
+  p->rev = (int *)malloc(sizeof(int) * (size_t)n);
+  const long double twopi = 6.283185307179586476925286766559L;
+  for (int k = 0; k < n / 2; k++) {
+    long double a = twopi * (long double)k / (long double)n;
+    p->w[k] = (double)cosl(a) - I * (double)sinl(a);
+  }
+  int lg = 0;
+  while ((1 << lg) < n) lg++;
+  for (int i = 0; i < n; i++) {
+    int r = 0;
+    for (int b = 0; b < lg; b++)
+      if (i & (1 << b)) r |= 1 << (lg - 1 - b);
+    p->rev[i] = r;
+  }
+}
+static void plan_free(fftplan *p) { free(p->w); free(p->rev); }
+
+/* sign = -1: sum_j a_j exp(-2 pi i jk/n); sign = +1: exp(+...).  Unnormalised. */
+static void fft_c(const fftplan *p, zc *a, int sign) {
+  const int n = p->n;
+  for (int i = 0; i < n; i++) {
+    int r = p->rev[i];
+    if (r > i) { zc t = a[i]; a[i] = a[r]; a[r] = t; }
+  }
+  for (int len = 2; len <= n; len <<= 1) {
+    const int half = len >> 1, step = n / len;
+    for (int i = 0; i < n; i += len)
+      for (int j = 0; j < half; j++) {
+        zc w = p->w[j * step];
+        if (sign > 0) w = conj(w);
+        zc u = a[i + j], v = a[i + j + half] * w;
+        a[i + j] = u + v;
+        a[i + j + half] = u - v;
+      }
+  }
+}
+
+/* ------------------------------------------------------------------------- */
+/* oracle state                                                               */
+/* ------------------------------------------------------------------------- */
+struct oracle {
+  dgpu_config c;
+  int nx, ny, nz, nth, nxp, nkx, nkz, twonkz, kzr, ny2;
+  size_t nf, np;                  /* elements per Fourier / physical array */
+  double *gy, *gyf, *dy, *dyf;    /* (0:Nyp+1) */
+  double *kx, *kz, *kx2, *kz2;    /* kx(0:Nxp-1), kz(0:2Nkz) */
+  zc *cu1, *cu2, *cu3, *cp, *cth; /* Fourier [j][k][i], i < Nxp, k <= 2Nkz */
+  zc *cf1, *cf2, *cf3, *cfth, *cr1, *cr2, *cr3, *crth, *cs1;
+  double *u1, *u2, *u3, *th, *r1, *r2, *r3, *rth, *s1; /* physical [j][z][x] */
+  double *nu_t, *kappa_t;
+  fftplan px, pz;
+};
+
+#define FI(j, k, i) ((((size_t)(j)) * o->kzr + (size_t)(k)) * o->nxp + (size_t)(i))
+#define PI(j, z, x) ((((size_t)(j)) * o->nz + (size_t)(z)) * o->nx + (size_t)(x))
+
+/* for/fft.f90:147, 209-225 — pi is 4*atan(1.0) in single precision */
+void oracle_wavenumbers(int nx, int nz, double lx, double lz, double *kx, double *kz) {
+  const double pi = (double)(4.0f * atanf(1.0f));
+  const int nxp = nx / 2, nkz = nz / 3, twonkz = 2 * nkz;
+  for (int i = 0; i < nxp; i++) kx[i] = ((double)i * (2.0 * pi)) / lx;
+  for (int k = 0; k <= nkz; k++) kz[k] = ((double)k * (2.0 * pi)) / lz;
+  for (int k = 1; k <= nkz; k++) kz[twonkz + 1 - k] = -(((double)k * (2.0 * pi)) / lz);
+}
+
+/* ReadGridHDF5 (for/domain.f90:533-564) + create_grid_chan (for/domain.f90:276-300).
+ * gy_file holds the ny+1 values of grid.h5:/grids/y; gy(0) stays 0 as in the
+ * reference, where it is never read. */
+void oracle_grid_metrics(int ny, const double *gy_file, double *gy, double *gyf,
+                         double *dy, double *dyf) {
+  const int n2 = ny + 2;
+  for (int j = 0; j < n2; j++) gy[j] = gyf[j] = dy[j] = dyf[j] = 0.0;
+  for (int j = 1; j <= ny + 1; j++) gy[j] = gy_file[j - 1];
+  for (int j = 1; j <= ny; j++) gyf[j] = 0.5 * (gy[j] + gy[j + 1]);
+  gyf[0] = 2.0 * gyf[1] - gyf[2];
+  gyf[ny + 1] = 2.0 * gyf[ny] - gyf[ny - 1];
+  for (int j = 1; j <= ny + 1; j++) dy[j] = gyf[j] - gyf[j - 1];
+  for (int j = 1; j <= ny; j++) dyf[j] = gy[j + 1] - gy[j];
+  dyf[0] = dyf[1];
+  dyf[ny + 1] = dyf[ny];
+}
+
+static void *xcalloc(size_t n, size_t sz) {
+  void *p = calloc(n ? n : 1, sz);
+  if (!p) { fprintf(stderr, "oracle: out of memory\n"); abort(); }
+  return p;
+}
+
+int oracle_create(const dgpu_config *cfg, oracle_t **out) {
+  if (!cfg || !out) { set_err("null argument"); return 1; }
+  if (!is_pow2(cfg->nx) || !is_pow2(cfg->nz) || cfg->nx < 4 || cfg->nz < 4) {
+    set_err("oracle: nx and nz must be powers of two >= 4");
+    return 2;
+  }
+  if (cfg->ny < 4 || cfg->n_th < 0 || cfg->n_th > DGPU_MAX_TH) {
+    set_err("oracle: bad ny or n_th");
+    return 2;
+  }
+  oracle_t *o = (oracle_t *)xcalloc(1, sizeof *o);
+  o->c = *cfg;
+  o->nx = cfg->nx; o->ny = cfg->ny; o->nz = cfg->nz; o->nth = cfg->n_th;
+  o->nxp = o->nx / 2; o->nkx = o->nx / 3; o->nkz = o->nz / 3;
+  o->twonkz = 2 * o->nkz; o->kzr = o->twonkz + 1; o->ny2 = o->ny + 2;
+  o->nf = (size_t)o->ny2 * o->kzr * o->nxp;
+  o->np = (size_t)o->ny2 * o->nz * o->nx;
+  const int n2 = o->ny2;
+  o->gy = xcalloc(n2, 8); o->gyf = xcalloc(n2, 8); o->dy = xcalloc(n2, 8); o->dyf = xcalloc(n2, 8);
+  memcpy(o->gy, cfg->gy, 8 * n2); memcpy(o->gyf, cfg->gyf, 8 * n2);
+  memcpy(o->dy, cfg->dy, 8 * n2); memcpy(o->dyf, cfg->dyf, 8 * n2);
+  o->kx = xcalloc(o->nxp, 8); o->kx2 = xcalloc(o->nxp, 8);
+  o->kz = xcalloc(o->kzr, 8); o->kz2 = xcalloc(o->kzr, 8);
+  memcpy(o->kx, cfg->kx, 8 * (size_t)o->nxp);
+  memcpy(o->kz, cfg->kz, 8 * (size_t)o->kzr);
+  for (int i = 0; i < o->nxp; i++) o->kx2[i] = o->kx[i] * o->kx[i];
+  for (int k = 0; k < o->kzr; k++) o->kz2[k] = o->kz[k] * o->kz[k];
+  const int nt = o->nth > 0 ? o->nth : 1;
+  o->cu1 = xcalloc(o->nf, sizeof(zc)); o->cu2 = xcalloc(o->nf, sizeof(zc));
+  o->cu3 = xcalloc(o->nf, sizeof(zc)); o->cp = xcalloc(o->nf, sizeof(zc));
+  o->cth = xcalloc(o->nf * nt, sizeof(zc));
+  o->cf1 = xcalloc(o->nf, sizeof(zc)); o->cf2 = xcalloc(o->nf, sizeof(zc));
+  o->cf3 = xcalloc(o->nf, sizeof(zc)); o->cfth = xcalloc(o->nf * nt, sizeof(zc));
+  o->cr1 = xcalloc(o->nf, sizeof(zc)); o->cr2 = xcalloc(o->nf, sizeof(zc));
+  o->cr3 = xcalloc(o->nf, sizeof(zc)); o->crth = xcalloc(o->nf * nt, sizeof(zc));
+  o->cs1 = xcalloc(o->nf, sizeof(zc));
+  o->u1 = xcalloc(o->np, 8); o->u2 = xcalloc(o->np, 8); o->u3 = xcalloc(o->np, 8);
+  o->th = xcalloc(o->np * nt, 8);
+  o->r1 = xcalloc(o->np, 8); o->r2 = xcalloc(o->np, 8); o->r3 = xcalloc(o->np, 8);
+  o->rth = xcalloc(o->np * nt, 8); o->s1 = xcalloc(o->np, 8);
+  o->nu_t = xcalloc(o->np, 8); o->kappa_t = xcalloc(o->np * nt, 8);
+  plan_init(&o->px, o->nx);
+  plan_init(&o->pz, o->nz);
+  *out = o;
+  return 0;
+}
+
+void oracle_destroy(oracle_t *o) {
+  if (!o) return;
+  void *ps[] = {o->gy, o->gyf, o->dy, o->dyf, o->kx, o->kz, o->kx2, o->kz2,
+                o->cu1, o->cu2, o->cu3, o->cp, o->cth, o->cf1, o->cf2, o->cf3,
+                o->cfth, o->cr1, o->cr2, o->cr3, o->crth, o->cs1, o->u1, o->u2,
+                o->u3, o->th, o->r1, o->r2, o->r3, o->rth, o->s1, o->nu_t, o->kappa_t};
+  for (size_t i = 0; i < sizeof ps / sizeof ps[0]; i++) free(ps[i]);
+  plan_free(&o->px);
+  plan_free(&o->pz);
+  free(o);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Fortran layout <-> oracle layout                                           */
+/* ------------------------------------------------------------------------- */
+static size_t ffort(const oracle_t *o, int i, int k, int j) {
+  return ((size_t)j * (o->nz + 2) + (size_t)k) * (o->nxp + 1) + (size_t)i;
+}
+static size_t pfort(const oracle_t *o, int x, int z, int j) {
+  return ((size_t)j * (o->nz + 2) + (size_t)z) * (o->nx + 2) + (size_t)x;
+}
+static void four_in(const oracle_t *o, const double *src, zc *dst) {
+  const zc *s = (const zc *)src;
+  for (int j = 0; j < o->ny2; j++)
+    for (int k = 0; k < o->kzr; k++)
+      for (int i = 0; i < o->nxp; i++)
+        dst[FI(j, k, i)] = (i <= o->nkx) ? s[ffort(o, i, k, j)] : 0.0;
+}
+static void four_out(const oracle_t *o, const zc *src, double *dst) {
+  zc *d = (zc *)dst;
+  for (int j = 0; j < o->ny2; j++)
+    for (int k = 0; k < o->kzr; k++)
+      for (int i = 0; i <= o->nxp; i++)
+        d[ffort(o, i, k, j)] = (i <= o->nkx) ? src[FI(j, k, i)] : 0.0;
+}
+
+int oracle_upload_state(oracle_t *o, const double *cu1, const double *cu2,
+                        const double *cu3, const double *cp, const double *cth) {
+  const size_t fd = (size_t)2 * (o->nxp + 1) * (o->nz + 2) * o->ny2;
+  four_in(o, cu1, o->cu1); four_in(o, cu2, o->cu2); four_in(o, cu3, o->cu3);
+  four_in(o, cp, o->cp);
+  for (int n = 0; n < o->nth; n++) four_in(o, cth + n * fd, o->cth + n * o->nf);
+  return 0;
+}
+int oracle_download_state(oracle_t *o, double *cu1, double *cu2, double *cu3,
+                          double *cp, double *cth) {
+  const size_t fd = (size_t)2 * (o->nxp + 1) * (o->nz + 2) * o->ny2;
+  four_out(o, o->cu1, cu1); four_out(o, o->cu2, cu2); four_out(o, o->cu3, cu3);
+  four_out(o, o->cp, cp);
+  for (int n = 0; n < o->nth; n++) four_out(o, o->cth + n * o->nf, cth + n * fd);
+  return 0;
+}
+int oracle_upload_rk(oracle_t *o, const double *cf1, const double *cf2,
+                     const double *cf3, const double *cfth) {
+  const size_t fd = (size_t)2 * (o->nxp + 1) * (o->nz + 2) * o->ny2;
+  four_in(o, cf1, o->cf1); four_in(o, cf2, o->cf2); four_in(o, cf3, o->cf3);
+  for (int n = 0; n < o->nth; n++) four_in(o, cfth + n * fd, o->cfth + n * o->nf);
+  return 0;
+}
+int oracle_download_rk(oracle_t *o, double *cf1, double *cf2, double *cf3, double *cfth) {
+  const size_t fd = (size_t)2 * (o->nxp + 1) * (o->nz + 2) * o->ny2;
+  four_out(o, o->cf1, cf1); four_out(o, o->cf2, cf2); four_out(o, o->cf3, cf3);
+  for (int n = 0; n < o->nth; n++) four_out(o, o->cfth + n * o->nf, cfth + n * fd);
+  return 0;
+}
+int oracle_download_les(oracle_t *o, double *nu_t, double *kappa_t) {
+  const size_t pd = (size_t)(o->nx + 2) * (o->nz + 2) * o->ny2;
+  for (int j = 0; j < o->ny2; j++)
+    for (int z = 0; z < o->nz; z++)
+      for (int x = 0; x < o->nx; x++) {
+        nu_t[pfort(o, x, z, j)] = o->nu_t[PI(j, z, x)];
+        for (int n = 0; n < o->nth; n++)
+          kappa_t[n * pd + pfort(o, x, z, j)] = o->kappa_t[n * o->np + PI(j, z, x)];
+      }
+  return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* 2D xz transforms over all planes j = 0..Nyp+1                              */
+/* ------------------------------------------------------------------------- */
+/* fft_xz_to_fourier (for/fft.f90:242-307): x r2c, /Nx, zero i > Nkx; z c2c
+ * forward; /Nz; pack kz < 0 into k = Nkz+1..2Nkz. */
+static void to_fourier(oracle_t *o, const double *v, zc *vv) {
+  const int nx = o->nx, nz = o->nz, nxp = o->nxp, nkx = o->nkx, nkz = o->nkz;
+#pragma omp parallel
+  {
+    zc *line = (zc *)malloc(sizeof(zc) * (size_t)(nx > nz ? nx : nz));
+    zc *half = (zc *)malloc(sizeof(zc) * (size_t)nz * nxp);
+#pragma omp for schedule(static)
+    for (int j = 0; j < o->ny2; j++) {
+      /* x: two real lines per complex transform */
+      for (int z = 0; z < nz; z += 2) {
+        const double *a = v + PI(j, z, 0), *b = v + PI(j, z + 1, 0);
+        for (int x = 0; x < nx; x++) line[x] = a[x] + I * b[x];
+        fft_c(&o->px, line, -1);
+        for (int i = 0; i < nxp; i++) {
+          zc zk = line[i], zn = conj(line[(nx - i) % nx]);
+          zc A = 0.5 * (zk + zn), B = -0.5 * I * (zk - zn);
+          half[(size_t)z * nxp + i] = (i <= nkx) ? A / nx : 0.0;
+          half[(size_t)(z + 1) * nxp + i] = (i <= nkx) ? B / nx : 0.0;
+        }
+      }
+      /* z: one complex transform per kx column i = 0..Nxp-1 */
+      for (int i = 0; i < nxp; i++) {
+        for (int z = 0; z < nz; z++) line[z] = half[(size_t)z * nxp + i];
+        fft_c(&o->pz, line, -1);
+        for (int k = 0; k <= nkz; k++) vv[FI(j, k, i)] = line[k] / nz;
+        for (int k = 1; k <= nkz; k++) vv[FI(j, nkz + k, i)] = line[nz - 1 + k - nkz] / nz;
+      }
+    }
+    free(line);
+    free(half);
+  }
+}
+
+/* fft_xz_to_physical (for/fft.f90:311-374): unpack, zero Nkz<k<Nz-Nkz, z c2c
+ * backward, zero i > Nkx, x c2r; no scaling. */
+static void to_physical(oracle_t *o, const zc *vv, double *v) {
+  const int nx = o->nx, nz = o->nz, nxp = o->nxp, nkx = o->nkx, nkz = o->nkz;
+#pragma omp parallel
+  {
+    zc *line = (zc *)malloc(sizeof(zc) * (size_t)(nx > nz ? nx : nz));
+    zc *half = (zc *)malloc(sizeof(zc) * (size_t)nz * (nxp + 1));
+#pragma omp for schedule(static)
+    for (int j = 0; j < o->ny2; j++) {
+      for (int i = 0; i < nxp; i++) {
+        for (int z = 0; z < nz; z++) line[z] = 0.0;
+        for (int k = 0; k <= nkz; k++) line[k] = vv[FI(j, k, i)];
+        for (int k = 1; k <= nkz; k++) line[nz - 1 + k - nkz] = vv[FI(j, nkz + k, i)];
+        fft_c(&o->pz, line, +1);
+        for (int z = 0; z < nz; z++) half[(size_t)z * (nxp + 1) + i] = line[z];
+      }
+      for (int z = 0; z < nz; z += 2) {
+        const zc *A = half + (size_t)z * (nxp + 1), *B = half + (size_t)(z + 1) * (nxp + 1);
+        for (int x = 0; x < nx; x++) line[x] = 0.0;
+        /* Hermitian completion; bins i > Nkx (incl. Nyquist) are zero; the
+         * imaginary part of the DC bin is not part of a real signal's spectrum */
+        line[0] = creal(A[0]) + I * creal(B[0]);
+        for (int i = 1; i <= nkx; i++) {
+          line[i] = A[i] + I * B[i];
+          line[nx - i] = conj(A[i]) + I * conj(B[i]);
+        }
+        fft_c(&o->px, line, +1);
+        double *a = v + PI(j, z, 0), *b = v + PI(j, z + 1, 0);
+        for (int x = 0; x < nx; x++) { a[x] = creal(line[x]); b[x] = cimag(line[x]); }
+      }
+    }
+    free(line);
+    free(half);
+  }
+}
+
+int oracle_fft_xz_to_fourier(oracle_t *o, const double *phys, double *four) {
+  double *v = (double *)xcalloc(o->np, 8);
+  zc *vv = (zc *)xcalloc(o->nf, sizeof(zc));
+  for (int j = 0; j < o->ny2; j++)
+    for (int z = 0; z < o->nz; z++)
+      for (int x = 0; x < o->nx; x++) v[PI(j, z, x)] = phys[pfort(o, x, z, j)];
+  to_fourier(o, v, vv);
+  four_out(o, vv, four);
+  free(v); free(vv);
+  return 0;
+}
+int oracle_fft_xz_to_physical(oracle_t *o, const double *four, double *phys) {
+  double *v = (double *)xcalloc(o->np, 8);
+  zc *vv = (zc *)xcalloc(o->nf, sizeof(zc));
+  four_in(o, four, vv);
+  to_physical(o, vv, v);
+  for (int j = 0; j < o->ny2; j++)
+    for (int z = 0; z < o->nz; z++)
+      for (int x = 0; x < o->nx; x++) phys[pfort(o, x, z, j)] = v[PI(j, z, x)];
+  free(v); free(vv);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Thomas solvers at one rank                                                 */
+/* ------------------------------------------------------------------------- */
+/* thomas_forward_real_mpi + thomas_backward_real_mpi (for/solver.f90:1691-1761,
+ * 1816-1852), rankY = 0 = NprocY-1: rows 0..Nyp+1. */
+void oracle_thomas_real(int ny, double *a, double *b, const double *c, double *g) {
+  for (int j = 1; j <= ny + 1; j++) {
+    a[j] = -a[j] / b[j - 1];
+    b[j] = b[j] + a[j] * c[j - 1];
+    g[j] = g[j] + a[j] * g[j - 1];
+  }
+  g[ny + 1] = g[ny + 1] / b[ny + 1];
+  for (int j = ny; j >= 0; j--) g[j] = (g[j] - c[j] * g[j + 1]) / b[j];
+}
+
+/* thomas_forward/backward_complex_mpi (for/solver.f90:1764-1813, 1855-1896): rows 1..Nyp */
+static void thomas_complex(int ny, double *a, double *b, const double *c, zc *g) {
+  for (int j = 2; j <= ny; j++) {
+    a[j] = -a[j] / b[j - 1];
+    b[j] = b[j] + a[j] * c[j - 1];
+    g[j] = g[j] + a[j] * g[j - 1];
+  }
+  g[ny] = g[ny] / b[ny];
+  for (int j = ny - 1; j >= 1; j--) g[j] = (g[j] - c[j] * g[j + 1]) / b[j];
+}
+void oracle_thomas_complex(int ny, double *a, double *b, const double *c, double *g) {
+  thomas_complex(ny, a, b, c, (zc *)g);
+}
+
+/* ------------------------------------------------------------------------- */
+/* BC rows of the implicit systems (for/solver.f90:1901-2232), one column     */
+/* ------------------------------------------------------------------------- */
+static void set_row(double *l, double *d, double *u, double *v, int j, double a,
+                    double b, double c, double val) {
+  l[j] = a; d[j] = b; u[j] = c; v[j] = val;
+}
+
+/* apply_BC_1_lower/upper, apply_BC_3_lower/upper, apply_BC_th_lower/upper: the
+ * three are identical in form for fields at GYF points. */
+static void bc_gyf(const oracle_t *o, int bc_lo, double c1_lo, int bc_hi, double c1_hi,
+                   double *l, double *d, double *u, double *v) {
+  const int ny = o->ny;
+  if (bc_lo == DGPU_BC_DIRICHLET) {
+    set_row(l, d, u, v, 0, 0., 1., 0., 0.);
+    set_row(l, d, u, v, 1, 0., 1., 0., c1_lo);
+  } else {
+    set_row(l, d, u, v, 0, 0., 1., 0., 0.);
+    set_row(l, d, u, v, 1, 0., -1., 1., o->dy[2] * c1_lo);
+  }
+  if (bc_hi == DGPU_BC_DIRICHLET) {
+    set_row(l, d, u, v, ny + 1, 0., 1., 0., 0.);
+    set_row(l, d, u, v, ny, 0., 1., 0., c1_hi);
+  } else {
+    set_row(l, d, u, v, ny, -1., 1., 0., o->dy[ny] * c1_hi);
+    set_row(l, d, u, v, ny + 1, 0., 1., 0., 0.);
+  }
+}
+
+/* apply_BC_2_lower/upper (for/solver.f90:1989-2054) */
+static void bc_u2(const oracle_t *o, double *l, double *d, double *u, double *v) {
+  const dgpu_config *c = &o->c;
+  const int ny = o->ny;
+  if (c->v_bc_ymin == DGPU_BC_DIRICHLET) {
+    set_row(l, d, u, v, 1, 0., 1., 0., c->v_bc_ymin_c1);
+    set_row(l, d, u, v, 2, 0., 1., 0., c->v_bc_ymin_c1);
+  } else if (c->v_bc_ymin == DGPU_BC_NEUMANN) {
+    set_row(l, d, u, v, 1, 0., -1., 1., o->dyf[1] * c->v_bc_ymin_c1);
+  }
+  set_row(l, d, u, v, 0, 0., 1., 0., 0.);
+  if (c->v_bc_ymax == DGPU_BC_DIRICHLET) {
+    set_row(l, d, u, v, ny + 1, 0., 1., 0., c->v_bc_ymax_c1);
+    set_row(l, d, u, v, ny, 0., 1., 0., c->v_bc_ymax_c1);
+  } else if (c->v_bc_ymax == DGPU_BC_NEUMANN) {
+    set_row(l, d, u, v, ny + 1, -1., 1., 0., o->dyf[ny] * c->v_bc_ymax_c1);
+  }
+}
+
+/* w_BC_Ymax_c1_transient (for/parameters.f90:279; for/solver.f90:2131) */
+static double w_top_c1(const oracle_t *o, double time) {
+  const dgpu_config *c = &o->c;
+  if (c->f_type == 4)
+    return c->w_bc_ymax_c1 +
+           c->amp_omega0 * sin(c->omega0 * (c->ro_inv / c->delta * time - c->force_start));
+  return c->w_bc_ymax_c1;
+}
+
+/* CN matrix of one column (rk_chan_1 matrix builds, for/solver.f90:1020-1045,
+ * 1081-1105, 1136-1161, 1192-1217) with an explicit nu_t/kappa_t column. */
+static void build_cn(const oracle_t *o, int which, double temp1, double temp2,
+                     const double *nut, const double *rhs, size_t nstride, double time,
+                     double *l, double *d, double *u, double *v) {
+  const dgpu_config *c = &o->c;
+  const int ny = o->ny;
+  const double *dy = o->dy, *dyf = o->dyf;
+  const double nvs = c->nu_v_scale;
+  for (int j = 0; j < o->ny2; j++) { l[j] = 0.; d[j] = 1.; u[j] = 0.; v[j] = 0.; }
+#define NT(jj) (nut ? nut[(size_t)(jj) * nstride] : 0.0)
+#define RHS(jj) (rhs ? rhs[(size_t)(jj) * nstride] : 0.0)
+  if (which >= 3) {
+    const int n = which - 3;
+    const double t = temp1 / c->pr[n] * nvs;
+    for (int j = c->jstart_th[n]; j <= c->jend_th[n]; j++) {
+      l[j] = -t / (dy[j] * dyf[j]);
+      d[j] = 1. + t / (dy[j + 1] * dyf[j]) + t / (dy[j] * dyf[j]);
+      u[j] = -t / (dy[j + 1] * dyf[j]);
+      v[j] = RHS(j);
+      if (c->use_les) {
+        l[j] = l[j] - temp2 * NT(j) / (dy[j] * dyf[j]);
+        d[j] = d[j] + temp2 * NT(j + 1) / (dy[j + 1] * dyf[j]) + temp2 * NT(j) / (dy[j] * dyf[j]);
+        u[j] = u[j] - temp2 * NT(j + 1) / (dy[j + 1] * dyf[j]);
+      }
+    }
+    bc_gyf(o, c->th_bc_ymin[n], c->th_bc_ymin_c1[n], c->th_bc_ymax[n], c->th_bc_ymax_c1[n],
+           l, d, u, v);
+  } else if (which == 1) {
+    for (int j = 2; j <= ny; j++) {
+      l[j] = -temp1 * nvs / (dyf[j - 1] * dy[j]);
+      d[j] = 1. + temp1 * nvs / (dyf[j] * dy[j]) + temp1 * nvs / (dyf[j - 1] * dy[j]);
+      u[j] = -temp1 * nvs / (dyf[j] * dy[j]);
+      v[j] = RHS(j);
+      if (c->use_les) {
+        l[j] = l[j] - temp2 * 0.5 * (NT(j) + NT(j - 1)) / (dyf[j - 1] * dy[j]);
+        d[j] = d[j] + temp2 * 0.5 * (NT(j) + NT(j + 1)) / (dyf[j] * dy[j]) +
+               temp2 * 0.5 * (NT(j) + NT(j - 1)) / (dyf[j - 1] * dy[j]);
+        u[j] = u[j] - temp2 * 0.5 * (NT(j) + NT(j + 1)) / (dyf[j] * dy[j]);
+      }
+    }
+    bc_u2(o, l, d, u, v);
+  } else {
+    for (int j = c->jstart; j <= c->jend; j++) {
+      l[j] = -temp1 * nvs / (dy[j] * dyf[j]);
+      d[j] = 1. - temp1 * nvs * (-1. / (dy[j + 1] * dyf[j]) - 1. / (dy[j] * dyf[j]));
+      u[j] = -temp1 * nvs / (dy[j + 1] * dyf[j]);
+      v[j] = RHS(j);
+      if (c->use_les) {
+        l[j] = l[j] - temp2 * NT(j) / (dy[j] * dyf[j]);
+        d[j] = d[j] + temp2 * NT(j + 1) / (dy[j + 1] * dyf[j]) + temp2 * NT(j) / (dy[j] * dyf[j]);
+        u[j] = u[j] - temp2 * NT(j + 1) / (dy[j + 1] * dyf[j]);
+      }
+    }
+    if (which == 0)
+      bc_gyf(o, c->u_bc_ymin, c->u_bc_ymin_c1, c->u_bc_ymax, c->u_bc_ymax_c1, l, d, u, v);
+    else
+      bc_gyf(o, c->w_bc_ymin, c->w_bc_ymin_c1, c->w_bc_ymax, w_top_c1(o, time), l, d, u, v);
+  }
+#undef NT
+#undef RHS
+}
+
+int oracle_cn_matrix(oracle_t *o, int which, int rk_step, double delta_t, double *matl,
+                     double *matd, double *matu, double *vec_bc) {
+  const double h[3] = {delta_t * (8.0 / 15.0), delta_t * (2.0 / 15.0), delta_t * (5.0 / 15.0)};
+  if (rk_step < 1 || rk_step > 3 || which < 0 || which >= 3 + o->nth) return 1;
+  const double temp1 = 0.5 * o->c.nu * h[rk_step - 1], temp2 = 0.5 * h[rk_step - 1];
+  build_cn(o, which, temp1, temp2, NULL, NULL, 0, 0.0, matl, matd, matu, vec_bc);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* rem_div_chan (for/solver.f90:1302-1379) + apply_BC_rem_div_mpi (2330-2371) */
+/* ------------------------------------------------------------------------- */
+static void rem_div(oracle_t *o) {
+  const int ny = o->ny, nxp = o->nxp;
+  const double *dy = o->dy, *dyf = o->dyf;
+#pragma omp parallel
+  {
+    double *a = (double *)malloc(8 * (size_t)o->ny2);
+    double *b = (double *)malloc(8 * (size_t)o->ny2);
+    double *c = (double *)malloc(8 * (size_t)o->ny2);
+    zc *g = (zc *)malloc(sizeof(zc) * (size_t)o->ny2);
+#pragma omp for schedule(static)
+    for (int k = 0; k < o->kzr; k++) {
+      for (int i = 0; i < nxp; i++) {
+        for (int j = 0; j < o->ny2; j++) { a[j] = 0.; b[j] = 1.; c[j] = 0.; g[j] = 0.; }
+        for (int j = 1; j <= ny; j++) {
+          a[j] = 1. / (dy[j] * dyf[j]);
+          b[j] = -o->kx2[i] - o->kz2[k] - 1. / (dy[j + 1] * dyf[j]) - 1. / (dy[j] * dyf[j]);
+          c[j] = 1. / (dy[j + 1] * dyf[j]);
+        }
+        for (int j = 1; j <= ny; j++) {
+          const zc ikx = I * o->kx[i], ikz = I * o->kz[k];
+          g[j] = ikx * o->cu1[FI(j, k, i)] +
+                 (o->cu2[FI(j + 1, k, i)] - o->cu2[FI(j, k, i)]) / dyf[j] +
+                 ikz * o->cu3[FI(j, k, i)];
+        }
+        if (k == 0 && i == 0) { a[1] = 0.; b[1] = 1.; c[1] = 0.; g[1] = 0.; }
+        else { a[1] = 0.; b[1] = 1.; c[1] = -1.; g[1] = 0.; }
+        a[ny] = 1.; b[ny] = -1.; c[ny] = 0.; g[ny] = 0.;
+        thomas_complex(ny, a, b, c, g);
+        for (int j = 1; j <= ny; j++) o->cr1[FI(j, k, i)] = g[j];
+      }
+    }
+    free(a); free(b); free(c); free(g);
+  }
+  for (int j = 1; j <= ny; j++)
+    for (int k = 0; k < o->kzr; k++)
+      for (int i = 0; i < nxp; i++) {
+        const zc ikx = I * o->kx[i], ikz = I * o->kz[k];
+        o->cu1[FI(j, k, i)] = o->cu1[FI(j, k, i)] - ikx * o->cr1[FI(j, k, i)];
+        o->cu3[FI(j, k, i)] = o->cu3[FI(j, k, i)] - ikz * o->cr1[FI(j, k, i)];
+      }
+  for (int j = 2; j <= ny; j++)
+    for (int k = 0; k < o->kzr; k++)
+      for (int i = 0; i < nxp; i++)
+        o->cu2[FI(j, k, i)] =
+            o->cu2[FI(j, k, i)] - (o->cr1[FI(j, k, i)] - o->cr1[FI(j - 1, k, i)]) / dy[j];
+}
+
+int oracle_rem_div(oracle_t *o) {
+  rem_div(o);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* rk_chan_1 (for/solver.f90:366-1273)                                        */
+/* ------------------------------------------------------------------------- */
+/* physical-space CN solve of one field over all z-planes: matrix build + BC +
+ * Thomas per (x, z) column, result written on rows [jlo, jhi]. */
+static void cn_solve_phys(oracle_t *o, int which, double temp1, double temp2,
+                          const double *rhs, double *out, int jlo, int jhi,
+                          const double *nut, double time) {
+  const int nx = o->nx, ny2 = o->ny2;
+  const size_t stride = (size_t)o->nz * nx;
+#pragma omp parallel
+  {
+    double *l = (double *)malloc(8 * (size_t)ny2), *d = (double *)malloc(8 * (size_t)ny2);
+    double *u = (double *)malloc(8 * (size_t)ny2), *v = (double *)malloc(8 * (size_t)ny2);
+#pragma omp for schedule(static)
+    for (int z = 0; z < o->nz; z++) {
+      for (int x = 0; x < nx; x++) {
+        const double *ncol = nut ? nut + PI(0, z, x) : NULL;
+        build_cn(o, which, temp1, temp2, ncol, rhs + PI(0, z, x), stride, time, l, d, u, v);
+        oracle_thomas_real(o->ny, l, d, u, v);
+        for (int j = jlo; j <= jhi; j++) out[PI(j, z, x)] = v[j];
+      }
+    }
+    free(l); free(d); free(u); free(v);
+  }
+}
+
+int oracle_rk_substep(oracle_t *o, int rk_step, double delta_t, double time,
+                      int time_step, int create_new_flow) {
+  const dgpu_config *c = &o->c;
+  if (rk_step < 1 || rk_step > 3) { set_err("rk_step must be 1..3"); return 1; }
+  const int ny = o->ny, nxp = o->nxp, kzr = o->kzr, nx = o->nx, nz = o->nz, nth = o->nth;
+  const int js = c->jstart, je = c->jend;
+  const double *dy = o->dy, *dyf = o->dyf, *gyf = o->gyf, *gy = o->gy;
+  /* RKW3 coefficients (for/parameters.f90:176-184) */
+  const double h_bar[3] = {delta_t * (8.0 / 15.0), delta_t * (2.0 / 15.0), delta_t * (5.0 / 15.0)};
+  const double beta_bar[3] = {1.0, 25.0 / 8.0, 9.0 / 4.0};
+  const double zeta_bar[3] = {0.0, -17.0 / 8.0, -5.0 / 4.0};
+  const int r = rk_step - 1;
+  const double temp1 = 0.5 * c->nu * h_bar[r];
+  const double temp2 = 0.5 * h_bar[r];
+  const double temp3 = zeta_bar[r] * h_bar[r];
+  const double temp4 = h_bar[r];
+  const double temp5 = beta_bar[r] * h_bar[r];
+  const int les_on = c->use_les && ((!create_new_flow) || (time_step > 100));
+  if (les_on) { set_err("oracle: LES branch not available"); return 3; }
+  (void)gy;
+
+#define CIKX(i) (I * o->kx[(i)])
+#define CIKZ(k) (I * o->kz[(k)])
+#define KPOW(i, k) pow(o->kx2[(i)] + o->kz2[(k)], c->beta)
+  /* cr <- cu (396-410), + temp3*cf (413-429) */
+  for (int j = js; j <= je; j++)
+    for (int k = 0; k < kzr; k++)
+      for (int i = 0; i < nxp; i++) {
+        o->cr1[FI(j, k, i)] = o->cu1[FI(j, k, i)];
+        o->cr3[FI(j, k, i)] = o->cu3[FI(j, k, i)];
+      }
+  for (int j = 2; j <= ny; j++)
+    for (int k = 0; k < kzr; k++)
+      for (int i = 0; i < nxp; i++) o->cr2[FI(j, k, i)] = o->cu2[FI(j, k, i)];
+  if (rk_step > 1) {
+    for (int j = js; j <= je; j++)
+      for (int k = 0; k < kzr; k++)
+        for (int i = 0; i < nxp; i++) {
+          o->cr1[FI(j, k, i)] = o->cr1[FI(j, k, i)] + temp3 * o->cf1[FI(j, k, i)];
+          o->cr3[FI(j, k, i)] = o->cr3[FI(j, k, i)] + temp3 * o->cf3[FI(j, k, i)];
+        }
+    for (int j = 2; j <= ny; j++)
+      for (int k = 0; k < kzr; k++)
+        for (int i = 0; i < nxp; i++)
+          o->cr2[FI(j, k, i)] = o->cr2[FI(j, k, i)] + temp3 * o->cf2[FI(j, k, i)];
+  }
+  /* pressure gradient (432-456) */
+  for (int j = 2; j <= ny; j++)
+    for (int k = 0; k < kzr; k++)
+      for (int i = 0; i < nxp; i++)
+        o->cs1[FI(j, k, i)] = (o->cp[FI(j, k, i)] - o->cp[FI(j - 1, k, i)]) / dy[j];
+  for (int j = js; j <= je; j++)
+    for (int k = 0; k < kzr; k++)
+      for (int i = 0; i < nxp; i++) {
+        o->cr1[FI(j, k, i)] = o->cr1[FI(j, k, i)] - temp4 * (CIKX(i) * o->cp[FI(j, k, i)]);
+        o->cr3[FI(j, k, i)] = o->cr3[FI(j, k, i)] - temp4 * (CIKZ(k) * o->cp[FI(j, k, i)]);
+      }
+  for (int j = 2; j <= ny; j++)
+    for (int k = 0; k < kzr; k++)
+      for (int i = 0; i < nxp; i++)
+        o->cr2[FI(j, k, i)] = o->cr2[FI(j, k, i)] - temp4 * o->cs1[FI(j, k, i)];
+  /* mean pressure-gradient forcing (464-476) */
+  if (c->f_type == 1) {
+    for (int j = js; j <= je; j++) o->cr1[FI(j, 0, 0)] = o->cr1[FI(j, 0, 0)] - temp4 * c->px0;
+  } else if (c->f_type == 2) {
+    for (int j = js; j <= je; j++)
+      o->cr1[FI(j, 0, 0)] =
+          o->cr1[FI(j, 0, 0)] - temp4 * (c->px0 + c->amp_omega0 * cos(c->omega0 * time));
+  }
+  /* horizontal viscous terms (484-498) and Coriolis (502-509) */
+  for (int j = js; j <= je; j++)
+    for (int k = 0; k < kzr; k++)
+      for (int i = 0; i < nxp; i++) {
+        o->cf1[FI(j, k, i)] = -(c->nu * KPOW(i, k) * o->cu1[FI(j, k, i)]);
+        o->cf3[FI(j, k, i)] = -(c->nu * KPOW(i, k) * o->cu3[FI(j, k, i)]);
+      }
+  for (int j = 2; j <= ny; j++)
+    for (int k = 0; k < kzr; k++)
+      for (int i = 0; i < nxp; i++)
+        o->cf2[FI(j, k, i)] = -(c->nu * KPOW(i, k) * o->cu2[FI(j, k, i)]);
+  for (int j = js; j <= je; j++)
+    for (int k = 0; k < kzr; k++)
+      for (int i = 0; i < nxp; i++) {
+        o->cf1[FI(j, k, i)] = o->cf1[FI(j, k, i)] + (c->ro_inv / c->delta) * o->cu3[FI(j, k, i)];
+        o->cf3[FI(j, k, i)] = o->cf3[FI(j, k, i)] - (c->ro_inv / c->delta) * o->cu1[FI(j, k, i)];
+      }
+  /* scalars: buoyancy (518-554), scalar RHS (565-602) */
+  for (int n = 0; n < nth; n++) {
+    zc *cth = o->cth + n * o->nf, *crth = o->crth + n * o->nf, *cfth = o->cfth + n * o->nf;
+    for (int j = 2; j <= ny; j++)
+      for (int k = 0; k < kzr; k++)
+        for (int i = (k == 0 ? 1 : 0); i < nxp; i++)
+          o->cf2[FI(j, k, i)] = o->cf2[FI(j, k, i)] +
+                                c->grav_y * (cth[FI(j, k, i)] * dyf[j - 1] + cth[FI(j - 1, k, i)] * dyf[j]) /
+                                    (2.0 * dy[j]);
+    for (int j = js; j <= je; j++)
+      for (int k = 0; k < kzr; k++)
+        for (int i = 0; i < nxp; i++) o->cf1[FI(j, k, i)] = o->cf1[FI(j, k, i)] + c->grav_x * cth[FI(j, k, i)];
+    for (int j = js; j <= je; j++)
+      for (int k = 0; k < kzr; k++)
+        for (int i = 0; i < nxp; i++) o->cf3[FI(j, k, i)] = o->cf3[FI(j, k, i)] + c->grav_z * cth[FI(j, k, i)];
+    const int jst = c->jstart_th[n], jet = c->jend_th[n];
+    for (int j = jst; j <= jet; j++)
+      for (int k = 0; k < kzr; k++)
+        for (int i = 0; i < nxp; i++) crth[FI(j, k, i)] = cth[FI(j, k, i)];
+    if (rk_step > 1)
+      for (int j = jst; j <= jet; j++)
+        for (int k = 0; k < kzr; k++)
+          for (int i = 0; i < nxp; i++) crth[FI(j, k, i)] = crth[FI(j, k, i)] + temp3 * cfth[FI(j, k, i)];
+    for (int j = jst; j <= jet; j++)
+      for (int k = 0; k < kzr; k++)
+        for (int i = 0; i < nxp; i++)
+          cfth[FI(j, k, i)] = -((c->nu / c->pr[n]) * KPOW(i, k) * cth[FI(j, k, i)]);
+    for (int j = jst; j <= jet; j++)
+      for (int k = 0; k < kzr; k++)
+        for (int i = 0; i < nxp; i++)
+          cfth[FI(j, k, i)] = cfth[FI(j, k, i)] - o->cu1[FI(j, k, i)] * c->dthdx[n] -
+                              o->cu3[FI(j, k, i)] * c->dthdz[n];
+  }
+  /* user_rhs_chan_fourier: thermal-wind advection (for/solver.f90:35-111) */
+  if (c->flavor_front && c->ro_inv != 0.0) {
+    for (int n = 0; n < nth; n++) {
+      const double dtx = c->dthdx[n], dtz = c->dthdz[n], dl = c->delta, ro = c->ro_inv;
+      const double ly = c->ly;
+      for (int j = js; j <= je; j++)
+        for (int k = 0; k < kzr; k++)
+          for (int i = 0; i < nxp; i++) {
+            const zc u2m = 0.5 * (o->cu2[FI(j, k, i)] + o->cu2[FI(j + 1, k, i)]);
+            o->cf1[FI(j, k, i)] = o->cf1[FI(j, k, i)] -
+                                  (dtx * (gyf[j] - 0.5 * ly) * dl / ro) * CIKZ(k) * o->cu1[FI(j, k, i)] -
+                                  (-1.0 * dtz * (gyf[j] - 0.5 * ly) * dl / ro) * CIKX(i) * o->cu1[FI(j, k, i)] -
+                                  (-1.0 * dtz * dl / ro) * u2m;
+            o->cf3[FI(j, k, i)] = o->cf3[FI(j, k, i)] -
+                                  (dtx * (gyf[j] - 0.5 * ly) * dl / ro) * CIKZ(k) * o->cu3[FI(j, k, i)] -
+                                  (-1.0 * dtz * (gyf[j] - 0.5 * ly) * dl / ro) * CIKX(i) * o->cu3[FI(j, k, i)] -
+                                  (dtx * dl / ro) * u2m;
+          }
+      for (int j = 2; j <= ny; j++)
+        for (int k = 0; k < kzr; k++)
+          for (int i = 0; i < nxp; i++)
+            o->cf2[FI(j, k, i)] = o->cf2[FI(j, k, i)] -
+                                  (dtx * (o->gy[j] - 0.5 * ly) * dl / ro) * CIKZ(k) * o->cu2[FI(j, k, i)] -
+                                  (-1.0 * dtz * (o->gy[j] - 0.5 * ly) * dl / ro) * CIKX(i) * o->cu2[FI(j, k, i)];
+      zc *cth = o->cth + n * o->nf, *cfth = o->cfth + n * o->nf;
+      for (int j = c->jstart_th[n]; j <= c->jend_th[n]; j++)
+        for (int k = 0; k < kzr; k++)
+          for (int i = 0; i < nxp; i++)
+            cfth[FI(j, k, i)] = cfth[FI(j, k, i)] -
+                                (dl / ro) * dtx * (gyf[j] - 0.5 * ly) * CIKZ(k) * cth[FI(j, k, i)] -
+                                (dl / ro) * (-1.0) * dtz * (gyf[j] - 0.5 * ly) * CIKX(i) * cth[FI(j, k, i)];
+    }
+  }
+  /* DNS branch: state to physical space (635-643) */
+  to_physical(o, o->cu1, o->u1);
+  to_physical(o, o->cu2, o->u2);
+  to_physical(o, o->cu3, o->u3);
+  for (int n = 0; n < nth; n++) to_physical(o, o->cth + n * o->nf, o->th + n * o->np);
+
+  const double *u1 = o->u1, *u2 = o->u2, *u3 = o->u3;
+  double *s1 = o->s1;
+  zc *cs1 = o->cs1;
+#define LOOP_P(jlo, jhi) \
+  for (int j = (jlo); j <= (jhi); j++) for (int z = 0; z < nz; z++) for (int x = 0; x < nx; x++)
+#define LOOP_F(jlo, jhi) \
+  for (int j = (jlo); j <= (jhi); j++) for (int k = 0; k < kzr; k++) for (int i = 0; i < nxp; i++)
+  /* nonlinear terms (655-815) */
+  LOOP_P(js, je) s1[PI(j, z, x)] = u3[PI(j, z, x)] * u1[PI(j, z, x)];
+  to_fourier(o, s1, cs1);
+  LOOP_F(js, je) {
+    o->cf1[FI(j, k, i)] = o->cf1[FI(j, k, i)] - CIKZ(k) * cs1[FI(j, k, i)];
+    o->cf3[FI(j, k, i)] = o->cf3[FI(j, k, i)] - CIKX(i) * cs1[FI(j, k, i)];
+  }
+  LOOP_P(js, je) s1[PI(j, z, x)] = u1[PI(j, z, x)] * u1[PI(j, z, x)];
+  to_fourier(o, s1, cs1);
+  LOOP_F(js, je) o->cf1[FI(j, k, i)] = o->cf1[FI(j, k, i)] - CIKX(i) * cs1[FI(j, k, i)];
+  LOOP_P(js, je) s1[PI(j, z, x)] = u3[PI(j, z, x)] * u3[PI(j, z, x)];
+  to_fourier(o, s1, cs1);
+  LOOP_F(js, je) o->cf3[FI(j, k, i)] = o->cf3[FI(j, k, i)] - CIKZ(k) * cs1[FI(j, k, i)];
+  LOOP_P(2, ny) s1[PI(j, z, x)] =
+      ((dyf[j] * u1[PI(j, z, x)] + dyf[j - 1] * u1[PI(j - 1, z, x)]) / (2. * dy[j])) * u2[PI(j, z, x)];
+  to_fourier(o, s1, cs1);
+  LOOP_F(2, ny) o->cf2[FI(j, k, i)] = o->cf2[FI(j, k, i)] - CIKX(i) * cs1[FI(j, k, i)];
+  LOOP_P(2, ny) s1[PI(j, z, x)] =
+      ((dyf[j] * u3[PI(j, z, x)] + dyf[j - 1] * u3[PI(j - 1, z, x)]) / (2. * dy[j])) * u2[PI(j, z, x)];
+  to_fourier(o, s1, cs1);
+  LOOP_F(2, ny) o->cf2[FI(j, k, i)] = o->cf2[FI(j, k, i)] - CIKZ(k) * cs1[FI(j, k, i)];
+  LOOP_P(js, je) s1[PI(j, z, x)] =
+      (u1[PI(j + 1, z, x)] * u2[PI(j + 1, z, x)] + u1[PI(j, z, x)] * u2[PI(j + 1, z, x)] -
+       u1[PI(j, z, x)] * u2[PI(j, z, x)] - u1[PI(j - 1, z, x)] * u2[PI(j, z, x)]) / (2.0 * dyf[j]);
+  to_fourier(o, s1, cs1);
+  LOOP_F(js, je) o->cf1[FI(j, k, i)] = o->cf1[FI(j, k, i)] - cs1[FI(j, k, i)];
+  LOOP_P(js, je) s1[PI(j, z, x)] =
+      (u3[PI(j + 1, z, x)] * u2[PI(j + 1, z, x)] + u3[PI(j, z, x)] * u2[PI(j + 1, z, x)] -
+       u3[PI(j, z, x)] * u2[PI(j, z, x)] - u3[PI(j - 1, z, x)] * u2[PI(j, z, x)]) / (2.0 * dyf[j]);
+  to_fourier(o, s1, cs1);
+  LOOP_F(js, je) o->cf3[FI(j, k, i)] = o->cf3[FI(j, k, i)] - cs1[FI(j, k, i)];
+  LOOP_P(2, ny) {
+    const double ap = u2[PI(j, z, x)] + u2[PI(j + 1, z, x)];
+    const double am = u2[PI(j, z, x)] + u2[PI(j - 1, z, x)];
+    s1[PI(j, z, x)] = (0.25 * (ap * ap) - 0.25 * (am * am)) / dy[j];
+  }
+  to_fourier(o, s1, cs1);
+  LOOP_F(2, ny) o->cf2[FI(j, k, i)] = o->cf2[FI(j, k, i)] - cs1[FI(j, k, i)];
+
+  /* cr += temp5*cf (824-838), cr -> r (841-843) */
+  LOOP_F(js, je) {
+    o->cr1[FI(j, k, i)] = o->cr1[FI(j, k, i)] + temp5 * o->cf1[FI(j, k, i)];
+    o->cr3[FI(j, k, i)] = o->cr3[FI(j, k, i)] + temp5 * o->cf3[FI(j, k, i)];
+  }
+  LOOP_F(2, ny) o->cr2[FI(j, k, i)] = o->cr2[FI(j, k, i)] + temp5 * o->cf2[FI(j, k, i)];
+  to_physical(o, o->cr1, o->r1);
+  to_physical(o, o->cr2, o->r2);
+  to_physical(o, o->cr3, o->r3);
+  /* explicit half of CN (850-870) and eddy viscosity (875-903) */
+  const double nvs = c->nu_v_scale;
+  double *r1 = o->r1, *r2 = o->r2, *r3 = o->r3;
+  const double *nt = o->nu_t;
+  LOOP_P(js, je) {
+    const size_t p = PI(j, z, x), pu = PI(j + 1, z, x), pd = PI(j - 1, z, x);
+    r1[p] = r1[p] + temp1 * nvs * (((u1[pu] - u1[p]) / dy[j + 1] - (u1[p] - u1[pd]) / dy[j]) / dyf[j]);
+    r3[p] = r3[p] + temp1 * nvs * (((u3[pu] - u3[p]) / dy[j + 1] - (u3[p] - u3[pd]) / dy[j]) / dyf[j]);
+  }
+  LOOP_P(2, ny) {
+    const size_t p = PI(j, z, x), pu = PI(j + 1, z, x), pd = PI(j - 1, z, x);
+    r2[p] = r2[p] + temp1 * nvs * (((u2[pu] - u2[p]) / dyf[j] - (u2[p] - u2[pd]) / dyf[j - 1]) / dy[j]);
+  }
+  if (c->use_les) {
+    LOOP_P(js, je) {
+      const size_t p = PI(j, z, x), pu = PI(j + 1, z, x), pd = PI(j - 1, z, x);
+      r1[p] = r1[p] + temp2 * ((nt[pu] * (u1[pu] - u1[p]) / dy[j + 1] - nt[p] * (u1[p] - u1[pd]) / dy[j]) / dyf[j]);
+      r3[p] = r3[p] + temp2 * ((nt[pu] * (u3[pu] - u3[p]) / dy[j + 1] - nt[p] * (u3[p] - u3[pd]) / dy[j]) / dyf[j]);
+    }
+    LOOP_P(2, ny) {
+      const size_t p = PI(j, z, x), pu = PI(j + 1, z, x), pd = PI(j - 1, z, x);
+      r2[p] = r2[p] + temp2 * ((0.5 * (nt[p] + nt[pu]) * (u2[pu] - u2[p]) / dyf[j] -
+                                0.5 * (nt[p] + nt[pd]) * (u2[p] - u2[pd]) / dyf[j - 1]) / dy[j]);
+    }
+  }
+  /* scalars: products, RHS, CN, solve (909-1066) */
+  for (int n = 0; n < nth; n++) {
+    const int jst = c->jstart_th[n], jet = c->jend_th[n];
+    double *th = o->th + n * o->np, *rth = o->rth + n * o->np;
+    zc *cfth = o->cfth + n * o->nf, *crth = o->crth + n * o->nf;
+    const double *kt = o->kappa_t + n * o->np;
+    LOOP_P(jst, jet) s1[PI(j, z, x)] = th[PI(j, z, x)] * u1[PI(j, z, x)];
+    to_fourier(o, s1, cs1);
+    LOOP_F(jst, jet) cfth[FI(j, k, i)] = cfth[FI(j, k, i)] - CIKX(i) * cs1[FI(j, k, i)];
+    LOOP_P(jst, jet) s1[PI(j, z, x)] = th[PI(j, z, x)] * u3[PI(j, z, x)];
+    to_fourier(o, s1, cs1);
+    LOOP_F(jst, jet) cfth[FI(j, k, i)] = cfth[FI(j, k, i)] - CIKZ(k) * cs1[FI(j, k, i)];
+    LOOP_P(jst, jet) s1[PI(j, z, x)] =
+        (th[PI(j + 1, z, x)] * u2[PI(j + 1, z, x)] + th[PI(j, z, x)] * u2[PI(j + 1, z, x)] -
+         th[PI(j, z, x)] * u2[PI(j, z, x)] - th[PI(j - 1, z, x)] * u2[PI(j, z, x)]) / (2.0 * dyf[j]);
+    to_fourier(o, s1, cs1);
+    LOOP_F(jst, jet) cfth[FI(j, k, i)] = cfth[FI(j, k, i)] - cs1[FI(j, k, i)];
+    LOOP_F(jst, jet) crth[FI(j, k, i)] = crth[FI(j, k, i)] + temp5 * cfth[FI(j, k, i)];
+    to_physical(o, crth, rth);
+    LOOP_P(jst, jet) {
+      const size_t p = PI(j, z, x), pu = PI(j + 1, z, x), pd = PI(j - 1, z, x);
+      rth[p] = rth[p] + (temp1 / c->pr[n]) * nvs *
+                            (((th[pu] - th[p]) / dy[j + 1] - (th[p] - th[pd]) / dy[j]) / dyf[j]);
+    }
+    if (c->use_les) {
+      LOOP_P(jst, jet) {
+        const size_t p = PI(j, z, x), pu = PI(j + 1, z, x), pd = PI(j - 1, z, x);
+        rth[p] = rth[p] + temp2 * ((kt[pu] * (th[pu] - th[p]) / dy[j + 1] - kt[p] * (th[p] - th[pd]) / dy[j]) / dyf[j]);
+      }
+    }
+    cn_solve_phys(o, 3 + n, temp1, temp2, rth, th, jst - 1, jet + 1, c->use_les ? kt : NULL, time);
+  }
+  /* velocity solves: u2 (1081-1121), u1 (1136-1178), u3 (1192-1233) */
+  cn_solve_phys(o, 1, temp1, temp2, r2, o->u2, 1, ny + 1, c->use_les ? nt : NULL, time);
+  cn_solve_phys(o, 0, temp1, temp2, r1, o->u1, js - 1, je + 1, c->use_les ? nt : NULL, time);
+  cn_solve_phys(o, 2, temp1, temp2, r3, o->u3, js - 1, je + 1, c->use_les ? nt : NULL, time);
+  /* back to Fourier space (1247-1252), projection (1257), pressure (1261-1267) */
+  to_fourier(o, o->u1, o->cu1);
+  to_fourier(o, o->u2, o->cu2);
+  to_fourier(o, o->u3, o->cu3);
+  for (int n = 0; n < nth; n++) to_fourier(o, o->th + n * o->np, o->cth + n * o->nf);
+  rem_div(o);
+  LOOP_F(js, je) o->cp[FI(j, k, i)] = o->cp[FI(j, k, i)] + o->cr1[FI(j, k, i)] / temp4;
+#undef LOOP_P
+#undef LOOP_F
+#undef CIKX
+#undef CIKZ
+#undef KPOW
+  return 0;
+}

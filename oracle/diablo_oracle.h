@@ -1,0 +1,60 @@
+/*
+ * diablo_oracle.h — CPU FP64 restatement of DIABLO's RKW3 + Crank–Nicolson step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity checker for the HIP
+ * path and the timed CPU baseline.  It is never linked into, loaded by, or
+ * called from the product (diablo3_amd/, libdiablo_gpu.so).
+ *
+ * It takes the same dgpu_config as the GPU library and the same Fortran-layout
+ * buffers, so a test can feed identical inputs to both.
+ */
+#ifndef DIABLO_ORACLE_H
+#define DIABLO_ORACLE_H
+#include "../include/diablo_gpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct oracle oracle_t;
+
+int oracle_create(const dgpu_config *cfg, oracle_t **out);
+void oracle_destroy(oracle_t *o);
+void oracle_set_threads(int nthreads);
+const char *oracle_last_error(void);
+
+int oracle_upload_state(oracle_t *o, const double *cu1, const double *cu2,
+                        const double *cu3, const double *cp, const double *cth);
+int oracle_download_state(oracle_t *o, double *cu1, double *cu2, double *cu3,
+                          double *cp, double *cth);
+int oracle_upload_rk(oracle_t *o, const double *cf1, const double *cf2,
+                     const double *cf3, const double *cfth);
+int oracle_download_rk(oracle_t *o, double *cf1, double *cf2, double *cf3,
+                       double *cfth);
+/* nu_t and kappa_t in the physical Fortran layout (read-only view of the LES state) */
+int oracle_download_les(oracle_t *o, double *nu_t, double *kappa_t);
+
+int oracle_rk_substep(oracle_t *o, int rk_step, double delta_t, double time,
+                      int time_step, int create_new_flow);
+int oracle_rem_div(oracle_t *o);
+
+int oracle_fft_xz_to_fourier(oracle_t *o, const double *phys, double *four);
+int oracle_fft_xz_to_physical(oracle_t *o, const double *four, double *phys);
+
+/* Restated helpers, exposed so tests can pin them one by one. */
+void oracle_wavenumbers(int nx, int nz, double lx, double lz, double *kx, double *kz);
+void oracle_grid_metrics(int ny, const double *gy_file, double *gy, double *gyf,
+                         double *dy, double *dyf);
+/* Builds the BC-augmented CN system of one physical column for field `which`
+ * (0 = u1, 1 = u2, 2 = u3, 3+n = scalar n) at rk_step, with nu_t = 0;
+ * matl/matd/matu have ny+2 entries (rows 0..Nyp+1). */
+int oracle_cn_matrix(oracle_t *o, int which, int rk_step, double delta_t,
+                     double *matl, double *matd, double *matu, double *vec_bc);
+/* Thomas solves exactly as thomas_forward/backward_*_mpi at one rank. */
+void oracle_thomas_real(int ny, double *a, double *b, const double *c, double *g);
+void oracle_thomas_complex(int ny, double *a, double *b, const double *c, double *g);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
