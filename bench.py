@@ -1,0 +1,169 @@
+#!/usr/bin/env python3
+"""RKW3 + Crank–Nicolson step throughput on one MI355X.
+
+Workload: the plain-DNS infinite front at 256x129x256 (code x, y, z), FP64 by
+default, synthetic seeded initial condition, resident in HBM before timing starts.
+One "step" is the full RKW3 step (three rk_chan_1 substeps) through the C ABI.
+
+Printed JSON line: value = Nx*Ny*Nz / t_step in Mpoint-updates/s; roofline for the
+dominant kernel (algorithmic bytes per launch / its mean HIP-event duration over
+the timed region); cpu_baseline = the CPU oracle timed on this host's cores on a
+bounded sample (one full step of the same workload).
+"""
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "RKW3 step Mpoint-updates/s, %HBM peak; learned-vs-AMD LES speedup at fixed error"
+HBM_PEAK_GBS = 8000.0
+
+
+def kernel_bytes(cfg, mixed):
+    """Algorithmic HBM bytes per launch of each kernel (each array read or written once).
+
+    C = one packed Fourier field (complex double), H = one x-spectrum field
+    (complex T); nf = 3 + n_th state fields, no = 6 + 2 n_th product spectra,
+    nj = planes that carry products."""
+    nf, no = 3 + cfg.n_th, 6 + 2 * cfg.n_th
+    s = 4 if mixed else 8
+    ny2 = cfg.ny2
+    jlo = min([cfg.jstart, 2] + list(cfg.jstart_th[:cfg.n_th]))
+    nj = cfg.ny - jlo + 1
+    C = cfg.kxr * cfg.kzr * 16 * ny2
+    Cj = cfg.kxr * cfg.kzr * 16 * nj
+    H = cfg.kxr * cfg.nz * 2 * s * ny2
+    Hj = cfg.kxr * cfg.nz * 2 * s * nj
+    return {
+        "k_inv_z": nf * C + nf * H,
+        "k_xprod": nf * H + no * Hj,
+        # products in; state, cp and (on rk 2, 3: 2 of 3 launches) old cf in; cf, cr out
+        "k_fwdz_asm": no * Hj + (nf + 1) * C + (2.0 / 3.0) * nf * Cj + 2 * nf * Cj,
+        "k_columns": (nf + 1) * C + (nf + 1) * C,
+    }
+
+
+def model_v1_bytes_per_step(cfg, s):
+    """The step-level 'canonical algorithm model v1': 30R + 30H + 57C per DNS substep."""
+    yg = cfg.ny + 2
+    R = cfg.nx * cfg.nz * yg * s
+    H = cfg.kxr * cfg.nz * yg * 2 * s
+    C = cfg.kxr * cfg.kzr * yg * 2 * s
+    return 3 * (30 * R + 30 * H + 57 * C)
+
+
+def read_traffic(kernel, workload):
+    """HBM bytes per launch measured with rocprofv3 --pmc (profiles/*), or None."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_traffic.md")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(cfg, threads, steps=1):
+    import oracle
+    from diablo3_amd.problem import Problem
+    p = Problem(cfg.copy(precision=0))
+    o = oracle.Oracle(p, threads=threads)
+    o.upload_state(p.initial_state(seed=0, theta_noise=1e-3))
+    o.rem_div()
+    t0 = time.perf_counter()
+    for n in range(steps):
+        o.step(time=n * cfg.delta_t, time_step=n + 1)
+    dt = (time.perf_counter() - t0) / steps
+    return {"value": cfg.nx * cfg.ny * cfg.nz / dt / 1e6, "unit": "Mpoint-updates/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{steps} full RKW3 step(s) of the same {cfg.nx}x{cfg.ny}x{cfg.nz} FP64 workload, "
+                      f"C restatement of rk_chan_1 (physical-space CN, 23 transforms/substep), "
+                      f"OpenMP {threads} threads, {dt:.2f} s/step"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="dns_256", choices=["dns_256", "dns_64"])
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs on one GPU only (no collectives, no sharding); "
+              "use --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    from diablo3_amd import configs
+    from diablo3_amd.config import FP64, MIXED
+    from diablo3_amd.problem import Problem
+    from diablo3_amd.solver import Solver
+
+    mixed = args.precision == "mixed"
+    cfg = configs.NAMED[args.workload](MIXED if mixed else FP64)
+    p = Problem(cfg)
+    g = Solver(p)
+    g.upload_state(p.initial_state(seed=0, theta_noise=1e-3))
+    g.rem_div()
+    t = 0.0
+    for n in range(args.warmup):
+        g.step(time=t, time_step=n + 1)
+        t += cfg.delta_t
+    g.timing(True)
+    g.synchronize()
+    t0 = time.perf_counter()
+    for n in range(args.steps):
+        g.step(time=t, time_step=args.warmup + n + 1)
+        t += cfg.delta_t
+    g.synchronize()
+    t1 = time.perf_counter()
+    ktimes = g.timing_read()
+    g.timing(False)
+    ms = (t1 - t0) / args.steps * 1e3
+    points = cfg.nx * cfg.ny * cfg.nz
+    value = points / (ms * 1e-3) / 1e6
+
+    kb = kernel_bytes(cfg, mixed)
+    dom = max(ktimes, key=lambda k: ktimes[k][0])
+    tot, n = ktimes[dom]
+    avg_s = tot / n * 1e-3
+    achieved = kb[dom] / avg_s / 1e9
+    workload = f"{args.workload}_{args.precision}"
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(dom, workload),
+            "kernel": dom, "algorithmic_bytes_per_launch": int(kb[dom]),
+            "mean_launch_ms": round(avg_s * 1e3, 5)}
+    per_kernel = {k: {"ms_per_launch": round(v[0] / max(v[1], 1), 5), "launches": v[1],
+                      "GB/s": round(kb[k] / (v[0] / max(v[1], 1) * 1e-3) / 1e9, 1)}
+                  for k, v in ktimes.items()}
+    s = 4 if mixed else 8
+    step_model = model_v1_bytes_per_step(cfg, s)
+    out = {
+        "metric": METRIC, "value": round(value, 2), "unit": "Mpoint-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if not mixed else "f32+f64",
+        "data": "synthetic (seeded front IC, projected divergence-free)",
+        "config": {"workload": f"front DNS {cfg.nx}x{cfg.ny}x{cfg.nz} (x,y,z), "
+                               f"{'FP64' if not mixed else 'mixed FP32 transforms / FP64 solves'}, "
+                               f"dt {cfg.delta_t}, Re {cfg.re}",
+                   "nx": cfg.nx, "ny": cfg.ny, "nz": cfg.nz, "precision": args.precision},
+        "roofline": roof,
+        "step_hbm_frac_model_v1": round(step_model / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "kernels": per_kernel,
+    }
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        out["cpu_baseline"] = cpu_baseline(configs.NAMED[args.workload](FP64), threads)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,181 @@
+"""ctypes binding of libdiablo_gpu.so — the drop-in for ``call rk_chan_1``.
+
+The binding is thin on purpose: every array crosses the C ABI in the reference's
+Fortran layout (see include/diablo_gpu.h), exactly as the Fortran driver would
+pass it through iso_c_binding.  There is no fallback: if the library is missing or
+no GPU is present, construction raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import List, Optional
+
+import numpy as np
+
+from . import layout
+from .ic import FourierState
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libdiablo_gpu.so")
+HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "diablo_gpu.h")
+
+_lib = None
+
+
+class LibraryMissing(RuntimeError):
+    pass
+
+
+def load():
+    """Load libdiablo_gpu.so (raises LibraryMissing if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise LibraryMissing(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+    L = ctypes.CDLL(LIB_PATH)
+    P = ctypes.c_void_p
+    D = ctypes.POINTER(ctypes.c_double)
+    L.dgpu_create.argtypes = [P, ctypes.POINTER(P)]
+    L.dgpu_destroy.argtypes = [P]
+    for name in ("dgpu_upload_state", "dgpu_download_state"):
+        getattr(L, name).argtypes = [P, D, D, D, D, D]
+    for name in ("dgpu_upload_rk", "dgpu_download_rk"):
+        getattr(L, name).argtypes = [P, D, D, D, D]
+    L.dgpu_rk_substep.argtypes = [P, ctypes.c_int32, ctypes.c_double, ctypes.c_double,
+                                  ctypes.c_int32, ctypes.c_int32]
+    L.dgpu_step.argtypes = [P, ctypes.c_double, ctypes.c_double, ctypes.c_int32, ctypes.c_int32]
+    L.dgpu_rem_div.argtypes = [P]
+    L.dgpu_fft_xz_to_fourier.argtypes = [P, D, D]
+    L.dgpu_fft_xz_to_physical.argtypes = [P, D, D]
+    L.dgpu_synchronize.argtypes = [P]
+    L.dgpu_last_step_ms.argtypes = [P, D]
+    L.dgpu_last_error.restype = ctypes.c_char_p
+    L.dgpu_timing_enable.argtypes = [P, ctypes.c_int32]
+    L.dgpu_timing_read.argtypes = [P, ctypes.c_int32, D, ctypes.POINTER(ctypes.c_int64),
+                                   ctypes.POINTER(ctypes.c_char_p)]
+    L.dgpu_field_doubles.argtypes = [P]
+    L.dgpu_field_doubles.restype = ctypes.c_int64
+    _lib = L
+    return L
+
+
+def exported_symbols() -> List[str]:
+    """Names of the functions include/diablo_gpu.h declares."""
+    import re
+    txt = open(HEADER_PATH).read()
+    return sorted(set(re.findall(r"\b(dgpu_[a-z0-9_]+)\s*\(", txt)))
+
+
+def _dp(a: np.ndarray):
+    assert a.flags.c_contiguous
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Solver:
+    """One device-resident channel state; methods mirror the reference routines."""
+
+    def __init__(self, problem):
+        L = load()
+        self.p = problem
+        self.cfg = problem.cfg
+        h = ctypes.c_void_p()
+        rc = L.dgpu_create(ctypes.cast(problem.cconfig.ptr, ctypes.c_void_p), ctypes.byref(h))
+        if rc:
+            raise RuntimeError(f"dgpu_create failed ({rc}): {L.dgpu_last_error().decode()}")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().dgpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc:
+            raise RuntimeError(f"libdiablo_gpu error {rc}: {load().dgpu_last_error().decode()}")
+
+    def _fshape(self):
+        return layout.fortran_fourier_shape(self.cfg)
+
+    # ---- state ----
+    def upload_state(self, s: FourierState):
+        f = [layout.to_fortran_fourier(self.cfg, a) for a in (s.cu1, s.cu2, s.cu3, s.cp)]
+        th = (np.stack([layout.to_fortran_fourier(self.cfg, t) for t in s.cth]) if s.cth
+              else np.zeros((1,) + self._fshape(), np.complex128))
+        self._check(load().dgpu_upload_state(self.h, *[_dp(a.view(np.float64)) for a in f],
+                                             _dp(th.view(np.float64))))
+
+    def download_state(self) -> FourierState:
+        shp = self._fshape()
+        f = [np.zeros(shp, np.complex128) for _ in range(4)]
+        th = np.zeros((max(self.cfg.n_th, 1),) + shp, np.complex128)
+        self._check(load().dgpu_download_state(self.h, *[_dp(a.view(np.float64)) for a in f],
+                                               _dp(th.view(np.float64))))
+        c = [layout.from_fortran_fourier(self.cfg, a) for a in f]
+        return FourierState(c[0], c[1], c[2], c[3],
+                            [layout.from_fortran_fourier(self.cfg, th[n]) for n in range(self.cfg.n_th)])
+
+    def download_rk(self):
+        shp = self._fshape()
+        f = [np.zeros(shp, np.complex128) for _ in range(3)]
+        th = np.zeros((max(self.cfg.n_th, 1),) + shp, np.complex128)
+        self._check(load().dgpu_download_rk(self.h, *[_dp(a.view(np.float64)) for a in f],
+                                            _dp(th.view(np.float64))))
+        return ([layout.from_fortran_fourier(self.cfg, a) for a in f],
+                [layout.from_fortran_fourier(self.cfg, th[n]) for n in range(self.cfg.n_th)])
+
+    # ---- time advance ----
+    def rk_substep(self, rk_step: int, delta_t: Optional[float] = None, time: float = 0.0,
+                   time_step: int = 1, create_new_flow: bool = True):
+        dt = self.cfg.delta_t if delta_t is None else delta_t
+        self._check(load().dgpu_rk_substep(self.h, rk_step, dt, time, time_step, int(create_new_flow)))
+
+    def step(self, delta_t: Optional[float] = None, time: float = 0.0, time_step: int = 1,
+             create_new_flow: bool = True):
+        dt = self.cfg.delta_t if delta_t is None else delta_t
+        self._check(load().dgpu_step(self.h, dt, time, time_step, int(create_new_flow)))
+
+    def rem_div(self):
+        self._check(load().dgpu_rem_div(self.h))
+
+    def synchronize(self):
+        self._check(load().dgpu_synchronize(self.h))
+
+    def last_step_ms(self) -> float:
+        v = ctypes.c_double()
+        self._check(load().dgpu_last_step_ms(self.h, ctypes.byref(v)))
+        return v.value
+
+    # ---- measurement ----
+    def timing(self, enable: bool):
+        self._check(load().dgpu_timing_enable(self.h, int(enable)))
+
+    def timing_read(self):
+        """{kernel name: (total device ms, launches)} since timing(True)."""
+        out = {}
+        for k in range(4):
+            ms, n, name = ctypes.c_double(), ctypes.c_int64(), ctypes.c_char_p()
+            self._check(load().dgpu_timing_read(self.h, k, ctypes.byref(ms), ctypes.byref(n),
+                                                ctypes.byref(name)))
+            out[name.value.decode()] = (ms.value, n.value)
+        return out
+
+    # ---- transforms ----
+    def fft_to_fourier(self, phys_compact: np.ndarray) -> np.ndarray:
+        pf = layout.to_fortran_physical(self.cfg, phys_compact)
+        out = np.zeros(self._fshape(), np.complex128)
+        self._check(load().dgpu_fft_xz_to_fourier(self.h, _dp(pf), _dp(out.view(np.float64))))
+        return layout.from_fortran_fourier(self.cfg, out)
+
+    def fft_to_physical(self, four_compact: np.ndarray) -> np.ndarray:
+        ff = layout.to_fortran_fourier(self.cfg, four_compact)
+        out = np.zeros(layout.fortran_physical_shape(self.cfg), np.float64)
+        self._check(load().dgpu_fft_xz_to_physical(self.h, _dp(ff.view(np.float64)), _dp(out)))
+        return layout.from_fortran_physical(self.cfg, out)

@@ -32,6 +32,7 @@ def lib():
         L.oracle_create.argtypes = [P, ctypes.POINTER(P)]
         L.oracle_destroy.argtypes = [P]
         L.oracle_set_threads.argtypes = [ctypes.c_int]
+        L.oracle_set_variant.argtypes = [ctypes.c_int]
         L.oracle_last_error.restype = ctypes.c_char_p
         for name in ("oracle_upload_state", "oracle_download_state"):
             getattr(L, name).argtypes = [P, D, D, D, D, D]

@@ -40,6 +40,10 @@
 typedef double complex zc;
 
 static __thread char g_err[512];
+/* Arithmetic variant switch for round-off sensitivity studies: bit 0 makes the
+ * back substitutions multiply by 1/b instead of dividing by b.  0 = the reference. */
+static int g_variant = 0;
+void oracle_set_variant(int flags) { g_variant = flags; }
 static void set_err(const char *m) { snprintf(g_err, sizeof g_err, "%s", m); }
 const char *oracle_last_error(void) { return g_err; }
 
@@ -386,6 +390,11 @@ void oracle_thomas_real(int ny, double *a, double *b, const double *c, double *g
     b[j] = b[j] + a[j] * c[j - 1];
     g[j] = g[j] + a[j] * g[j - 1];
   }
+  if (g_variant & 1) {
+    g[ny + 1] = g[ny + 1] * (1.0 / b[ny + 1]);
+    for (int j = ny; j >= 0; j--) g[j] = (g[j] - c[j] * g[j + 1]) * (1.0 / b[j]);
+    return;
+  }
   g[ny + 1] = g[ny + 1] / b[ny + 1];
   for (int j = ny; j >= 0; j--) g[j] = (g[j] - c[j] * g[j + 1]) / b[j];
 }
@@ -396,6 +405,11 @@ static void thomas_complex(int ny, double *a, double *b, const double *c, zc *g)
     a[j] = -a[j] / b[j - 1];
     b[j] = b[j] + a[j] * c[j - 1];
     g[j] = g[j] + a[j] * g[j - 1];
+  }
+  if (g_variant & 1) {
+    g[ny] = g[ny] * (1.0 / b[ny]);
+    for (int j = ny - 1; j >= 1; j--) g[j] = (g[j] - c[j] * g[j + 1]) * (1.0 / b[j]);
+    return;
   }
   g[ny] = g[ny] / b[ny];
   for (int j = ny - 1; j >= 1; j--) g[j] = (g[j] - c[j] * g[j + 1]) / b[j];

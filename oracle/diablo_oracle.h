@@ -21,6 +21,8 @@ typedef struct oracle oracle_t;
 int oracle_create(const dgpu_config *cfg, oracle_t **out);
 void oracle_destroy(oracle_t *o);
 void oracle_set_threads(int nthreads);
+/* bit 0: back substitution multiplies by 1/b instead of dividing (round-off studies) */
+void oracle_set_variant(int flags);
 const char *oracle_last_error(void);
 
 int oracle_upload_state(oracle_t *o, const double *cu1, const double *cu2,
