@@ -39,12 +39,14 @@ def kernel_bytes(cfg, mixed):
     Cj = cfg.kxr * cfg.kzr * 16 * nj
     H = cfg.kxr * cfg.nz * 2 * s * ny2
     Hj = cfg.kxr * cfg.nz * 2 * s * nj
+    Cj_t = cfg.kxr * cfg.kzr * 2 * s * nj   # nonlinear terms, complex T
     return {
         "k_inv_z": nf * C + nf * H,
         "k_xprod": nf * H + no * Hj,
-        # products in; state, cp and (on rk 2, 3: 2 of 3 launches) old cf in; cf, cr out
-        "k_fwdz_asm": no * Hj + (nf + 1) * C + (2.0 / 3.0) * nf * Cj + 2 * nf * Cj,
-        "k_columns": (nf + 1) * C + (nf + 1) * C,
+        "k_fwdz_nl": no * Hj + nf * Cj_t,
+        # nonlinear terms, state, cp and (rk 2, 3: 2 of 3 launches) old cf in; cf and state out
+        "k_cn": nf * Cj_t + (nf + 1) * C + (2.0 / 3.0) * nf * Cj + nf * Cj + nf * C,
+        "k_project": 4 * C + 4 * C,
     }
 
 

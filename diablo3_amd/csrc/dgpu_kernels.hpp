@@ -1,11 +1,13 @@
 // dgpu_kernels.hpp — HIP kernels of the RKW3 + CN channel step (gfx950).
 //
 // Device layouts (one rank, Nyp = ny):
-//   Fourier  C[j][k][i]  complex double, i = 0..Nkx (kxr), k = 0..2Nkz (kzr),
-//                        j = 0..ny+1; planes are contiguous (pc = kxr*kzr).
+//   Fourier  C[j][k][i]  complex double, i = 0..Nkx active (kxr of them) in rows of
+//                        pitch kxp = kxr rounded up to 8 (128-byte rows for complex
+//                        double), k = 0..2Nkz (kzr), j = 0..ny+1; pc = kxp*kzr.
 //   x-spectra H[j][z][i] complex T: the x transform of physical line (j, z)
-//                        restricted to i = 0..Nkx (ph = nz*kxr).
+//                        restricted to i = 0..Nkx, same pitch kxp (ph = nz*kxp).
 //   physical R[j][z][x]  T (pp = nz*nx).
+// Padding columns i = kxr..kxp-1 are zero and never read as modes.
 // Every kernel indexes only inside these extents; grids are sized by the host to
 // the same extents (the host checks them before launch).
 #pragma once
@@ -18,7 +20,7 @@ constexpr int MAXF = 3 + MAXTH;         // u1, u2, u3, th_1..th_n
 constexpr int MAXO = 6 + 2 * MAXTH;     // fused product spectra
 
 struct Geo {
-  int nx, nz, ny, ny2, nkx, nkz, kxr, kzr;
+  int nx, nz, ny, ny2, nkx, nkz, kxr, kzr, kxp;
   long long pc, ph, pp;
 };
 
@@ -47,7 +49,7 @@ __global__ __launch_bounds__(NT) void k_inv_z(Ptrs<const cd*, MAXO> src, Ptrs<cx
   for (int idx = tid; idx < TI * NZ; idx += NT) {
     const int ii = idx % TI, z = idx / TI, i = i0 + ii, p = zpack(g, z);
     cx<T> v = {T(0), T(0)};
-    if (i < g.kxr && p >= 0) v = cvt<T>(C[(long long)p * g.kxr + i]);
+    if (i < g.kxr && p >= 0) v = cvt<T>(C[(long long)p * g.kxp + i]);
     s[ii * LD + z] = v;
   }
   __syncthreads();
@@ -55,7 +57,7 @@ __global__ __launch_bounds__(NT) void k_inv_z(Ptrs<const cd*, MAXO> src, Ptrs<cx
   cx<T>* __restrict__ H = dst.p[f] + (long long)j * g.ph;
   for (int idx = tid; idx < TI * NZ; idx += NT) {
     const int ii = idx % TI, z = idx / TI, i = i0 + ii;
-    if (i < g.kxr) H[(long long)z * g.kxr + i] = s[ii * LD + z];
+    if (i < g.kxr) H[(long long)z * g.kxp + i] = s[ii * LD + z];
   }
 }
 
@@ -68,7 +70,7 @@ __global__ __launch_bounds__(NT) void k_fwd_z(Ptrs<const cx<T>*, MAXO> src, Ptrs
   const cx<T>* __restrict__ H = src.p[f] + (long long)j * g.ph;
   for (int idx = tid; idx < TI * NZ; idx += NT) {
     const int ii = idx % TI, z = idx / TI, i = i0 + ii;
-    s[ii * LD + z] = (i < g.kxr) ? H[(long long)z * g.kxr + i] : cx<T>{T(0), T(0)};
+    s[ii * LD + z] = (i < g.kxr) ? H[(long long)z * g.kxp + i] : cx<T>{T(0), T(0)};
   }
   __syncthreads();
   fft_lds<T, NZ, TI, NT, -1>(s, LD, tw, tid);
@@ -78,7 +80,7 @@ __global__ __launch_bounds__(NT) void k_fwd_z(Ptrs<const cx<T>*, MAXO> src, Ptrs
     const int ii = idx % TI, p = idx / TI, i = i0 + ii;
     if (i < g.kxr) {
       cd v = cvt<double>(s[ii * LD + zpos(g, p)]);
-      C[(long long)p * g.kxr + i] = rnz * v;
+      C[(long long)p * g.kxp + i] = rnz * v;
     }
   }
 }
@@ -122,7 +124,7 @@ __global__ __launch_bounds__(NT) void k_inv_x(Ptrs<const cx<T>*, MAXO> src, Ptrs
   const cx<T>* __restrict__ H = src.p[f] + (long long)j * g.ph;
   for (int idx = tid; idx < ZP * NX; idx += NT) {
     const int q = idx / NX, n = idx % NX, z = zb + 2 * q;
-    s[idx] = pair_spectrum(H + (long long)z * g.kxr, H + (long long)(z + 1) * g.kxr, n, NX, g.nkx);
+    s[idx] = pair_spectrum(H + (long long)z * g.kxp, H + (long long)(z + 1) * g.kxp, n, NX, g.nkx);
   }
   __syncthreads();
   fft_lds<T, NX, ZP, NT, +1>(s, NX, tw, tid);
@@ -153,8 +155,8 @@ __global__ __launch_bounds__(NT) void k_fwd_x(Ptrs<const T*, MAXO> src, Ptrs<cx<
     const int q = idx / g.kxr, i = idx % g.kxr, z = zb + 2 * q;
     cx<T> A, B;
     pair_split(s + q * NX, i, NX, A, B);
-    H[(long long)z * g.kxr + i] = rnx * A;
-    H[(long long)(z + 1) * g.kxr + i] = rnx * B;
+    H[(long long)z * g.kxp + i] = rnx * A;
+    H[(long long)(z + 1) * g.kxp + i] = rnx * B;
   }
 }
 
@@ -193,15 +195,14 @@ __global__ __launch_bounds__(NT) void k_xprod(XProdArgs a) {
   for (int idx = tid; idx < NL * NX; idx += NT) {
     const int l = idx / NX, n = idx % NX, f = l / 3, jj = j - 1 + (l - 3 * f);
     const cx<T>* H = (const cx<T>*)a.hin[f] + (long long)jj * g.ph;
-    s[idx] = pair_spectrum(H + (long long)z * g.kxr, H + (long long)(z + 1) * g.kxr, n, NX, g.nkx);
+    s[idx] = pair_spectrum(H + (long long)z * g.kxp, H + (long long)(z + 1) * g.kxp, n, NX, g.nkx);
   }
   __syncthreads();
   fft_lds<T, NX, NL, NT, +1>(s, NX, tw, tid);
   const bool vrow = (j >= a.jstart && j <= a.jend);
   const bool v2row = (j >= 2 && j <= g.ny);
-  const T dyj = (T)a.dy[j], dyj1 = (T)a.dy[j + 1 <= g.ny + 1 ? j + 1 : j];
+  const T dyj = (T)a.dy[j];
   const T dyfj = (T)a.dyf[j], dyfm = (T)a.dyf[j - 1];
-  (void)dyj1;
   const T cint = T(1) / (T(2) * dyj);  // interpolation to GY(j): (dyf_j u_j + dyf_{j-1} u_{j-1})/(2 dy_j)
   for (int x = tid; x < NX; x += NT) {
     cx<T> U1[3], U2[3], U3[3];
@@ -301,351 +302,433 @@ __global__ __launch_bounds__(NT) void k_xprod(XProdArgs a) {
       rb = B0;
     }
     cx<T>* H = (cx<T>*)a.hout[o] + (long long)j * g.ph;
-    H[(long long)z * g.kxr + i] = rnx * ra;
-    H[(long long)(z + 1) * g.kxr + i] = rnx * rb;
+    H[(long long)z * g.kxp + i] = rnx * ra;
+    H[(long long)(z + 1) * g.kxp + i] = rnx * rb;
   }
 }
 
 // ---------------------------------------------------------------------------
-// Fused z pass + Fourier-space assembly of rk_chan_1 (for/solver.f90:385-838,
-// 909-973 and user_rhs_chan_fourier 35-111) for one plane j and a tile of kx.
-// Writes the new RK memory cf and the full right-hand side cr including the
-// explicit half of Crank–Nicolson (for/solver.f90:850-870, 981-1002).
+// z forward transform of the fused product spectra: the nonlinear tendency of
+// every equation, NL_e = Fz(N_e) - ikz Fz(Z_e), scaled by 1/Nz, on the rows the
+// equation owns (u1, u3: jstart..jend; u2: 2..Nyp; th_n: jstart_th..jend_th).
 // ---------------------------------------------------------------------------
-struct AsmArgs {
+struct NlArgs {
   Geo g;
-  int jlo, rk;
+  int jlo;
+  int jstart, jend, jstart_th[MAXTH], jend_th[MAXTH];
+  const double* __restrict__ kz;
+  const void* h[MAXO];
+  void* nl[MAXF];
+  const void* tw;
+};
+
+__device__ __forceinline__ void eq_rows(int e, int jstart, int jend, const int* jst, const int* jet,
+                                        int ny, int& lo, int& hi) {
+  if (e == 0 || e == 2) { lo = jstart; hi = jend; }
+  else if (e == 1) { lo = 2; hi = ny; }
+  else { lo = jst[e - 3]; hi = jet[e - 3]; }
+}
+
+template <typename T, int NZ, int TI, int NT>
+__global__ __launch_bounds__(NT) void k_fwdz_nl(NlArgs a) {
+  constexpr int LD = NZ + 1;
+  __shared__ cx<T> s[2 * TI * LD];
+  const Geo& g = a.g;
+  const int tid = threadIdx.x, j = a.jlo + blockIdx.y, i0 = blockIdx.x * TI, e = blockIdx.z;
+  const cx<T>* __restrict__ tw = (const cx<T>*)a.tw;
+  const T rnz = T(1) / T(NZ);
+  int lo, hi;
+  eq_rows(e, a.jstart, a.jend, a.jstart_th, a.jend_th, g.ny, lo, hi);
+  if (j < lo || j > hi) return;  // uniform over the block
+  const int oN = e < 3 ? e : 6 + 2 * (e - 3);
+  const int oZ = e < 3 ? 3 + e : 7 + 2 * (e - 3);
+  const cx<T>* HN = (const cx<T>*)a.h[oN] + (long long)j * g.ph;
+  const cx<T>* HZ = (const cx<T>*)a.h[oZ] + (long long)j * g.ph;
+  for (int idx = tid; idx < 2 * TI * NZ; idx += NT) {
+    const int ii = idx % TI, r = idx / TI, which = r / NZ, z = r - which * NZ, i = i0 + ii;
+    const cx<T>* Hs = which ? HZ : HN;
+    s[(which * TI + ii) * LD + z] = (i < g.kxr) ? Hs[(long long)z * g.kxp + i] : cx<T>{T(0), T(0)};
+  }
+  __syncthreads();
+  fft_lds<T, NZ, 2 * TI, NT, -1>(s, LD, tw, tid);
+  cx<T>* NL = (cx<T>*)a.nl[e] + (long long)j * g.pc;
+  for (int idx = tid; idx < TI * g.kzr; idx += NT) {
+    const int ii = idx % TI, k = idx / TI, i = i0 + ii;
+    if (i >= g.kxr) continue;
+    const int zp = zpos(g, k);
+    const cx<T> fn = s[ii * LD + zp], fz = s[(TI + ii) * LD + zp];
+    NL[(long long)k * g.kxp + i] = rnz * (fn - ik((T)a.kz[k], fz));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Column kernel.  Per (kx, kz) column and all rows j = 0..Nyp+1:
+//  1. Fourier-space assembly of rk_chan_1 (for/solver.f90:385-838, 909-973, and
+//     user_rhs_chan_fourier 35-111): new RK memory cf and the right-hand side cr,
+//     including the explicit half of Crank–Nicolson (for/solver.f90:850-870,
+//     981-1002) and the BC rows of apply_BC_* (for/solver.f90:1901-2232);
+//  2. the Crank–Nicolson solves of every scalar, then u2, u1, u3
+//     (for/solver.f90:1004-1233), in Fourier space (the DNS operator does not
+//     depend on x or z), by parallel cyclic reduction with coefficients that the
+//     host precomputes from the reference's matrices;
+//  3. rem_div_chan (for/solver.f90:1302-1379, BC rows 2330-2371) by parallel cyclic
+//     reduction, then the velocity correction and the pressure update
+//     (for/solver.f90:1261-1267).
+// ---------------------------------------------------------------------------
+constexpr int PCR_LMAX = 10;  // up to 1024 rows
+
+struct SolveRows {
+  int blo, bhi;      // build range: rows whose cr comes from the assembly
+  int wlo, whi;      // rows written back
+  int kind[5];       // rows 0, 1, 2, ny, ny+1: -1 = build-range rule, 0 = zero, 1 = BC value
+  double val[5];     // BC value of the mean mode at those rows
+};
+
+// Per-row coefficients of the assembly, computed once on the host from the grid
+// (the reference evaluates them inline; the host table holds the same quotients).
+enum RowCoef {
+  RC_RDY = 0,    // 1/dy(j)
+  RC_RDYP,       // 1/dy(j+1)
+  RC_RDYF,       // 1/dyf(j)
+  RC_RDYFM,      // 1/dyf(j-1)
+  RC_WTH,        // dyf(j-1)/(2 dy(j)): weight of th(j) in the GY interpolation
+  RC_WTHM,       // dyf(j)/(2 dy(j)):   weight of th(j-1)
+  RC_TW0,        // per scalar n, 4 entries at RC_TW0 + 4n:
+  RC_COUNT = RC_TW0 + 4 * MAXTH
+  // +0 dthdx (gyf-Ly/2) delta/Ro_inv, +1 -dthdz (gyf-Ly/2) delta/Ro_inv,
+  // +2 dthdx (gy-Ly/2) delta/Ro_inv,  +3 -dthdz (gy-Ly/2) delta/Ro_inv
+};
+
+struct ColArgs {
+  Geo g;
+  int rk, do_cn, levels, plevels;
   int jstart, jend, jstart_th[MAXTH], jend_th[MAXTH];
   int front, f_type;
   double nu, beta, nu_v_scale, ro_inv, delta, grav_x, grav_y, grav_z, ly, px0, omega0, amp_omega0, time;
   double pr[MAXTH], dthdx[MAXTH], dthdz[MAXTH];
   double temp1, temp3, temp4, temp5;
+  // host-precomputed scalars: Ro_inv/delta; per scalar dthdx delta/Ro_inv,
+  // -dthdz delta/Ro_inv, nu/Pr, temp1 nu_v_scale/Pr; temp1 nu_v_scale
+  double cori, twu_x[MAXTH], twu_z[MAXTH], nupr[MAXTH], t1pr[MAXTH], t1v;
+  const double* __restrict__ rowc;  // [ny2][RC_COUNT]
   const double *__restrict__ dy, *__restrict__ dyf, *__restrict__ gy, *__restrict__ gyf;
   const double *__restrict__ kx, *__restrict__ kz;
-  const cd* u[MAXF];     // cu1, cu2, cu3, cth_n (state at the start of the substep)
-  const cd* p;           // cp
-  cd* cf[MAXF];          // RK memory (read when rk > 1, then overwritten)
-  cd* cr[MAXF];          // right-hand side
-  const void* h[MAXO];   // product x-spectra from k_xprod
-  const void* tw;
-};
-
-template <typename T, int NZ, int TI, int NT, int NTH>
-__global__ __launch_bounds__(NT) void k_fwdz_asm(AsmArgs a) {
-  constexpr int LD = NZ + 1;
-  __shared__ cx<T> s[2 * TI * LD];
-  const Geo& g = a.g;
-  const int tid = threadIdx.x, j = a.jlo + blockIdx.y, i0 = blockIdx.x * TI;
-  const cx<T>* __restrict__ tw = (const cx<T>*)a.tw;
-  const long long pc = g.pc;
-  const double rnz = 1.0 / NZ;
-  const double dyj = a.dy[j], dyfj = a.dyf[j];
-  const double cori = a.ro_inv / a.delta;
-  constexpr int NE = 3 + NTH;
-  for (int e = 0; e < NE; ++e) {
-    // e: 0 = u1, 1 = u2, 2 = u3, 3+n = th_n ; product outputs per equation
-    int oN, oZ;
-    bool valid;
-    if (e == 0) { oN = 0; oZ = 3; valid = (j >= a.jstart && j <= a.jend); }
-    else if (e == 1) { oN = 1; oZ = 4; valid = (j >= 2 && j <= g.ny); }
-    else if (e == 2) { oN = 2; oZ = 5; valid = (j >= a.jstart && j <= a.jend); }
-    else {
-      const int n = e - 3;
-      oN = 6 + 2 * n; oZ = 7 + 2 * n;
-      valid = (j >= a.jstart_th[n] && j <= a.jend_th[n]);
-    }
-    if (!valid) continue;  // uniform over the block
-    const cx<T>* HN = (const cx<T>*)a.h[oN] + (long long)j * g.ph;
-    const cx<T>* HZ = (const cx<T>*)a.h[oZ] + (long long)j * g.ph;
-    for (int idx = tid; idx < 2 * TI * NZ; idx += NT) {
-      const int ii = idx % TI, r = idx / TI, which = r / NZ, z = r - which * NZ, i = i0 + ii;
-      const cx<T>* Hs = which ? HZ : HN;
-      s[(which * TI + ii) * LD + z] = (i < g.kxr) ? Hs[(long long)z * g.kxr + i] : cx<T>{T(0), T(0)};
-    }
-    __syncthreads();
-    fft_lds<T, NZ, 2 * TI, NT, -1>(s, LD, tw, tid);
-    for (int idx = tid; idx < TI * g.kzr; idx += NT) {
-      const int ii = idx % TI, k = idx / TI, i = i0 + ii;
-      if (i >= g.kxr) continue;
-      const int zp = zpos(g, k);
-      const cd fn = cvt<double>(s[ii * LD + zp]), fz = cvt<double>(s[(TI + ii) * LD + zp]);
-      const double kxi = a.kx[i], kzk = a.kz[k];
-      const cd nl = rnz * (fn - ik(kzk, fz));
-      const double k2 = a.kx[i] * a.kx[i] + a.kz[k] * a.kz[k];
-      const double kk = (a.beta == 1.0) ? k2 : pow(k2, a.beta);
-      const long long o = (long long)j * pc + (long long)k * g.kxr + i;
-      const long long om = o - pc, op = o + pc;
-      cd cf, cr;
-      if (e == 0 || e == 2) {
-        const cd* U = a.u[e];
-        const cd* V = a.u[2 - e];  // Coriolis partner
-        const cd uc = U[o];
-        cf = -1.0 * ((a.nu * kk) * uc);
-        if (e == 0) cf = cf + cori * V[o];
-        else cf = cf - cori * V[o];
-        for (int n = 0; n < NTH; ++n) cf = cf + (e == 0 ? a.grav_x : a.grav_z) * a.u[3 + n][o];
-        if (a.front && a.ro_inv != 0.0) {
-          const cd u2m = 0.5 * (a.u[1][o] + a.u[1][op]);
-          const double yo = a.gyf[j] - 0.5 * a.ly;
-          for (int n = 0; n < NTH; ++n) {
-            cf = cf - (a.dthdx[n] * yo * a.delta / a.ro_inv) * ik(kzk, uc) -
-                 (-1.0 * a.dthdz[n] * yo * a.delta / a.ro_inv) * ik(kxi, uc);
-            if (e == 0) cf = cf - (-1.0 * a.dthdz[n] * a.delta / a.ro_inv) * u2m;
-            else cf = cf - (a.dthdx[n] * a.delta / a.ro_inv) * u2m;
-          }
-        }
-        cf = cf + nl;
-        cr = uc;
-        if (a.rk > 1) cr = cr + a.temp3 * a.cf[e][o];
-        cr = cr - a.temp4 * (e == 0 ? ik(kxi, a.p[o]) : ik(kzk, a.p[o]));
-        if (e == 0 && k == 0 && i == 0) {
-          if (a.f_type == 1) cr.x = cr.x - a.temp4 * a.px0;
-          else if (a.f_type == 2) cr.x = cr.x - a.temp4 * (a.px0 + a.amp_omega0 * cos(a.omega0 * a.time));
-        }
-        cr = cr + a.temp5 * cf;
-        const cd up = U[op], um = U[om];
-        const double dyp = a.dy[j + 1];
-        cr = cr + (a.temp1 * a.nu_v_scale) * ((1.0 / dyfj) * ((1.0 / dyp) * (up - uc) - (1.0 / dyj) * (uc - um)));
-      } else if (e == 1) {
-        const cd* U = a.u[1];
-        const cd uc = U[o];
-        cf = -1.0 * ((a.nu * kk) * uc);
-        if (!(k == 0 && i == 0)) {
-          for (int n = 0; n < NTH; ++n) {
-            const cd* TH = a.u[3 + n];
-            cf = cf + a.grav_y * ((1.0 / (2.0 * dyj)) * (a.dyf[j - 1] * TH[o] + dyfj * TH[om]));
-          }
-        }
-        if (a.front && a.ro_inv != 0.0) {
-          const double yo = a.gy[j] - 0.5 * a.ly;
-          for (int n = 0; n < NTH; ++n)
-            cf = cf - (a.dthdx[n] * yo * a.delta / a.ro_inv) * ik(kzk, uc) -
-                 (-1.0 * a.dthdz[n] * yo * a.delta / a.ro_inv) * ik(kxi, uc);
-        }
-        cf = cf + nl;
-        cr = uc;
-        if (a.rk > 1) cr = cr + a.temp3 * a.cf[1][o];
-        cr = cr - a.temp4 * ((1.0 / dyj) * (a.p[o] - a.p[om]));
-        cr = cr + a.temp5 * cf;
-        const cd up = U[op], um = U[om];
-        cr = cr + (a.temp1 * a.nu_v_scale) *
-                      ((1.0 / dyj) * ((1.0 / dyfj) * (up - uc) - (1.0 / a.dyf[j - 1]) * (uc - um)));
-      } else {
-        const int n = e - 3;
-        const cd* TH = a.u[e];
-        const cd tc = TH[o];
-        cf = -1.0 * (((a.nu / a.pr[n]) * kk) * tc);
-        cf = cf - a.dthdx[n] * a.u[0][o] - a.dthdz[n] * a.u[2][o];
-        if (a.front && a.ro_inv != 0.0) {
-          const double yo = a.gyf[j] - 0.5 * a.ly;
-          cf = cf - ((a.delta / a.ro_inv) * a.dthdx[n] * yo) * ik(kzk, tc) -
-               ((a.delta / a.ro_inv) * (-1.0) * a.dthdz[n] * yo) * ik(kxi, tc);
-        }
-        cf = cf + nl;
-        cr = tc;
-        if (a.rk > 1) cr = cr + a.temp3 * a.cf[e][o];
-        cr = cr + a.temp5 * cf;
-        const cd up = TH[op], um = TH[om];
-        const double dyp = a.dy[j + 1];
-        cr = cr + ((a.temp1 / a.pr[n]) * a.nu_v_scale) *
-                      ((1.0 / dyfj) * ((1.0 / dyp) * (up - tc) - (1.0 / dyj) * (tc - um)));
-      }
-      a.cf[e][o] = cf;
-      a.cr[e][o] = cr;
-    }
-    __syncthreads();
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Column kernel: per (kx, kz) column, the Crank–Nicolson solves of every scalar
-// and of u2, u1, u3 (for/solver.f90:1004-1233, in Fourier space: the DNS
-// operator does not depend on x or z), then rem_div_chan (for/solver.f90:
-// 1302-1379) and the pressure update (for/solver.f90:1261-1267).
-// ---------------------------------------------------------------------------
-struct SolveRows {
-  int blo, bhi;      // rows filled from cr (the build range)
-  int wlo, whi;      // rows written back
-  int kind[5];       // rows 0, 1, 2, ny, ny+1: -1 = from the build range rule, 0 = zero, 1 = BC value
-  double val[5];     // BC value of the mean mode at those rows
-};
-
-struct ColArgs {
-  Geo g;
-  int ncols;
-  int jstart, jend;
-  double temp4;
-  SolveRows rows[MAXF];            // 0 = u1, 1 = u2, 2 = u3, 3+n = th_n
-  const double* __restrict__ fac;  // [field][3][ny2]: l, c, 1/b of the LU factors
-  const double *__restrict__ dy, *__restrict__ dyf, *__restrict__ kx, *__restrict__ kz;
+  SolveRows rows[MAXF];
+  const double* __restrict__ pcr;  // [field][2*PCR_LMAX+1][ny2]: alpha_l, gamma_l, 1/b_final
   cd* u[MAXF];
   cd* p;
-  const cd* cr[MAXF];
-  int do_cn;                       // 0: projection only (rem_div_chan)
+  cd* cf[MAXF];
+  const void* nl[MAXF];
 };
+
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2).
+// The grid is padded to a multiple of 8 and block b takes tile
+// (b % 8) * (grid / 8) + b / 8, so that every XCD works on one contiguous range of
+// tiles and neighbouring tiles, which share cache lines, meet in one L2.
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+  const int per = (int)gridDim.x / 8;
+  const int t = (b % 8) * per + b / 8;
+  return t < nb ? t : -1;
+}
 
 __device__ __forceinline__ int special_row(int j, int ny) {
   return j <= 2 ? j : (j == ny ? 3 : (j == ny + 1 ? 4 : -1));
 }
 
-template <int TC, int NTH>
-__global__ __launch_bounds__(64) void k_columns(ColArgs a) {
+// Inputs of the assembly at one (row j, column) point.  Loaded before any store
+// so that the loads of a thread's rows are independent of its cf stores.
+template <int NTH>
+struct RowIn {
+  cd u1m, u1c, u1p, u2m, u2c, u2p, u3m, u3c, u3p, pm, pc_;
+  cd thm[NTH > 0 ? NTH : 1], thc[NTH > 0 ? NTH : 1], thp[NTH > 0 ? NTH : 1];
+  cd cfo[3 + NTH], nl[3 + NTH];
+};
+
+template <typename T, int NTH>
+__device__ __forceinline__ void load_row(RowIn<NTH>& r, const ColArgs& a, long long o, long long pc,
+                                         const cd* __restrict__ u1, const cd* __restrict__ u2,
+                                         const cd* __restrict__ u3, const cd* __restrict__ t0,
+                                         const cd* __restrict__ t1, const cd* __restrict__ p) {
+  const long long om = o - pc, op = o + pc;
+  r.u1m = u1[om]; r.u1c = u1[o]; r.u1p = u1[op];
+  r.u2m = u2[om]; r.u2c = u2[o]; r.u2p = u2[op];
+  r.u3m = u3[om]; r.u3c = u3[o]; r.u3p = u3[op];
+  r.pm = p[om]; r.pc_ = p[o];
+  if constexpr (NTH >= 1) { r.thm[0] = t0[om]; r.thc[0] = t0[o]; r.thp[0] = t0[op]; }
+  if constexpr (NTH >= 2) { r.thm[1] = t1[om]; r.thc[1] = t1[o]; r.thp[1] = t1[op]; }
+#pragma unroll
+  for (int e = 0; e < 3 + NTH; ++e) {
+    r.cfo[e] = (a.rk > 1) ? ((const cd* __restrict__)a.cf[e])[o] : cd{0.0, 0.0};
+    r.nl[e] = cvt<double>(((const cx<T>* __restrict__)a.nl[e])[o]);
+  }
+}
+
+// cf and cr of equation e at row j, column (i, k), from the loaded row inputs;
+// kk = (kx^2 + kz^2)^beta of the column, rc the row's coefficient table
+template <int NTH>
+__device__ __forceinline__ void assemble(const ColArgs& a, const RowIn<NTH>& r, const double* __restrict__ rc,
+                                         int e, int i, int k, double kxi, double kzk, double kk,
+                                         cd& cf, cd& cr) {
+  const bool tws = a.front && a.ro_inv != 0.0;
+  if (e == 0 || e == 2) {
+    const cd uc = (e == 0) ? r.u1c : r.u3c;
+    const cd up = (e == 0) ? r.u1p : r.u3p;
+    const cd um = (e == 0) ? r.u1m : r.u3m;
+    cf = -1.0 * ((a.nu * kk) * uc);
+    if (e == 0) cf = cf + a.cori * r.u3c;
+    else cf = cf - a.cori * r.u1c;
+    for (int n = 0; n < NTH; ++n) cf = cf + (e == 0 ? a.grav_x : a.grav_z) * r.thc[n];
+    if (tws) {
+      const cd u2m = 0.5 * (r.u2c + r.u2p);
+      for (int n = 0; n < NTH; ++n) {
+        cf = cf - rc[RC_TW0 + 4 * n] * ik(kzk, uc) - rc[RC_TW0 + 4 * n + 1] * ik(kxi, uc);
+        if (e == 0) cf = cf - a.twu_z[n] * u2m;
+        else cf = cf - a.twu_x[n] * u2m;
+      }
+    }
+    cf = cf + r.nl[e];
+    cr = uc;
+    if (a.rk > 1) cr = cr + a.temp3 * r.cfo[e];
+    cr = cr - a.temp4 * (e == 0 ? ik(kxi, r.pc_) : ik(kzk, r.pc_));
+    if (e == 0 && k == 0 && i == 0) {
+      if (a.f_type == 1) cr.x = cr.x - a.temp4 * a.px0;
+      else if (a.f_type == 2) cr.x = cr.x - a.temp4 * (a.px0 + a.amp_omega0 * cos(a.omega0 * a.time));
+    }
+    cr = cr + a.temp5 * cf;
+    cr = cr + a.t1v * (rc[RC_RDYF] * (rc[RC_RDYP] * (up - uc) - rc[RC_RDY] * (uc - um)));
+  } else if (e == 1) {
+    const cd uc = r.u2c;
+    cf = -1.0 * ((a.nu * kk) * uc);
+    if (!(k == 0 && i == 0)) {
+      for (int n = 0; n < NTH; ++n) cf = cf + a.grav_y * (rc[RC_WTH] * r.thc[n] + rc[RC_WTHM] * r.thm[n]);
+    }
+    if (tws) {
+      for (int n = 0; n < NTH; ++n)
+        cf = cf - rc[RC_TW0 + 4 * n + 2] * ik(kzk, uc) - rc[RC_TW0 + 4 * n + 3] * ik(kxi, uc);
+    }
+    cf = cf + r.nl[1];
+    cr = uc;
+    if (a.rk > 1) cr = cr + a.temp3 * r.cfo[1];
+    cr = cr - a.temp4 * (rc[RC_RDY] * (r.pc_ - r.pm));
+    cr = cr + a.temp5 * cf;
+    cr = cr + a.t1v * (rc[RC_RDY] * (rc[RC_RDYF] * (r.u2p - uc) - rc[RC_RDYFM] * (uc - r.u2m)));
+  } else {
+    const int n = e - 3;
+    const cd tc = r.thc[n];
+    cf = -1.0 * ((a.nupr[n] * kk) * tc);
+    cf = cf - a.dthdx[n] * r.u1c - a.dthdz[n] * r.u3c;
+    if (tws) cf = cf - rc[RC_TW0 + 4 * n] * ik(kzk, tc) - rc[RC_TW0 + 4 * n + 1] * ik(kxi, tc);
+    cf = cf + r.nl[e];
+    cr = tc;
+    if (a.rk > 1) cr = cr + a.temp3 * r.cfo[e];
+    cr = cr + a.temp5 * cf;
+    cr = cr + a.t1pr[n] * (rc[RC_RDYF] * (rc[RC_RDYP] * (r.thp[n] - tc) - rc[RC_RDY] * (tc - r.thm[n])));
+  }
+}
+
+// Assembly + Crank-Nicolson solves.  T: precision of the nonlinear terms;
+// TC columns x RG row groups per block; RPT >= ceil(ny2 / RG) rows per thread.
+// The CN solution overwrites the state rows [wlo, whi]; k_project follows.
+template <typename T, int TC, int NT, int RPT, int NTH>
+__global__ __launch_bounds__(NT) void k_cn(ColArgs a) {
+  constexpr int NF = 3 + NTH;
+  constexpr int RG = NT / TC;
   extern __shared__ unsigned char smem[];
   const Geo& g = a.g;
   const int ny2 = g.ny2, ny = g.ny, tid = threadIdx.x;
-  const int c0 = blockIdx.x * TC;
-  cd* sU1 = (cd*)smem;
-  cd* sU2 = sU1 + TC * ny2;
-  cd* sU3 = sU2 + TC * ny2;
-  cd* sX = sU3 + TC * ny2;
-  double* sB = (double*)(sX + TC * ny2);
+  const int cc = tid % TC, rg = tid / TC;
+  const int tile = xcd_tile(blockIdx.x, (int)(g.pc / TC));
+  if (tile < 0) return;
+  const int c = tile * TC + cc;  // column in the padded plane: c = k*kxp + i
+  const int i = c % g.kxp, k = c / g.kxp;
+  const bool active = i < g.kxr;
   const long long pc = g.pc;
-  cd* slots[3] = {sU1, sU2, sU3};
+  const int lvl = a.levels;
+  cd* sR = (cd*)smem;
+  double* sCo = (double*)(sR + NF * TC * ny2);  // one field's PCR table: (2*lvl+1) rows of ny2
+  auto R = [&](int f, int j) -> cd& { return sR[(f * TC + cc) * ny2 + j]; };
+  const cd* __restrict__ u1 = a.u[0];
+  const cd* __restrict__ u2 = a.u[1];
+  const cd* __restrict__ u3 = a.u[2];
+  const cd* __restrict__ t0 = NTH >= 1 ? a.u[3] : a.u[0];
+  const cd* __restrict__ t1 = NTH >= 2 ? a.u[4] : a.u[0];
+  const cd* __restrict__ pp = a.p;
 
-  auto load_rhs = [&](int F, cd* dst) {
-    const SolveRows& R = a.rows[F];
-    const cd* src = a.cr[F];
-    for (int idx = tid; idx < TC * ny2; idx += 64) {
-      const int cc = idx % TC, j = idx / TC, c = c0 + cc;
+  // ---- 1. assembly of cf and cr; BC rows of the right-hand side
+  const double kxi = active ? a.kx[i] : 0.0, kzk = a.kz[k];
+  const double k2 = kxi * kxi + kzk * kzk;
+  const double kk = (a.beta == 1.0) ? k2 : pow(k2, a.beta);
+  for (int m = 0; m < RPT; ++m) {
+    const int j = rg + m * RG;
+    if (j >= ny2) break;
+    const long long o = (long long)j * pc + c;
+    const bool inner = active && j >= 1 && j <= ny;
+    RowIn<NTH> r;
+    if (inner) load_row<T, NTH>(r, a, o, pc, u1, u2, u3, t0, t1, pp);
+    cd crv[NF];
+#pragma unroll
+    for (int e = 0; e < NF; ++e) {
+      const SolveRows& SR = a.rows[e];
       cd v = {0.0, 0.0};
-      if (c < a.ncols) {
-        const int sr = special_row(j, ny);
-        const int kd = sr >= 0 ? R.kind[sr] : -1;
-        if (kd == 1) {
-          if (c == 0) v = {R.val[sr], 0.0};
-        } else if (kd == -1) {
-          if (j >= R.blo && j <= R.bhi) v = src[(long long)j * pc + c];
+      if (inner && j >= SR.blo && j <= SR.bhi) {
+        cd cf;
+        assemble<NTH>(a, r, a.rowc + (long long)j * RC_COUNT, e, i, k, kxi, kzk, kk, cf, v);
+        a.cf[e][o] = cf;
+      }
+      const int sr = special_row(j, ny);
+      const int kd = sr >= 0 ? SR.kind[sr] : -1;
+      if (kd == 0) v = {0.0, 0.0};
+      else if (kd == 1) v = (c == 0) ? cd{SR.val[sr], 0.0} : cd{0.0, 0.0};
+      else if (!(j >= SR.blo && j <= SR.bhi)) v = {0.0, 0.0};
+      crv[e] = v;
+    }
+#pragma unroll
+    for (int e = 0; e < NF; ++e) R(e, j) = crv[e];
+  }
+  __syncthreads();
+
+  // ---- 2. PCR with the host's coefficients, scalars first (for/solver.f90:1004-1007)
+  for (int e0 = 0; e0 < NF; ++e0) {
+    const int e = (e0 < NTH) ? 3 + e0 : e0 - NTH;
+    const double* co = a.pcr + (long long)e * (2 * PCR_LMAX + 1) * ny2;
+    for (int q = tid; q < 2 * lvl * ny2; q += NT) sCo[q] = co[q];
+    for (int q = tid; q < ny2; q += NT) sCo[2 * lvl * ny2 + q] = co[2 * PCR_LMAX * ny2 + q];
+    __syncthreads();
+    for (int l = 0; l < lvl; ++l) {
+      const int s = 1 << l;
+      const double* Al = sCo + (2 * l) * ny2;
+      const double* Gl = sCo + (2 * l + 1) * ny2;
+      cd nv[RPT];
+#pragma unroll
+      for (int m = 0; m < RPT; ++m) {
+        const int j = rg + m * RG;
+        if (j < ny2) {
+          const cd dm = (j - s >= 0) ? R(e, j - s) : cd{0.0, 0.0};
+          const cd dp = (j + s < ny2) ? R(e, j + s) : cd{0.0, 0.0};
+          nv[m] = R(e, j) + Al[j] * dm + Gl[j] * dp;
         }
       }
-      dst[cc * ny2 + j] = v;
+      __syncthreads();
+#pragma unroll
+      for (int m = 0; m < RPT; ++m) {
+        const int j = rg + m * RG;
+        if (j < ny2) R(e, j) = nv[m];
+      }
+      __syncthreads();
     }
-  };
-  auto cn_sweep = [&](int F, cd* col) {
-    const double* L = a.fac + (long long)F * 3 * ny2;
-    const double* U = L + ny2;
-    const double* RB = U + ny2;
-    cd prev = col[0];
-    for (int j = 1; j < ny2; ++j) {
-      const cd v = col[j] + L[j] * prev;
-      col[j] = v;
-      prev = v;
+    const SolveRows& SR = a.rows[e];
+    const double* Bf = sCo + (2 * lvl) * ny2;
+    for (int m = 0; m < RPT; ++m) {
+      const int j = rg + m * RG;
+      if (active && j < ny2 && j >= SR.wlo && j <= SR.whi) a.u[e][(long long)j * pc + c] = Bf[j] * R(e, j);
     }
-    cd nxt = RB[ny2 - 1] * col[ny2 - 1];
-    col[ny2 - 1] = nxt;
-    for (int j = ny2 - 2; j >= 0; --j) {
-      const cd v = RB[j] * (col[j] - U[j] * nxt);
-      col[j] = v;
-      nxt = v;
-    }
-  };
-  auto store_rows = [&](int F, const cd* src, cd* dst) {
-    const SolveRows& R = a.rows[F];
-    for (int idx = tid; idx < TC * ny2; idx += 64) {
-      const int cc = idx % TC, j = idx / TC, c = c0 + cc;
-      if (c < a.ncols && j >= R.wlo && j <= R.whi) dst[(long long)j * pc + c] = src[cc * ny2 + j];
-    }
-  };
+    __syncthreads();
+  }
+}
 
-  if (a.do_cn) {
-    // scalars first, with the velocity of the previous substep (for/solver.f90:1004-1007)
-    for (int n = 0; n < NTH; ++n) {
-      load_rhs(3 + n, sX);
-      __syncthreads();
-      if (tid < TC) cn_sweep(3 + n, sX + tid * ny2);
-      __syncthreads();
-      store_rows(3 + n, sX, a.u[3 + n]);
-      __syncthreads();
-    }
-    for (int F = 0; F < 3; ++F) load_rhs(F, slots[F]);
-    __syncthreads();
-    if (tid < 3 * TC) {
-      const int F = tid / TC, cc = tid - F * TC;
-      cn_sweep(F, slots[F] + cc * ny2);
-    }
-    __syncthreads();
-  } else {
-    for (int F = 0; F < 3; ++F)
-      for (int idx = tid; idx < TC * ny2; idx += 64) {
-        const int cc = idx % TC, j = idx / TC, c = c0 + cc;
-        slots[F][cc * ny2 + j] = (c < a.ncols) ? a.u[F][(long long)j * pc + c] : cd{0.0, 0.0};
+// rem_div_chan (for/solver.f90:1302-1379, BC rows 2330-2371): Poisson rows 1..Nyp
+// by PCR with per-column coefficients, velocity correction, and with
+// update_p the pressure update of rk_chan_1 (for/solver.f90:1261-1267).
+template <int TC, int NT, int RPT>
+__global__ __launch_bounds__(NT) void k_project(ColArgs a, int update_p) {
+  constexpr int RG = NT / TC;
+  extern __shared__ unsigned char smem[];
+  const Geo& g = a.g;
+  const int ny2 = g.ny2, ny = g.ny, tid = threadIdx.x;
+  const int cc = tid % TC, rg = tid / TC;
+  const int tile = xcd_tile(blockIdx.x, (int)(g.pc / TC));
+  if (tile < 0) return;
+  const int c = tile * TC + cc;
+  const int i = c % g.kxp, k = c / g.kxp;
+  const bool active = i < g.kxr;
+  const long long pc = g.pc;
+  cd* sD = (cd*)smem;
+  double* sA = (double*)(sD + TC * ny2);
+  double* sB = sA + TC * ny2;
+  double* sC = sB + TC * ny2;
+  cd* __restrict__ u1 = a.u[0];
+  cd* __restrict__ u2 = a.u[1];
+  cd* __restrict__ u3 = a.u[2];
+  const double kxi = active ? a.kx[i] : 0.0, kzk = a.kz[k];
+  const double k2 = -(kxi * kxi) - kzk * kzk;
+  for (int m = 0; m < RPT; ++m) {
+    const int j = rg + m * RG;
+    if (j >= ny2) break;
+    const int q = cc * ny2 + j;
+    double aj = 0.0, bj = 1.0, cj = 0.0;
+    cd d = {0.0, 0.0};
+    if (j == 1) {
+      cj = (c == 0) ? 0.0 : -1.0;
+    } else if (j == ny) {
+      aj = 1.0; bj = -1.0;
+    } else if (j > 1 && j < ny) {
+      aj = 1.0 / (a.dy[j] * a.dyf[j]);
+      bj = k2 - 1.0 / (a.dy[j + 1] * a.dyf[j]) - 1.0 / (a.dy[j] * a.dyf[j]);
+      cj = 1.0 / (a.dy[j + 1] * a.dyf[j]);
+      if (active) {
+        const long long o = (long long)j * pc + c;
+        d = ik(kxi, u1[o]) + (1.0 / a.dyf[j]) * (u2[o + pc] - u2[o]) + ik(kzk, u3[o]);
       }
-    __syncthreads();
-  }
-  // rem_div_chan: right-hand side rows 1..ny, BC rows 1 and ny (for/solver.f90:1325-1350, 2330-2371)
-  for (int idx = tid; idx < TC * ny2; idx += 64) {
-    const int cc = idx % TC, j = idx / TC, c = c0 + cc;
-    cd v = {0.0, 0.0};
-    if (c < a.ncols && j >= 2 && j <= ny - 1) {
-      const int i = c % g.kxr, k = c / g.kxr;
-      const cd* u1 = sU1 + cc * ny2;
-      const cd* u2 = sU2 + cc * ny2;
-      const cd* u3 = sU3 + cc * ny2;
-      v = ik(a.kx[i], u1[j]) + (1.0 / a.dyf[j]) * (u2[j + 1] - u2[j]) + ik(a.kz[k], u3[j]);
     }
-    sX[cc * ny2 + j] = v;
+    sA[q] = aj; sB[q] = bj; sC[q] = cj; sD[q] = d;
   }
   __syncthreads();
-  if (tid < TC && c0 + tid < a.ncols) {
-    // complex Thomas over rows 1..ny (thomas_forward/backward_complex_mpi at one rank)
-    const int c = c0 + tid, i = c % g.kxr, k = c / g.kxr;
-    cd* phi = sX + tid * ny2;
-    double* bp = sB + tid * ny2;
-    const double k2 = -(a.kx[i] * a.kx[i]) - a.kz[k] * a.kz[k];
-    // row 1
-    double bprev = 1.0, cprev = (c == 0) ? 0.0 : -1.0;
-    bp[1] = bprev;
-    cd gprev = phi[1];
-    for (int j = 2; j <= ny; ++j) {
-      double aj, bj, cj;
-      if (j < ny) {
-        aj = 1.0 / (a.dy[j] * a.dyf[j]);
-        bj = k2 - 1.0 / (a.dy[j + 1] * a.dyf[j]) - 1.0 / (a.dy[j] * a.dyf[j]);
-        cj = 1.0 / (a.dy[j + 1] * a.dyf[j]);
-      } else {
-        aj = 1.0; bj = -1.0; cj = 0.0;
+  for (int l = 0; l < a.plevels; ++l) {
+    const int s = 1 << l;
+    double na[RPT], nb[RPT], nc[RPT];
+    cd nd[RPT];
+#pragma unroll
+    for (int m = 0; m < RPT; ++m) {
+      const int j = rg + m * RG;
+      if (j >= 1 && j <= ny) {
+        const int q = cc * ny2 + j;
+        const int jm = j - s, jp = j + s;
+        double a2 = 0.0, c2 = 0.0, b2 = sB[q];
+        cd d2 = sD[q];
+        if (jm >= 1) {
+          const double al = -sA[q] / sB[q - s];
+          a2 = al * sA[q - s];
+          b2 = b2 + al * sC[q - s];
+          d2 = d2 + al * sD[q - s];
+        }
+        if (jp <= ny) {
+          const double ga = -sC[q] / sB[q + s];
+          c2 = ga * sC[q + s];
+          b2 = b2 + ga * sA[q + s];
+          d2 = d2 + ga * sD[q + s];
+        }
+        na[m] = a2; nb[m] = b2; nc[m] = c2; nd[m] = d2;
       }
-      const double m = -aj / bprev;
-      bj = bj + m * cprev;
-      const cd gj = phi[j] + m * gprev;
-      phi[j] = gj;
-      bp[j] = bj;
-      bprev = bj; cprev = cj; gprev = gj;
     }
-    cd nxt = (1.0 / bp[ny]) * phi[ny];
-    phi[ny] = nxt;
-    for (int j = ny - 1; j >= 1; --j) {
-      const double cj = (j == 1) ? ((c == 0) ? 0.0 : -1.0) : 1.0 / (a.dy[j + 1] * a.dyf[j]);
-      const cd v = (1.0 / bp[j]) * (phi[j] - cj * nxt);
-      phi[j] = v;
-      nxt = v;
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < RPT; ++m) {
+      const int j = rg + m * RG;
+      if (j >= 1 && j <= ny) {
+        const int q = cc * ny2 + j;
+        sA[q] = na[m]; sB[q] = nb[m]; sC[q] = nc[m]; sD[q] = nd[m];
+      }
     }
+    __syncthreads();
+  }
+  for (int m = 0; m < RPT; ++m) {
+    const int j = rg + m * RG;
+    if (j >= ny2) break;
+    const int q = cc * ny2 + j;
+    sD[q] = (j >= 1 && j <= ny) ? (1.0 / sB[q]) * sD[q] : cd{0.0, 0.0};
   }
   __syncthreads();
-  // velocity correction and pressure update
-  for (int idx = tid; idx < TC * ny2; idx += 64) {
-    const int cc = idx % TC, j = idx / TC, c = c0 + cc;
-    if (c >= a.ncols) continue;
-    const int i = c % g.kxr, k = c / g.kxr;
-    const cd* phi = sX + cc * ny2;
-    if (j >= 1 && j <= ny) {
-      sU1[cc * ny2 + j] = sU1[cc * ny2 + j] - ik(a.kx[i], phi[j]);
-      sU3[cc * ny2 + j] = sU3[cc * ny2 + j] - ik(a.kz[k], phi[j]);
-    }
-    if (j >= 2 && j <= ny) sU2[cc * ny2 + j] = sU2[cc * ny2 + j] - (1.0 / a.dy[j]) * (phi[j] - phi[j - 1]);
-    if (a.do_cn && j >= a.jstart && j <= a.jend) {
-      const long long o = (long long)j * pc + c;
-      a.p[o] = a.p[o] + (1.0 / a.temp4) * phi[j];
-    }
-  }
-  __syncthreads();
-  if (a.do_cn) {
-    for (int F = 0; F < 3; ++F) store_rows(F, slots[F], a.u[F]);
-  } else {
-    for (int F = 0; F < 3; ++F)
-      for (int idx = tid; idx < TC * ny2; idx += 64) {
-        const int cc = idx % TC, j = idx / TC, c = c0 + cc;
-        if (c < a.ncols && j >= 1 && j <= ny) a.u[F][(long long)j * pc + c] = slots[F][cc * ny2 + j];
-      }
+  if (!active) return;
+  for (int m = 0; m < RPT; ++m) {
+    const int j = rg + m * RG;
+    if (j >= ny2) break;
+    if (j < 1 || j > ny) continue;
+    const long long o = (long long)j * pc + c;
+    const cd phi = sD[cc * ny2 + j];
+    u1[o] = u1[o] - ik(kxi, phi);
+    u3[o] = u3[o] - ik(kzk, phi);
+    if (j >= 2) u2[o] = u2[o] - (1.0 / a.dy[j]) * (phi - sD[cc * ny2 + j - 1]);
+    if (update_p && j >= a.jstart && j <= a.jend) a.p[o] = a.p[o] + (1.0 / a.temp4) * phi;
   }
 }
 

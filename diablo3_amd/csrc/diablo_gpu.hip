@@ -14,6 +14,7 @@
 #include "../../include/diablo_gpu.h"
 #include "dgpu_kernels.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <type_traits>
 #include <cstdio>
@@ -69,12 +70,13 @@ struct dgpu_handle {
   cd* U[MAXF] = {};
   cd* P = nullptr;
   cd* CF[MAXF] = {};
-  cd* CR[MAXF] = {};
+  void* NL[MAXF] = {};     // nonlinear tendencies (complex T), packed layout
   void* Hin[MAXF] = {};
   void* Hout[MAXO] = {};
   void *twx64 = nullptr, *twz64 = nullptr, *twx32 = nullptr, *twz32 = nullptr;
   double* d_fac = nullptr;
   double fac_dt = NAN;
+  double* d_rowc = nullptr;  // [ny2][RC_COUNT] assembly row coefficients
   double* d_R = nullptr;   // one physical FP64 field (transform entry points)
   cd* d_H = nullptr;       // one FP64 x-spectrum field
   cd* d_C = nullptr;       // one packed field
@@ -86,8 +88,8 @@ struct dgpu_handle {
   std::vector<hipEvent_t> ev_pool;
   struct Mark { int id; hipEvent_t a, b; };
   std::vector<Mark> marks;
-  double t_ms[4] = {0, 0, 0, 0};
-  long long t_n[4] = {0, 0, 0, 0};
+  double t_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  long long t_n[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
 
 namespace {
@@ -150,31 +152,99 @@ void cn_matrix(const dgpu_handle* h, int F, double temp1, std::vector<double>& a
   }
 }
 
+int pcr_levels(int n) {
+  int l = 0;
+  while ((1 << l) < n) ++l;
+  return l;
+}
+
+// Parallel-cyclic-reduction coefficients of one tridiagonal system (rows 0..n-1,
+// a[0] = c[n-1] = 0): at level l (stride s = 2^l) every row j is replaced by
+// row_j + alpha_l[j] row_{j-s} + gamma_l[j] row_{j+s}; after the last level the
+// system is diagonal and x_j = d_j / b_j.  Computed in long double and stored as
+// double; out = [alpha_0, gamma_0, ..., alpha_{L-1}, gamma_{L-1}, (unused), 1/b].
+int pcr_coefficients(const std::vector<double>& a0, const std::vector<double>& b0,
+                     const std::vector<double>& c0, double* out, int lmax) {
+  const int n = (int)b0.size();
+  const int L = pcr_levels(n);
+  if (L > lmax) return fail(2, "too many rows for the PCR coefficient table");
+  std::vector<long double> a(a0.begin(), a0.end()), b(b0.begin(), b0.end()), c(c0.begin(), c0.end());
+  std::vector<long double> na(n), nb(n), nc(n);
+  for (int l = 0; l < L; ++l) {
+    const int s = 1 << l;
+    double* A = out + (size_t)(2 * l) * n;
+    double* G = out + (size_t)(2 * l + 1) * n;
+    for (int j = 0; j < n; ++j) {
+      long double al = 0, ga = 0, a2 = 0, c2 = 0, b2 = b[j];
+      if (j - s >= 0) {
+        if (b[j - s] == 0) return fail(20, "zero pivot in cyclic reduction");
+        al = -a[j] / b[j - s];
+        a2 = al * a[j - s];
+        b2 += al * c[j - s];
+      }
+      if (j + s < n) {
+        if (b[j + s] == 0) return fail(20, "zero pivot in cyclic reduction");
+        ga = -c[j] / b[j + s];
+        c2 = ga * c[j + s];
+        b2 += ga * a[j + s];
+      }
+      A[j] = (double)al;
+      G[j] = (double)ga;
+      na[j] = a2; nb[j] = b2; nc[j] = c2;
+    }
+    a.swap(na); b.swap(nb); c.swap(nc);
+  }
+  double* B = out + (size_t)(2 * lmax) * n;
+  for (int j = 0; j < n; ++j) {
+    if (b[j] == 0) return fail(20, "singular Crank-Nicolson system");
+    B[j] = (double)(1.0L / b[j]);
+  }
+  // self-check against a long-double Thomas solve of a fixed right-hand side
+  std::vector<long double> g(n), bp(n), x(n);
+  std::vector<double> d(n);
+  for (int j = 0; j < n; ++j) d[j] = std::sin(0.37 * j + 0.1) + 0.5;
+  bp[0] = b0[0];
+  g[0] = d[0];
+  for (int j = 1; j < n; ++j) {
+    const long double m = -(long double)a0[j] / bp[j - 1];
+    bp[j] = b0[j] + m * c0[j - 1];
+    g[j] = d[j] + m * g[j - 1];
+  }
+  x[n - 1] = g[n - 1] / bp[n - 1];
+  for (int j = n - 2; j >= 0; --j) x[j] = (g[j] - c0[j] * x[j + 1]) / bp[j];
+  std::vector<double> y(d), ny_(n);
+  for (int l = 0; l < L; ++l) {
+    const int s = 1 << l;
+    const double* A = out + (size_t)(2 * l) * n;
+    const double* G = out + (size_t)(2 * l + 1) * n;
+    for (int j = 0; j < n; ++j)
+      ny_[j] = y[j] + A[j] * (j - s >= 0 ? y[j - s] : 0.0) + G[j] * (j + s < n ? y[j + s] : 0.0);
+    y.swap(ny_);
+  }
+  long double err = 0, nrm = 0;
+  for (int j = 0; j < n; ++j) {
+    const long double v = B[j] * y[j];
+    err = std::max(err, std::fabs(v - x[j]));
+    nrm = std::max(nrm, std::fabs(x[j]));
+  }
+  if (!(err <= 1e-12L * (nrm > 0 ? nrm : 1.0L))) return fail(21, "cyclic reduction disagrees with Thomas");
+  return 0;
+}
+
 int ensure_factors(dgpu_handle* h, double delta_t) {
   if (h->fac_dt == delta_t) return 0;
   const int n2 = h->c.ny + 2, nf = 3 + h->nth;
-  std::vector<double> buf((size_t)3 * MAXF * 3 * n2, 0.0), a, b, cc;
+  const size_t per_field = (size_t)(2 * PCR_LMAX + 1) * n2;
+  std::vector<double> buf((size_t)3 * MAXF * per_field, 0.0), a, b, cc;
   const double hb[3] = {delta_t * (8.0 / 15.0), delta_t * (2.0 / 15.0), delta_t * (5.0 / 15.0)};
   for (int r = 0; r < 3; ++r) {
     const double temp1 = 0.5 * h->c.nu * hb[r];
     for (int F = 0; F < nf; ++F) {
       cn_matrix(h, F, temp1, a, b, cc);
-      double* L = &buf[((size_t)r * MAXF + F) * 3 * n2];
-      double* Uc = L + n2;
-      double* RB = Uc + n2;
-      // thomas_forward_real_mpi at one rank: rows 1..ny+1 eliminated against row j-1
-      std::vector<double> bp(n2);
-      bp[0] = b[0];
-      L[0] = 0.0;
-      for (int j = 1; j < n2; ++j) {
-        L[j] = -a[j] / bp[j - 1];
-        bp[j] = b[j] + L[j] * cc[j - 1];
-      }
-      for (int j = 0; j < n2; ++j) {
-        if (bp[j] == 0.0) return fail(20, "singular Crank-Nicolson system");
-        Uc[j] = cc[j];
-        RB[j] = 1.0 / bp[j];
-      }
+      a[0] = 0.0;
+      cc[n2 - 1] = 0.0;
+      int rc = pcr_coefficients(a, b, cc, &buf[((size_t)r * MAXF + F) * per_field], PCR_LMAX);
+      if (rc) return rc;
     }
   }
   HIPCHK(hipMemcpyAsync(h->d_fac, buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice, h->st));
@@ -257,7 +327,8 @@ int ensure_fft_scratch(dgpu_handle* h) {
 // --------------------------------------------------------------------------------
 // per-kernel timing
 // --------------------------------------------------------------------------------
-const char* const kKernelNames[4] = {"k_inv_z", "k_xprod", "k_fwdz_asm", "k_columns"};
+constexpr int NKERN = 5;
+const char* const kKernelNames[NKERN] = {"k_inv_z", "k_xprod", "k_fwdz_nl", "k_cn", "k_project"};
 
 hipEvent_t pool_event(dgpu_handle* h) {
   if (!h->ev_pool.empty()) {
@@ -407,34 +478,84 @@ int launch_xprod(dgpu_handle* h, XProdArgs& a, int nj) {
   return 0;
 }
 
-template <typename T, int NTH>
-int launch_fwdz_asm(dgpu_handle* h, AsmArgs& a, int nj) {
+template <typename T>
+int launch_fwdz_nl(dgpu_handle* h, NlArgs& a, int nj) {
   a.tw = sizeof(T) == 8 ? h->twz64 : h->twz32;
   DG_SWITCH_N(h->g.nz, {
     constexpr int TI = ti_for(N_, sizeof(cx<T>));
-    dim3 grid((h->g.kxr + TI - 1) / TI, nj, 1);
-    k_fwdz_asm<T, N_, TI, 256, NTH><<<grid, 256, 0, h->st>>>(a);
+    dim3 grid((h->g.kxr + TI - 1) / TI, nj, 3 + h->nth);
+    k_fwdz_nl<T, N_, TI, 256><<<grid, 256, 0, h->st>>>(a);
   });
   HIPCHK(hipGetLastError());
   return 0;
 }
 
-template <int NTH>
-int launch_columns(dgpu_handle* h, ColArgs& a) {
-  const int ny2 = h->g.ny2;
-  const int tc = (ny2 * 72 * 8 <= 96 * 1024) ? 8 : 4;
-  const size_t lds = (size_t)tc * ny2 * (4 * sizeof(cd) + sizeof(double));
-  if (lds > 160 * 1024) return fail(2, "ny too large for the column kernel");
-  const int blocks = (a.ncols + tc - 1) / tc;
-  if (tc == 8) {
-    HIPCHK(hipFuncSetAttribute((const void*)&k_columns<8, NTH>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    k_columns<8, NTH><<<blocks, 64, lds, h->st>>>(a);
-  } else {
-    HIPCHK(hipFuncSetAttribute((const void*)&k_columns<4, NTH>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    k_columns<4, NTH><<<blocks, 64, lds, h->st>>>(a);
+// columns per block (TC) and rows per thread (RPT) of the column kernels: the
+// largest TC whose LDS lets at least two blocks share a CU (<= 80 KiB)
+struct ColShape {
+  int tc, rpt;
+  size_t lds;
+};
+ColShape col_shape(const dgpu_handle* h, bool cn) {
+  const int ny2 = h->g.ny2, nf = 3 + h->nth;
+  const int lvl = pcr_levels(ny2);
+  ColShape cs{};
+  for (int tc : {8, 4, 2}) {
+    const size_t lds = cn ? (size_t)tc * ny2 * 16 * nf + (size_t)(2 * lvl + 1) * ny2 * 8
+                          : (size_t)tc * ny2 * 40;
+    if (lds <= 80 * 1024 || (tc == 2 && lds <= 160 * 1024)) {
+      const int rg = 256 / tc, need = (ny2 + rg - 1) / rg;
+      int rpt = 1;
+      while (rpt < need) rpt *= 2;
+      cs = {tc, rpt, lds};
+      break;
+    }
   }
+  return cs;
+}
+
+int grid_xcd(long long tiles) { return (int)((tiles + 7) / 8 * 8); }
+
+template <typename T, int NTH, int TC, int RPT>
+int launch_cn_t(dgpu_handle* h, ColArgs& a, size_t lds) {
+  HIPCHK(hipFuncSetAttribute((const void*)&k_cn<T, TC, 256, RPT, NTH>,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  k_cn<T, TC, 256, RPT, NTH><<<grid_xcd(h->g.pc / TC), 256, lds, h->st>>>(a);
   HIPCHK(hipGetLastError());
   return 0;
+}
+
+template <int TC, int RPT>
+int launch_project_t(dgpu_handle* h, ColArgs& a, size_t lds, int update_p) {
+  HIPCHK(hipFuncSetAttribute((const void*)&k_project<TC, 256, RPT>,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  k_project<TC, 256, RPT><<<grid_xcd(h->g.pc / TC), 256, lds, h->st>>>(a, update_p);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+#define DG_SHAPES(X) X(8, 1) X(8, 2) X(8, 4) X(8, 8) X(4, 1) X(4, 2) X(4, 4) X(4, 8) \
+                     X(2, 1) X(2, 2) X(2, 4) X(2, 8) X(2, 16)
+
+template <typename T, int NTH>
+int launch_cn(dgpu_handle* h, ColArgs& a) {
+  const ColShape cs = col_shape(h, true);
+  if (cs.tc == 0 || h->g.pc % cs.tc) return fail(2, "ny too large for the column kernels");
+#define DG_X(TC_, RPT_) \
+  if (cs.tc == TC_ && cs.rpt == RPT_) return launch_cn_t<T, NTH, TC_, RPT_>(h, a, cs.lds);
+  DG_SHAPES(DG_X)
+#undef DG_X
+  return fail(2, "ny too large for the column kernels");
+}
+
+int launch_project(dgpu_handle* h, ColArgs& a, int update_p) {
+  const ColShape cs = col_shape(h, false);
+  if (cs.tc == 0 || h->g.pc % cs.tc) return fail(2, "ny too large for the column kernels");
+#define DG_X(TC_, RPT_) \
+  if (cs.tc == TC_ && cs.rpt == RPT_) return launch_project_t<TC_, RPT_>(h, a, cs.lds, update_p);
+  DG_SHAPES(DG_X)
+#undef DG_X
+  return fail(2, "ny too large for the column kernels");
 }
 
 template <typename F>
@@ -480,8 +601,9 @@ void pack_fourier(const dgpu_handle* h, const double* fort, cd* compact) {
   for (int j = 0; j < g.ny2; ++j)
     for (int k = 0; k < g.kzr; ++k) {
       const double* src = fort + 2 * (((size_t)j * ldk + k) * ldi);
-      cd* dst = compact + (size_t)j * g.pc + (size_t)k * g.kxr;
+      cd* dst = compact + (size_t)j * g.pc + (size_t)k * g.kxp;
       std::memcpy(dst, src, sizeof(cd) * g.kxr);
+      std::memset((void*)(dst + g.kxr), 0, sizeof(cd) * (g.kxp - g.kxr));
     }
 }
 void unpack_fourier(const dgpu_handle* h, const cd* compact, double* fort) {
@@ -490,7 +612,7 @@ void unpack_fourier(const dgpu_handle* h, const cd* compact, double* fort) {
   for (int j = 0; j < g.ny2; ++j)
     for (int k = 0; k < g.kzr; ++k) {
       double* dst = fort + 2 * (((size_t)j * ldk + k) * ldi);
-      std::memcpy(dst, compact + (size_t)j * g.pc + (size_t)k * g.kxr, sizeof(cd) * g.kxr);
+      std::memcpy(dst, compact + (size_t)j * g.pc + (size_t)k * g.kxp, sizeof(cd) * g.kxr);
       std::memset(dst + 2 * g.kxr, 0, sizeof(cd) * (ldi - g.kxr));
     }
 }
@@ -510,6 +632,34 @@ int download_field(dgpu_handle* h, const cd* dev, double* fort) {
   HIPCHK(hipStreamSynchronize(h->st));
   unpack_fourier(h, stage, fort);
   return 0;
+}
+
+void fill_col_args(dgpu_handle* h, ColArgs& ca, double time) {
+  const dgpu_config& c = h->c;
+  ca.g = h->g;
+  ca.levels = pcr_levels(h->g.ny2);
+  ca.plevels = pcr_levels(h->g.ny);
+  ca.jstart = c.jstart; ca.jend = c.jend;
+  for (int n = 0; n < MAXTH; ++n) {
+    ca.jstart_th[n] = c.jstart_th[n]; ca.jend_th[n] = c.jend_th[n];
+    ca.pr[n] = c.pr[n]; ca.dthdx[n] = c.dthdx[n]; ca.dthdz[n] = c.dthdz[n];
+  }
+  ca.front = c.flavor_front; ca.f_type = c.f_type;
+  ca.nu = c.nu; ca.beta = c.beta; ca.nu_v_scale = c.nu_v_scale; ca.ro_inv = c.ro_inv; ca.delta = c.delta;
+  ca.grav_x = c.grav_x; ca.grav_y = c.grav_y; ca.grav_z = c.grav_z; ca.ly = c.ly;
+  ca.px0 = c.px0; ca.omega0 = c.omega0; ca.amp_omega0 = c.amp_omega0; ca.time = time;
+  ca.temp4 = 1.0;
+  ca.cori = c.ro_inv / c.delta;
+  for (int n = 0; n < MAXTH; ++n) {
+    ca.twu_x[n] = (c.ro_inv != 0.0 && n < h->nth) ? c.dthdx[n] * c.delta / c.ro_inv : 0.0;
+    ca.twu_z[n] = (c.ro_inv != 0.0 && n < h->nth) ? -1.0 * c.dthdz[n] * c.delta / c.ro_inv : 0.0;
+    ca.nupr[n] = n < h->nth ? c.nu / c.pr[n] : 0.0;
+  }
+  ca.rowc = h->d_rowc;
+  ca.dy = h->d_dy; ca.dyf = h->d_dyf; ca.gy = h->d_gy; ca.gyf = h->d_gyf; ca.kx = h->d_kx; ca.kz = h->d_kz;
+  solve_rows(h, time, ca.rows);
+  for (int f = 0; f < 3 + h->nth; ++f) { ca.u[f] = h->U[f]; ca.cf[f] = h->CF[f]; ca.nl[f] = h->NL[f]; }
+  ca.p = h->P;
 }
 
 int substep(dgpu_handle* h, int rk, double delta_t, double time, int time_step, int create_new_flow) {
@@ -552,40 +702,32 @@ int substep(dgpu_handle* h, int rk, double delta_t, double time, int time_step, 
     return mixed ? launch_xprod<float, NTH>(h, xa, nj) : launch_xprod<double, NTH>(h, xa, nj);
   });
   if (rc || (rc = tend(h))) return rc;
-  // 3. z forward + assembly
-  AsmArgs aa{};
-  aa.g = g; aa.jlo = jlo; aa.rk = rk; aa.jstart = c.jstart; aa.jend = c.jend;
-  for (int n = 0; n < MAXTH; ++n) {
-    aa.jstart_th[n] = c.jstart_th[n]; aa.jend_th[n] = c.jend_th[n];
-    aa.pr[n] = c.pr[n]; aa.dthdx[n] = c.dthdx[n]; aa.dthdz[n] = c.dthdz[n];
-  }
-  aa.front = c.flavor_front; aa.f_type = c.f_type;
-  aa.nu = c.nu; aa.beta = c.beta; aa.nu_v_scale = c.nu_v_scale; aa.ro_inv = c.ro_inv; aa.delta = c.delta;
-  aa.grav_x = c.grav_x; aa.grav_y = c.grav_y; aa.grav_z = c.grav_z; aa.ly = c.ly;
-  aa.px0 = c.px0; aa.omega0 = c.omega0; aa.amp_omega0 = c.amp_omega0; aa.time = time;
-  aa.temp1 = temp1; aa.temp3 = temp3; aa.temp4 = temp4; aa.temp5 = temp5;
-  aa.dy = h->d_dy; aa.dyf = h->d_dyf; aa.gy = h->d_gy; aa.gyf = h->d_gyf; aa.kx = h->d_kx; aa.kz = h->d_kz;
-  for (int f = 0; f < nf; ++f) { aa.u[f] = h->U[f]; aa.cf[f] = h->CF[f]; aa.cr[f] = h->CR[f]; }
-  aa.p = h->P;
-  for (int o = 0; o < 6 + 2 * h->nth; ++o) aa.h[o] = h->Hout[o];
+  // 3. z forward transform of the products -> nonlinear tendencies
+  NlArgs na{};
+  na.g = g; na.jlo = jlo; na.jstart = c.jstart; na.jend = c.jend;
+  for (int n = 0; n < MAXTH; ++n) { na.jstart_th[n] = c.jstart_th[n]; na.jend_th[n] = c.jend_th[n]; }
+  na.kz = h->d_kz;
+  for (int o = 0; o < 6 + 2 * h->nth; ++o) na.h[o] = h->Hout[o];
+  for (int f = 0; f < nf; ++f) na.nl[f] = h->NL[f];
   if ((rc = tbeg(h, 2))) return rc;
+  rc = mixed ? launch_fwdz_nl<float>(h, na, nj) : launch_fwdz_nl<double>(h, na, nj);
+  if (rc || (rc = tend(h))) return rc;
+  // 4. columns: assembly, CN, projection, pressure
+  ColArgs ca{};
+  fill_col_args(h, ca, time);
+  ca.rk = rk; ca.do_cn = 1;
+  ca.temp1 = temp1; ca.temp3 = temp3; ca.temp4 = temp4; ca.temp5 = temp5;
+  ca.t1v = temp1 * c.nu_v_scale;
+  for (int n = 0; n < h->nth; ++n) ca.t1pr[n] = temp1 / c.pr[n] * c.nu_v_scale;
+  ca.pcr = h->d_fac + (size_t)r * MAXF * (2 * PCR_LMAX + 1) * g.ny2;
+  if ((rc = tbeg(h, 3))) return rc;
   rc = by_nth(h->nth, [&](auto NTHc) {
     constexpr int NTH = decltype(NTHc)::value;
-    return mixed ? launch_fwdz_asm<float, NTH>(h, aa, nj) : launch_fwdz_asm<double, NTH>(h, aa, nj);
+    return mixed ? launch_cn<float, NTH>(h, ca) : launch_cn<double, NTH>(h, ca);
   });
   if (rc || (rc = tend(h))) return rc;
-  // 4. columns
-  ColArgs ca{};
-  ca.g = g; ca.ncols = (int)g.pc; ca.jstart = c.jstart; ca.jend = c.jend; ca.temp4 = temp4;
-  solve_rows(h, time, ca.rows);
-  ca.fac = h->d_fac + (size_t)r * MAXF * 3 * g.ny2;
-  ca.dy = h->d_dy; ca.dyf = h->d_dyf; ca.kx = h->d_kx; ca.kz = h->d_kz;
-  for (int f = 0; f < nf; ++f) { ca.u[f] = h->U[f]; ca.cr[f] = h->CR[f]; }
-  ca.p = h->P;
-  ca.do_cn = 1;
-  if ((rc = tbeg(h, 3))) return rc;
-  rc = by_nth(h->nth, [&](auto NTHc) { return launch_columns<decltype(NTHc)::value>(h, ca); });
-  if (rc) return rc;
+  if ((rc = tbeg(h, 4))) return rc;
+  if ((rc = launch_project(h, ca, 1))) return rc;
   return tend(h);
 }
 
@@ -620,8 +762,9 @@ int dgpu_create(const dgpu_config* cfg, dgpu_handle** out) {
   Geo& g = h->g;
   g.nx = cfg->nx; g.nz = cfg->nz; g.ny = cfg->ny; g.ny2 = n2;
   g.nkx = cfg->nx / 3; g.nkz = cfg->nz / 3; g.kxr = g.nkx + 1; g.kzr = 2 * g.nkz + 1;
-  g.pc = (long long)g.kxr * g.kzr;
-  g.ph = (long long)g.nz * g.kxr;
+  g.kxp = (g.kxr + 7) / 8 * 8;
+  g.pc = (long long)g.kxp * g.kzr;
+  g.ph = (long long)g.nz * g.kxp;
   g.pp = (long long)g.nz * g.nx;
   auto bail = [&](int code) {
     dgpu_destroy(h);
@@ -642,16 +785,19 @@ int dgpu_create(const dgpu_config* cfg, dgpu_handle** out) {
   } while (0)
   ALLOC(h->d_gy, 8 * n2); ALLOC(h->d_gyf, 8 * n2); ALLOC(h->d_dy, 8 * n2); ALLOC(h->d_dyf, 8 * n2);
   ALLOC(h->d_kx, 8 * (size_t)g.kxr); ALLOC(h->d_kz, 8 * (size_t)g.kzr);
-  for (int f = 0; f < nf; ++f) { ALLOC(h->U[f], cbytes); ALLOC(h->CF[f], cbytes); ALLOC(h->CR[f], cbytes); }
+  const size_t nlbytes = hsz * g.pc * n2;
+  for (int f = 0; f < nf; ++f) { ALLOC(h->U[f], cbytes); ALLOC(h->CF[f], cbytes); ALLOC(h->NL[f], nlbytes); }
   ALLOC(h->P, cbytes);
   for (int f = 0; f < nf; ++f) ALLOC(h->Hin[f], hbytes);
   for (int o = 0; o < no; ++o) ALLOC(h->Hout[o], hbytes);
   ALLOC(h->twx64, sizeof(cd) * g.nx); ALLOC(h->twz64, sizeof(cd) * g.nz);
   ALLOC(h->twx32, sizeof(cf) * g.nx); ALLOC(h->twz32, sizeof(cf) * g.nz);
-  ALLOC(h->d_fac, sizeof(double) * 3 * MAXF * 3 * n2);
+  ALLOC(h->d_fac, sizeof(double) * 3 * MAXF * (2 * PCR_LMAX + 1) * n2);
+  ALLOC(h->d_rowc, sizeof(double) * RC_COUNT * n2);
 #undef ALLOC
   h->fort_doubles = fortran_doubles(cfg);
-  if (hipHostMalloc((void**)&h->hstage, sizeof(double) * h->fort_doubles, hipHostMallocDefault) != hipSuccess)
+  const size_t stage_doubles = std::max<size_t>(h->fort_doubles, (size_t)2 * g.pc * n2);
+  if (hipHostMalloc((void**)&h->hstage, sizeof(double) * stage_doubles, hipHostMallocDefault) != hipSuccess)
     return bail(fail(11, "hipHostMalloc"));
   auto up = [&](void* dst, const void* src, size_t bytes) {
     return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, h->st) == hipSuccess;
@@ -663,6 +809,26 @@ int dgpu_create(const dgpu_config* cfg, dgpu_handle** out) {
             up(h->d_kx, h->kx.data(), 8 * (size_t)g.kxr) && up(h->d_kz, h->kz.data(), 8 * (size_t)g.kzr) &&
             up(h->twx64, tx64.data(), sizeof(cd) * g.nx) && up(h->twz64, tz64.data(), sizeof(cd) * g.nz) &&
             up(h->twx32, tx32.data(), sizeof(cf) * g.nx) && up(h->twz32, tz32.data(), sizeof(cf) * g.nz);
+  std::vector<double> rowc((size_t)RC_COUNT * n2, 0.0);
+  for (int j = 1; j <= cfg->ny; ++j) {
+    double* r = &rowc[(size_t)j * RC_COUNT];
+    const double dy = h->dy[j], dyp = h->dy[j + 1], dyf = h->dyf[j], dyfm = h->dyf[j - 1];
+    r[RC_RDY] = 1.0 / dy;
+    r[RC_RDYP] = 1.0 / dyp;
+    r[RC_RDYF] = 1.0 / dyf;
+    r[RC_RDYFM] = 1.0 / dyfm;
+    r[RC_WTH] = dyfm / (2.0 * dy);
+    r[RC_WTHM] = dyf / (2.0 * dy);
+    if (cfg->ro_inv != 0.0)
+      for (int n = 0; n < cfg->n_th; ++n) {
+        const double yf = h->gyf[j] - 0.5 * cfg->ly, yg = h->gy[j] - 0.5 * cfg->ly;
+        r[RC_TW0 + 4 * n + 0] = cfg->dthdx[n] * yf * cfg->delta / cfg->ro_inv;
+        r[RC_TW0 + 4 * n + 1] = -1.0 * cfg->dthdz[n] * yf * cfg->delta / cfg->ro_inv;
+        r[RC_TW0 + 4 * n + 2] = cfg->dthdx[n] * yg * cfg->delta / cfg->ro_inv;
+        r[RC_TW0 + 4 * n + 3] = -1.0 * cfg->dthdz[n] * yg * cfg->delta / cfg->ro_inv;
+      }
+  }
+  ok = ok && up(h->d_rowc, rowc.data(), sizeof(double) * rowc.size());
   if (!ok || hipStreamSynchronize(h->st) != hipSuccess) return bail(fail(10, "upload of constants failed"));
   *out = h;
   g_err.clear();
@@ -754,12 +920,8 @@ int dgpu_step(dgpu_handle* h, double delta_t, double time, int32_t time_step, in
 int dgpu_rem_div(dgpu_handle* h) {
   if (!h) return fail(1, "null handle");
   ColArgs ca{};
-  ca.g = h->g; ca.ncols = (int)h->g.pc; ca.jstart = h->c.jstart; ca.jend = h->c.jend; ca.temp4 = 1.0;
-  ca.dy = h->d_dy; ca.dyf = h->d_dyf; ca.kx = h->d_kx; ca.kz = h->d_kz;
-  for (int f = 0; f < 3; ++f) ca.u[f] = h->U[f];
-  ca.p = h->P;
-  ca.do_cn = 0;
-  int rc = launch_columns<0>(h, ca);
+  fill_col_args(h, ca, 0.0);
+  int rc = launch_project(h, ca, 0);
   if (rc) return rc;
   HIPCHK(hipStreamSynchronize(h->st));
   return 0;
@@ -812,13 +974,13 @@ int dgpu_timing_enable(dgpu_handle* h, int32_t enable) {
   int rc = harvest(h);
   if (rc) return rc;
   h->timing = enable != 0;
-  for (int k = 0; k < 4; ++k) { h->t_ms[k] = 0.0; h->t_n[k] = 0; }
+  for (int k = 0; k < NKERN; ++k) { h->t_ms[k] = 0.0; h->t_n[k] = 0; }
   return 0;
 }
 
 int dgpu_timing_read(dgpu_handle* h, int32_t kernel, double* total_ms, int64_t* launches, const char** name) {
   if (!h || !total_ms || !launches) return fail(1, "null argument");
-  if (kernel < 0 || kernel > 3) return fail(2, "kernel id must be 0..3");
+  if (kernel < 0 || kernel >= NKERN) return fail(2, "kernel id out of range");
   HIPCHK(hipStreamSynchronize(h->st));
   int rc = harvest(h);
   if (rc) return rc;

@@ -140,8 +140,9 @@ int dgpu_last_step_ms(dgpu_handle *h, double *ms);
 /* Per-kernel device timing, for measurement only (no counterpart in the
  * reference).  While enabled, every launch of the step's kernels is bracketed by
  * HIP events on the handle's stream.  Kernel ids: 0 = z inverse transform of the
- * state, 1 = fused x pass (products), 2 = z forward transform + Fourier assembly,
- * 3 = column solves (CN + projection + pressure).  dgpu_timing_read returns the
+ * state, 1 = fused x pass (products), 2 = z forward transform of the products,
+ * 3 = Fourier assembly + Crank-Nicolson solves, 4 = projection + pressure update.
+ * dgpu_timing_read returns the
  * device time and launch count accumulated since the last enable, and the name. */
 int dgpu_timing_enable(dgpu_handle *h, int32_t enable);
 int dgpu_timing_read(dgpu_handle *h, int32_t kernel, double *total_ms, int64_t *launches,
