@@ -4,10 +4,12 @@
 // x[j + r*N/R] (r < R) for butterfly j, multiplies input r by w^(r*k) with
 // k = j mod Ns, w = exp(-+2 pi i/(Ns*R)), applies a radix-R DFT and writes
 // y[(j/Ns)*Ns*R + k + r*Ns]; Ns grows by R per stage and the output is in
-// natural order.  Radix-4 stages first, one radix-2 stage when log2(N) is odd.
-// Lines live in LDS at s[l*ld + n]; every thread of the block takes part, and a
-// stage reads all of its butterflies into registers before any write, so one LDS
-// buffer suffices (two barriers per stage).
+// natural order.  Radix-16 stages (in-register 4x4 butterflies), with one radix
+// 8 or 4 stage for leftover bits: 256 points take two passes through LDS.
+// Lines live in LDS at s[l*ld + swz(n)] (an XOR swizzle of the index within the
+// line that keeps the strided stage writes free of bank conflicts); every thread
+// of the block takes part, and a stage reads all of its butterflies into
+// registers before any write, so one LDS buffer suffices (two barriers per stage).
 //
 // Sign and scale follow FFTW: DIR = -1 is exp(-i...), DIR = +1 is exp(+i...),
 // neither is normalised.  Twiddles come from a table tw[m] = exp(-2 pi i m/N),
@@ -62,6 +64,74 @@ __device__ __forceinline__ void dft4(cx<T>& v0, cx<T>& v1, cx<T>& v2, cx<T>& v3)
   v3 = a1 - a3;
 }
 
+// exp(DIR * 2 pi i m / 16) for the m that the radix-8 and radix-16 butterflies use
+template <int DIR, typename T>
+__device__ __forceinline__ cx<T> w16(int m) {
+  constexpr double C = 0.92387953251128675613;  // cos(pi/8)
+  constexpr double S = 0.38268343236508977173;  // sin(pi/8)
+  constexpr double H = 0.70710678118654752440;  // cos(pi/4)
+  constexpr T sg = DIR < 0 ? T(-1) : T(1);
+  switch (m) {
+    case 1: return {T(C), sg * T(S)};
+    case 2: return {T(H), sg * T(H)};
+    case 3: return {T(S), sg * T(C)};
+    case 6: return {T(-H), sg * T(H)};
+    case 9: return {T(-C), -sg * T(S)};
+    default: return {T(1), T(0)};
+  }
+}
+
+// In-register DFTs of length 8 and 16 in natural order (Cooley-Tukey 2x4 and 4x4).
+template <int DIR, typename T>
+__device__ __forceinline__ void dft8(cx<T>* v) {
+  cx<T> e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+  cx<T> o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+  dft4<DIR>(e0, e1, e2, e3);
+  dft4<DIR>(o0, o1, o2, o3);
+  o1 = o1 * w16<DIR, T>(2);
+  o2 = rot<DIR>(o2);
+  o3 = o3 * w16<DIR, T>(6);
+  v[0] = e0 + o0; v[4] = e0 - o0;
+  v[1] = e1 + o1; v[5] = e1 - o1;
+  v[2] = e2 + o2; v[6] = e2 - o2;
+  v[3] = e3 + o3; v[7] = e3 - o3;
+}
+
+template <int DIR, typename T>
+__device__ __forceinline__ void dft16(cx<T>* v) {
+  cx<T> y[4][4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    y[r][0] = v[r]; y[r][1] = v[r + 4]; y[r][2] = v[r + 8]; y[r][3] = v[r + 12];
+    dft4<DIR>(y[r][0], y[r][1], y[r][2], y[r][3]);
+  }
+  y[1][1] = y[1][1] * w16<DIR, T>(1);
+  y[1][2] = y[1][2] * w16<DIR, T>(2);
+  y[1][3] = y[1][3] * w16<DIR, T>(3);
+  y[2][1] = y[2][1] * w16<DIR, T>(2);
+  y[2][2] = rot<DIR>(y[2][2]);
+  y[2][3] = y[2][3] * w16<DIR, T>(6);
+  y[3][1] = y[3][1] * w16<DIR, T>(3);
+  y[3][2] = y[3][2] * w16<DIR, T>(6);
+  y[3][3] = y[3][3] * w16<DIR, T>(9);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    cx<T> a = y[0][q], b = y[1][q], c = y[2][q], d = y[3][q];
+    dft4<DIR>(a, b, c, d);
+    v[q] = a; v[q + 4] = b; v[q + 8] = c; v[q + 12] = d;
+  }
+}
+
+// Within-line LDS index swizzle: the low bits of the element index are XORed with
+// the next bits so that the strided writes of the early Stockham stages (stride
+// R*Ns elements) spread over all banks.  A bijection on [0, N) for every N.
+template <typename T>
+__device__ __forceinline__ int swz(int n) {
+  constexpr int SH = sizeof(cx<T>) == 16 ? 4 : 5;  // 16 (complex double) or 32 (complex float) per 256 B
+  constexpr int MASK = (1 << SH) - 1;
+  return n ^ ((n >> SH) & MASK);
+}
+
 // One Stockham stage of radix R over LINES lines of length N.
 template <typename T, int N, int LINES, int NT, int DIR, int R>
 __device__ __forceinline__ void fft_stage(cx<T>* __restrict__ s, int ld,
@@ -69,17 +139,18 @@ __device__ __forceinline__ void fft_stage(cx<T>* __restrict__ s, int ld,
   constexpr int NB = LINES * (N / R);
   constexpr int PER = (NB + NT - 1) / NT;
   cx<T> v[PER][R];
-  int ob[PER];
+  int ob[PER], ol[PER];
 #pragma unroll
   for (int p = 0; p < PER; ++p) {
     const int b = tid + p * NT;
     ob[p] = -1;
+    ol[p] = 0;
     if (NB % NT == 0 || b < NB) {
       const int l = b / (N / R);
       const int j = b - l * (N / R);
-      cx<T>* line = s + l * ld;
+      const cx<T>* line = s + l * ld;
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[p][r] = line[j + r * (N / R)];
+      for (int r = 0; r < R; ++r) v[p][r] = line[swz<T>(j + r * (N / R))];
       const int k = j & (Ns - 1);
       const int tstep = N / (Ns * R);
 #pragma unroll
@@ -88,14 +159,19 @@ __device__ __forceinline__ void fft_stage(cx<T>* __restrict__ s, int ld,
         if constexpr (DIR > 0) w = conjg(w);
         v[p][r] = v[p][r] * w;
       }
-      if constexpr (R == 4) {
+      if constexpr (R == 16) {
+        dft16<DIR>(v[p]);
+      } else if constexpr (R == 8) {
+        dft8<DIR>(v[p]);
+      } else if constexpr (R == 4) {
         dft4<DIR>(v[p][0], v[p][1], v[p][2], v[p][3]);
       } else {
         cx<T> a = v[p][0], c = v[p][1];
         v[p][0] = a + c;
         v[p][1] = a - c;
       }
-      ob[p] = l * ld + (j / Ns) * Ns * R + k;
+      ob[p] = (j / Ns) * Ns * R + k;
+      ol[p] = l * ld;
     }
   }
   __syncthreads();
@@ -103,26 +179,45 @@ __device__ __forceinline__ void fft_stage(cx<T>* __restrict__ s, int ld,
   for (int p = 0; p < PER; ++p) {
     if (ob[p] >= 0) {
 #pragma unroll
-      for (int r = 0; r < R; ++r) s[ob[p] + r * Ns] = v[p][r];
+      for (int r = 0; r < R; ++r) s[ol[p] + swz<T>(ob[p] + r * Ns)] = v[p][r];
     }
   }
   __syncthreads();
 }
 
-// Batched in-place FFT of LINES lines of length N (power of two, >= 2) in LDS.
-// Caller must __syncthreads() after filling s; the routine ends with a barrier.
+// radix of stage st for a transform of 2^L points: radix-16 stages, then one
+// stage of 2, 4 or 8 for the remaining bits (2 bits left over: a radix-4 stage;
+// 1 bit: the last radix-16 becomes 8 x 4 rather than 16 x 2)
+constexpr int stage_radix(int L, int st) {
+  const int full = L / 4, rem = L % 4;
+  if (rem == 0) return 16;
+  if (rem == 1) {
+    if (full == 0) return 2;
+    if (st < full - 1) return 16;
+    return st == full - 1 ? 8 : 4;
+  }
+  if (st < full) return 16;
+  return 1 << rem;  // 4 or 8
+}
+constexpr int stage_count(int L) { return L % 4 == 0 ? L / 4 : L / 4 + 1; }
+
+template <typename T, int N, int LINES, int NT, int DIR, int ST, int NST>
+__device__ __forceinline__ void fft_stages(cx<T>* __restrict__ s, int ld, const cx<T>* __restrict__ tw,
+                                           int Ns, int tid) {
+  if constexpr (ST < NST) {
+    constexpr int R = stage_radix(ilog2(N), ST);
+    fft_stage<T, N, LINES, NT, DIR, R>(s, ld, tw, Ns, tid);
+    fft_stages<T, N, LINES, NT, DIR, ST + 1, NST>(s, ld, tw, Ns * R, tid);
+  }
+}
+
+// Batched in-place FFT of LINES lines of length N (power of two, >= 2) in LDS,
+// element n of line l at s[l*ld + swz<T>(n)].  Caller must __syncthreads() after
+// filling s; the routine ends with a barrier.
 template <typename T, int N, int LINES, int NT, int DIR>
 __device__ __forceinline__ void fft_lds(cx<T>* __restrict__ s, int ld,
                                         const cx<T>* __restrict__ tw, int tid) {
-  constexpr int L2 = ilog2(N);
-  constexpr int N4 = L2 / 2;
-  int Ns = 1;
-#pragma unroll
-  for (int st = 0; st < N4; ++st) {
-    fft_stage<T, N, LINES, NT, DIR, 4>(s, ld, tw, Ns, tid);
-    Ns *= 4;
-  }
-  if constexpr (L2 & 1) fft_stage<T, N, LINES, NT, DIR, 2>(s, ld, tw, Ns, tid);
+  fft_stages<T, N, LINES, NT, DIR, 0, stage_count(ilog2(N))>(s, ld, tw, 1, tid);
 }
 
 }  // namespace dgpu

@@ -50,14 +50,14 @@ __global__ __launch_bounds__(NT) void k_inv_z(Ptrs<const cd*, MAXO> src, Ptrs<cx
     const int ii = idx % TI, z = idx / TI, i = i0 + ii, p = zpack(g, z);
     cx<T> v = {T(0), T(0)};
     if (i < g.kxr && p >= 0) v = cvt<T>(C[(long long)p * g.kxp + i]);
-    s[ii * LD + z] = v;
+    s[ii * LD + swz<T>(z)] = v;
   }
   __syncthreads();
   fft_lds<T, NZ, TI, NT, +1>(s, LD, tw, tid);
   cx<T>* __restrict__ H = dst.p[f] + (long long)j * g.ph;
   for (int idx = tid; idx < TI * NZ; idx += NT) {
     const int ii = idx % TI, z = idx / TI, i = i0 + ii;
-    if (i < g.kxr) H[(long long)z * g.kxp + i] = s[ii * LD + z];
+    if (i < g.kxr) H[(long long)z * g.kxp + i] = s[ii * LD + swz<T>(z)];
   }
 }
 
@@ -70,7 +70,7 @@ __global__ __launch_bounds__(NT) void k_fwd_z(Ptrs<const cx<T>*, MAXO> src, Ptrs
   const cx<T>* __restrict__ H = src.p[f] + (long long)j * g.ph;
   for (int idx = tid; idx < TI * NZ; idx += NT) {
     const int ii = idx % TI, z = idx / TI, i = i0 + ii;
-    s[ii * LD + z] = (i < g.kxr) ? H[(long long)z * g.kxp + i] : cx<T>{T(0), T(0)};
+    s[ii * LD + swz<T>(z)] = (i < g.kxr) ? H[(long long)z * g.kxp + i] : cx<T>{T(0), T(0)};
   }
   __syncthreads();
   fft_lds<T, NZ, TI, NT, -1>(s, LD, tw, tid);
@@ -79,7 +79,7 @@ __global__ __launch_bounds__(NT) void k_fwd_z(Ptrs<const cx<T>*, MAXO> src, Ptrs
   for (int idx = tid; idx < TI * g.kzr; idx += NT) {
     const int ii = idx % TI, p = idx / TI, i = i0 + ii;
     if (i < g.kxr) {
-      cd v = cvt<double>(s[ii * LD + zpos(g, p)]);
+      cd v = cvt<double>(s[ii * LD + swz<T>(zpos(g, p))]);
       C[(long long)p * g.kxp + i] = rnz * v;
     }
   }
@@ -109,7 +109,7 @@ __device__ __forceinline__ cx<T> pair_spectrum(const cx<T>* __restrict__ A,
 // split the spectrum Z of a + i b into the half spectra of a and b at bin i
 template <typename T>
 __device__ __forceinline__ void pair_split(const cx<T>* __restrict__ Zl, int i, int nx, cx<T>& A, cx<T>& B) {
-  const cx<T> zk = Zl[i], zm = conjg(Zl[(nx - i) & (nx - 1)]);
+  const cx<T> zk = Zl[swz<T>(i)], zm = conjg(Zl[swz<T>((nx - i) & (nx - 1))]);
   const cx<T> sm = zk + zm, df = zk - zm;
   A = {T(0.5) * sm.x, T(0.5) * sm.y};
   B = {T(0.5) * df.y, T(-0.5) * df.x};
@@ -124,15 +124,16 @@ __global__ __launch_bounds__(NT) void k_inv_x(Ptrs<const cx<T>*, MAXO> src, Ptrs
   const cx<T>* __restrict__ H = src.p[f] + (long long)j * g.ph;
   for (int idx = tid; idx < ZP * NX; idx += NT) {
     const int q = idx / NX, n = idx % NX, z = zb + 2 * q;
-    s[idx] = pair_spectrum(H + (long long)z * g.kxp, H + (long long)(z + 1) * g.kxp, n, NX, g.nkx);
+    s[q * NX + swz<T>(n)] = pair_spectrum(H + (long long)z * g.kxp, H + (long long)(z + 1) * g.kxp, n, NX, g.nkx);
   }
   __syncthreads();
   fft_lds<T, NX, ZP, NT, +1>(s, NX, tw, tid);
   T* __restrict__ R = dst.p[f] + (long long)j * g.pp;
   for (int idx = tid; idx < ZP * NX; idx += NT) {
     const int q = idx / NX, n = idx % NX, z = zb + 2 * q;
-    R[(long long)z * NX + n] = s[idx].x;
-    R[(long long)(z + 1) * NX + n] = s[idx].y;
+    const cx<T> v = s[q * NX + swz<T>(n)];
+    R[(long long)z * NX + n] = v.x;
+    R[(long long)(z + 1) * NX + n] = v.y;
   }
 }
 
@@ -145,7 +146,7 @@ __global__ __launch_bounds__(NT) void k_fwd_x(Ptrs<const T*, MAXO> src, Ptrs<cx<
   const T* __restrict__ R = src.p[f] + (long long)j * g.pp;
   for (int idx = tid; idx < ZP * NX; idx += NT) {
     const int q = idx / NX, n = idx % NX, z = zb + 2 * q;
-    s[idx] = {R[(long long)z * NX + n], R[(long long)(z + 1) * NX + n]};
+    s[q * NX + swz<T>(n)] = {R[(long long)z * NX + n], R[(long long)(z + 1) * NX + n]};
   }
   __syncthreads();
   fft_lds<T, NX, ZP, NT, -1>(s, NX, tw, tid);
@@ -195,7 +196,7 @@ __global__ __launch_bounds__(NT) void k_xprod(XProdArgs a) {
   for (int idx = tid; idx < NL * NX; idx += NT) {
     const int l = idx / NX, n = idx % NX, f = l / 3, jj = j - 1 + (l - 3 * f);
     const cx<T>* H = (const cx<T>*)a.hin[f] + (long long)jj * g.ph;
-    s[idx] = pair_spectrum(H + (long long)z * g.kxp, H + (long long)(z + 1) * g.kxp, n, NX, g.nkx);
+    s[l * NX + swz<T>(n)] = pair_spectrum(H + (long long)z * g.kxp, H + (long long)(z + 1) * g.kxp, n, NX, g.nkx);
   }
   __syncthreads();
   fft_lds<T, NX, NL, NT, +1>(s, NX, tw, tid);
@@ -208,15 +209,15 @@ __global__ __launch_bounds__(NT) void k_xprod(XProdArgs a) {
     cx<T> U1[3], U2[3], U3[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      U1[d] = s[(0 * 3 + d) * NX + x];
-      U2[d] = s[(1 * 3 + d) * NX + x];
-      U3[d] = s[(2 * 3 + d) * NX + x];
+      U1[d] = s[(0 * 3 + d) * NX + swz<T>(x)];
+      U2[d] = s[(1 * 3 + d) * NX + swz<T>(x)];
+      U3[d] = s[(2 * 3 + d) * NX + swz<T>(x)];
     }
     cx<T> TH[NTH > 0 ? NTH : 1][3];
 #pragma unroll
     for (int n = 0; n < NTH; ++n)
 #pragma unroll
-      for (int d = 0; d < 3; ++d) TH[n][d] = s[((3 + n) * 3 + d) * NX + x];
+      for (int d = 0; d < 3; ++d) TH[n][d] = s[((3 + n) * 3 + d) * NX + swz<T>(x)];
     cx<T> P[NP];
     // the two real lines of the pair are the .x and .y components
 #define DG_PROD(expr_x, expr_y) cx<T>{expr_x, expr_y}
@@ -269,7 +270,7 @@ __global__ __launch_bounds__(NT) void k_xprod(XProdArgs a) {
 #undef C
     // each thread rewrites only column x, which only it has read
 #pragma unroll
-    for (int p = 0; p < NP; ++p) s[p * NX + x] = P[p];
+    for (int p = 0; p < NP; ++p) s[p * NX + swz<T>(x)] = P[p];
   }
   __syncthreads();
   fft_lds<T, NX, NP, NT, -1>(s, NX, tw, tid);
@@ -347,7 +348,7 @@ __global__ __launch_bounds__(NT) void k_fwdz_nl(NlArgs a) {
   for (int idx = tid; idx < 2 * TI * NZ; idx += NT) {
     const int ii = idx % TI, r = idx / TI, which = r / NZ, z = r - which * NZ, i = i0 + ii;
     const cx<T>* Hs = which ? HZ : HN;
-    s[(which * TI + ii) * LD + z] = (i < g.kxr) ? Hs[(long long)z * g.kxp + i] : cx<T>{T(0), T(0)};
+    s[(which * TI + ii) * LD + swz<T>(z)] = (i < g.kxr) ? Hs[(long long)z * g.kxp + i] : cx<T>{T(0), T(0)};
   }
   __syncthreads();
   fft_lds<T, NZ, 2 * TI, NT, -1>(s, LD, tw, tid);
@@ -356,7 +357,7 @@ __global__ __launch_bounds__(NT) void k_fwdz_nl(NlArgs a) {
     const int ii = idx % TI, k = idx / TI, i = i0 + ii;
     if (i >= g.kxr) continue;
     const int zp = zpos(g, k);
-    const cx<T> fn = s[ii * LD + zp], fz = s[(TI + ii) * LD + zp];
+    const cx<T> fn = s[ii * LD + swz<T>(zp)], fz = s[(TI + ii) * LD + swz<T>(zp)];
     NL[(long long)k * g.kxp + i] = rnz * (fn - ik((T)a.kz[k], fz));
   }
 }

@@ -77,7 +77,7 @@ def noise_floor(p, s0, substeps):
     return {k: max(rel(fa[k], base[k]), rel(fb[k], base[k])) for k in base}
 
 
-@pytest.mark.parametrize("nx,ny,nz", [(16, 17, 16), (64, 33, 32), (32, 9, 128)])
+@pytest.mark.parametrize("nx,ny,nz", [(16, 17, 16), (64, 33, 32), (32, 9, 128), (256, 5, 64), (512, 5, 16), (16, 5, 512), (1024, 5, 32), (32, 5, 1024)])
 def test_transforms_match_oracle(nx, ny, nz):
     cfg = configs.small_dns(nx, ny, nz)
     p = Problem(cfg)
