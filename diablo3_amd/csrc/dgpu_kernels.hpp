@@ -547,7 +547,8 @@ __global__ __launch_bounds__(NT) void k_cn(ColArgs a) {
   const int lvl = a.levels;
   cd* sR = (cd*)smem;
   double* sCo = (double*)(sR + NF * TC * ny2);  // one field's PCR table: (2*lvl+1) rows of ny2
-  auto R = [&](int f, int j) -> cd& { return sR[(f * TC + cc) * ny2 + j]; };
+  // rows of the TC columns interleaved: lanes of one row group touch consecutive 16-byte slots
+  auto R = [&](int f, int j) -> cd& { return sR[(f * ny2 + j) * TC + cc]; };
   const cd* __restrict__ u1 = a.u[0];
   const cd* __restrict__ u2 = a.u[1];
   const cd* __restrict__ u3 = a.u[2];
@@ -655,7 +656,7 @@ __global__ __launch_bounds__(NT) void k_project(ColArgs a, int update_p) {
   for (int m = 0; m < RPT; ++m) {
     const int j = rg + m * RG;
     if (j >= ny2) break;
-    const int q = cc * ny2 + j;
+    const int q = j * TC + cc;
     double aj = 0.0, bj = 1.0, cj = 0.0;
     cd d = {0.0, 0.0};
     if (j == 1) {
@@ -682,21 +683,21 @@ __global__ __launch_bounds__(NT) void k_project(ColArgs a, int update_p) {
     for (int m = 0; m < RPT; ++m) {
       const int j = rg + m * RG;
       if (j >= 1 && j <= ny) {
-        const int q = cc * ny2 + j;
+        const int q = j * TC + cc, qm = q - s * TC, qp = q + s * TC;
         const int jm = j - s, jp = j + s;
         double a2 = 0.0, c2 = 0.0, b2 = sB[q];
         cd d2 = sD[q];
         if (jm >= 1) {
-          const double al = -sA[q] / sB[q - s];
-          a2 = al * sA[q - s];
-          b2 = b2 + al * sC[q - s];
-          d2 = d2 + al * sD[q - s];
+          const double al = -sA[q] / sB[qm];
+          a2 = al * sA[qm];
+          b2 = b2 + al * sC[qm];
+          d2 = d2 + al * sD[qm];
         }
         if (jp <= ny) {
-          const double ga = -sC[q] / sB[q + s];
-          c2 = ga * sC[q + s];
-          b2 = b2 + ga * sA[q + s];
-          d2 = d2 + ga * sD[q + s];
+          const double ga = -sC[q] / sB[qp];
+          c2 = ga * sC[qp];
+          b2 = b2 + ga * sA[qp];
+          d2 = d2 + ga * sD[qp];
         }
         na[m] = a2; nb[m] = b2; nc[m] = c2; nd[m] = d2;
       }
@@ -706,7 +707,7 @@ __global__ __launch_bounds__(NT) void k_project(ColArgs a, int update_p) {
     for (int m = 0; m < RPT; ++m) {
       const int j = rg + m * RG;
       if (j >= 1 && j <= ny) {
-        const int q = cc * ny2 + j;
+        const int q = j * TC + cc;
         sA[q] = na[m]; sB[q] = nb[m]; sC[q] = nc[m]; sD[q] = nd[m];
       }
     }
@@ -715,7 +716,7 @@ __global__ __launch_bounds__(NT) void k_project(ColArgs a, int update_p) {
   for (int m = 0; m < RPT; ++m) {
     const int j = rg + m * RG;
     if (j >= ny2) break;
-    const int q = cc * ny2 + j;
+    const int q = j * TC + cc;
     sD[q] = (j >= 1 && j <= ny) ? (1.0 / sB[q]) * sD[q] : cd{0.0, 0.0};
   }
   __syncthreads();
@@ -725,10 +726,10 @@ __global__ __launch_bounds__(NT) void k_project(ColArgs a, int update_p) {
     if (j >= ny2) break;
     if (j < 1 || j > ny) continue;
     const long long o = (long long)j * pc + c;
-    const cd phi = sD[cc * ny2 + j];
+    const cd phi = sD[j * TC + cc];
     u1[o] = u1[o] - ik(kxi, phi);
     u3[o] = u3[o] - ik(kzk, phi);
-    if (j >= 2) u2[o] = u2[o] - (1.0 / a.dy[j]) * (phi - sD[cc * ny2 + j - 1]);
+    if (j >= 2) u2[o] = u2[o] - (1.0 / a.dy[j]) * (phi - sD[(j - 1) * TC + cc]);
     if (update_p && j >= a.jstart && j <= a.jend) a.p[o] = a.p[o] + (1.0 / a.temp4) * phi;
   }
 }

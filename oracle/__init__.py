@@ -116,6 +116,15 @@ class Oracle:
         return ([self._layout.from_fortran_fourier(self.cfg, a) for a in f],
                 [self._layout.from_fortran_fourier(self.cfg, th[n]) for n in range(self.cfg.n_th)])
 
+    def download_les(self):
+        """nu_t and kappa_t[n] as compact physical [j][z][x] arrays."""
+        shp = self._layout.fortran_physical_shape(self.cfg)
+        nt = np.zeros(shp, np.float64)
+        kt = np.zeros((max(self.cfg.n_th, 1),) + shp, np.float64)
+        self._check(lib().oracle_download_les(self.h, _dp(nt), _dp(kt)))
+        return (self._layout.from_fortran_physical(self.cfg, nt),
+                [self._layout.from_fortran_physical(self.cfg, kt[n]) for n in range(self.cfg.n_th)])
+
     def rk_substep(self, rk_step, delta_t=None, time=0.0, time_step=1, create_new_flow=1):
         dt = self.cfg.delta_t if delta_t is None else delta_t
         self._check(lib().oracle_rk_substep(self.h, rk_step, dt, time, time_step, int(create_new_flow)))

@@ -119,8 +119,17 @@ struct oracle {
   zc *cf1, *cf2, *cf3, *cfth, *cr1, *cr2, *cr3, *crth, *cs1;
   double *u1, *u2, *u3, *th, *r1, *r2, *r3, *rth, *s1; /* physical [j][z][x] */
   double *nu_t, *kappa_t;
+  /* LES work arrays (allocated when use_les): physical [j][z][x] and a
+   * Fourier scratch; the index names follow les.f90 */
+  double *les_p;   /* NLESP physical arrays, see les_phys() */
+  zc *les_c;       /* one Fourier scratch array */
   fftplan px, pz;
 };
+/* physical LES arrays: Sij1..6, SIij1..6, Oij4..6, du1dx du2dy du3dz du2dx du3dx
+ * du1dz du2dz du1dy du3dy, temp, temp_th, s1_th, then dthetadx/dy/dz per scalar */
+enum { L_S1 = 0, L_SI1 = 6, L_O4 = 12, L_DU1DX = 15, L_DU2DY, L_DU3DZ, L_DU2DX, L_DU3DX,
+       L_DU1DZ, L_DU2DZ, L_DU1DY, L_DU3DY, L_TEMP, L_TEMPTH, L_S1TH, L_TH0 };
+#define NLESP(nth) (L_TH0 + 3 * (nth))
 
 #define FI(j, k, i) ((((size_t)(j)) * o->kzr + (size_t)(k)) * o->nxp + (size_t)(i))
 #define PI(j, z, x) ((((size_t)(j)) * o->nz + (size_t)(z)) * o->nx + (size_t)(x))
@@ -198,6 +207,15 @@ int oracle_create(const dgpu_config *cfg, oracle_t **out) {
   o->r1 = xcalloc(o->np, 8); o->r2 = xcalloc(o->np, 8); o->r3 = xcalloc(o->np, 8);
   o->rth = xcalloc(o->np * nt, 8); o->s1 = xcalloc(o->np, 8);
   o->nu_t = xcalloc(o->np, 8); o->kappa_t = xcalloc(o->np * nt, 8);
+  if (cfg->use_les) {
+    if (cfg->les_model_type != 5) {
+      set_err("oracle: only les_model_type 5 (AMD) is restated");
+      oracle_destroy(o);
+      return 2;
+    }
+    o->les_p = xcalloc(o->np * NLESP(nt), 8);
+    o->les_c = xcalloc(o->nf * 6, sizeof(zc));
+  }
   plan_init(&o->px, o->nx);
   plan_init(&o->pz, o->nz);
   *out = o;
@@ -209,7 +227,8 @@ void oracle_destroy(oracle_t *o) {
   void *ps[] = {o->gy, o->gyf, o->dy, o->dyf, o->kx, o->kz, o->kx2, o->kz2,
                 o->cu1, o->cu2, o->cu3, o->cp, o->cth, o->cf1, o->cf2, o->cf3,
                 o->cfth, o->cr1, o->cr2, o->cr3, o->crth, o->cs1, o->u1, o->u2,
-                o->u3, o->th, o->r1, o->r2, o->r3, o->rth, o->s1, o->nu_t, o->kappa_t};
+                o->u3, o->th, o->r1, o->r2, o->r3, o->rth, o->s1, o->nu_t, o->kappa_t,
+                o->les_p, o->les_c};
   for (size_t i = 0; i < sizeof ps / sizeof ps[0]; i++) free(ps[i]);
   plan_free(&o->px);
   plan_free(&o->pz);
@@ -629,6 +648,479 @@ static void cn_solve_phys(oracle_t *o, int which, double temp1, double temp2,
   }
 }
 
+/* ------------------------------------------------------------------------- */
+/* LES: AMD closure (les_model_type 5), for/les.f90                           */
+/* ------------------------------------------------------------------------- */
+/* apply_BC_vel_mpi_post / apply_BC_th_mpi_post at one rank: the lower and the
+ * upper routines both run (for/flow.f90:1161-1360, 1596-1732).  Loops run over
+ * i < Nxp, k <= 2Nkz; the mean mode (0,0) is the rankZ == 0 special case. */
+static void bc_post(oracle_t *o, double time) {
+  const dgpu_config *c = &o->c;
+  const int ny = o->ny, kzr = o->kzr, nxp = o->nxp;
+  const double *dy = o->dy, *dyf = o->dyf;
+  zc *u1 = o->cu1, *u2 = o->cu2, *u3 = o->cu3;
+#define PL(a, j) for (int k = 0; k < kzr; k++) for (int i = 0; i < nxp; i++)
+  /* lower: u1, u3 (1168-1220) */
+  zc *uu[2] = {u1, u3};
+  const int bl[2] = {c->u_bc_ymin, c->w_bc_ymin};
+  const double cl[2] = {c->u_bc_ymin_c1, c->w_bc_ymin_c1};
+  for (int m = 0; m < 2; m++) {
+    zc *a = uu[m];
+    if (bl[m] == DGPU_BC_DIRICHLET) {
+      PL(a, 0) { a[FI(0, k, i)] = 0.0; a[FI(1, k, i)] = 0.0; }
+      a[FI(1, 0, 0)] = cl[m];
+      a[FI(0, 0, 0)] = cl[m];
+    } else {
+      PL(a, 0) { a[FI(1, k, i)] = a[FI(2, k, i)]; a[FI(0, k, i)] = a[FI(1, k, i)]; }
+      a[FI(1, 0, 0)] = a[FI(2, 0, 0)] - dy[2] * cl[m];
+      a[FI(0, 0, 0)] = a[FI(1, 0, 0)] - dy[1] * cl[m];
+    }
+  }
+  /* lower: u2 (1222-1243); the Dirichlet form adds 2 c1 to every mode */
+  if (c->v_bc_ymin == DGPU_BC_DIRICHLET) {
+    PL(u2, 0) { u2[FI(1, k, i)] = 2.0 * c->v_bc_ymin_c1 - u2[FI(2, k, i)]; u2[FI(0, k, i)] = u2[FI(1, k, i)]; }
+  } else {
+    PL(u2, 0) { u2[FI(1, k, i)] = u2[FI(2, k, i)]; u2[FI(0, k, i)] = u2[FI(1, k, i)]; }
+    u2[FI(1, 0, 0)] = u2[FI(2, 0, 0)] - dyf[1] * c->v_bc_ymin_c1;
+    u2[FI(0, 0, 0)] = u2[FI(1, 0, 0)] - dyf[1] * c->v_bc_ymin_c1;
+  }
+  /* upper: u1 (1266-1288), u3 (1290-1334) */
+  const int bu[2] = {c->u_bc_ymax, c->w_bc_ymax};
+  const double cu[2] = {c->u_bc_ymax_c1, w_top_c1(o, time)};
+  for (int m = 0; m < 2; m++) {
+    zc *a = uu[m];
+    if (bu[m] == DGPU_BC_DIRICHLET) {
+      PL(a, 0) { a[FI(ny, k, i)] = 0.0; a[FI(ny + 1, k, i)] = 0.0; }
+      a[FI(ny, 0, 0)] = (m == 0) ? c->u_bc_ymax_c1 : c->w_bc_ymax_c1;
+      a[FI(ny + 1, 0, 0)] = (m == 0) ? c->u_bc_ymax_c1 : c->w_bc_ymax_c1;
+      if (m == 1) a[FI(ny + 1, 0, 0)] = 0.0; /* for/flow.f90:1311 */
+    } else {
+      PL(a, 0) { a[FI(ny, k, i)] = a[FI(ny - 1, k, i)]; a[FI(ny + 1, k, i)] = a[FI(ny, k, i)]; }
+      a[FI(ny, 0, 0)] = a[FI(ny - 1, 0, 0)] + dy[ny] * cu[m];
+      a[FI(ny + 1, 0, 0)] = a[FI(ny, 0, 0)] + dy[ny] * cu[m];
+    }
+  }
+  /* upper: u2 (1336-1356); the Dirichlet form touches only the mean mode */
+  if (c->v_bc_ymax == DGPU_BC_DIRICHLET) {
+    u2[FI(ny + 1, 0, 0)] = 2.0 * c->v_bc_ymax_c1 - u2[FI(ny, 0, 0)];
+  } else {
+    PL(u2, 0) { u2[FI(ny, k, i)] = u2[FI(ny - 1, k, i)]; u2[FI(ny + 1, k, i)] = u2[FI(ny, k, i)]; }
+    u2[FI(ny + 1, 0, 0)] = u2[FI(ny, 0, 0)] + dy[ny] * c->v_bc_ymax_c1;
+  }
+  /* scalars (1596-1689) */
+  for (int n = 0; n < o->nth; n++) {
+    zc *a = o->cth + n * o->nf;
+    if (c->th_bc_ymin[n] == DGPU_BC_DIRICHLET) {
+      PL(a, 0) { a[FI(0, k, i)] = 0.0; a[FI(1, k, i)] = 0.0; }
+      a[FI(1, 0, 0)] = c->th_bc_ymin_c1[n];
+      a[FI(0, 0, 0)] = c->th_bc_ymin_c1[n];
+    } else {
+      PL(a, 0) { a[FI(1, k, i)] = a[FI(2, k, i)]; a[FI(0, k, i)] = a[FI(1, k, i)]; }
+      a[FI(1, 0, 0)] = a[FI(2, 0, 0)] - dy[2] * c->th_bc_ymin_c1[n];
+      a[FI(0, 0, 0)] = a[FI(1, 0, 0)] - dy[1] * c->th_bc_ymin_c1[n];
+    }
+    if (c->th_bc_ymax[n] == DGPU_BC_DIRICHLET) {
+      PL(a, 0) { a[FI(ny, k, i)] = 0.0; a[FI(ny + 1, k, i)] = 0.0; }
+      a[FI(ny, 0, 0)] = c->th_bc_ymax_c1[n];
+      a[FI(ny + 1, 0, 0)] = c->th_bc_ymax_c1[n];
+    } else {
+      PL(a, 0) { a[FI(ny, k, i)] = a[FI(ny - 1, k, i)]; a[FI(ny + 1, k, i)] = a[FI(ny, k, i)]; }
+      a[FI(ny, 0, 0)] = a[FI(ny - 1, 0, 0)] + dy[ny] * c->th_bc_ymax_c1[n];
+      a[FI(ny + 1, 0, 0)] = a[FI(ny, 0, 0)] + dy[ny] * c->th_bc_ymax_c1[n];
+    }
+  }
+}
+
+/* apply_BC_les / apply_BC_th_les (for/les.f90:1980-2079): with a Neumann
+ * condition the two ghost planes copy the first interior plane, mean included */
+static void bc_les(oracle_t *o) {
+  const dgpu_config *c = &o->c;
+  const int ny = o->ny, kzr = o->kzr, nxp = o->nxp;
+  zc *uu[2] = {o->cu1, o->cu3};
+  const int bu[2] = {c->u_bc_ymax, c->w_bc_ymax}, bl[2] = {c->u_bc_ymin, c->w_bc_ymin};
+  for (int m = 0; m < 2; m++) {
+    zc *a = uu[m];
+    if (bu[m] == DGPU_BC_NEUMANN)
+      PL(a, 0) { a[FI(ny, k, i)] = a[FI(ny - 1, k, i)]; a[FI(ny + 1, k, i)] = a[FI(ny, k, i)]; }
+  }
+  for (int m = 0; m < 2; m++) {
+    zc *a = uu[m];
+    if (bl[m] == DGPU_BC_NEUMANN)
+      PL(a, 0) { a[FI(1, k, i)] = a[FI(2, k, i)]; a[FI(0, k, i)] = a[FI(1, k, i)]; }
+  }
+  for (int n = 0; n < o->nth; n++) {
+    zc *a = o->cth + n * o->nf;
+    if (c->th_bc_ymax[n] == DGPU_BC_NEUMANN)
+      PL(a, 0) { a[FI(ny, k, i)] = a[FI(ny - 1, k, i)]; a[FI(ny + 1, k, i)] = a[FI(ny, k, i)]; }
+    if (c->th_bc_ymin[n] == DGPU_BC_NEUMANN)
+      PL(a, 0) { a[FI(1, k, i)] = a[FI(2, k, i)]; a[FI(0, k, i)] = a[FI(1, k, i)]; }
+  }
+#undef PL
+}
+
+#define LOOP_PX(jlo, jhi) \
+  for (int j = (jlo); j <= (jhi); j++) for (int z = 0; z < nz; z++) for (int x = 0; x < nx; x++)
+#define LOOP_FX(jlo, jhi) \
+  for (int j = (jlo); j <= (jhi); j++) for (int k = 0; k < kzr; k++) for (int i = 0; i < nxp; i++)
+static double *les_phys(oracle_t *o, int slot) { return o->les_p + (size_t)slot * o->np; }
+
+/* The filter-width factors of les_chan type 5 (for/les.f90:756-780, 1316-1338):
+ * delta(j) = -2 C_AMD^2 * 3 / (1/(dx beta_sgs)^2 + 1/(2 h_j)^2 + 1/(dz beta_sgs)^2),
+ * with h = dyf (GYF points) or dy (GY points). */
+#define C_AMD 0.2887
+#define BETA_SGS 3.0
+static double amd_delta(const oracle_t *o, double h) {
+  const double dx = o->c.lx / o->nx, dz = o->c.lz / o->nz;
+  const double a = dx * BETA_SGS, b = h * 2.0, cz = dz * BETA_SGS;
+  return -(2.0 * (C_AMD * C_AMD) * 3.0 / (1.0 / (a * a) + 1.0 / (b * b) + 1.0 / (cz * cz)));
+}
+
+/* Fourier field built per plane by f(j, k, i), then transformed to physical
+ * space (the compute_* routines of for/les.f90:1443-1746).  Planes outside
+ * [jlo, jhi] are zero here; in the reference they hold stale values that no
+ * later statement reads. */
+static void les_grad(oracle_t *o, int jlo, int jhi, zc (*f)(const oracle_t *, int, int, int, int),
+                     int which, double *out) {
+  zc *cs = o->les_c;
+  memset(cs, 0, sizeof(zc) * o->nf);
+#pragma omp parallel for schedule(static)
+  for (int j = jlo; j <= jhi; j++)
+    for (int k = 0; k < o->kzr; k++)
+      for (int i = 0; i < o->nxp; i++) cs[FI(j, k, i)] = f(o, j, k, i, which);
+  to_physical(o, cs, out);
+}
+
+/* Every spectral field of compute_strain_chan, compute_strain_chan_invariant,
+ * compute_rotation_chan_invariant and compute_all_gradients_chan, by id. */
+enum { G_S1, G_S2, G_S3, G_S4, G_S5, G_S6, G_SI1, G_SI2, G_SI3, G_SI4, G_SI5, G_SI6,
+       G_O4, G_O5, G_O6, G_DU1DX, G_DU2DY, G_DU3DZ, G_DU2DX, G_DU3DX, G_DU1DZ, G_DU2DZ,
+       G_DU1DY, G_DU3DY, G_DTHDX, G_DTHDY, G_DTHDZ };
+static zc les_spec(const oracle_t *o, int j, int k, int i, int which) {
+  const zc ikx = I * o->kx[i], ikz = I * o->kz[k];
+  const double dx = o->c.lx / o->nx * BETA_SGS, dz = o->c.lz / o->nz * BETA_SGS;
+  const zc *u1 = o->cu1, *u2 = o->cu2, *u3 = o->cu3;
+  const size_t p = FI(j, k, i);
+  switch (which) {
+    /* compute_strain_chan (1452-1474) */
+    case G_S1: return ikx * u1[p];
+    case G_S3: return ikz * u3[p];
+    case G_S5: return 0.5 * (ikz * u1[p] + ikx * u3[p]);
+    case G_S2: return (u2[FI(j + 1, k, i)] - u2[p]) / o->dyf[j];
+    case G_S4: return 0.5 * ((u1[p] - u1[FI(j - 1, k, i)]) / o->dy[j] + ikx * u2[p]);
+    case G_S6: return 0.5 * (ikz * u2[p] + (u3[p] - u3[FI(j - 1, k, i)]) / o->dy[j]);
+    /* compute_strain_chan_invariant (1510-1542) */
+    case G_SI1: return ikx * u1[p];
+    case G_SI3: return ikz * u3[p];
+    case G_SI5: return 0.5 * (ikz * u1[p] * (dz / dx) + ikx * u3[p] * (dx / dz));
+    case G_SI2: return (u2[FI(j + 1, k, i)] - u2[p]) / o->dyf[j];
+    case G_SI4: {
+      const double dyl = o->dy[j] * 2.0;
+      return 0.5 * ((u1[p] - u1[FI(j - 1, k, i)]) / o->dy[j] * (dyl / dx) + ikx * u2[p] * (dx / dyl));
+    }
+    case G_SI6: {
+      const double dyl = o->dy[j] * 2.0;
+      return 0.5 * (ikz * u2[p] * (dz / dyl) + (u3[p] - u3[FI(j - 1, k, i)]) / o->dy[j] * (dyl / dz));
+    }
+    /* compute_rotation_chan_invariant (1617-1645) */
+    case G_O5: return 0.5 * (ikz * u1[p] * (dz / dx) - ikx * u3[p] * (dx / dz));
+    case G_O4: {
+      const double dyl = o->dy[j] * 2.0;
+      return 0.5 * ((u1[p] - u1[FI(j - 1, k, i)]) / o->dy[j] * (dyl / dx) - ikx * u2[p] * (dx / dyl));
+    }
+    case G_O6: {
+      const double dyl = o->dy[j] * 2.0;
+      return 0.5 * (ikz * u2[p] * (dz / dyl) - (u3[p] - u3[FI(j - 1, k, i)]) / o->dy[j] * (dyl / dz));
+    }
+    /* compute_all_gradients_chan (1663-1713) */
+    case G_DU1DX: return ikx * u1[p];
+    case G_DU1DZ: return ikz * u1[p];
+    case G_DU2DX: return ikx * u2[p];
+    case G_DU2DY: return (u2[FI(j + 1, k, i)] - u2[p]) / o->dyf[j];
+    case G_DU2DZ: return ikz * u2[p];
+    case G_DU3DX: return ikx * u3[p];
+    case G_DU3DZ: return ikz * u3[p];
+    case G_DU1DY: return (u1[p] - u1[FI(j - 1, k, i)]) / o->dy[j];
+    case G_DU3DY: return (u3[p] - u3[FI(j - 1, k, i)]) / o->dy[j];
+    default: return 0.0;
+  }
+}
+/* the scalar gradients of compute_all_gradients_chan (1690-1706), scalar n */
+static int g_les_n = 0;
+static zc les_spec_th(const oracle_t *o, int j, int k, int i, int which) {
+  const zc *t = o->cth + (size_t)g_les_n * o->nf;
+  const size_t p = FI(j, k, i);
+  if (which == G_DTHDX) return I * o->kx[i] * t[p];
+  if (which == G_DTHDZ) return I * o->kz[k] * t[p];
+  return (t[p] - t[FI(j - 1, k, i)]) / o->dy[j];
+}
+
+/* les_chan, les_model_type 5 (for/les.f90:3-75, 551-901): AMD eddy viscosity
+ * nu_t at GY points and the SGS stress divergence added to cf1..cf3; leaves
+ * u1..u3 in physical space.  Then les_chan_th type 5 (for/les.f90:1072-1117,
+ * 1201-1395): kappa_t and the SGS flux divergence added to cfth; th in
+ * physical space. */
+static void les_amd(oracle_t *o, double time) {
+  const dgpu_config *c = &o->c;
+  const int ny = o->ny, nx = o->nx, nz = o->nz, nxp = o->nxp, kzr = o->kzr;
+  const int js = c->jstart, je = c->jend;
+  const double *dy = o->dy, *dyf = o->dyf;
+  /* for/les.f90:63-75 */
+  bc_post(o, time);
+  bc_les(o);
+  /* compute_strain_chan: Sij1..Sij6.  CSij2(Nyp+1) holds the -10000 sentinel of
+   * ghost_CS2_mpi (for/les.f90:2509-2513); Sij2 is not read by type 5, so it is
+   * not transformed. */
+  double *S[6], *SI[6], *O[3];
+  for (int m = 0; m < 6; m++) { S[m] = les_phys(o, L_S1 + m); SI[m] = les_phys(o, L_SI1 + m); }
+  for (int m = 0; m < 3; m++) O[m] = les_phys(o, L_O4 + m);
+  les_grad(o, 0, ny + 1, les_spec, G_S1, S[0]);
+  les_grad(o, 0, ny + 1, les_spec, G_S3, S[2]);
+  les_grad(o, 0, ny + 1, les_spec, G_S5, S[4]);
+  les_grad(o, 1, ny + 1, les_spec, G_S4, S[3]);
+  les_grad(o, 1, ny + 1, les_spec, G_S6, S[5]);
+  /* compute_strain_chan_invariant, with the sentinel plane of CSIij2 */
+  les_grad(o, 0, ny + 1, les_spec, G_SI1, SI[0]);
+  {
+    zc *cs = o->les_c;
+    memset(cs, 0, sizeof(zc) * o->nf);
+    for (int j = 0; j <= ny; j++)
+      for (int k = 0; k < kzr; k++)
+        for (int i = 0; i < nxp; i++) cs[FI(j, k, i)] = les_spec(o, j, k, i, G_SI2);
+    for (int k = 0; k < kzr; k++)
+      for (int i = 0; i < nxp; i++) cs[FI(ny + 1, k, i)] = -10000.0;
+    to_physical(o, cs, SI[1]);
+  }
+  les_grad(o, 0, ny + 1, les_spec, G_SI3, SI[2]);
+  les_grad(o, 1, ny + 1, les_spec, G_SI4, SI[3]);
+  les_grad(o, 0, ny + 1, les_spec, G_SI5, SI[4]);
+  les_grad(o, 1, ny + 1, les_spec, G_SI6, SI[5]);
+  /* compute_rotation_chan_invariant */
+  les_grad(o, 1, ny + 1, les_spec, G_O4, O[0]);
+  les_grad(o, 0, ny + 1, les_spec, G_O5, O[1]);
+  les_grad(o, 1, ny + 1, les_spec, G_O6, O[2]);
+  /* compute_all_gradients_chan; du2dy at Nyp+1 would read cu2(Nyp+2), outside
+   * the array, and feeds only temp_th(Nyp+1), which nothing reads */
+  double *du1dx = les_phys(o, L_DU1DX), *du2dy = les_phys(o, L_DU2DY), *du3dz = les_phys(o, L_DU3DZ);
+  double *du2dx = les_phys(o, L_DU2DX), *du3dx = les_phys(o, L_DU3DX), *du1dz = les_phys(o, L_DU1DZ);
+  double *du2dz = les_phys(o, L_DU2DZ), *du1dy = les_phys(o, L_DU1DY), *du3dy = les_phys(o, L_DU3DY);
+  les_grad(o, 0, ny + 1, les_spec, G_DU1DX, du1dx);
+  les_grad(o, 0, ny, les_spec, G_DU2DY, du2dy);
+  les_grad(o, 0, ny + 1, les_spec, G_DU3DZ, du3dz);
+  les_grad(o, 0, ny + 1, les_spec, G_DU2DX, du2dx);
+  les_grad(o, 0, ny + 1, les_spec, G_DU3DX, du3dx);
+  les_grad(o, 0, ny + 1, les_spec, G_DU1DZ, du1dz);
+  les_grad(o, 0, ny + 1, les_spec, G_DU2DZ, du2dz);
+  les_grad(o, 1, ny + 1, les_spec, G_DU1DY, du1dy);
+  les_grad(o, 1, ny + 1, les_spec, G_DU3DY, du3dy);
+  for (int n = 0; n < o->nth; n++) {
+    g_les_n = n;
+    les_grad(o, 1, ny + 1, les_spec_th, G_DTHDX, les_phys(o, L_TH0 + 3 * n));
+    les_grad(o, 1, ny + 1, les_spec_th, G_DTHDY, les_phys(o, L_TH0 + 3 * n + 1));
+    les_grad(o, 1, ny + 1, les_spec_th, G_DTHDZ, les_phys(o, L_TH0 + 3 * n + 2));
+  }
+  to_physical(o, o->cu1, o->u1);
+  to_physical(o, o->cu2, o->u2);
+  to_physical(o, o->cu3, o->u3);
+  const double *u1 = o->u1, *u3 = o->u3;
+  double *s1 = o->s1, *temp = les_phys(o, L_TEMP);
+
+  /* max{0,-(I3-I4)}/(I1-I2) at GYF points (for/les.f90:574-619) */
+#pragma omp parallel for schedule(static)
+  for (int j = js; j <= je; j++)
+    for (int z = 0; z < nz; z++)
+      for (int x = 0; x < nx; x++) {
+        const size_t p = PI(j, z, x), q = PI(j + 1, z, x);
+        const double a1 = SI[0][p], a2 = SI[1][p], a3 = SI[2][p], a5 = SI[4][p];
+        const double a4 = 0.5 * (SI[3][q] + SI[3][p]), a6 = 0.5 * (SI[5][q] + SI[5][p]);
+        const double o4 = 0.5 * (O[0][q] + O[0][p]), o5 = O[1][p], o6 = 0.5 * (O[2][q] + O[2][p]);
+        double v = a1 * a1 * a1 + a2 * a2 * a2 + a3 * a3 * a3 + 3.0 * a1 * (a4 * a4) + 3.0 * a1 * (a5 * a5) +
+                   3.0 * a2 * (a4 * a4) + 3.0 * a2 * (a6 * a6) + 3.0 * a3 * (a5 * a5) + 3.0 * a3 * (a6 * a6) +
+                   6.0 * a4 * a5 * a6;
+        v = -v - a1 * (o4 * o4) - a1 * (o5 * o5) - a2 * (o4 * o4) - a2 * (o6 * o6) - a3 * (o5 * o5) -
+            a3 * (o6 * o6) - 2.0 * a4 * o5 * o6 + 2.0 * a5 * o4 * o6 - 2.0 * a6 * o4 * o5;
+        if (v <= 0.0) v = 0.0;
+        else
+          v = v / (a1 * a1 + a2 * a2 + a3 * a3 + 2.0 * (a4 * a4) + 2.0 * (o4 * o4) + 2.0 * (a5 * a5) +
+                   2.0 * (o5 * o5) + 2.0 * (a6 * a6) + 2.0 * (o6 * o6));
+        s1[p] = v;
+      }
+  /* the same at GY points, GYF values interpolated (for/les.f90:625-740) */
+#pragma omp parallel for schedule(static)
+  for (int j = 2; j <= ny; j++) {
+    const double wc = dyf[j - 1], wm = dyf[j], den = 2.0 * dy[j];
+    for (int z = 0; z < nz; z++)
+      for (int x = 0; x < nx; x++) {
+        const size_t p = PI(j, z, x), m = PI(j - 1, z, x);
+        const double a1 = (SI[0][p] * wc + SI[0][m] * wm) / den;
+        const double a2 = (SI[1][p] * wc + SI[1][m] * wm) / den;
+        const double a3 = (SI[2][p] * wc + SI[2][m] * wm) / den;
+        const double a5 = (SI[4][p] * wc + SI[4][m] * wm) / den;
+        const double o5 = (O[1][p] * wc + O[1][m] * wm) / den;
+        const double a4 = SI[3][p], a6 = SI[5][p], o4 = O[0][p], o6 = O[2][p];
+        double v = a1 * a1 * a1 + a2 * a2 * a2 + a3 * a3 * a3 + 3.0 * a1 * (a4 * a4) + 3.0 * a1 * (a5 * a5) +
+                   3.0 * a2 * (a4 * a4) + 3.0 * a2 * (a6 * a6) + 3.0 * a3 * (a5 * a5) + 3.0 * a3 * (a6 * a6) +
+                   6.0 * a4 * a6 * a5;
+        v = -v - a1 * (o4 * o4) - a1 * (o5 * o5) - a2 * (o4 * o4) - a2 * (o6 * o6) - a3 * (o5 * o5) -
+            a3 * (o6 * o6) - 2.0 * a4 * o5 * o6 + 2.0 * a5 * o4 * o6 - 2.0 * a6 * o4 * o5;
+        if (v <= 0.0) v = 0.0;
+        else
+          v = v / (a1 * a1 + a2 * a2 + a3 * a3 + 2.0 * (a4 * a4) + 2.0 * (o4 * o4) + 2.0 * (a5 * a5) +
+                   2.0 * (o5 * o5) + 2.0 * (a6 * a6) + 2.0 * (o6 * o6));
+        temp[p] = v;
+      }
+  }
+  /* coefficient times S_ij (for/les.f90:745-773); temp(Nyp+1) is never set by
+   * the reference (a stale value that feeds only planes nothing reads): 0 here */
+  for (size_t q = PI(ny + 1, 0, 0); q < PI(ny + 2, 0, 0); q++) temp[q] = 0.0;
+  LOOP_PX(js, je) {
+    const size_t p = PI(j, z, x);
+    S[0][p] = s1[p] * S[0][p];
+    S[4][p] = s1[p] * S[4][p];
+    S[1][p] = 0.0;
+    S[2][p] = s1[p] * S[2][p];
+  }
+  LOOP_PX(2, ny + 1) {
+    const size_t p = PI(j, z, x), m = PI(j - 1, z, x);
+    S[3][p] = temp[p] * (S[3][p] - 0.5 * (u1[p] - u1[m]) / dy[j]);
+    S[5][p] = temp[p] * (S[5][p] - 0.5 * (u3[p] - u3[m]) / dy[j]);
+  }
+  double *dyfac[2];
+  dyfac[0] = (double *)xcalloc((size_t)o->ny2, 8);
+  dyfac[1] = (double *)xcalloc((size_t)o->ny2, 8);
+  for (int j = 1; j <= ny + 1; j++) { dyfac[0][j] = amd_delta(o, dyf[j]); dyfac[1][j] = amd_delta(o, dy[j]); }
+  /* nu_t = -delta_y/2 * coefficient at GY, zero at j = 1, 2, Nyp, Nyp+1 (ghost_les_mpi) */
+  LOOP_PX(2, ny + 1) o->nu_t[PI(j, z, x)] = -0.5 * dyfac[1][j] * temp[PI(j, z, x)];
+  LOOP_PX(1, 2) o->nu_t[PI(j, z, x)] = 0.0;
+  LOOP_PX(ny, ny + 1) o->nu_t[PI(j, z, x)] = 0.0;
+  /* tau_ij in Fourier space and its divergence into cf (for/les.f90:800-901) */
+  zc *C1 = o->les_c + o->nf, *C3 = o->les_c + 2 * o->nf, *C4 = o->les_c + 3 * o->nf;
+  zc *C5 = o->les_c + 4 * o->nf, *C6 = o->les_c + 5 * o->nf;
+  to_fourier(o, S[0], C1);
+  to_fourier(o, S[2], C3);
+  to_fourier(o, S[3], C4);
+  to_fourier(o, S[4], C5);
+  to_fourier(o, S[5], C6);
+  LOOP_FX(1, ny) {
+    const size_t p = FI(j, k, i);
+    C1[p] = dyfac[0][j] * C1[p];
+    C5[p] = dyfac[0][j] * C5[p];
+    C3[p] = dyfac[0][j] * C3[p];
+  }
+  LOOP_FX(1, ny + 1) {
+    const size_t p = FI(j, k, i);
+    C4[p] = dyfac[1][j] * C4[p];
+    C6[p] = dyfac[1][j] * C6[p];
+  }
+  /* one rank: rankY == NprocY-1 sets j1 = jstart, j2 = Nyp-1 (for/les.f90:45-57) */
+  LOOP_FX(js, ny - 1) {
+    const size_t p = FI(j, k, i), pu = FI(j + 1, k, i);
+    const zc ikx = I * o->kx[i], ikz = I * o->kz[k];
+    o->cf1[p] = o->cf1[p] - ikx * C1[p] - (C4[pu] - C4[p]) / dyf[j] - ikz * C5[p];
+    o->cf3[p] = o->cf3[p] - ikx * C5[p] - (C6[pu] - C6[p]) / dyf[j] - ikz * C3[p];
+  }
+  LOOP_FX(2, ny) {
+    const size_t p = FI(j, k, i);
+    o->cf2[p] = o->cf2[p] - I * o->kx[i] * C4[p] - I * o->kz[k] * C6[p];
+  }
+  /* the cross terms treated explicitly (for/les.f90:874-901) */
+  for (int m = 0; m < 2; m++) {
+    const double *uu = m == 0 ? u1 : u3;
+    memset(temp, 0, 8 * o->np);
+    LOOP_PX(2, ny) {
+      const size_t p = PI(j, z, x);
+      temp[p] = o->nu_t[p] * (uu[p] - uu[PI(j - 1, z, x)]) / dy[j];
+    }
+    to_fourier(o, temp, C6);
+    LOOP_FX(2, ny) {
+      const size_t p = FI(j, k, i);
+      o->cf2[p] = o->cf2[p] + (m == 0 ? I * o->kx[i] : I * o->kz[k]) * C6[p];
+    }
+  }
+
+  /* ---- les_chan_th, type 5 ---- */
+  const double dxs = c->lx / nx * BETA_SGS, dzs = c->lz / nz * BETA_SGS;
+  double *s1th = les_phys(o, L_S1TH), *tth = les_phys(o, L_TEMPTH);
+  for (int n = 0; n < o->nth; n++) to_physical(o, o->cth + n * o->nf, o->th + n * o->np);
+  for (int n = 0; n < o->nth; n++) {
+    double *tx = les_phys(o, L_TH0 + 3 * n), *ty = les_phys(o, L_TH0 + 3 * n + 1);
+    double *tz = les_phys(o, L_TH0 + 3 * n + 2);
+    const int jst = c->jstart_th[n], jet = c->jend_th[n];
+    /* numerator / denominator at GYF (for/les.f90:1221-1256) */
+#pragma omp parallel for schedule(static)
+    for (int j = jst; j <= jet; j++) {
+      const double dyl = dyf[j] * 2.0;
+      for (int z = 0; z < nz; z++)
+        for (int x = 0; x < nx; x++) {
+          const size_t p = PI(j, z, x), q = PI(j + 1, z, x);
+          const double tyc = 0.5 * (ty[q] + ty[p]);
+          double v = -(dxs * dxs * du1dx[p] * (tx[p] * tx[p]) + dyl * dyl * du2dy[p] * (tyc * tyc) +
+                       dzs * dzs * du3dz[p] * (tz[p] * tz[p]) +
+                       (dyl * dyl * 0.5 * (du1dy[q] + du1dy[p]) + dxs * dxs * 0.5 * (du2dx[q] + du2dx[p])) *
+                           tx[p] * tyc +
+                       (dzs * dzs * du1dz[p] + dxs * dxs * du3dx[p]) * tx[p] * tz[p] +
+                       (dzs * dzs * 0.5 * (du2dz[q] + du2dz[p]) + dyl * dyl * 0.5 * (du3dy[q] + du3dy[p])) *
+                           tyc * tz[p]);
+          if (v <= 0.0) v = 0.0;
+          else {
+            const double ax = dxs * tx[p], ay = dyl * tyc, az = dzs * tz[p];
+            v = v / (ax * ax + ay * ay + az * az);
+          }
+          s1th[p] = v;
+        }
+    }
+    /* at GY (for/les.f90:1259-1306), j = 2..Nyp: temp_th(Nyp+1) is not read */
+#pragma omp parallel for schedule(static)
+    for (int j = 2; j <= ny; j++) {
+      const double dyl = dy[j] * 2.0, wc = dyf[j - 1], wm = dyf[j], den = 2.0 * dy[j];
+      for (int z = 0; z < nz; z++)
+        for (int x = 0; x < nx; x++) {
+          const size_t p = PI(j, z, x), m = PI(j - 1, z, x);
+          /* evaluation order of the reference expression */
+          const double S2 = tx[p] * wc + tx[m] * wm, S5 = tz[p] * wc + tz[m] * wm;
+          const double X2 = dzs * dzs * (du1dz[p] * wc + du1dz[m] * wm) / den +
+                            dxs * dxs * (du3dx[p] * wc + du3dx[m] * wm) / den;
+          double v = -(dxs * dxs * (du1dx[p] * wc + du1dx[m] * wm) / den * ((S2 / den) * (S2 / den)) +
+                       dyl * dyl * (du2dy[p] * wc + du2dy[m] * wm) / den * (ty[p] * ty[p]) +
+                       dzs * dzs * (du3dz[p] * wc + du3dz[m] * wm) / den * ((S5 / den) * (S5 / den)) +
+                       (dyl * dyl * du1dy[p] + dxs * dxs * du2dx[p]) * S2 / den * ty[p] +
+                       X2 * S2 / den * S5 / den +
+                       (dzs * dzs * du2dz[p] + dyl * dyl * du3dy[p]) * ty[p] * S5 / den);
+          if (v <= 0.0) v = 0.0;
+          else {
+            const double ax = dxs * S2 / den, ay = dyl * ty[p], az = dzs * S5 / den;
+            v = v / (ax * ax + ay * ay + az * az);
+          }
+          tth[p] = v;
+        }
+    }
+    LOOP_PX(jst, jet) {
+      const size_t p = PI(j, z, x);
+      tx[p] = s1th[p] * tx[p];
+      tz[p] = s1th[p] * tz[p];
+    }
+    double *kt = o->kappa_t + n * o->np;
+    LOOP_PX(2, ny) kt[PI(j, z, x)] = -0.5 * dyfac[1][j] * tth[PI(j, z, x)];
+    LOOP_PX(1, 2) kt[PI(j, z, x)] = 0.0;
+    LOOP_PX(ny, ny + 1) kt[PI(j, z, x)] = 0.0;
+    to_fourier(o, tx, C1);
+    to_fourier(o, tz, C5);
+    LOOP_FX(1, ny + 1) {
+      const size_t p = FI(j, k, i);
+      C1[p] = 0.5 * dyfac[0][j] * C1[p];
+      C5[p] = 0.5 * dyfac[0][j] * C5[p];
+    }
+    zc *cfth = o->cfth + n * o->nf;
+    LOOP_FX(jst, jet) {
+      const size_t p = FI(j, k, i);
+      cfth[p] = cfth[p] - I * o->kx[i] * C1[p] - I * o->kz[k] * C5[p];
+    }
+  }
+  free(dyfac[0]);
+  free(dyfac[1]);
+}
+#undef LOOP_PX
+#undef LOOP_FX
+
 int oracle_rk_substep(oracle_t *o, int rk_step, double delta_t, double time,
                       int time_step, int create_new_flow) {
   const dgpu_config *c = &o->c;
@@ -647,7 +1139,6 @@ int oracle_rk_substep(oracle_t *o, int rk_step, double delta_t, double time,
   const double temp4 = h_bar[r];
   const double temp5 = beta_bar[r] * h_bar[r];
   const int les_on = c->use_les && ((!create_new_flow) || (time_step > 100));
-  if (les_on) { set_err("oracle: LES branch not available"); return 3; }
   (void)gy;
 
 #define CIKX(i) (I * o->kx[(i)])
@@ -781,11 +1272,16 @@ int oracle_rk_substep(oracle_t *o, int rk_step, double delta_t, double time,
                                 (dl / ro) * (-1.0) * dtz * (gyf[j] - 0.5 * ly) * CIKX(i) * cth[FI(j, k, i)];
     }
   }
-  /* DNS branch: state to physical space (635-643) */
-  to_physical(o, o->cu1, o->u1);
-  to_physical(o, o->cu2, o->u2);
-  to_physical(o, o->cu3, o->u3);
-  for (int n = 0; n < nth; n++) to_physical(o, o->cth + n * o->nf, o->th + n * o->np);
+  if (les_on) {
+    /* les_chan + les_chan_th (611-623): SGS terms into cf, state to physical space */
+    les_amd(o, time);
+  } else {
+    /* state to physical space (635-643) */
+    to_physical(o, o->cu1, o->u1);
+    to_physical(o, o->cu2, o->u2);
+    to_physical(o, o->cu3, o->u3);
+    for (int n = 0; n < nth; n++) to_physical(o, o->cth + n * o->nf, o->th + n * o->np);
+  }
 
   const double *u1 = o->u1, *u2 = o->u2, *u3 = o->u3;
   double *s1 = o->s1;
