@@ -17,7 +17,7 @@ namespace dgpu {
 
 constexpr int MAXTH = 4;
 constexpr int MAXF = 3 + MAXTH;         // u1, u2, u3, th_1..th_n
-constexpr int MAXO = 6 + 2 * MAXTH;     // fused product spectra
+constexpr int MAXO = 3 * MAXF;         // largest transform batch (LES: v, dv/dx, dv/dz)
 
 struct Geo {
   int nx, nz, ny, ny2, nkx, nkz, kxr, kzr, kxp;
@@ -39,17 +39,24 @@ __device__ __forceinline__ int zpack(const Geo& g, int z) {
 // ---------------------------------------------------------------------------
 // z transforms: inverse (packed C -> H) and forward (H -> packed C)
 // ---------------------------------------------------------------------------
+// Field f of a batch with bit f of dzmask set is multiplied by i kz before the
+// transform (the x-spectrum of its z derivative).
 template <typename T, int NZ, int TI, int NT>
 __global__ __launch_bounds__(NT) void k_inv_z(Ptrs<const cd*, MAXO> src, Ptrs<cx<T>*, MAXO> dst,
-                                              Geo g, int j0, const cx<T>* __restrict__ tw) {
+                                              Geo g, int j0, const cx<T>* __restrict__ tw,
+                                              const double* __restrict__ kz, unsigned dzmask) {
   constexpr int LD = NZ + 1;
   __shared__ cx<T> s[TI * LD];
   const int f = blockIdx.z, j = j0 + blockIdx.y, i0 = blockIdx.x * TI, tid = threadIdx.x;
   const cd* __restrict__ C = src.p[f] + (long long)j * g.pc;
+  const bool dz = (dzmask >> f) & 1u;
   for (int idx = tid; idx < TI * NZ; idx += NT) {
     const int ii = idx % TI, z = idx / TI, i = i0 + ii, p = zpack(g, z);
     cx<T> v = {T(0), T(0)};
-    if (i < g.kxr && p >= 0) v = cvt<T>(C[(long long)p * g.kxp + i]);
+    if (i < g.kxr && p >= 0) {
+      const cd c = C[(long long)p * g.kxp + i];
+      v = cvt<T>(dz ? ik(kz[p], c) : c);
+    }
     s[ii * LD + swz<T>(z)] = v;
   }
   __syncthreads();
@@ -115,16 +122,38 @@ __device__ __forceinline__ void pair_split(const cx<T>* __restrict__ Zl, int i, 
   B = {T(0.5) * df.y, T(-0.5) * df.x};
 }
 
+// the pair spectrum of the x derivatives of lines z and z+1 (bins times i kx)
+template <typename T>
+__device__ __forceinline__ cx<T> pair_spectrum_dx(const cx<T>* __restrict__ A, const cx<T>* __restrict__ B,
+                                                  int n, int nx, int nkx, const double* __restrict__ kx) {
+  if (n >= 1 && n <= nkx) {
+    const T k = (T)kx[n];
+    const cx<T> a = ik(k, A[n]), b = ik(k, B[n]);
+    return {a.x - b.y, a.y + b.x};
+  }
+  if (n >= nx - nkx) {
+    const T k = (T)kx[nx - n];
+    const cx<T> a = ik(k, A[nx - n]), b = ik(k, B[nx - n]);
+    return {a.x + b.y, b.x - a.y};
+  }
+  return {T(0), T(0)};  // DC (kx = 0) and de-aliased bins
+}
+
+// Field f of a batch with bit f of dxmask set yields its x derivative.
 template <typename T, int NX, int ZP, int NT>
 __global__ __launch_bounds__(NT) void k_inv_x(Ptrs<const cx<T>*, MAXO> src, Ptrs<T*, MAXO> dst,
-                                              Geo g, int j0, const cx<T>* __restrict__ tw) {
+                                              Geo g, int j0, const cx<T>* __restrict__ tw,
+                                              const double* __restrict__ kx, unsigned dxmask) {
   __shared__ cx<T> s[ZP * NX];
   const int f = blockIdx.z, j = j0 + blockIdx.y, tid = threadIdx.x;
   const int zb = blockIdx.x * ZP * 2;
   const cx<T>* __restrict__ H = src.p[f] + (long long)j * g.ph;
+  const bool dx = (dxmask >> f) & 1u;
   for (int idx = tid; idx < ZP * NX; idx += NT) {
     const int q = idx / NX, n = idx % NX, z = zb + 2 * q;
-    s[q * NX + swz<T>(n)] = pair_spectrum(H + (long long)z * g.kxp, H + (long long)(z + 1) * g.kxp, n, NX, g.nkx);
+    const cx<T>* A = H + (long long)z * g.kxp;
+    const cx<T>* B = H + (long long)(z + 1) * g.kxp;
+    s[q * NX + swz<T>(n)] = dx ? pair_spectrum_dx(A, B, n, NX, g.nkx, kx) : pair_spectrum(A, B, n, NX, g.nkx);
   }
   __syncthreads();
   fft_lds<T, NX, ZP, NT, +1>(s, NX, tw, tid);
@@ -182,9 +211,13 @@ struct XProdArgs {
   const void* hin[MAXF];   // x-spectra of u1, u2, u3, th_n (all planes)
   void* hout[MAXO];        // N1, N2, N3, Z1, Z2, Z3, (Nt, Zt) per scalar
   const void* tw;
+  // LES: physical SGS fields added to products 1, 0, 2, 5, 6, 3, 4, (8, 9) + 3n
+  // (tau_11, tau_13, tau_33, the two stress fluxes, the two GY stresses, the
+  // scalar fluxes); null for DNS
+  const void* aug[7 + 2 * MAXTH];
 };
 
-template <typename T, int NX, int NT, int NTH>
+template <typename T, int NX, int NT, int NTH, bool LES>
 __global__ __launch_bounds__(NT) void k_xprod(XProdArgs a) {
   constexpr int NF = 3 + NTH, NL = 3 * NF, NP = 8 + 3 * NTH, NO = 6 + 2 * NTH;
   static_assert(NP <= NL, "product lines overlay input lines");
@@ -268,6 +301,25 @@ __global__ __launch_bounds__(NT) void k_xprod(XProdArgs a) {
     }
 #undef DG_PROD
 #undef C
+    if constexpr (LES) {
+      const long long pz = (long long)j * g.pp + (long long)z * NX + x;
+      auto A = [&](int m) {
+        const T* p = (const T*)a.aug[m];
+        return cx<T>{p[pz], p[pz + NX]};
+      };
+      P[1] = P[1] + A(0);
+      P[0] = P[0] + A(1);
+      P[2] = P[2] + A(2);
+      P[5] = P[5] + A(3);
+      P[6] = P[6] + A(4);
+      P[3] = P[3] + A(5);
+      P[4] = P[4] + A(6);
+#pragma unroll
+      for (int n = 0; n < NTH; ++n) {
+        P[8 + 3 * n] = P[8 + 3 * n] + A(7 + 2 * n);
+        P[9 + 3 * n] = P[9 + 3 * n] + A(8 + 2 * n);
+      }
+    }
     // each thread rewrites only column x, which only it has read
 #pragma unroll
     for (int p = 0; p < NP; ++p) s[p * NX + swz<T>(x)] = P[p];
@@ -420,6 +472,7 @@ struct ColArgs {
   cd* p;
   cd* cf[MAXF];
   const void* nl[MAXF];
+  cd* crout[MAXF];  // do_cn == 0: right-hand side rows [blo, bhi] of the physical-space solve
 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2).
@@ -577,16 +630,27 @@ __global__ __launch_bounds__(NT) void k_cn(ColArgs a) {
         assemble<NTH>(a, r, a.rowc + (long long)j * RC_COUNT, e, i, k, kxi, kzk, kk, cf, v);
         a.cf[e][o] = cf;
       }
-      const int sr = special_row(j, ny);
-      const int kd = sr >= 0 ? SR.kind[sr] : -1;
-      if (kd == 0) v = {0.0, 0.0};
-      else if (kd == 1) v = (c == 0) ? cd{SR.val[sr], 0.0} : cd{0.0, 0.0};
-      else if (!(j >= SR.blo && j <= SR.bhi)) v = {0.0, 0.0};
+      if (a.do_cn) {
+        const int sr = special_row(j, ny);
+        const int kd = sr >= 0 ? SR.kind[sr] : -1;
+        if (kd == 0) v = {0.0, 0.0};
+        else if (kd == 1) v = (c == 0) ? cd{SR.val[sr], 0.0} : cd{0.0, 0.0};
+        else if (!(j >= SR.blo && j <= SR.bhi)) v = {0.0, 0.0};
+      } else if (!(inner && j >= SR.blo && j <= SR.bhi)) {
+        v = {0.0, 0.0};  // physical-space solve: its BC rows are set per column there
+      }
       crv[e] = v;
+    }
+    if (!a.do_cn) {
+      if (active)
+#pragma unroll
+        for (int e = 0; e < NF; ++e) a.crout[e][o] = crv[e];
+      continue;
     }
 #pragma unroll
     for (int e = 0; e < NF; ++e) R(e, j) = crv[e];
   }
+  if (!a.do_cn) return;  // uniform over the grid
   __syncthreads();
 
   // ---- 2. PCR with the host's coefficients, scalars first (for/solver.f90:1004-1007)

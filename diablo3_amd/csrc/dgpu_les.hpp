@@ -1,0 +1,449 @@
+// dgpu_les.hpp — kernels of the LES branch of rk_chan_1 (use_LES after the
+// spin-up gate of for/solver.f90:615), les_model_type 5: the anisotropic
+// minimum-dissipation (AMD) closure of les_chan / les_chan_th (for/les.f90).
+//
+// The reference forms 27 spectral gradient fields, transforms each to physical
+// space, evaluates the model, and transforms 9 stress fields back (for/les.f90:
+// 551-901, 1201-1395).  Here:
+//  * the state's ghost planes get the BC posts of les_chan (k_bc_post);
+//  * x-spectra of u, du/dz (k_inv_z with the ikz option) give, by one inverse x
+//    transform each, u, du/dx (ikx option) and du/dz in physical space: 3 fields
+//    per transported variable instead of 27 transforms (every strain, rotation
+//    and invariant tensor entry is a pointwise combination of these, and
+//    d/dy is the reference's own difference stencil applied in physical space);
+//  * k_amd evaluates nu_t, kappa_t and the SGS stresses per point and writes them
+//    as additions to the nonlinear product fields whose spectral derivatives the
+//    reference applies to the stress (tau_11 joins u1u1, tau_13 joins u1u3, ...),
+//    so the stress divergence rides on the product transforms of k_xprod;
+//  * the Crank–Nicolson solve with nu_t runs per physical (x, z) column
+//    (k_cn_phys), as in the reference, since nu_t varies in x and z.
+// Transforms are linear, so moving a linear operation across one changes only
+// round-off; the placement of every non-linear operation is the reference's.
+#pragma once
+#include "dgpu_kernels.hpp"
+
+namespace dgpu {
+
+constexpr int MAXA = 7 + 2 * MAXTH;  // SGS addition fields
+constexpr double C_AMD = 0.2887;     // for/les.f90:25
+constexpr double BETA_SGS = 3.0;     // for/les.f90:43
+
+// ---------------------------------------------------------------------------
+// apply_BC_vel_mpi_post + apply_BC_th_mpi_post at one rank (for/flow.f90:
+// 1161-1360, 1596-1732), then apply_BC_les / apply_BC_th_les (for/les.f90:
+// 1980-2079), on the Fourier state.  One thread per (kx, kz) column; the mean
+// mode is column 0.
+// ---------------------------------------------------------------------------
+struct BcArgs {
+  Geo g;
+  int nth;
+  int u_lo, v_lo, w_lo, u_hi, v_hi, w_hi, th_lo[MAXTH], th_hi[MAXTH];
+  double u_lo_c1, v_lo_c1, w_lo_c1, u_hi_c1, v_hi_c1, w_hi_c1, w_hi_tr, th_lo_c1[MAXTH], th_hi_c1[MAXTH];
+  double dy1, dy2, dyny, dyf1;
+  cd* u[MAXF];
+};
+
+__global__ __launch_bounds__(256) void k_bc_post(BcArgs a) {
+  const Geo& g = a.g;
+  const long long c = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (c >= g.pc) return;
+  if (c % g.kxp >= g.kxr) return;
+  const bool mean = (c == 0);
+  const int ny = g.ny;
+  auto at = [&](cd* f, int j) -> cd& { return f[(long long)j * g.pc + c]; };
+  const cd z0 = {0.0, 0.0};
+  // GYF fields u1, u3 and every scalar share the form of the lower/upper posts
+  auto gyf_lo = [&](cd* f, int bc, double c1) {
+    if (bc == DGPU_BC_DIRICHLET) {
+      at(f, 0) = z0; at(f, 1) = z0;
+      if (mean) { at(f, 1) = cd{c1, 0.0}; at(f, 0) = cd{c1, 0.0}; }
+    } else {
+      at(f, 1) = at(f, 2); at(f, 0) = at(f, 1);
+      if (mean) { at(f, 1) = at(f, 2) - cd{a.dy2 * c1, 0.0}; at(f, 0) = at(f, 1) - cd{a.dy1 * c1, 0.0}; }
+    }
+  };
+  auto gyf_hi = [&](cd* f, int bc, double c1d, double c1n, bool w) {
+    if (bc == DGPU_BC_DIRICHLET) {
+      at(f, ny) = z0; at(f, ny + 1) = z0;
+      if (mean) { at(f, ny) = cd{c1d, 0.0}; at(f, ny + 1) = cd{c1d, 0.0}; }
+      if (w && mean) at(f, ny + 1) = z0;  // for/flow.f90:1311
+    } else {
+      at(f, ny) = at(f, ny - 1); at(f, ny + 1) = at(f, ny);
+      if (mean) {
+        at(f, ny) = at(f, ny - 1) + cd{a.dyny * c1n, 0.0};
+        at(f, ny + 1) = at(f, ny) + cd{a.dyny * c1n, 0.0};
+      }
+    }
+  };
+  cd *u1 = a.u[0], *u2 = a.u[1], *u3 = a.u[2];
+  // lower: u1, u3, then u2
+  gyf_lo(u1, a.u_lo, a.u_lo_c1);
+  gyf_lo(u3, a.w_lo, a.w_lo_c1);
+  if (a.v_lo == DGPU_BC_DIRICHLET) {
+    at(u2, 1) = cd{2.0 * a.v_lo_c1, 0.0} - at(u2, 2);
+    at(u2, 0) = at(u2, 1);
+  } else {
+    at(u2, 1) = at(u2, 2); at(u2, 0) = at(u2, 1);
+    if (mean) {
+      at(u2, 1) = at(u2, 2) - cd{a.dyf1 * a.v_lo_c1, 0.0};
+      at(u2, 0) = at(u2, 1) - cd{a.dyf1 * a.v_lo_c1, 0.0};
+    }
+  }
+  // upper: u1, u3, then u2
+  gyf_hi(u1, a.u_hi, a.u_hi_c1, a.u_hi_c1, false);
+  gyf_hi(u3, a.w_hi, a.w_hi_c1, a.w_hi_tr, true);
+  if (a.v_hi == DGPU_BC_DIRICHLET) {
+    if (mean) at(u2, ny + 1) = cd{2.0 * a.v_hi_c1, 0.0} - at(u2, ny);
+  } else {
+    at(u2, ny) = at(u2, ny - 1); at(u2, ny + 1) = at(u2, ny);
+    if (mean) at(u2, ny + 1) = at(u2, ny) + cd{a.dyny * a.v_hi_c1, 0.0};
+  }
+  for (int n = 0; n < a.nth; ++n) {
+    gyf_lo(a.u[3 + n], a.th_lo[n], a.th_lo_c1[n]);
+    gyf_hi(a.u[3 + n], a.th_hi[n], a.th_hi_c1[n], a.th_hi_c1[n], false);
+  }
+  // apply_BC_les: Neumann ghosts copy the first interior plane, mean included
+  if (a.u_hi == DGPU_BC_NEUMANN) { at(u1, ny) = at(u1, ny - 1); at(u1, ny + 1) = at(u1, ny); }
+  if (a.w_hi == DGPU_BC_NEUMANN) { at(u3, ny) = at(u3, ny - 1); at(u3, ny + 1) = at(u3, ny); }
+  if (a.u_lo == DGPU_BC_NEUMANN) { at(u1, 1) = at(u1, 2); at(u1, 0) = at(u1, 1); }
+  if (a.w_lo == DGPU_BC_NEUMANN) { at(u3, 1) = at(u3, 2); at(u3, 0) = at(u3, 1); }
+  for (int n = 0; n < a.nth; ++n) {
+    cd* t = a.u[3 + n];
+    if (a.th_hi[n] == DGPU_BC_NEUMANN) { at(t, ny) = at(t, ny - 1); at(t, ny + 1) = at(t, ny); }
+    if (a.th_lo[n] == DGPU_BC_NEUMANN) { at(t, 1) = at(t, 2); at(t, 0) = at(t, 1); }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// AMD model per point (for/les.f90:571-797, 1213-1346).
+// Inputs (physical, [j][z][x], all planes): for each transported field v in
+// (u1, u2, u3, th_n): v, dv/dx, dv/dz.  Outputs, on every plane (zero where the
+// reference adds nothing):
+//   nu_t, kappa_t_n            at GY points, zero at j = 0, 1, 2, Nyp, Nyp+1
+//   A0 = tau_11  (joins u1u1)       A1 = tau_13 (joins u1u3)   rows jstart..Nyp-1
+//   A2 = tau_33  (joins u3u3)       A3 = (tau_12(j+1)-tau_12(j))/dyf(j)  (u1 flux)
+//   A4 = (tau_23(j+1)-tau_23(j))/dyf(j)  (u3 flux)                        rows jstart..Nyp-1
+//   A5 = tau_12 - nu_t du1/dy (joins u1 u2 at GY)   A6 = tau_23 - nu_t du3/dy   rows 2..Nyp
+//   A7+2n = flux_x of th_n (joins th u1)  A8+2n = flux_z (joins th u3)  rows jstart_th..jend_th
+// with tau including the filter-width factor delta(j) of the reference.
+// ---------------------------------------------------------------------------
+struct AmdArgs {
+  Geo g;
+  int nth, jstart, jend, jstart_th[MAXTH], jend_th[MAXTH];
+  double dxs, dzs;                      // dx * beta_sgs, dz * beta_sgs
+  const double* __restrict__ dy;
+  const double* __restrict__ dyf;
+  const double* __restrict__ dlt;       // [2][ny2]: delta_yf(j), delta_y(j)
+  const void* f[3 * MAXF];              // v, dv/dx, dv/dz per transported field
+  void* nut;
+  void* kap[MAXTH];
+  void* aug[MAXA];
+};
+
+// the invariant-scaled strain/rotation entries of one GYF plane
+struct Gyf {
+  double a1, a2, a3, a5, o5;
+};
+// ... and of one GY plane
+struct Gy {
+  double a4, o4, a6, o6;
+};
+
+template <typename T>
+struct AmdIn {
+  const T* __restrict__ v[3 * MAXF];
+  long long pp;
+  __device__ double at(int f, int j, long long q) const { return (double)v[f][(long long)j * pp + q]; }
+};
+
+// field slots: 3*f + {0: v, 1: dv/dx, 2: dv/dz}, f = 0 u1, 1 u2, 2 u3, 3+n th_n
+template <typename T>
+__device__ __forceinline__ Gyf gyf_plane(const AmdIn<T>& in, const AmdArgs& a, int j, long long q) {
+  Gyf r;
+  const double du1dz = in.at(2, j, q), du3dx = in.at(7, j, q);
+  r.a1 = in.at(1, j, q);                                              // SI11 = du1/dx
+  r.a2 = (in.at(3, j + 1, q) - in.at(3, j, q)) / a.dyf[j];             // SI22 = du2/dy
+  r.a3 = in.at(8, j, q);                                              // SI33 = du3/dz
+  r.a5 = 0.5 * (du1dz * (a.dzs / a.dxs) + du3dx * (a.dxs / a.dzs));   // SI13
+  r.o5 = 0.5 * (du1dz * (a.dzs / a.dxs) - du3dx * (a.dxs / a.dzs));   // O13
+  return r;
+}
+template <typename T>
+__device__ __forceinline__ Gy gy_plane(const AmdIn<T>& in, const AmdArgs& a, int j, long long q) {
+  Gy r;
+  const double dyl = a.dy[j] * 2.0;
+  const double du1dy = (in.at(0, j, q) - in.at(0, j - 1, q)) / a.dy[j];
+  const double du3dy = (in.at(6, j, q) - in.at(6, j - 1, q)) / a.dy[j];
+  const double du2dx = in.at(4, j, q), du2dz = in.at(5, j, q);
+  r.a4 = 0.5 * (du1dy * (dyl / a.dxs) + du2dx * (a.dxs / dyl));
+  r.o4 = 0.5 * (du1dy * (dyl / a.dxs) - du2dx * (a.dxs / dyl));
+  r.a6 = 0.5 * (du2dz * (a.dzs / dyl) + du3dy * (dyl / a.dzs));
+  r.o6 = 0.5 * (du2dz * (a.dzs / dyl) - du3dy * (dyl / a.dzs));
+  return r;
+}
+
+// max{0, -(I3 - I4)} / (I1 - I2) from the six strain and three rotation entries
+__device__ __forceinline__ double amd_coef(double a1, double a2, double a3, double a4, double a5, double a6,
+                                           double o4, double o5, double o6) {
+  double v = a1 * a1 * a1 + a2 * a2 * a2 + a3 * a3 * a3 + 3.0 * a1 * (a4 * a4) + 3.0 * a1 * (a5 * a5) +
+             3.0 * a2 * (a4 * a4) + 3.0 * a2 * (a6 * a6) + 3.0 * a3 * (a5 * a5) + 3.0 * a3 * (a6 * a6) +
+             6.0 * a4 * a5 * a6;
+  v = -v - a1 * (o4 * o4) - a1 * (o5 * o5) - a2 * (o4 * o4) - a2 * (o6 * o6) - a3 * (o5 * o5) -
+      a3 * (o6 * o6) - 2.0 * a4 * o5 * o6 + 2.0 * a5 * o4 * o6 - 2.0 * a6 * o4 * o5;
+  if (v <= 0.0) return 0.0;
+  return v / (a1 * a1 + a2 * a2 + a3 * a3 + 2.0 * (a4 * a4) + 2.0 * (o4 * o4) + 2.0 * (a5 * a5) +
+              2.0 * (o5 * o5) + 2.0 * (a6 * a6) + 2.0 * (o6 * o6));
+}
+
+// coefficient at GYF j (GY entries averaged from j and j+1)
+template <typename T>
+__device__ __forceinline__ double coef_gyf(const AmdIn<T>& in, const AmdArgs& a, int j, long long q) {
+  const Gyf f = gyf_plane(in, a, j, q);
+  const Gy p = gy_plane(in, a, j + 1, q), m = gy_plane(in, a, j, q);
+  return amd_coef(f.a1, f.a2, f.a3, 0.5 * (p.a4 + m.a4), f.a5, 0.5 * (p.a6 + m.a6), 0.5 * (p.o4 + m.o4), f.o5,
+                  0.5 * (p.o6 + m.o6));
+}
+// coefficient at GY j (GYF entries interpolated from j-1 and j)
+template <typename T>
+__device__ __forceinline__ double coef_gy(const AmdIn<T>& in, const AmdArgs& a, int j, long long q) {
+  const Gyf f = gyf_plane(in, a, j, q), fm = gyf_plane(in, a, j - 1, q);
+  const Gy p = gy_plane(in, a, j, q);
+  const double wc = a.dyf[j - 1], wm = a.dyf[j], den = 2.0 * a.dy[j];
+  auto w = [&](double c, double m) { return (c * wc + m * wm) / den; };
+  return amd_coef(w(f.a1, fm.a1), w(f.a2, fm.a2), w(f.a3, fm.a3), p.a4, w(f.a5, fm.a5), p.a6, p.o4,
+                  w(f.o5, fm.o5), p.o6);
+}
+
+// tau_12 (e = 0) or tau_23 (e = 1) at GY j: delta_y(j) coef(j) (S - du/dy / 2).
+// At j = 1 the reference scales the strain itself (its coefficient loop starts
+// at j = 2, for/les.f90:762); that row feeds only rows the BC rows overwrite.
+template <typename T>
+__device__ __forceinline__ double tau_gy(const AmdIn<T>& in, const AmdArgs& a, int e, int j, long long q) {
+  const double dlt = a.dlt[a.g.ny2 + j];
+  if (j == 1) {
+    const double dvdy = (in.at(e ? 6 : 0, 1, q) - in.at(e ? 6 : 0, 0, q)) / a.dy[1];
+    return dlt * 0.5 * (dvdy + in.at(e ? 5 : 4, 1, q));
+  }
+  return dlt * (coef_gy(in, a, j, q) * (0.5 * in.at(e ? 5 : 4, j, q)));
+}
+
+template <typename T, int NTH>
+__global__ __launch_bounds__(256) void k_amd(AmdArgs a) {
+  const Geo& g = a.g;
+  const long long q = (long long)blockIdx.x * 256 + threadIdx.x;
+  const int j = blockIdx.y;
+  if (q >= g.pp) return;
+  AmdIn<T> in;
+  for (int s = 0; s < 3 * (3 + NTH); ++s) in.v[s] = (const T*)a.f[s];
+  in.pp = g.pp;
+  const int ny = g.ny;
+  const long long o = (long long)j * g.pp + q;
+  auto put = [&](void* p, double v) { ((T*)p)[o] = (T)v; };
+  // nu_t at GY j (zero on the wall rows of ghost_les_mpi)
+  double nut = 0.0;
+  if (j >= 3 && j <= ny - 1) nut = -0.5 * a.dlt[g.ny2 + j] * coef_gy(in, a, j, q);
+  put(a.nut, nut);
+  // GYF stresses, rows j1..j2 = jstart..Nyp-1
+  double A0 = 0.0, A1 = 0.0, A2 = 0.0, A3 = 0.0, A4 = 0.0;
+  if (j >= a.jstart && j <= ny - 1) {
+    const double dyf = a.dlt[j] * coef_gyf(in, a, j, q);
+    const double du1dz = in.at(2, j, q), du3dx = in.at(7, j, q);
+    A0 = dyf * in.at(1, j, q);
+    A1 = dyf * (0.5 * (du1dz + du3dx));
+    A2 = dyf * in.at(8, j, q);
+    A3 = (tau_gy(in, a, 0, j + 1, q) - tau_gy(in, a, 0, j, q)) / a.dyf[j];
+    A4 = (tau_gy(in, a, 1, j + 1, q) - tau_gy(in, a, 1, j, q)) / a.dyf[j];
+  }
+  put(a.aug[0], A0); put(a.aug[1], A1); put(a.aug[2], A2); put(a.aug[3], A3); put(a.aug[4], A4);
+  // GY stresses and the explicit cross terms, rows 2..Nyp
+  double A5 = 0.0, A6 = 0.0;
+  if (j >= 2 && j <= ny) {
+    A5 = tau_gy(in, a, 0, j, q) - nut * (in.at(0, j, q) - in.at(0, j - 1, q)) / a.dy[j];
+    A6 = tau_gy(in, a, 1, j, q) - nut * (in.at(6, j, q) - in.at(6, j - 1, q)) / a.dy[j];
+  }
+  put(a.aug[5], A5); put(a.aug[6], A6);
+  // scalars (les_chan_th type 5)
+#pragma unroll
+  for (int n = 0; n < NTH; ++n) {
+    const int ft = 3 * (3 + n);
+    double kap = 0.0, fx = 0.0, fz = 0.0;
+    const double dxs2 = a.dxs * a.dxs, dzs2 = a.dzs * a.dzs;
+    // kappa_t at GY j (for/les.f90:1259-1306, 1344-1348)
+    if (j >= 3 && j <= ny - 1) {
+      const double dyl = a.dy[j] * 2.0, dyl2 = dyl * dyl;
+      const double wc = a.dyf[j - 1], wm = a.dyf[j], den = 2.0 * a.dy[j];
+      auto w = [&](int s) { return (in.at(s, j, q) * wc + in.at(s, j - 1, q) * wm) / den; };
+      const double du1dx = w(1), du3dz = w(8), du1dz = w(2), du3dx = w(7);
+      const double du2dy = ((in.at(3, j + 1, q) - in.at(3, j, q)) / a.dyf[j] * wc +
+                            (in.at(3, j, q) - in.at(3, j - 1, q)) / a.dyf[j - 1] * wm) / den;
+      const double tx = w(ft + 1), tz = w(ft + 2);
+      const double ty = (in.at(ft, j, q) - in.at(ft, j - 1, q)) / a.dy[j];
+      const double du1dy = (in.at(0, j, q) - in.at(0, j - 1, q)) / a.dy[j];
+      const double du3dy = (in.at(6, j, q) - in.at(6, j - 1, q)) / a.dy[j];
+      const double du2dx = in.at(4, j, q), du2dz = in.at(5, j, q);
+      double v = -(dxs2 * du1dx * (tx * tx) + dyl2 * du2dy * (ty * ty) + dzs2 * du3dz * (tz * tz) +
+                   (dyl2 * du1dy + dxs2 * du2dx) * tx * ty + (dzs2 * du1dz + dxs2 * du3dx) * tx * tz +
+                   (dzs2 * du2dz + dyl2 * du3dy) * ty * tz);
+      if (v > 0.0) {
+        const double ax = a.dxs * tx, ay = dyl * ty, az = a.dzs * tz;
+        kap = -0.5 * a.dlt[g.ny2 + j] * (v / (ax * ax + ay * ay + az * az));
+      }
+    }
+    put(a.kap[n], kap);
+    // SGS flux at GYF j (for/les.f90:1221-1256, 1310-1318, 1372-1393)
+    if (j >= a.jstart_th[n] && j <= a.jend_th[n]) {
+      const double dyl = a.dyf[j] * 2.0, dyl2 = dyl * dyl;
+      auto gyv = [&](int s, int jj) { return (in.at(s, jj, q) - in.at(s, jj - 1, q)) / a.dy[jj]; };
+      const double tx = in.at(ft + 1, j, q), tz = in.at(ft + 2, j, q);
+      const double ty = 0.5 * (gyv(ft, j + 1) + gyv(ft, j));
+      const double du1dx = in.at(1, j, q), du3dz = in.at(8, j, q);
+      const double du2dy = (in.at(3, j + 1, q) - in.at(3, j, q)) / a.dyf[j];
+      const double du1dy = 0.5 * (gyv(0, j + 1) + gyv(0, j)), du3dy = 0.5 * (gyv(6, j + 1) + gyv(6, j));
+      const double du2dx = 0.5 * (in.at(4, j + 1, q) + in.at(4, j, q));
+      const double du2dz = 0.5 * (in.at(5, j + 1, q) + in.at(5, j, q));
+      const double du1dz = in.at(2, j, q), du3dx = in.at(7, j, q);
+      double v = -(dxs2 * du1dx * (tx * tx) + dyl2 * du2dy * (ty * ty) + dzs2 * du3dz * (tz * tz) +
+                   (dyl2 * du1dy + dxs2 * du2dx) * tx * ty + (dzs2 * du1dz + dxs2 * du3dx) * tx * tz +
+                   (dzs2 * du2dz + dyl2 * du3dy) * ty * tz);
+      double s1 = 0.0;
+      if (v > 0.0) {
+        const double ax = a.dxs * tx, ay = dyl * ty, az = a.dzs * tz;
+        s1 = v / (ax * ax + ay * ay + az * az);
+      }
+      const double f = 0.5 * a.dlt[j] * s1;
+      fx = f * tx;
+      fz = f * tz;
+    }
+    put(a.aug[7 + 2 * n], fx);
+    put(a.aug[8 + 2 * n], fz);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Crank–Nicolson solve per physical (x, z) column with the eddy viscosity /
+// diffusivity (rk_chan_1, for/solver.f90:875-903, 990-1233): the explicit half
+// of the nu_t term is added to the right-hand side r (the inverse transform of
+// the Fourier right-hand side, which already holds the molecular part), the
+// matrix gets the implicit half, the BC rows of apply_BC_* are set for every
+// column, and Thomas (thomas_forward/backward_real_mpi at one rank,
+// for/solver.f90:1691-1761, 1816-1852) writes rows [wlo, whi] of the field.
+// One thread per column; c' is kept in scratch, d' overwrites r.
+// ---------------------------------------------------------------------------
+struct PhysCnArgs {
+  Geo g;
+  int nf;
+  int field[MAXF];                    // slot -> field (0 u1, 1 u2, 2 u3, 3+n th_n)
+  int blo[MAXF], bhi[MAXF], wlo[MAXF], whi[MAXF];
+  int lo_kind[MAXF], hi_kind[MAXF];   // BC code at each wall (0 Dirichlet, 1 Neumann)
+  double lo_c1[MAXF], hi_c1[MAXF];
+  double tcn[MAXF];                   // temp1 nu_v_scale (/Pr for scalars)
+  double temp2;
+  const double* __restrict__ dy;
+  const double* __restrict__ dyf;
+  double* u[MAXF];                    // physical field, solution written in place
+  double* r[MAXF];                    // right-hand side; becomes d'
+  const double* visc[MAXF];           // nu_t (velocity) or kappa_t_n (scalar)
+  double* cp;                         // scratch, 2 * ny2 planes
+};
+
+__global__ __launch_bounds__(256) void k_cn_phys(PhysCnArgs a) {
+  const Geo& g = a.g;
+  const long long q = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (q >= g.pp) return;
+  const int ny = g.ny, n2 = g.ny2;
+  const long long pp = g.pp;
+  const double* __restrict__ dy = a.dy;
+  const double* __restrict__ dyf = a.dyf;
+  double* __restrict__ cp = a.cp;
+  for (int e = 0; e < a.nf; ++e) {
+    double* __restrict__ u = a.u[e];
+    double* __restrict__ r = a.r[e];
+    const double* __restrict__ nt = a.visc[e];
+    const double T = a.tcn[e], t2 = a.temp2;
+    const int fld = a.field[e];
+    const bool v2 = (fld == 1);
+    // row j of the system: (l, d, c | rhs)
+    auto row = [&](int j, double& l, double& d, double& c, double& rhs) {
+      l = 0.0; d = 1.0; c = 0.0; rhs = 0.0;
+      if (j >= a.blo[e] && j <= a.bhi[e]) {
+        const long long o = (long long)j * pp + q;
+        const double um = u[o - pp], uc = u[o], up = u[o + pp];
+        const double nm = nt[o - pp], nc = nt[o], np = nt[o + pp];
+        rhs = r[o];
+        if (!v2) {
+          l = -T / (dy[j] * dyf[j]);
+          // the u1/u3 builds write the diagonal as 1 - T(-a - b) (for/solver.f90:1136-1145)
+          d = (fld >= 3) ? 1. + T / (dy[j + 1] * dyf[j]) + T / (dy[j] * dyf[j])
+                       : 1. - T * (-1. / (dy[j + 1] * dyf[j]) - 1. / (dy[j] * dyf[j]));
+          c = -T / (dy[j + 1] * dyf[j]);
+          rhs = rhs + t2 * ((np * (up - uc) / dy[j + 1] - nc * (uc - um) / dy[j]) / dyf[j]);
+          l = l - t2 * nc / (dy[j] * dyf[j]);
+          d = d + t2 * np / (dy[j + 1] * dyf[j]) + t2 * nc / (dy[j] * dyf[j]);
+          c = c - t2 * np / (dy[j + 1] * dyf[j]);
+        } else {
+          l = -T / (dyf[j - 1] * dy[j]);
+          d = 1. + T / (dyf[j] * dy[j]) + T / (dyf[j - 1] * dy[j]);
+          c = -T / (dyf[j] * dy[j]);
+          rhs = rhs + t2 * ((0.5 * (nc + np) * (up - uc) / dyf[j] - 0.5 * (nc + nm) * (uc - um) / dyf[j - 1]) / dy[j]);
+          l = l - t2 * 0.5 * (nc + nm) / (dyf[j - 1] * dy[j]);
+          d = d + t2 * 0.5 * (nc + np) / (dyf[j] * dy[j]) + t2 * 0.5 * (nc + nm) / (dyf[j - 1] * dy[j]);
+          c = c - t2 * 0.5 * (nc + np) / (dyf[j] * dy[j]);
+        }
+      }
+      // BC rows (apply_BC_* of the reference, for/solver.f90:1901-2232)
+      if (!v2) {
+        if (j == 0) { l = 0.; d = 1.; c = 0.; rhs = 0.; }
+        if (j == 1) {
+          if (a.lo_kind[e] == DGPU_BC_DIRICHLET) { l = 0.; d = 1.; c = 0.; rhs = a.lo_c1[e]; }
+          else { l = 0.; d = -1.; c = 1.; rhs = dy[2] * a.lo_c1[e]; }
+        }
+        if (j == ny) {
+          if (a.hi_kind[e] == DGPU_BC_DIRICHLET) { l = 0.; d = 1.; c = 0.; rhs = a.hi_c1[e]; }
+          else { l = -1.; d = 1.; c = 0.; rhs = dy[ny] * a.hi_c1[e]; }
+        }
+        if (j == ny + 1) { l = 0.; d = 1.; c = 0.; rhs = 0.; }
+      } else {
+        if (a.lo_kind[e] == DGPU_BC_DIRICHLET) {
+          if (j == 1 || j == 2) { l = 0.; d = 1.; c = 0.; rhs = a.lo_c1[e]; }
+        } else if (a.lo_kind[e] == DGPU_BC_NEUMANN) {
+          if (j == 1) { l = 0.; d = -1.; c = 1.; rhs = dyf[1] * a.lo_c1[e]; }
+        }
+        if (j == 0) { l = 0.; d = 1.; c = 0.; rhs = 0.; }
+        if (a.hi_kind[e] == DGPU_BC_DIRICHLET) {
+          if (j == ny + 1 || j == ny) { l = 0.; d = 1.; c = 0.; rhs = a.hi_c1[e]; }
+        } else if (a.hi_kind[e] == DGPU_BC_NEUMANN) {
+          if (j == ny + 1) { l = -1.; d = 1.; c = 0.; rhs = dyf[ny] * a.hi_c1[e]; }
+        }
+      }
+    };
+    // forward elimination, rows 0..ny+1; c_j and the eliminated b'_j are kept in
+    // the scratch (planes 0..ny+1 and ny2..2*ny2-1), g'_j overwrites r
+    double l, d, c, rhs;
+    row(0, l, d, c, rhs);
+    cp[q] = c;
+    cp[(long long)n2 * pp + q] = d;
+    r[q] = rhs;
+    double bprev = d, cprev = c, gprev = rhs;
+    for (int j = 1; j <= ny + 1; ++j) {
+      row(j, l, d, c, rhs);
+      const double m = -l / bprev;
+      d = d + m * cprev;
+      rhs = rhs + m * gprev;
+      const long long o = (long long)j * pp + q;
+      cp[o] = c;
+      cp[(long long)(n2 + j) * pp + q] = d;
+      r[o] = rhs;
+      bprev = d; cprev = c; gprev = rhs;
+    }
+    // backward substitution; rows outside [wlo, whi] keep their values
+    double x = gprev / bprev;
+    if (ny + 1 >= a.wlo[e] && ny + 1 <= a.whi[e]) u[(long long)(ny + 1) * pp + q] = x;
+    for (int j = ny; j >= 0; --j) {
+      const long long o = (long long)j * pp + q;
+      x = (r[o] - cp[o] * x) / cp[(long long)(n2 + j) * pp + q];
+      if (j >= a.wlo[e] && j <= a.whi[e]) u[o] = x;
+    }
+  }
+}
+
+}  // namespace dgpu

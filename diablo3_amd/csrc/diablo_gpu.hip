@@ -1,18 +1,22 @@
 // diablo_gpu.hip — C ABI of the MI355X RKW3 + Crank–Nicolson channel step.
 //
-// One substep (rk_chan_1, for/solver.f90:366-1273) of DNS is four launches:
+// One DNS substep (rk_chan_1, for/solver.f90:366-1273) is five launches:
 //   k_inv_z     packed Fourier state -> x-spectra per z row (z inverse transform)
 //   k_xprod     x inverse transform, nonlinear products with their y stencils,
 //               x forward transform, ikx factors (physical space stays in LDS)
-//   k_fwdz_asm  z forward transform of the products + Fourier-space assembly of
-//               cf (RK memory) and cr (right-hand side incl. explicit CN half)
-//   k_columns   per (kx,kz) column: CN solves, rem_div_chan, pressure update
+//   k_fwdz_nl   z forward transform of the products -> nonlinear tendencies
+//   k_cn        per (kx,kz) column: Fourier-space assembly of cf (RK memory) and
+//               cr (right-hand side incl. explicit CN half), then the CN solves
+//   k_project   per (kx,kz) column: rem_div_chan and the pressure update
 // The CN operator of DNS is independent of x and z, so solving it per Fourier
 // column is the same linear map as the reference's per-physical-column solve
 // followed by the forward transform; it saves the 8 transforms the reference
 // spends on cr -> r and u -> cu.
+// The LES substep (use_LES past the spin-up gate) adds the AMD closure and solves
+// CN per physical column (substep_les, kernels in dgpu_les.hpp).
 #include "../../include/diablo_gpu.h"
 #include "dgpu_kernels.hpp"
+#include "dgpu_les.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -80,6 +84,18 @@ struct dgpu_handle {
   double* d_R = nullptr;   // one physical FP64 field (transform entry points)
   cd* d_H = nullptr;       // one FP64 x-spectrum field
   cd* d_C = nullptr;       // one packed field
+  // LES branch (use_les): physical fields of u, du/dx, du/dz per transported
+  // variable, z-derivative x-spectra, eddy viscosity/diffusivity, SGS additions,
+  // physical-space CN right-hand sides and scratch
+  void* HinZ[MAXF] = {};
+  void* Pg[3 * MAXF] = {};
+  void* NuT = nullptr;
+  void* KapT[MAXTH] = {};
+  void* Aug[MAXA] = {};
+  cd* CR[MAXF] = {};
+  double* Rp[MAXF] = {};
+  double* CPs = nullptr;
+  double* d_dlt = nullptr;  // [2][ny2]: delta_yf, delta_y of the AMD filter width
   double* hstage = nullptr;  // pinned staging, one Fortran array
   size_t fort_doubles = 0;
   std::vector<void*> allocs;
@@ -88,8 +104,8 @@ struct dgpu_handle {
   std::vector<hipEvent_t> ev_pool;
   struct Mark { int id; hipEvent_t a, b; };
   std::vector<Mark> marks;
-  double t_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  long long t_n[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  double t_ms[16] = {};
+  long long t_n[16] = {};
 };
 
 namespace {
@@ -327,8 +343,9 @@ int ensure_fft_scratch(dgpu_handle* h) {
 // --------------------------------------------------------------------------------
 // per-kernel timing
 // --------------------------------------------------------------------------------
-constexpr int NKERN = 5;
-const char* const kKernelNames[NKERN] = {"k_inv_z", "k_xprod", "k_fwdz_nl", "k_cn", "k_project"};
+constexpr int NKERN = 10;
+const char* const kKernelNames[NKERN] = {"k_inv_z",   "k_xprod", "k_fwdz_nl", "k_cn",      "k_project",
+                                         "k_bc_post", "les_grad", "k_amd",    "k_cn_phys", "les_cn_fft"};
 
 hipEvent_t pool_event(dgpu_handle* h) {
   if (!h->ev_pool.empty()) {
@@ -384,7 +401,8 @@ int harvest(dgpu_handle* h) {
   }
 
 template <typename T>
-int launch_inv_z(dgpu_handle* h, int nf, const cd* const* src, void* const* dst, int j0, int nj) {
+int launch_inv_z(dgpu_handle* h, int nf, const cd* const* src, void* const* dst, int j0, int nj,
+                 unsigned dzmask = 0) {
   Ptrs<const cd*, MAXO> s{};
   Ptrs<cx<T>*, MAXO> d{};
   for (int f = 0; f < nf; ++f) { s.p[f] = src[f]; d.p[f] = (cx<T>*)dst[f]; }
@@ -392,7 +410,7 @@ int launch_inv_z(dgpu_handle* h, int nf, const cd* const* src, void* const* dst,
   DG_SWITCH_N(h->g.nz, {
     constexpr int TI = ti_for(N_, sizeof(cx<T>));
     dim3 grid((h->g.kxr + TI - 1) / TI, nj, nf);
-    k_inv_z<T, N_, TI, 256><<<grid, 256, 0, h->st>>>(s, d, h->g, j0, tw);
+    k_inv_z<T, N_, TI, 256><<<grid, 256, 0, h->st>>>(s, d, h->g, j0, tw, h->d_kz, dzmask);
   });
   HIPCHK(hipGetLastError());
   return 0;
@@ -416,7 +434,8 @@ int launch_fwd_z(dgpu_handle* h, int nf, void* const* src, cd* const* dst, int j
 constexpr int zp_for(int nx) { return nx >= 2048 ? 1 : 2048 / nx; }
 
 template <typename T>
-int launch_inv_x(dgpu_handle* h, int nf, void* const* src, T* const* dst, int j0, int nj) {
+int launch_inv_x(dgpu_handle* h, int nf, void* const* src, T* const* dst, int j0, int nj,
+                 unsigned dxmask = 0) {
   Ptrs<const cx<T>*, MAXO> s{};
   Ptrs<T*, MAXO> d{};
   for (int f = 0; f < nf; ++f) { s.p[f] = (const cx<T>*)src[f]; d.p[f] = dst[f]; }
@@ -428,10 +447,10 @@ int launch_inv_x(dgpu_handle* h, int nf, void* const* src, T* const* dst, int j0
     if (pairs % ZP) {
       // fall back to one pair per block when nz/2 is smaller than the batch
       dim3 grid(pairs, nj, nf);
-      k_inv_x<T, N_, 1, 256><<<grid, 256, 0, h->st>>>(s, d, h->g, j0, tw);
+      k_inv_x<T, N_, 1, 256><<<grid, 256, 0, h->st>>>(s, d, h->g, j0, tw, h->d_kx, dxmask);
     } else {
       dim3 grid(pairs / ZP, nj, nf);
-      k_inv_x<T, N_, ZP, 256><<<grid, 256, 0, h->st>>>(s, d, h->g, j0, tw);
+      k_inv_x<T, N_, ZP, 256><<<grid, 256, 0, h->st>>>(s, d, h->g, j0, tw, h->d_kx, dxmask);
     }
   });
   HIPCHK(hipGetLastError());
@@ -461,7 +480,7 @@ int launch_fwd_x(dgpu_handle* h, int nf, T* const* src, void* const* dst, int j0
 
 constexpr int xprod_nt(int nx) { return nx <= 32 ? 64 : (nx == 64 ? 128 : 256); }
 
-template <typename T, int NTH>
+template <typename T, int NTH, bool LES = false>
 int launch_xprod(dgpu_handle* h, XProdArgs& a, int nj) {
   a.tw = sizeof(T) == 8 ? h->twx64 : h->twx32;
   DG_SWITCH_N(h->g.nx, {
@@ -471,7 +490,7 @@ int launch_xprod(dgpu_handle* h, XProdArgs& a, int nj) {
     } else {
       constexpr int NT = xprod_nt(N_);
       dim3 grid(h->g.nz / 2, nj, 1);
-      k_xprod<T, N_, NT, NTH><<<grid, NT, 0, h->st>>>(a);
+      k_xprod<T, N_, NT, NTH, LES><<<grid, NT, 0, h->st>>>(a);
     }
   });
   HIPCHK(hipGetLastError());
@@ -583,6 +602,14 @@ int validate(const dgpu_config* c) {
     if (c->pr[n] == 0.0) return fail(2, "Pr must be non-zero");
   }
   if (c->delta == 0.0) return fail(2, "delta must be non-zero");
+  if (c->use_les) {
+    if (c->les_model_type != 5) return fail(2, "only les_model_type 5 (AMD) is built");
+    if (c->precision != DGPU_FP64) return fail(2, "the LES branch runs in FP64 precision in this build");
+    // the Fourier terms that rk_chan_1 forms before les_chan's BC posts differ
+    // from those after them only on rows that the BC rows of Dirichlet v overwrite
+    if (c->v_bc_ymin != DGPU_BC_DIRICHLET || c->v_bc_ymax != DGPU_BC_DIRICHLET)
+      return fail(2, "the LES branch needs Dirichlet v at both walls");
+  }
   for (int j = 1; j <= c->ny + 1; ++j)
     if (!(c->dy[j] > 0.0)) return fail(2, "dy must be positive");
   for (int j = 0; j <= c->ny + 1; ++j)
@@ -662,11 +689,180 @@ void fill_col_args(dgpu_handle* h, ColArgs& ca, double time) {
   ca.p = h->P;
 }
 
+double w_top_c1(const dgpu_config& c, double time) {
+  // w_BC_Ymax_c1_transient (for/parameters.f90:279, for/solver.f90:2131)
+  if (c.f_type == 4)
+    return c.w_bc_ymax_c1 + c.amp_omega0 * std::sin(c.omega0 * (c.ro_inv / c.delta * time - c.force_start));
+  return c.w_bc_ymax_c1;
+}
+
+// One substep of the LES branch: rk_chan_1 with les_chan / les_chan_th
+// (for/solver.f90:611-623) and the physical-space Crank–Nicolson solves with the
+// eddy viscosity (for/solver.f90:875-903, 990-1233).
+template <int NTH>
+int substep_les(dgpu_handle* h, int rk, double delta_t, double time) {
+  const dgpu_config& c = h->c;
+  int rc = ensure_factors(h, delta_t);
+  if (rc) return rc;
+  const double hb[3] = {delta_t * (8.0 / 15.0), delta_t * (2.0 / 15.0), delta_t * (5.0 / 15.0)};
+  const double bb[3] = {1.0, 25.0 / 8.0, 9.0 / 4.0};
+  const double zb[3] = {0.0, -17.0 / 8.0, -5.0 / 4.0};
+  const int r = rk - 1;
+  const double temp1 = 0.5 * c.nu * hb[r], temp2 = 0.5 * hb[r], temp3 = zb[r] * hb[r], temp4 = hb[r],
+               temp5 = bb[r] * hb[r];
+  const Geo& g = h->g;
+  constexpr int nf = 3 + NTH;
+  const int ny = g.ny;
+  int jlo = c.jstart < 2 ? c.jstart : 2;
+  for (int n = 0; n < NTH; ++n) jlo = c.jstart_th[n] < jlo ? c.jstart_th[n] : jlo;
+  const int nj = ny - jlo + 1;
+  const double wtop = w_top_c1(c, time);
+
+  // 0. BC posts of les_chan on the state's ghost planes
+  BcArgs ba{};
+  ba.g = g; ba.nth = NTH;
+  ba.u_lo = c.u_bc_ymin; ba.v_lo = c.v_bc_ymin; ba.w_lo = c.w_bc_ymin;
+  ba.u_hi = c.u_bc_ymax; ba.v_hi = c.v_bc_ymax; ba.w_hi = c.w_bc_ymax;
+  ba.u_lo_c1 = c.u_bc_ymin_c1; ba.v_lo_c1 = c.v_bc_ymin_c1; ba.w_lo_c1 = c.w_bc_ymin_c1;
+  ba.u_hi_c1 = c.u_bc_ymax_c1; ba.v_hi_c1 = c.v_bc_ymax_c1; ba.w_hi_c1 = c.w_bc_ymax_c1; ba.w_hi_tr = wtop;
+  for (int n = 0; n < NTH; ++n) {
+    ba.th_lo[n] = c.th_bc_ymin[n]; ba.th_hi[n] = c.th_bc_ymax[n];
+    ba.th_lo_c1[n] = c.th_bc_ymin_c1[n]; ba.th_hi_c1[n] = c.th_bc_ymax_c1[n];
+  }
+  ba.dy1 = h->dy[1]; ba.dy2 = h->dy[2]; ba.dyny = h->dy[ny]; ba.dyf1 = h->dyf[1];
+  for (int f = 0; f < nf; ++f) ba.u[f] = h->U[f];
+  if ((rc = tbeg(h, 5))) return rc;
+  k_bc_post<<<(unsigned)((g.pc + 255) / 256), 256, 0, h->st>>>(ba);
+  HIPCHK(hipGetLastError());
+  if ((rc = tend(h))) return rc;
+
+  // 1. x-spectra of v and dv/dz, then physical v, dv/dx, dv/dz on all planes
+  const cd* zs[2 * MAXF];
+  void* zd[2 * MAXF];
+  for (int f = 0; f < nf; ++f) {
+    zs[f] = h->U[f]; zd[f] = h->Hin[f];
+    zs[nf + f] = h->U[f]; zd[nf + f] = h->HinZ[f];
+  }
+  if ((rc = tbeg(h, 6))) return rc;
+  if ((rc = launch_inv_z<double>(h, 2 * nf, zs, zd, 0, g.ny2, ((1u << nf) - 1u) << nf))) return rc;
+  void* xs[3 * MAXF];
+  double* xd[3 * MAXF];
+  unsigned dxmask = 0;
+  for (int f = 0; f < nf; ++f) {
+    xs[3 * f] = h->Hin[f]; xd[3 * f] = (double*)h->Pg[3 * f];
+    xs[3 * f + 1] = h->Hin[f]; xd[3 * f + 1] = (double*)h->Pg[3 * f + 1];
+    xs[3 * f + 2] = h->HinZ[f]; xd[3 * f + 2] = (double*)h->Pg[3 * f + 2];
+    dxmask |= 1u << (3 * f + 1);
+  }
+  if ((rc = launch_inv_x<double>(h, 3 * nf, xs, xd, 0, g.ny2, dxmask))) return rc;
+  if ((rc = tend(h))) return rc;
+
+  // 2. AMD: nu_t, kappa_t and the SGS additions to the products
+  AmdArgs aa{};
+  aa.g = g; aa.nth = NTH; aa.jstart = c.jstart; aa.jend = c.jend;
+  for (int n = 0; n < MAXTH; ++n) { aa.jstart_th[n] = c.jstart_th[n]; aa.jend_th[n] = c.jend_th[n]; }
+  aa.dxs = c.lx / c.nx * BETA_SGS; aa.dzs = c.lz / c.nz * BETA_SGS;
+  aa.dy = h->d_dy; aa.dyf = h->d_dyf; aa.dlt = h->d_dlt;
+  for (int s = 0; s < 3 * nf; ++s) aa.f[s] = h->Pg[s];
+  aa.nut = h->NuT;
+  for (int n = 0; n < NTH; ++n) aa.kap[n] = h->KapT[n];
+  for (int m = 0; m < 7 + 2 * NTH; ++m) aa.aug[m] = h->Aug[m];
+  if ((rc = tbeg(h, 7))) return rc;
+  k_amd<double, NTH><<<dim3((unsigned)((g.pp + 255) / 256), g.ny2), 256, 0, h->st>>>(aa);
+  HIPCHK(hipGetLastError());
+  if ((rc = tend(h))) return rc;
+
+  // 3. fused x pass: products + SGS stresses; 4. z forward -> nonlinear tendencies
+  XProdArgs xa{};
+  xa.g = g; xa.jlo = jlo; xa.jstart = c.jstart; xa.jend = c.jend;
+  for (int n = 0; n < MAXTH; ++n) { xa.jstart_th[n] = c.jstart_th[n]; xa.jend_th[n] = c.jend_th[n]; }
+  xa.dy = h->d_dy; xa.dyf = h->d_dyf; xa.kx = h->d_kx;
+  for (int f = 0; f < nf; ++f) xa.hin[f] = h->Hin[f];
+  for (int o = 0; o < 6 + 2 * NTH; ++o) xa.hout[o] = h->Hout[o];
+  for (int m = 0; m < 7 + 2 * NTH; ++m) xa.aug[m] = h->Aug[m];
+  if ((rc = tbeg(h, 1))) return rc;
+  if ((rc = launch_xprod<double, NTH, true>(h, xa, nj))) return rc;
+  if ((rc = tend(h))) return rc;
+  NlArgs na{};
+  na.g = g; na.jlo = jlo; na.jstart = c.jstart; na.jend = c.jend;
+  for (int n = 0; n < MAXTH; ++n) { na.jstart_th[n] = c.jstart_th[n]; na.jend_th[n] = c.jend_th[n]; }
+  na.kz = h->d_kz;
+  for (int o = 0; o < 6 + 2 * NTH; ++o) na.h[o] = h->Hout[o];
+  for (int f = 0; f < nf; ++f) na.nl[f] = h->NL[f];
+  if ((rc = tbeg(h, 2))) return rc;
+  if ((rc = launch_fwdz_nl<double>(h, na, nj))) return rc;
+  if ((rc = tend(h))) return rc;
+
+  // 5. Fourier assembly of cf and of the right-hand side (molecular explicit half included)
+  ColArgs ca{};
+  fill_col_args(h, ca, time);
+  ca.rk = rk; ca.do_cn = 0;
+  ca.temp1 = temp1; ca.temp3 = temp3; ca.temp4 = temp4; ca.temp5 = temp5;
+  ca.t1v = temp1 * c.nu_v_scale;
+  for (int n = 0; n < NTH; ++n) ca.t1pr[n] = temp1 / c.pr[n] * c.nu_v_scale;
+  ca.pcr = h->d_fac + (size_t)r * MAXF * (2 * PCR_LMAX + 1) * g.ny2;
+  for (int f = 0; f < nf; ++f) ca.crout[f] = h->CR[f];
+  if ((rc = tbeg(h, 3))) return rc;
+  if ((rc = launch_cn<double, NTH>(h, ca))) return rc;
+  if ((rc = tend(h))) return rc;
+
+  // 6. right-hand sides to physical space
+  const cd* cs[MAXF];
+  double* rs[MAXF];
+  for (int f = 0; f < nf; ++f) { cs[f] = h->CR[f]; rs[f] = h->Rp[f]; }
+  if ((rc = tbeg(h, 9))) return rc;
+  if ((rc = launch_inv_z<double>(h, nf, cs, h->Hin, 0, g.ny2))) return rc;
+  if ((rc = launch_inv_x<double>(h, nf, h->Hin, rs, 0, g.ny2))) return rc;
+  if ((rc = tend(h))) return rc;
+
+  // 7. per-column CN with nu_t / kappa_t, scalars first (for/solver.f90:1004-1007)
+  PhysCnArgs pa{};
+  pa.g = g; pa.nf = nf; pa.temp2 = temp2; pa.dy = h->d_dy; pa.dyf = h->d_dyf; pa.cp = h->CPs;
+  for (int e0 = 0; e0 < nf; ++e0) {
+    const int e = e0 < NTH ? 3 + e0 : e0 - NTH;  // slot e0 solves field e
+    const SolveRows& R = ca.rows[e];
+    pa.blo[e0] = R.blo; pa.bhi[e0] = R.bhi; pa.wlo[e0] = R.wlo; pa.whi[e0] = R.whi;
+    pa.u[e0] = (double*)h->Pg[3 * e];
+    pa.r[e0] = h->Rp[e];
+    pa.visc[e0] = (const double*)(e < 3 ? h->NuT : h->KapT[e - 3]);
+    pa.field[e0] = e;
+    if (e == 0) { pa.lo_kind[e0] = c.u_bc_ymin; pa.hi_kind[e0] = c.u_bc_ymax; pa.lo_c1[e0] = c.u_bc_ymin_c1; pa.hi_c1[e0] = c.u_bc_ymax_c1; }
+    else if (e == 1) { pa.lo_kind[e0] = c.v_bc_ymin; pa.hi_kind[e0] = c.v_bc_ymax; pa.lo_c1[e0] = c.v_bc_ymin_c1; pa.hi_c1[e0] = c.v_bc_ymax_c1; }
+    else if (e == 2) { pa.lo_kind[e0] = c.w_bc_ymin; pa.hi_kind[e0] = c.w_bc_ymax; pa.lo_c1[e0] = c.w_bc_ymin_c1; pa.hi_c1[e0] = wtop; }
+    else {
+      const int n = e - 3;
+      pa.lo_kind[e0] = c.th_bc_ymin[n]; pa.hi_kind[e0] = c.th_bc_ymax[n];
+      pa.lo_c1[e0] = c.th_bc_ymin_c1[n]; pa.hi_c1[e0] = c.th_bc_ymax_c1[n];
+    }
+    pa.tcn[e0] = e < 3 ? temp1 * c.nu_v_scale : temp1 / c.pr[e - 3] * c.nu_v_scale;
+  }
+  if ((rc = tbeg(h, 8))) return rc;
+  k_cn_phys<<<(unsigned)((g.pp + 255) / 256), 256, 0, h->st>>>(pa);
+  HIPCHK(hipGetLastError());
+  if ((rc = tend(h))) return rc;
+
+  // 8. solved fields back to Fourier space (every plane, as the reference), then
+  //    rem_div_chan and the pressure update
+  double* us[MAXF];
+  cd* ud[MAXF];
+  for (int f = 0; f < nf; ++f) { us[f] = (double*)h->Pg[3 * f]; ud[f] = h->U[f]; }
+  if ((rc = tbeg(h, 9))) return rc;
+  if ((rc = launch_fwd_x<double>(h, nf, us, h->Hin, 0, g.ny2))) return rc;
+  if ((rc = launch_fwd_z<double>(h, nf, h->Hin, ud, 0, g.ny2))) return rc;
+  if ((rc = tend(h))) return rc;
+  if ((rc = tbeg(h, 4))) return rc;
+  if ((rc = launch_project(h, ca, 1))) return rc;
+  return tend(h);
+}
+
 int substep(dgpu_handle* h, int rk, double delta_t, double time, int time_step, int create_new_flow) {
   const dgpu_config& c = h->c;
   if (rk < 1 || rk > 3) return fail(3, "rk_step must be 1, 2 or 3");
   if (c.use_les && (!create_new_flow || time_step > 100))
-    return fail(4, "the LES branch (les_chan) is not available in this library build");
+    return by_nth(h->nth, [&](auto NTHc) {
+      constexpr int NTH = decltype(NTHc)::value;
+      return substep_les<NTH>(h, rk, delta_t, time);
+    });
   int rc = ensure_factors(h, delta_t);
   if (rc) return rc;
   const double hb[3] = {delta_t * (8.0 / 15.0), delta_t * (2.0 / 15.0), delta_t * (5.0 / 15.0)};
@@ -794,6 +990,20 @@ int dgpu_create(const dgpu_config* cfg, dgpu_handle** out) {
   ALLOC(h->twx32, sizeof(cf) * g.nx); ALLOC(h->twz32, sizeof(cf) * g.nz);
   ALLOC(h->d_fac, sizeof(double) * 3 * MAXF * (2 * PCR_LMAX + 1) * n2);
   ALLOC(h->d_rowc, sizeof(double) * RC_COUNT * n2);
+  if (cfg->use_les) {
+    const size_t rbytes = sizeof(double) * g.pp * n2;
+    for (int f = 0; f < nf; ++f) {
+      ALLOC(h->HinZ[f], hbytes);
+      ALLOC(h->CR[f], cbytes);
+      ALLOC(h->Rp[f], rbytes);
+      for (int m = 0; m < 3; ++m) ALLOC(h->Pg[3 * f + m], rbytes);
+    }
+    ALLOC(h->NuT, rbytes);
+    for (int n = 0; n < h->nth; ++n) ALLOC(h->KapT[n], rbytes);
+    for (int m = 0; m < 7 + 2 * h->nth; ++m) ALLOC(h->Aug[m], rbytes);
+    ALLOC(h->CPs, 2 * rbytes);
+    ALLOC(h->d_dlt, sizeof(double) * 2 * n2);
+  }
 #undef ALLOC
   h->fort_doubles = fortran_doubles(cfg);
   const size_t stage_doubles = std::max<size_t>(h->fort_doubles, (size_t)2 * g.pc * n2);
@@ -829,6 +1039,18 @@ int dgpu_create(const dgpu_config* cfg, dgpu_handle** out) {
       }
   }
   ok = ok && up(h->d_rowc, rowc.data(), sizeof(double) * rowc.size());
+  std::vector<double> dlt(2 * (size_t)n2, 0.0);
+  if (cfg->use_les) {
+    // delta_yf(j), delta_y(j) of les_chan type 5 (for/les.f90:756-780):
+    // -2 C_AMD^2 * 3 / (1/(dx beta)^2 + 1/(2 h)^2 + 1/(dz beta)^2)
+    const double ax = cfg->lx / cfg->nx * BETA_SGS, az = cfg->lz / cfg->nz * BETA_SGS;
+    for (int j = 1; j <= cfg->ny + 1; ++j)
+      for (int w = 0; w < 2; ++w) {
+        const double b = (w == 0 ? h->dyf[j] : h->dy[j]) * 2.0;
+        dlt[(size_t)w * n2 + j] = -(2.0 * (C_AMD * C_AMD) * 3.0 / (1.0 / (ax * ax) + 1.0 / (b * b) + 1.0 / (az * az)));
+      }
+    ok = ok && up(h->d_dlt, dlt.data(), sizeof(double) * dlt.size());
+  }
   if (!ok || hipStreamSynchronize(h->st) != hipSuccess) return bail(fail(10, "upload of constants failed"));
   *out = h;
   g_err.clear();
@@ -891,6 +1113,31 @@ int dgpu_download_rk(dgpu_handle* h, double* cf1, double* cf2, double* cf3, doub
     return rc;
   for (int n = 0; n < h->nth; ++n)
     if ((rc = download_field(h, h->CF[3 + n], cfth ? cfth + (size_t)n * h->fort_doubles : nullptr))) return rc;
+  return 0;
+}
+
+int dgpu_download_les(dgpu_handle* h, double* nu_t, double* kappa_t) {
+  if (!h || !nu_t) return fail(1, "null argument");
+  const Geo& g = h->g;
+  const size_t fd = fortran_doubles(&h->c);
+  const int nfld = 1 + h->nth;
+  std::vector<double> tmp((size_t)g.pp * g.ny2);
+  for (int m = 0; m < nfld; ++m) {
+    double* out = m == 0 ? nu_t : (kappa_t ? kappa_t + (size_t)(m - 1) * fd : nullptr);
+    if (!out) return fail(1, "null kappa_t");
+    const void* src = m == 0 ? h->NuT : h->KapT[m - 1];
+    if (src) {
+      HIPCHK(hipMemcpyAsync(tmp.data(), src, tmp.size() * 8, hipMemcpyDeviceToHost, h->st));
+      HIPCHK(hipStreamSynchronize(h->st));
+    } else {
+      std::fill(tmp.begin(), tmp.end(), 0.0);
+    }
+    std::fill(out, out + fd, 0.0);
+    for (int j = 0; j < g.ny2; ++j)
+      for (int z = 0; z < g.nz; ++z)
+        std::memcpy(out + ((size_t)j * (g.nz + 2) + z) * (g.nx + 2), &tmp[((size_t)j * g.nz + z) * g.nx],
+                    sizeof(double) * g.nx);
+  }
   return 0;
 }
 

@@ -55,6 +55,7 @@ def load():
     L.dgpu_timing_enable.argtypes = [P, ctypes.c_int32]
     L.dgpu_timing_read.argtypes = [P, ctypes.c_int32, D, ctypes.POINTER(ctypes.c_int64),
                                    ctypes.POINTER(ctypes.c_char_p)]
+    L.dgpu_download_les.argtypes = [P, D, D]
     L.dgpu_field_doubles.argtypes = [P]
     L.dgpu_field_doubles.restype = ctypes.c_int64
     _lib = L
@@ -131,6 +132,15 @@ class Solver:
         return ([layout.from_fortran_fourier(self.cfg, a) for a in f],
                 [layout.from_fortran_fourier(self.cfg, th[n]) for n in range(self.cfg.n_th)])
 
+    def download_les(self):
+        """nu_t and kappa_t[n] of the last LES substep, compact physical [j][z][x]."""
+        shp = layout.fortran_physical_shape(self.cfg)
+        nt = np.zeros(shp, np.float64)
+        kt = np.zeros((max(self.cfg.n_th, 1),) + shp, np.float64)
+        self._check(load().dgpu_download_les(self.h, _dp(nt), _dp(kt)))
+        return (layout.from_fortran_physical(self.cfg, nt),
+                [layout.from_fortran_physical(self.cfg, kt[n]) for n in range(self.cfg.n_th)])
+
     # ---- time advance ----
     def rk_substep(self, rk_step: int, delta_t: Optional[float] = None, time: float = 0.0,
                    time_step: int = 1, create_new_flow: bool = True):
@@ -158,13 +168,14 @@ class Solver:
         self._check(load().dgpu_timing_enable(self.h, int(enable)))
 
     def timing_read(self):
-        """{kernel name: (total device ms, launches)} since timing(True)."""
+        """{kernel name: (total device ms, launches)} since timing(True), launched kernels only."""
         out = {}
-        for k in range(5):
+        for k in range(10):
             ms, n, name = ctypes.c_double(), ctypes.c_int64(), ctypes.c_char_p()
             self._check(load().dgpu_timing_read(self.h, k, ctypes.byref(ms), ctypes.byref(n),
                                                 ctypes.byref(name)))
-            out[name.value.decode()] = (ms.value, n.value)
+            if n.value:
+                out[name.value.decode()] = (ms.value, n.value)
         return out
 
     # ---- transforms ----

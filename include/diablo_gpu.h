@@ -111,8 +111,16 @@ int dgpu_upload_rk(dgpu_handle *h, const double *cf1, const double *cf2,
 int dgpu_download_rk(dgpu_handle *h, double *cf1, double *cf2, double *cf3,
                      double *cfth);
 
+/* Eddy viscosity nu_t and diffusivities kappa_t(.., 1:N_th) of the last LES
+ * substep, in the physical Fortran layout real(8) (0:Nx+1, 0:Nzp+1, 0:Nyp+1)
+ * (for/flow.f90:49-50; GY points).  Read-only view for stats
+ * (save_stats_LES_OOL, for/les.f90:1030-1063).  Zero before the first LES substep. */
+int dgpu_download_les(dgpu_handle *h, double *nu_t, double *kappa_t);
+
 /* One RKW3 substep.  Replaces `call rk_chan_1` (for/solver.f90:366-1273) with the
- * module values rk_step, delta_t, time, time_step and create_new_flow passed in. */
+ * module values rk_step, delta_t, time, time_step and create_new_flow passed in.
+ * With use_les and the spin-up gate passed (not create_new_flow, or time_step >
+ * 100; for/solver.f90:615) it runs the LES branch, les_model_type 5 (AMD). */
 int dgpu_rk_substep(dgpu_handle *h, int32_t rk_step, double delta_t, double time,
                     int32_t time_step, int32_t create_new_flow);
 
@@ -141,7 +149,9 @@ int dgpu_last_step_ms(dgpu_handle *h, double *ms);
  * reference).  While enabled, every launch of the step's kernels is bracketed by
  * HIP events on the handle's stream.  Kernel ids: 0 = z inverse transform of the
  * state, 1 = fused x pass (products), 2 = z forward transform of the products,
- * 3 = Fourier assembly + Crank-Nicolson solves, 4 = projection + pressure update.
+ * 3 = Fourier assembly + Crank-Nicolson solves, 4 = projection + pressure update;
+ * LES substeps add 5 = BC posts, 6 = gradient fields, 7 = AMD model,
+ * 8 = physical-column CN solves, 9 = right-hand-side and solution transforms.
  * dgpu_timing_read returns the
  * device time and launch count accumulated since the last enable, and the name. */
 int dgpu_timing_enable(dgpu_handle *h, int32_t enable);

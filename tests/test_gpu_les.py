@@ -1,0 +1,129 @@
+"""Parity of the HIP LES branch (AMD closure, les_model_type 5) with the CPU
+oracle, through the C ABI.
+
+Bars: after 3 RKW3 steps past the spin-up gate, u1, u2, u3 and theta match the
+oracle to relative L2 <= 1e-11; nu_t and kappa_t of the last substep to 1e-11;
+cp to max(1e-11, 30 x the oracle's own round-off disagreement on cp), for the
+reason the DNS parity tests give (cp = phi / h_bar amplifies round-off).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from diablo3_amd import configs
+from diablo3_amd.problem import Problem
+from diablo3_amd.solver import Solver
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+def fields(s):
+    return {"cu1": s.cu1, "cu2": s.cu2, "cu3": s.cu3, "cp": s.cp, **{f"cth{n}": t for n, t in enumerate(s.cth)}}
+
+
+def les_problem(nx, ny, nz, amp=0.05, seed=11):
+    cfg = configs.small_dns(nx, ny, nz)
+    cfg.use_les = True
+    cfg.les_model_type = 5
+    p = Problem(cfg)
+    s = p.initial_state(seed=seed, theta_noise=1e-3)
+    rng = np.random.default_rng(seed)
+    # resolved low modes large enough that the model is active on most points
+    shp = (cfg.ny - 3, 4, 4)
+    for a in (s.cu1, s.cu2, s.cu3, s.cth[0]):
+        a[2:cfg.ny - 1, 1:5, 1:5] += amp * (rng.standard_normal(shp) + 1j * rng.standard_normal(shp))
+    o = oracle.Oracle(p, threads=16)
+    o.upload_state(s)
+    o.rem_div()
+    return p, o.download_state()
+
+
+def run_oracle(p, s0, nsteps, variant=0, perturb=False):
+    oracle.lib().oracle_set_variant(variant)
+    try:
+        o = oracle.Oracle(p, threads=16)
+        s = s0.copy()
+        if perturb:
+            for name in ("cu1", "cu2", "cu3"):
+                setattr(s, name, o.fft_to_fourier(o.fft_to_physical(getattr(s, name))))
+            s.cth = [o.fft_to_fourier(o.fft_to_physical(t)) for t in s.cth]
+        o.upload_state(s)
+        t = 0.0
+        for n in range(nsteps):
+            for rk in (1, 2, 3):
+                o.rk_substep(rk, time=t, time_step=n + 1, create_new_flow=0)
+            t += p.cfg.delta_t
+        return o
+    finally:
+        oracle.lib().oracle_set_variant(0)
+
+
+@pytest.mark.parametrize("nx,ny,nz", [(32, 33, 32), (64, 33, 16)])
+def test_les_steps_match_oracle(nx, ny, nz):
+    p, s0 = les_problem(nx, ny, nz)
+    nsteps = 3
+    print(f"LES parity {nx}x{ny}x{nz}, {nsteps} steps", flush=True)
+    g = Solver(p)
+    g.upload_state(s0)
+    t = 0.0
+    for n in range(nsteps):
+        for rk in (1, 2, 3):
+            g.rk_substep(rk, time=t, time_step=n + 1, create_new_flow=False)
+        t += p.cfg.delta_t
+    o = run_oracle(p, s0, nsteps)
+    fg, fo = fields(g.download_state()), fields(o.download_state())
+    base = fo
+    fa = fields(run_oracle(p, s0, nsteps, perturb=True).download_state())
+    fb = fields(run_oracle(p, s0, nsteps, variant=1).download_state())
+    cp_floor = max(rel(fa["cp"], base["cp"]), rel(fb["cp"], base["cp"]))
+    errs = {k: rel(fg[k], fo[k]) for k in fo}
+    (ntg, ktg), (nto, kto) = g.download_les(), o.download_les()
+    errs["nu_t"] = rel(ntg, nto)
+    errs["kappa_t"] = rel(ktg[0], kto[0])
+    print({k: f"{v:.2e}" for k, v in errs.items()}, f"cp floor {cp_floor:.2e}",
+          f"nu_t max {nto.max():.3e}, active {(nto > 0).mean():.2f}", flush=True)
+    assert nto.max() > 0 and (nto > 0).mean() > 0.1 and (kto[0] > 0).mean() > 0.05
+    for k, e in errs.items():
+        bar = max(1e-11, 30 * cp_floor) if k == "cp" else 1e-11
+        assert e <= bar, (k, e, bar)
+
+
+def test_les_substep_rk_memory():
+    """cf after one LES substep (the SGS divergence is in it) matches the oracle."""
+    p, s0 = les_problem(32, 17, 32)
+    g = Solver(p)
+    g.upload_state(s0)
+    g.rk_substep(1, time=0.0, time_step=1, create_new_flow=False)
+    o = oracle.Oracle(p, threads=16)
+    o.upload_state(s0)
+    o.rk_substep(1, time=0.0, time_step=1, create_new_flow=0)
+    (cg, tg), (co, to) = g.download_rk(), o.download_rk()
+    cfg = p.cfg
+    # the rows a BC row overwrites are not part of the step's result
+    rows = {0: slice(cfg.jstart, cfg.ny - 1), 1: slice(3, cfg.ny), 2: slice(cfg.jstart, cfg.ny - 1)}
+    for e in range(3):
+        err = rel(cg[e][rows[e]], co[e][rows[e]])
+        print("cf", e + 1, f"{err:.2e}", flush=True)
+        assert err <= 1e-12
+    err = rel(tg[0][2:cfg.ny], to[0][2:cfg.ny])
+    print("cfth", f"{err:.2e}", flush=True)
+    assert err <= 1e-12
+
+
+def test_les_gate_runs_dns_before_spin_up():
+    """create_new_flow with time_step <= 100: the LES handle steps exactly as DNS."""
+    p, s0 = les_problem(32, 17, 32)
+    p0 = Problem(configs.small_dns(32, 17, 32))
+    out = []
+    for prob in (p, p0):
+        g = Solver(prob)
+        g.upload_state(s0)
+        g.step(time=0.0, time_step=5, create_new_flow=True)
+        out.append(fields(g.download_state()))
+    for k in out[0]:
+        assert np.array_equal(out[0][k], out[1][k]), k
