@@ -40,7 +40,7 @@ def kernel_bytes(cfg, mixed):
     H = cfg.kxr * cfg.nz * 2 * s * ny2
     Hj = cfg.kxr * cfg.nz * 2 * s * nj
     Cj_t = cfg.kxr * cfg.kzr * 2 * s * nj   # nonlinear terms, complex T
-    return {
+    out = {
         "k_inv_z": nf * C + nf * H,
         "k_xprod": nf * H + no * Hj,
         "k_fwdz_nl": no * Hj + nf * Cj_t,
@@ -48,14 +48,34 @@ def kernel_bytes(cfg, mixed):
         "k_cn": nf * Cj_t + (nf + 1) * C + (2.0 / 3.0) * nf * Cj + nf * Cj + nf * C,
         "k_project": 4 * C + 4 * C,
     }
+    if cfg.use_les:
+        # physical field R (FP64: the LES branch runs in FP64); na SGS addition fields
+        R = cfg.nx * cfg.nz * 8 * ny2
+        Rj = cfg.nx * cfg.nz * 8 * nj
+        na = 7 + 2 * cfg.n_th
+        out["k_bc_post"] = nf * 6 * C / ny2
+        # z inverse of v and dv/dz (2 nf), x inverse of v, dv/dx, dv/dz (3 nf)
+        out["les_grad"] = 2 * nf * C + 2 * nf * H + 3 * nf * H + 3 * nf * R
+        out["k_amd"] = 3 * nf * R + (1 + cfg.n_th + na) * R
+        out["k_xprod"] += na * Rj
+        # assembly only: the right-hand side goes out instead of the solve
+        out["k_cn"] = nf * Cj_t + (nf + 1) * C + (2.0 / 3.0) * nf * Cj + nf * Cj + nf * C
+        # per field: r, u, nu_t in, c', b', g' out and back, u out
+        out["k_cn_phys"] = nf * 10 * R
+        # rhs: C -> H -> R; solution: R -> H -> C
+        out["les_cn_fft"] = 2 * nf * C + 4 * nf * H + 2 * nf * R
+    return out
 
 
 def model_v1_bytes_per_step(cfg, s):
-    """The step-level 'canonical algorithm model v1': 30R + 30H + 57C per DNS substep."""
+    """The step-level 'canonical algorithm model v1': 30R + 30H + 57C per DNS
+    substep, 96R + 88H + 89C per LES substep."""
     yg = cfg.ny + 2
     R = cfg.nx * cfg.nz * yg * s
     H = cfg.kxr * cfg.nz * yg * 2 * s
     C = cfg.kxr * cfg.kzr * yg * 2 * s
+    if cfg.use_les:
+        return 3 * (96 * R + 88 * H + 89 * C)
     return 3 * (30 * R + 30 * H + 57 * C)
 
 
@@ -69,20 +89,30 @@ def read_traffic(kernel, workload):
 
 
 def cpu_baseline(cfg, threads, steps=1):
+    """The oracle on this host's cores: one full step (DNS) or one substep scaled
+    by 3 (LES, whose substep costs the same at every rk_step)."""
     import oracle
     from diablo3_amd.problem import Problem
     p = Problem(cfg.copy(precision=0))
     o = oracle.Oracle(p, threads=threads)
     o.upload_state(p.initial_state(seed=0, theta_noise=1e-3))
     o.rem_div()
+    cnf = 0 if cfg.use_les else 1
     t0 = time.perf_counter()
-    for n in range(steps):
-        o.step(time=n * cfg.delta_t, time_step=n + 1)
-    dt = (time.perf_counter() - t0) / steps
+    if cfg.use_les:
+        o.rk_substep(1, time=0.0, time_step=1, create_new_flow=cnf)
+        dt = 3.0 * (time.perf_counter() - t0)
+        what = "one LES substep (x3)"
+    else:
+        for n in range(steps):
+            o.step(time=n * cfg.delta_t, time_step=n + 1, create_new_flow=cnf)
+        dt = (time.perf_counter() - t0) / steps
+        what = f"{steps} full RKW3 step(s)"
     return {"value": cfg.nx * cfg.ny * cfg.nz / dt / 1e6, "unit": "Mpoint-updates/s", "cores": threads,
             "kind": "port",
-            "sample": f"{steps} full RKW3 step(s) of the same {cfg.nx}x{cfg.ny}x{cfg.nz} FP64 workload, "
-                      f"C restatement of rk_chan_1 (physical-space CN, 23 transforms/substep), "
+            "sample": f"{what} of the same {cfg.nx}x{cfg.ny}x{cfg.nz} FP64 workload, "
+                      f"C restatement of rk_chan_1 (physical-space CN, "
+                      f"{59 if cfg.use_les else 23} transforms/substep), "
                       f"OpenMP {threads} threads, {dt:.2f} s/step"}
 
 
@@ -91,7 +121,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="dns_256", choices=["dns_256", "dns_64"])
+    ap.add_argument("--workload", default="dns_256", choices=["dns_256", "dns_64", "les_512", "les_128"])
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -112,15 +142,17 @@ def main():
     g = Solver(p)
     g.upload_state(p.initial_state(seed=0, theta_noise=1e-3))
     g.rem_div()
+    # LES: a restart-style run (create_new_flow false), so the SGS model is on from step 1
+    cnf = not cfg.use_les
     t = 0.0
     for n in range(args.warmup):
-        g.step(time=t, time_step=n + 1)
+        g.step(time=t, time_step=n + 1, create_new_flow=cnf)
         t += cfg.delta_t
     g.timing(True)
     g.synchronize()
     t0 = time.perf_counter()
     for n in range(args.steps):
-        g.step(time=t, time_step=args.warmup + n + 1)
+        g.step(time=t, time_step=args.warmup + n + 1, create_new_flow=cnf)
         t += cfg.delta_t
     g.synchronize()
     t1 = time.perf_counter()
@@ -151,7 +183,8 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64" if not mixed else "f32+f64",
         "data": "synthetic (seeded front IC, projected divergence-free)",
-        "config": {"workload": f"front DNS {cfg.nx}x{cfg.ny}x{cfg.nz} (x,y,z), "
+        "config": {"workload": f"front {'LES (AMD closure)' if cfg.use_les else 'DNS'} "
+                               f"{cfg.nx}x{cfg.ny}x{cfg.nz} (x,y,z), "
                                f"{'FP64' if not mixed else 'mixed FP32 transforms / FP64 solves'}, "
                                f"dt {cfg.delta_t}, Re {cfg.re}",
                    "nx": cfg.nx, "ny": cfg.ny, "nz": cfg.nz, "precision": args.precision},

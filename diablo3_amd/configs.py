@@ -58,4 +58,38 @@ def small_dns(nx: int = 16, ny: int = 17, nz: int = 16, precision: int = FP64) -
     return (front_dns(nx, ny, nz, 1000.0, 0.5, 1.0, 1.0, 1e-3, 1e-3, precision))
 
 
-NAMED = {"dns_64": dns_64, "dns_256": dns_256}
+def front_les(nx: int, ny: int, nz: int, re: float, dt: float, amp: float, n2: float = 1.0,
+              precision: int = FP64) -> Config:
+    """Finite front with the AMD closure (les_model_type 5, C_AMD 0.2887 of
+    for/les.f90:25): Lx = Lz = 8 pi, f = Ro_inv / delta = 1 with Ro_inv = delta =
+    0.5, so the tanh front of IC type 6 (for/flow.f90:629-646, 366-385) has width
+    0.5 and M^2 = 1 at its core, stratified with Ri(1) = N^2, in thermal-wind
+    balance; the background gradient dTHdX = 2 Ro_inv / Lx and the top Neumann
+    compensation of w come from input_chan (for/parameters.f90:266-273).  Bottom
+    no-slip, top free-slip, scalar no-flux walls."""
+    return Config(
+        nx=nx, ny=ny, nz=nz, n_th=1, flavor="Front", use_les=True, les_model_type=5,
+        re=re, beta=1.0, lx=8.0 * math.pi, ly=2.0, lz=8.0 * math.pi, nu_v_scale=1.0,
+        delta_t=dt, ri=[n2], pr=[1.0], ic_type=6, kick=2.0 * amp, physical_noise=True,
+        ro=2.0, delta=0.5, grav_x=0.0, grav_y=1.0, grav_z=0.0, f_type=3,
+        u_bc_ymin=DIRICHLET, u_bc_ymin_c1=0.0, w_bc_ymin=DIRICHLET, w_bc_ymin_c1=0.0,
+        v_bc_ymin=DIRICHLET, v_bc_ymin_c1=0.0,
+        u_bc_ymax=NEUMANN, u_bc_ymax_c1=0.0, w_bc_ymax=NEUMANN, w_bc_ymax_c1=0.0,
+        v_bc_ymax=DIRICHLET, v_bc_ymax_c1=0.0,
+        th_bc_ymin=[NEUMANN], th_bc_ymin_c1=[0.0], th_bc_ymax=[NEUMANN], th_bc_ymax_c1=[0.0],
+        grid_type=1, grid_cs=1.75, precision=precision, bc_compensation=True,
+    ).derive()
+
+
+def les_512(precision: int = FP64) -> Config:
+    """LES with the AMD closure, finite front, 512x257x512: Re 1e5, N^2 1, dt 2e-4,
+    perturbation amplitude 1e-2."""
+    return front_les(512, 257, 512, 1e5, 2e-4, 1e-2, precision=precision)
+
+
+def les_128(precision: int = FP64) -> Config:
+    """The same front at the 128x129x128 LES grid, Re 5000."""
+    return front_les(128, 129, 128, 5000.0, 2e-4, 1e-2, precision=precision)
+
+
+NAMED = {"dns_64": dns_64, "dns_256": dns_256, "les_128": les_128, "les_512": les_512}
