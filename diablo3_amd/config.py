@@ -331,7 +331,8 @@ class DgpuConfig(ctypes.Structure):
         ("u_bc_ymin", ctypes.c_int32), ("v_bc_ymin", ctypes.c_int32),
         ("w_bc_ymin", ctypes.c_int32), ("u_bc_ymax", ctypes.c_int32),
         ("v_bc_ymax", ctypes.c_int32), ("w_bc_ymax", ctypes.c_int32),
-        ("th_bc_ymin", _I4), ("th_bc_ymax", _I4), ("reserved_i", _I4),
+        ("th_bc_ymin", _I4), ("th_bc_ymax", _I4), ("homogeneous_x", ctypes.c_int32),
+        ("reserved_i", ctypes.c_int32 * 3),
         ("lx", ctypes.c_double), ("ly", ctypes.c_double), ("lz", ctypes.c_double),
         ("nu", ctypes.c_double), ("beta", ctypes.c_double), ("nu_v_scale", ctypes.c_double),
         ("ro_inv", ctypes.c_double), ("delta", ctypes.c_double),
@@ -360,6 +361,8 @@ class CConfig:
             setattr(s, name, int(getattr(cfg, name)))
         s.use_les = int(bool(cfg.use_les))
         s.flavor_front = int(cfg.flavor.strip() == "Front")
+        # homogeneousX (for/parameters.f90:282-287)
+        s.homogeneous_x = int(cfg.ic_type not in (5, 6, 7, 8))
         for name in ("lx", "ly", "lz", "nu", "beta", "nu_v_scale", "ro_inv", "delta",
                      "grav_x", "grav_y", "grav_z", "px0", "omega0", "amp_omega0",
                      "force_start", "u_bc_ymin_c1", "v_bc_ymin_c1", "w_bc_ymin_c1",

@@ -36,7 +36,7 @@ constexpr double BETA_SGS = 3.0;     // for/les.f90:43
 // ---------------------------------------------------------------------------
 struct BcArgs {
   Geo g;
-  int nth;
+  int nth, les;  // les: also apply_BC_les / apply_BC_th_les
   int u_lo, v_lo, w_lo, u_hi, v_hi, w_hi, th_lo[MAXTH], th_hi[MAXTH];
   double u_lo_c1, v_lo_c1, w_lo_c1, u_hi_c1, v_hi_c1, w_hi_c1, w_hi_tr, th_lo_c1[MAXTH], th_hi_c1[MAXTH];
   double dy1, dy2, dyny, dyf1;
@@ -102,6 +102,7 @@ __global__ __launch_bounds__(256) void k_bc_post(BcArgs a) {
     gyf_lo(a.u[3 + n], a.th_lo[n], a.th_lo_c1[n]);
     gyf_hi(a.u[3 + n], a.th_hi[n], a.th_hi_c1[n], a.th_hi_c1[n], false);
   }
+  if (!a.les) return;
   // apply_BC_les: Neumann ghosts copy the first interior plane, mean included
   if (a.u_hi == DGPU_BC_NEUMANN) { at(u1, ny) = at(u1, ny - 1); at(u1, ny + 1) = at(u1, ny); }
   if (a.w_hi == DGPU_BC_NEUMANN) { at(u3, ny) = at(u3, ny - 1); at(u3, ny + 1) = at(u3, ny); }

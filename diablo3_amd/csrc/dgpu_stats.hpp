@@ -1,0 +1,295 @@
+// dgpu_stats.hpp — energy-budget statistics of save_stats_chan on the device
+// (for/diagnostics.f90:3-669, compute_averages 676-799, compute_TKE_diss 804-1092,
+// compute_MKE_diss 1097-1249, compute_TKE 1254-1360, compute_TKE_Production
+// 1366-1519), all in FP64.
+//
+// Inputs are the physical fields v, dv/dx, dv/dz of u1, u2, u3 and each scalar
+// (the LES gradient layout, [j][z][x]) and the Fourier state (mean modes).
+//  k_stat_mean   per (x, j): the z-line means of u1, u2, u3, th_n and of du_i/dx
+//                (ume_xy, vme_xy, wme_xy, thme_xy, dudx_m, dvdx_m, dwdx_m); for an
+//                x-homogeneous flow the plane means and zero gradients instead
+//  k_stat_zsum   per (x, j): the z sums of every fluctuation product
+//  k_stat_plane  per j: the x reductions and the x-only (mean-field) terms
+// The z and x sums replace the reference's sequential loops (round-off only).
+#pragma once
+#include "dgpu_kernels.hpp"
+
+namespace dgpu {
+
+// z-line mean slots
+enum { SM_U1 = 0, SM_U2, SM_U3, SM_DXU1, SM_DXU2, SM_DXU3, SM_TH0 };
+// z-sum slots
+enum { SZ_UU = 0, SZ_VV, SZ_WW, SZ_UW, SZ_UV, SZ_WV, SZ_EPS, SZ_SHEAR, SZ_U1, SZ_TH0 };
+constexpr int SZ_PER_TH = 3;  // (th')^2, th' w', |grad th|^2
+// output profiles (include/diablo_gpu.h: dgpu_stats)
+enum {
+  SP_UME = 0, SP_VME, SP_WME, SP_MKE, SP_URMS, SP_VRMS, SP_WRMS, SP_UW, SP_UV, SP_WV,
+  SP_UU_DUDX, SP_VV_DVDY, SP_WU_DWDX, SP_VU_DVDX, SP_UV_DUDY, SP_WV_DWDY,
+  SP_DUDY, SP_DWDY, SP_SHEAR, SP_EPS, SP_EPSM, SP_CP, SP_U1Y, SP_NBASE,
+  SP_PER_TH = 6  // thme, thrms, thv, thv_m, dthdy, pe_diss
+};
+
+struct StatArgs {
+  Geo g;
+  int nth, hx;
+  double nu, twf, ly, lx;
+  double dthdx[MAXTH], dthdz[MAXTH];
+  const double* __restrict__ dy;
+  const double* __restrict__ dyf;
+  const double* __restrict__ gy;
+  const double* __restrict__ gyf;
+  const double* f[3 * MAXF];  // v, dv/dx, dv/dz per field (u1, u2, u3, th_n)
+  const cd* u[MAXF];          // Fourier state (mean modes)
+  const cd* p;
+  double* zm;                 // [6 + nth][ny2][nx]
+  double* zs;                 // [SZ_TH0 + 3 nth][ny2][nx]
+  double* prof;               // [SP_NBASE + 6 nth][ny2]
+};
+
+__global__ __launch_bounds__(256) void k_stat_mean(StatArgs a) {
+  const Geo& g = a.g;
+  const int x = blockIdx.x * 256 + threadIdx.x, j = blockIdx.y;
+  if (x >= g.nx) return;
+  const long long pl = (long long)g.ny2 * g.nx;
+  const long long o = (long long)j * g.nx + x;
+  const int nm = 6 + a.nth;
+  const int ny = g.ny;
+  // slot -> (field slot in f, valid row range); vme_xy runs to Nyp+1
+  for (int m = 0; m < nm; ++m) {
+    double v = 0.0;
+    const int fs = m < 3 ? 3 * m : (m < 6 ? 3 * (m - 3) + 1 : 3 * (3 + m - 6));
+    const int jhi = (m == SM_U2) ? ny + 1 : ny;
+    if (j >= 1 && j <= jhi) {
+      if (a.hx) {
+        // plane means from the mean mode; the x derivatives of the mean are zero
+        // (compute_MKE_diss returns before setting them, for/diagnostics.f90:1117)
+        if (m < 3 || m >= 6) {
+          const cd* C = m < 3 ? a.u[m] : a.u[3 + m - 6];
+          v = C[(long long)j * g.pc].x;
+        }
+      } else {
+        const double* __restrict__ F = a.f[fs] + (long long)j * g.pp + x;
+        for (int z = 0; z < g.nz; ++z) v += F[(long long)z * g.nx];
+        v = v / (double)g.nz;
+      }
+    }
+    a.zm[m * pl + o] = v;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_stat_zsum(StatArgs a) {
+  const Geo& g = a.g;
+  const int x = blockIdx.x * 256 + threadIdx.x, j = blockIdx.y;
+  if (x >= g.nx) return;
+  const int ny = g.ny, nx = g.nx;
+  const long long pl = (long long)g.ny2 * nx;
+  const long long pp = g.pp;
+  auto M = [&](int m, int jj) { return a.zm[m * pl + (long long)jj * nx + x]; };
+  auto F = [&](int s, int jj, int z) { return a.f[s][(long long)jj * pp + (long long)z * nx + x]; };
+  const double* dy = a.dy;
+  const double* dyf = a.dyf;
+  const double* gy = a.gy;
+  const double* gyf = a.gyf;
+  const bool gyf_row = (j >= 1 && j <= ny), gy_row = (j >= 2 && j <= ny);
+  double uu = 0, vv = 0, ww = 0, uw = 0, uv = 0, wv = 0, eps = 0, sh = 0, s1 = 0;
+  double tt[MAXTH] = {}, tv[MAXTH] = {}, pe[MAXTH] = {};
+  // the x derivatives whose kz = 0 row compute_TKE_diss drops when x is not
+  // homogeneous (k_start = 1): the z-line mean of the derivative
+  const double mdx1 = a.hx ? 0.0 : M(SM_DXU1, j), mdx1m = a.hx ? 0.0 : (j >= 1 ? M(SM_DXU1, j - 1) : 0.0);
+  const double mdx2 = a.hx ? 0.0 : M(SM_DXU2, j);
+  const double mdx3 = a.hx ? 0.0 : M(SM_DXU3, j), mdx3m = a.hx ? 0.0 : (j >= 1 ? M(SM_DXU3, j - 1) : 0.0);
+  if (gyf_row || gy_row) {
+    const double mu1 = M(SM_U1, j), mu2 = M(SM_U2, j), mu3 = M(SM_U3, j);
+    for (int z = 0; z < g.nz; ++z) {
+      const double u1 = F(0, j, z), u2 = F(3, j, z), u3 = F(6, j, z);
+      const double f1 = u1 - mu1, f2 = u2 - mu2, f3 = u3 - mu3;
+      if (gyf_row) {
+        uu += f1 * f1; vv += f2 * f2; ww += f3 * f3; uw += f1 * f3;
+        const double d1 = (F(0, j + 1, z) - F(0, j - 1, z)) / (2.0 * dyf[j]);
+        const double d3 = (F(6, j + 1, z) - F(6, j - 1, z)) / (2.0 * dyf[j]);
+        sh += d1 * d1 + d3 * d3;
+        s1 += u1;
+      }
+      if (gy_row) {
+        const double f1m = F(0, j - 1, z) - M(SM_U1, j - 1), f3m = F(6, j - 1, z) - M(SM_U3, j - 1);
+        const double iu = (dyf[j - 1] * f1 + dyf[j] * f1m) / (2.0 * dy[j]);
+        const double iw = (dyf[j - 1] * f3 + dyf[j] * f3m) / (2.0 * dy[j]);
+        uv += iu * f2;
+        wv += iw * f2;
+        // dissipation integrand (for/diagnostics.f90:834-1016)
+        auto itp = [&](double c, double m) { return (dyf[j - 1] * c * c + dyf[j] * m * m) / (2.0 * dy[j]); };
+        double e = itp(F(1, j, z) - mdx1, F(1, j - 1, z) - mdx1m);
+        const double a2 = F(4, j, z) - mdx2;
+        e += a2 * a2;
+        const double t3 = (f1 - f1m) / dy[j];
+        e += t3 * t3;
+        e += itp(F(7, j, z) - mdx3, F(7, j - 1, z) - mdx3m);
+        e += itp(F(2, j, z), F(2, j - 1, z));
+        const double t6 = ((F(3, j + 1, z) - M(SM_U2, j + 1)) - (F(3, j - 1, z) - M(SM_U2, j - 1))) /
+                          (gy[j + 1] - gy[j - 1]);
+        e += t6 * t6;
+        const double t7 = (f3 - f3m) / dy[j];
+        e += t7 * t7;
+        const double a8 = F(5, j, z);
+        e += a8 * a8;
+        e += itp(F(8, j, z), F(8, j - 1, z));
+        eps += e;
+      }
+      if (gyf_row) {
+        for (int n = 0; n < a.nth; ++n) {
+          const int ts = 3 * (3 + n);
+          const double ft = F(ts, j, z) - M(SM_TH0 + n, j);
+          tt[n] += ft * ft;
+          tv[n] += ft * (0.5 * (u2 + F(3, j + 1, z)) - 0.5 * (mu2 + M(SM_U2, j + 1)));
+          // |grad th + background|^2; the y difference is of scalar 1 for every n
+          // (crth(.., 1) in for/diagnostics.f90:147)
+          const double gx_ = F(ts + 1, j, z) + a.dthdx[n];
+          const double gy_ = (F(9, j + 1, z) - F(9, j - 1, z)) / (gyf[j + 1] - gyf[j - 1]);
+          const double gz_ = F(ts + 2, j, z) + a.dthdz[n];
+          pe[n] += gx_ * gx_ + gy_ * gy_ + gz_ * gz_;
+        }
+      }
+    }
+  }
+  const long long o = (long long)j * nx + x;
+  a.zs[SZ_UU * pl + o] = uu; a.zs[SZ_VV * pl + o] = vv; a.zs[SZ_WW * pl + o] = ww;
+  a.zs[SZ_UW * pl + o] = uw; a.zs[SZ_UV * pl + o] = uv; a.zs[SZ_WV * pl + o] = wv;
+  a.zs[SZ_EPS * pl + o] = eps; a.zs[SZ_SHEAR * pl + o] = sh; a.zs[SZ_U1 * pl + o] = s1;
+  for (int n = 0; n < a.nth; ++n) {
+    a.zs[(SZ_TH0 + SZ_PER_TH * n) * pl + o] = tt[n];
+    a.zs[(SZ_TH0 + SZ_PER_TH * n + 1) * pl + o] = tv[n];
+    a.zs[(SZ_TH0 + SZ_PER_TH * n + 2) * pl + o] = pe[n];
+  }
+}
+
+// block-wide sum of 256 values (fixed tree: the result does not depend on timing)
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) sh[t] += sh[t + s];
+    __syncthreads();
+  }
+  const double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+constexpr int NSTAT_X = 16 + 4 * MAXTH;
+
+__global__ __launch_bounds__(256) void k_stat_plane(StatArgs a) {
+  __shared__ double sh[256];
+  const Geo& g = a.g;
+  const int j = blockIdx.x, t = threadIdx.x, nx = g.nx, ny = g.ny, nth = a.nth;
+  const long long pl = (long long)g.ny2 * nx;
+  auto M = [&](int m, int jj, int x) { return a.zm[m * pl + (long long)jj * nx + x]; };
+  auto S = [&](int s, int x) { return a.zs[s * pl + (long long)j * nx + x]; };
+  const double* dy = a.dy;
+  const double* dyf = a.dyf;
+  const double* gy = a.gy;
+  const double* gyf = a.gyf;
+  const double NxNz = (double)nx * (double)g.nz, Nz = (double)g.nz, Nx = (double)nx;
+  const bool gyf_row = (j >= 1 && j <= ny), gy_row = (j >= 2 && j <= ny);
+  // per-thread partial sums over x of every profile term
+  double acc[NSTAT_X] = {};
+  for (int x = t; x < nx; x += 256) {
+    const double suu = S(SZ_UU, x), svv = S(SZ_VV, x), sww = S(SZ_WW, x), suw = S(SZ_UW, x);
+    const double suv = S(SZ_UV, x), swv = S(SZ_WV, x);
+    acc[0] += suu; acc[1] += svv; acc[2] += sww; acc[3] += suw; acc[4] += suv; acc[5] += swv;
+    acc[6] += S(SZ_EPS, x); acc[7] += S(SZ_SHEAR, x);
+    if (gyf_row) {
+      acc[8] += (suu / Nz) * M(SM_DXU1, j, x);     // uu_dudx
+      acc[9] += (suw / Nz) * M(SM_DXU3, j, x);     // wu_dwdx
+      // compute_MKE_diss (x not homogeneous): interpolated and GY terms
+      if (!a.hx) {
+        const double m1 = M(SM_DXU1, j, x), m1m = M(SM_DXU1, j - 1, x);
+        const double m3 = M(SM_DXU3, j, x), m3m = M(SM_DXU3, j - 1, x);
+        const double m2 = M(SM_DXU2, j, x);
+        double e = (dyf[j - 1] * m1 * m1 + dyf[j] * m1m * m1m) / (2.0 * dy[j]) + m2 * m2 +
+                   (dyf[j - 1] * m3 * m3 + dyf[j] * m3m * m3m) / (2.0 * dy[j]);
+        if (gy_row) {
+          const double t3 = (M(SM_U1, j, x) - M(SM_U1, j - 1, x)) / dy[j];
+          const double t5 = (M(SM_U2, j + 1, x) - M(SM_U2, j - 1, x)) / (gy[j + 1] - gy[j - 1]);
+          const double t6 = (M(SM_U3, j, x) - M(SM_U3, j - 1, x)) / dy[j] + a.twf;
+          e += t3 * t3 + t5 * t5 + t6 * t6;
+        }
+        acc[10] += e;
+      }
+    }
+    if (gy_row) {
+      acc[11] += (suv / Nz) * M(SM_DXU2, j, x);    // vu_dvdx
+      acc[12] += (M(SM_U2, j + 1, x) - M(SM_U2, j - 1, x)) / (gy[j + 1] - gy[j - 1]) * (svv / Nz);
+      acc[13] += (M(SM_U1, j, x) - M(SM_U1, j - 1, x)) / dy[j] * (suv / Nz);
+      acc[14] += ((M(SM_U3, j, x) - M(SM_U3, j - 1, x)) / dy[j] + a.twf) * (swv / Nz);
+      // mke integrand (for/diagnostics.f90:44-61), same for every z
+      const double um = M(SM_U1, j, x), umm = M(SM_U1, j - 1, x);
+      const double wm = M(SM_U3, j, x) + a.twf * (gyf[j] - 0.5 * a.ly);
+      const double wmm = M(SM_U3, j - 1, x) + a.twf * (gyf[j - 1] - 0.5 * a.ly);
+      const double vm = M(SM_U2, j, x);
+      acc[15] += (dyf[j - 1] * um * um + dyf[j] * umm * umm) / (2.0 * dy[j]) +
+                 (dyf[j - 1] * wm * wm + dyf[j] * wmm * wmm) / (2.0 * dy[j]) + vm * vm;
+    }
+    for (int n = 0; n < nth; ++n) {
+      acc[16 + 4 * n] += S(SZ_TH0 + SZ_PER_TH * n, x);
+      acc[17 + 4 * n] += S(SZ_TH0 + SZ_PER_TH * n + 1, x);
+      acc[18 + 4 * n] += S(SZ_TH0 + SZ_PER_TH * n + 2, x);
+      if (gyf_row) {
+        const double gxv = ((double)x * a.lx) / (double)nx;
+        acc[19 + 4 * n] += (M(SM_TH0 + n, j, x) + a.dthdx[n] * (gxv - 0.5 * a.lx)) *
+                           (0.5 * (M(SM_U2, j, x) + M(SM_U2, j + 1, x)));
+      }
+    }
+  }
+  double tot[NSTAT_X];
+  for (int k = 0; k < 16 + 4 * nth; ++k) tot[k] = block_sum(acc[k], sh);
+  if (t != 0) return;
+  const int n2 = g.ny2;
+  double* P = a.prof;
+  auto put = [&](int p, double v) { P[(long long)p * n2 + j] = v; };
+  const double ume = a.u[0][(long long)j * g.pc].x, vme = a.u[1][(long long)j * g.pc].x;
+  const double wme = a.u[2][(long long)j * g.pc].x;
+  put(SP_UME, ume); put(SP_VME, vme); put(SP_WME, wme);
+  put(SP_CP, a.p[(long long)j * g.pc].x);
+  double urms = 0, vrms = 0, wrms = 0, uw = 0, uv = 0, wv = 0, eps = 0, epsm = 0, shr = 0, mke = 0;
+  double uud = 0, vvd = 0, wud = 0, vud = 0, uvd = 0, wvd = 0, dudy = 0, dwdy = 0, u1y = 0;
+  if (gyf_row) {
+    urms = sqrt(tot[0] / NxNz); vrms = sqrt(tot[1] / NxNz); wrms = sqrt(tot[2] / NxNz);
+    uw = tot[3] / NxNz;
+    shr = tot[7] / NxNz;
+    uud = tot[8] / Nx; wud = tot[9] / Nx;
+    epsm = a.hx ? 0.0 : a.nu * tot[10] / Nx;
+    const double um1 = a.u[0][(long long)(j - 1) * g.pc].x, wm1 = a.u[2][(long long)(j - 1) * g.pc].x;
+    dudy = (ume - um1) / (gyf[j] - gyf[j - 1]);
+    dwdy = (wme - wm1) / (gyf[j] - gyf[j - 1]);
+    u1y = gyf[j] * S(SZ_U1, 0) / Nz;
+  }
+  if (gy_row) {
+    uv = tot[4] / NxNz; wv = tot[5] / NxNz;
+    eps = a.nu * tot[6] / NxNz;
+    vud = tot[11] / Nx; vvd = tot[12] / Nx; uvd = tot[13] / Nx; wvd = tot[14] / Nx;
+    mke = 0.5 * (Nz * tot[15]) / NxNz;
+  }
+  put(SP_URMS, urms); put(SP_VRMS, vrms); put(SP_WRMS, wrms);
+  put(SP_UW, uw); put(SP_UV, uv); put(SP_WV, wv);
+  put(SP_UU_DUDX, uud); put(SP_VV_DVDY, vvd); put(SP_WU_DWDX, wud); put(SP_VU_DVDX, vud);
+  put(SP_UV_DUDY, uvd); put(SP_WV_DWDY, wvd);
+  put(SP_DUDY, dudy); put(SP_DWDY, dwdy); put(SP_SHEAR, shr); put(SP_EPS, eps); put(SP_EPSM, epsm);
+  put(SP_MKE, mke); put(SP_U1Y, u1y);
+  for (int n = 0; n < nth; ++n) {
+    const int b = SP_NBASE + SP_PER_TH * n;
+    const cd* C = a.u[3 + n];
+    const double thme = C[(long long)j * g.pc].x;
+    double thrms = 0, thv = 0, thvm = 0, dthdy = 0, ped = 0;
+    if (gyf_row) {
+      thrms = sqrt(tot[16 + 4 * n] / NxNz);
+      thv = tot[17 + 4 * n] / NxNz;
+      ped = tot[18 + 4 * n] / NxNz;
+      thvm = tot[19 + 4 * n] / Nx;
+      dthdy = (thme - C[(long long)(j - 1) * g.pc].x) / (gyf[j] - gyf[j - 1]);
+    }
+    put(b + 0, thme); put(b + 1, thrms); put(b + 2, thv); put(b + 3, thvm); put(b + 4, dthdy); put(b + 5, ped);
+  }
+}
+
+}  // namespace dgpu

@@ -17,6 +17,7 @@
 #include "../../include/diablo_gpu.h"
 #include "dgpu_kernels.hpp"
 #include "dgpu_les.hpp"
+#include "dgpu_stats.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -76,6 +77,7 @@ struct dgpu_handle {
   cd* CF[MAXF] = {};
   void* NL[MAXF] = {};     // nonlinear tendencies (complex T), packed layout
   void* Hin[MAXF] = {};
+  void* HinD[MAXF] = {};  // FP64 x-spectra of a mixed-precision handle (statistics only)
   void* Hout[MAXO] = {};
   void *twx64 = nullptr, *twz64 = nullptr, *twx32 = nullptr, *twz32 = nullptr;
   double* d_fac = nullptr;
@@ -96,6 +98,8 @@ struct dgpu_handle {
   double* Rp[MAXF] = {};
   double* CPs = nullptr;
   double* d_dlt = nullptr;  // [2][ny2]: delta_yf, delta_y of the AMD filter width
+  // statistics (allocated on the first dgpu_stats call of a DNS handle)
+  double *d_zm = nullptr, *d_zs = nullptr, *d_prof = nullptr;
   double* hstage = nullptr;  // pinned staging, one Fortran array
   size_t fort_doubles = 0;
   std::vector<void*> allocs;
@@ -696,6 +700,75 @@ double w_top_c1(const dgpu_config& c, double time) {
   return c.w_bc_ymax_c1;
 }
 
+// apply_BC_vel_mpi_post + apply_BC_th_mpi_post on the device state, and with
+// les also apply_BC_les / apply_BC_th_les
+int launch_bc_post(dgpu_handle* h, double time, int les) {
+  const dgpu_config& c = h->c;
+  const Geo& g = h->g;
+  const int ny = g.ny;
+  BcArgs ba{};
+  ba.g = g; ba.nth = h->nth; ba.les = les;
+  ba.u_lo = c.u_bc_ymin; ba.v_lo = c.v_bc_ymin; ba.w_lo = c.w_bc_ymin;
+  ba.u_hi = c.u_bc_ymax; ba.v_hi = c.v_bc_ymax; ba.w_hi = c.w_bc_ymax;
+  ba.u_lo_c1 = c.u_bc_ymin_c1; ba.v_lo_c1 = c.v_bc_ymin_c1; ba.w_lo_c1 = c.w_bc_ymin_c1;
+  ba.u_hi_c1 = c.u_bc_ymax_c1; ba.v_hi_c1 = c.v_bc_ymax_c1; ba.w_hi_c1 = c.w_bc_ymax_c1;
+  ba.w_hi_tr = w_top_c1(c, time);
+  for (int n = 0; n < h->nth; ++n) {
+    ba.th_lo[n] = c.th_bc_ymin[n]; ba.th_hi[n] = c.th_bc_ymax[n];
+    ba.th_lo_c1[n] = c.th_bc_ymin_c1[n]; ba.th_hi_c1[n] = c.th_bc_ymax_c1[n];
+  }
+  ba.dy1 = h->dy[1]; ba.dy2 = h->dy[2]; ba.dyny = h->dy[ny]; ba.dyf1 = h->dyf[1];
+  for (int f = 0; f < 3 + h->nth; ++f) ba.u[f] = h->U[f];
+  k_bc_post<<<(unsigned)((g.pc + 255) / 256), 256, 0, h->st>>>(ba);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// physical v, dv/dx, dv/dz of every transported field on all planes (FP64), into
+// the LES gradient buffers
+int physical_gradients(dgpu_handle* h) {
+  const Geo& g = h->g;
+  const int nf = 3 + h->nth;
+  const cd* zs[2 * MAXF];
+  void* zd[2 * MAXF];
+  void* const* H = (h->prec == DGPU_MIXED) ? h->HinD : h->Hin;  // FP64 x-spectra
+  for (int f = 0; f < nf; ++f) {
+    zs[f] = h->U[f]; zd[f] = H[f];
+    zs[nf + f] = h->U[f]; zd[nf + f] = h->HinZ[f];
+  }
+  int rc;
+  if ((rc = launch_inv_z<double>(h, 2 * nf, zs, zd, 0, g.ny2, ((1u << nf) - 1u) << nf))) return rc;
+  void* xs[3 * MAXF];
+  double* xd[3 * MAXF];
+  unsigned dxmask = 0;
+  for (int f = 0; f < nf; ++f) {
+    xs[3 * f] = H[f]; xd[3 * f] = (double*)h->Pg[3 * f];
+    xs[3 * f + 1] = H[f]; xd[3 * f + 1] = (double*)h->Pg[3 * f + 1];
+    xs[3 * f + 2] = h->HinZ[f]; xd[3 * f + 2] = (double*)h->Pg[3 * f + 2];
+    dxmask |= 1u << (3 * f + 1);
+  }
+  return launch_inv_x<double>(h, 3 * nf, xs, xd, 0, g.ny2, dxmask);
+}
+
+// the physical-gradient buffers of the LES branch, for a handle built without it
+int ensure_grad_buffers(dgpu_handle* h) {
+  const Geo& g = h->g;
+  const int nf = 3 + h->nth;
+  const size_t rbytes = sizeof(double) * g.pp * g.ny2;
+  const size_t hbytes = sizeof(cd) * g.ph * g.ny2;
+  void* p = nullptr;
+  int rc;
+  for (int f = 0; f < nf; ++f) {
+    if (!h->HinZ[f]) { if ((rc = dalloc(h, &p, hbytes))) return rc; h->HinZ[f] = p; }
+    for (int m = 0; m < 3; ++m)
+      if (!h->Pg[3 * f + m]) { if ((rc = dalloc(h, &p, rbytes))) return rc; h->Pg[3 * f + m] = p; }
+  }
+  // FP64 x-spectra for the transforms (the mixed handle keeps float ones in Hin)
+  if (h->prec == DGPU_MIXED && !h->HinD[0])
+    for (int f = 0; f < nf; ++f) { if ((rc = dalloc(h, &p, hbytes))) return rc; h->HinD[f] = p; }
+  return 0;
+}
+
 // One substep of the LES branch: rk_chan_1 with les_chan / les_chan_th
 // (for/solver.f90:611-623) and the physical-space Crank–Nicolson solves with the
 // eddy viscosity (for/solver.f90:875-903, 990-1233).
@@ -719,42 +792,13 @@ int substep_les(dgpu_handle* h, int rk, double delta_t, double time) {
   const double wtop = w_top_c1(c, time);
 
   // 0. BC posts of les_chan on the state's ghost planes
-  BcArgs ba{};
-  ba.g = g; ba.nth = NTH;
-  ba.u_lo = c.u_bc_ymin; ba.v_lo = c.v_bc_ymin; ba.w_lo = c.w_bc_ymin;
-  ba.u_hi = c.u_bc_ymax; ba.v_hi = c.v_bc_ymax; ba.w_hi = c.w_bc_ymax;
-  ba.u_lo_c1 = c.u_bc_ymin_c1; ba.v_lo_c1 = c.v_bc_ymin_c1; ba.w_lo_c1 = c.w_bc_ymin_c1;
-  ba.u_hi_c1 = c.u_bc_ymax_c1; ba.v_hi_c1 = c.v_bc_ymax_c1; ba.w_hi_c1 = c.w_bc_ymax_c1; ba.w_hi_tr = wtop;
-  for (int n = 0; n < NTH; ++n) {
-    ba.th_lo[n] = c.th_bc_ymin[n]; ba.th_hi[n] = c.th_bc_ymax[n];
-    ba.th_lo_c1[n] = c.th_bc_ymin_c1[n]; ba.th_hi_c1[n] = c.th_bc_ymax_c1[n];
-  }
-  ba.dy1 = h->dy[1]; ba.dy2 = h->dy[2]; ba.dyny = h->dy[ny]; ba.dyf1 = h->dyf[1];
-  for (int f = 0; f < nf; ++f) ba.u[f] = h->U[f];
   if ((rc = tbeg(h, 5))) return rc;
-  k_bc_post<<<(unsigned)((g.pc + 255) / 256), 256, 0, h->st>>>(ba);
-  HIPCHK(hipGetLastError());
+  if ((rc = launch_bc_post(h, time, 1))) return rc;
   if ((rc = tend(h))) return rc;
 
   // 1. x-spectra of v and dv/dz, then physical v, dv/dx, dv/dz on all planes
-  const cd* zs[2 * MAXF];
-  void* zd[2 * MAXF];
-  for (int f = 0; f < nf; ++f) {
-    zs[f] = h->U[f]; zd[f] = h->Hin[f];
-    zs[nf + f] = h->U[f]; zd[nf + f] = h->HinZ[f];
-  }
   if ((rc = tbeg(h, 6))) return rc;
-  if ((rc = launch_inv_z<double>(h, 2 * nf, zs, zd, 0, g.ny2, ((1u << nf) - 1u) << nf))) return rc;
-  void* xs[3 * MAXF];
-  double* xd[3 * MAXF];
-  unsigned dxmask = 0;
-  for (int f = 0; f < nf; ++f) {
-    xs[3 * f] = h->Hin[f]; xd[3 * f] = (double*)h->Pg[3 * f];
-    xs[3 * f + 1] = h->Hin[f]; xd[3 * f + 1] = (double*)h->Pg[3 * f + 1];
-    xs[3 * f + 2] = h->HinZ[f]; xd[3 * f + 2] = (double*)h->Pg[3 * f + 2];
-    dxmask |= 1u << (3 * f + 1);
-  }
-  if ((rc = launch_inv_x<double>(h, 3 * nf, xs, xd, 0, g.ny2, dxmask))) return rc;
+  if ((rc = physical_gradients(h))) return rc;
   if ((rc = tend(h))) return rc;
 
   // 2. AMD: nu_t, kappa_t and the SGS additions to the products
@@ -1138,6 +1182,58 @@ int dgpu_download_les(dgpu_handle* h, double* nu_t, double* kappa_t) {
         std::memcpy(out + ((size_t)j * (g.nz + 2) + z) * (g.nx + 2), &tmp[((size_t)j * g.nz + z) * g.nx],
                     sizeof(double) * g.nx);
   }
+  return 0;
+}
+
+int64_t dgpu_stats_doubles(const dgpu_config* cfg) {
+  if (!cfg) return -1;
+  return (int64_t)(SP_NBASE + SP_PER_TH * cfg->n_th) * (cfg->ny + 2) + 1;
+}
+
+int dgpu_stats(dgpu_handle* h, double time, double* out) {
+  if (!h || !out) return fail(1, "null argument");
+  const dgpu_config& c = h->c;
+  const Geo& g = h->g;
+  const int nth = h->nth, n2 = g.ny2;
+  int rc = ensure_grad_buffers(h);
+  if (rc) return rc;
+  void* p = nullptr;
+  if (!h->d_zm) {
+    if ((rc = dalloc(h, &p, sizeof(double) * (6 + MAXTH) * n2 * g.nx))) return rc;
+    h->d_zm = (double*)p;
+    if ((rc = dalloc(h, &p, sizeof(double) * (SZ_TH0 + SZ_PER_TH * MAXTH) * n2 * g.nx))) return rc;
+    h->d_zs = (double*)p;
+    if ((rc = dalloc(h, &p, sizeof(double) * (SP_NBASE + SP_PER_TH * MAXTH) * n2))) return rc;
+    h->d_prof = (double*)p;
+  }
+  // save_stats_chan starts with the BC posts on the state (for/diagnostics.f90:26-27)
+  if ((rc = launch_bc_post(h, time, 0))) return rc;
+  if ((rc = physical_gradients(h))) return rc;
+  StatArgs a{};
+  a.g = g; a.nth = nth;
+  a.hx = c.homogeneous_x != 0;
+  a.nu = c.nu; a.ly = c.ly; a.lx = c.lx;
+  // the thermal-wind shear (1/f) dTHdX(1) that the stats add to dw/dy and wme
+  a.twf = (nth > 0 && c.ro_inv != 0.0) ? (1.0 / (c.ro_inv / c.delta)) * c.dthdx[0] : 0.0;
+  for (int n = 0; n < MAXTH; ++n) { a.dthdx[n] = c.dthdx[n]; a.dthdz[n] = c.dthdz[n]; }
+  a.dy = h->d_dy; a.dyf = h->d_dyf; a.gy = h->d_gy; a.gyf = h->d_gyf;
+  for (int s = 0; s < 3 * (3 + nth); ++s) a.f[s] = (const double*)h->Pg[s];
+  for (int f = 0; f < 3 + nth; ++f) a.u[f] = h->U[f];
+  a.p = h->P;
+  a.zm = h->d_zm; a.zs = h->d_zs; a.prof = h->d_prof;
+  const dim3 gxj((unsigned)((g.nx + 255) / 256), (unsigned)n2);
+  k_stat_mean<<<gxj, 256, 0, h->st>>>(a);
+  k_stat_zsum<<<gxj, 256, 0, h->st>>>(a);
+  k_stat_plane<<<(unsigned)n2, 256, 0, h->st>>>(a);
+  HIPCHK(hipGetLastError());
+  const size_t np_ = (size_t)(SP_NBASE + SP_PER_TH * nth) * n2;
+  HIPCHK(hipMemcpyAsync(out, h->d_prof, sizeof(double) * np_, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  // u1z_x0: integrate_y_var of u1y_left (for/diagnostics.f90:1964-1980)
+  const double* u1y = out + (size_t)SP_U1Y * n2;
+  double res = 0.0;
+  for (int j = 2; j <= g.ny; ++j) res = res + 0.5 * (u1y[j] + u1y[j - 1]) * h->dy[j];
+  out[np_] = res / c.ly;
   return 0;
 }
 

@@ -22,7 +22,8 @@ module diablo_gpu
     integer(c_int32_t) :: u_bc_ymin, v_bc_ymin, w_bc_ymin
     integer(c_int32_t) :: u_bc_ymax, v_bc_ymax, w_bc_ymax
     integer(c_int32_t) :: th_bc_ymin(DGPU_MAX_TH), th_bc_ymax(DGPU_MAX_TH)
-    integer(c_int32_t) :: reserved_i(4)
+    integer(c_int32_t) :: homogeneous_x
+    integer(c_int32_t) :: reserved_i(3)
     real(c_double) :: lx, ly, lz
     real(c_double) :: nu, beta, nu_v_scale
     real(c_double) :: ro_inv, delta

@@ -56,6 +56,9 @@ def load():
     L.dgpu_timing_read.argtypes = [P, ctypes.c_int32, D, ctypes.POINTER(ctypes.c_int64),
                                    ctypes.POINTER(ctypes.c_char_p)]
     L.dgpu_download_les.argtypes = [P, D, D]
+    L.dgpu_stats.argtypes = [P, ctypes.c_double, D]
+    L.dgpu_stats_doubles.argtypes = [P]
+    L.dgpu_stats_doubles.restype = ctypes.c_int64
     L.dgpu_field_doubles.argtypes = [P]
     L.dgpu_field_doubles.restype = ctypes.c_int64
     _lib = L
@@ -140,6 +143,18 @@ class Solver:
         self._check(load().dgpu_download_les(self.h, _dp(nt), _dp(kt)))
         return (layout.from_fortran_physical(self.cfg, nt),
                 [layout.from_fortran_physical(self.cfg, kt[n]) for n in range(self.cfg.n_th)])
+
+    def stats(self, time: float = 0.0):
+        """save_stats_chan's energy-budget profiles: {name: (Ny+2,) array}, 'u1z_x0'."""
+        from .stats import PROFILES, PER_SCALAR
+        n = load().dgpu_stats_doubles(ctypes.cast(self.p.cconfig.ptr, ctypes.c_void_p))
+        out = np.zeros(n, np.float64)
+        self._check(load().dgpu_stats(self.h, time, _dp(out)))
+        n2 = self.cfg.ny2
+        names = PROFILES + [f"{b}{k + 1:02d}" for k in range(self.cfg.n_th) for b in PER_SCALAR]
+        res = {name: out[i * n2:(i + 1) * n2].copy() for i, name in enumerate(names)}
+        res["u1z_x0"] = float(out[-1])
+        return res
 
     # ---- time advance ----
     def rk_substep(self, rk_step: int, delta_t: Optional[float] = None, time: float = 0.0,

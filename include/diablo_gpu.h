@@ -65,7 +65,10 @@ typedef struct dgpu_config {
   int32_t u_bc_ymin, v_bc_ymin, w_bc_ymin;
   int32_t u_bc_ymax, v_bc_ymax, w_bc_ymax;
   int32_t th_bc_ymin[DGPU_MAX_TH], th_bc_ymax[DGPU_MAX_TH];
-  int32_t reserved_i[4];
+  /* homogeneousX of init_parameters (for/parameters.f90:282-287): 0 for IC types
+   * 5-8 (finite fronts), else 1; selects plane or z-line averages in dgpu_stats */
+  int32_t homogeneous_x;
+  int32_t reserved_i[3];
 
   double lx, ly, lz;        /* domain lengths (for/parameters.f90:101) */
   double nu, beta, nu_v_scale;
@@ -157,6 +160,24 @@ int dgpu_last_step_ms(dgpu_handle *h, double *ms);
 int dgpu_timing_enable(dgpu_handle *h, int32_t enable);
 int dgpu_timing_read(dgpu_handle *h, int32_t kernel, double *total_ms, int64_t *launches,
                      const char **name);
+
+/* Energy-budget statistics of save_stats_chan (for/diagnostics.f90:3-669, with
+ * compute_averages, compute_TKE_diss, compute_MKE_diss, compute_TKE and
+ * compute_TKE_Production, 676-1519), computed on the device in FP64 from the
+ * current state, which first gets the BC posts save_stats_chan applies
+ * (for/diagnostics.f90:26-27).  out holds dgpu_stats_doubles(cfg) doubles: profiles
+ * of Ny+2 entries (index j = 0..Nyp+1 as the reference's arrays; entries it never
+ * sets are 0), in this order (code variable names; the mean.h5 dataset names of
+ * for/diagnostics.f90:140-480 swap v and w):
+ *   0 ume  1 vme  2 wme  3 mke  4 urms  5 vrms  6 wrms  7 uw  8 uv  9 wv
+ *   10 uu_dudx  11 vv_dvdy  12 wu_dwdx  13 vu_dvdx  14 uv_dudy  15 wv_dwdy
+ *   16 dudy  17 dwdy  18 shear  19 epsilon  20 epsilon_m  21 cp (mean mode)
+ *   22 u1y_left
+ * then per scalar n: thme, thrms, thv, thv_m, dthdy, pe_diss; then one double,
+ * u1z_x0 (the y-average of u1y_left, integrate_y_var, for/diagnostics.f90:1964).
+ * Not computed: movie planes, z* bins, the BPE PDF, FTx_uu, the vorticity rms. */
+int dgpu_stats(dgpu_handle *h, double time, double *out);
+int64_t dgpu_stats_doubles(const dgpu_config *cfg);
 
 /* Text of the last error on this thread ("" if none). */
 const char *dgpu_last_error(void);

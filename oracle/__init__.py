@@ -42,6 +42,7 @@ def lib():
         L.oracle_rk_substep.argtypes = [P, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_int, ctypes.c_int]
         L.oracle_rem_div.argtypes = [P]
+        L.oracle_bc_post.argtypes = [P, ctypes.c_double]
         L.oracle_fft_xz_to_fourier.argtypes = [P, D, D]
         L.oracle_fft_xz_to_physical.argtypes = [P, D, D]
         L.oracle_wavenumbers.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double,
@@ -135,6 +136,9 @@ class Oracle:
 
     def rem_div(self):
         self._check(lib().oracle_rem_div(self.h))
+
+    def bc_post(self, time=0.0):
+        self._check(lib().oracle_bc_post(self.h, time))
 
     def fft_to_fourier(self, phys_compact):
         pf = self._layout.to_fortran_physical(self.cfg, phys_compact)

@@ -1121,6 +1121,13 @@ static void les_amd(oracle_t *o, double time) {
 #undef LOOP_PX
 #undef LOOP_FX
 
+/* apply_BC_vel_mpi_post + apply_BC_th_mpi_post on the Fourier state, as
+ * save_stats_chan starts (for/diagnostics.f90:26-27) */
+int oracle_bc_post(oracle_t *o, double time) {
+  bc_post(o, time);
+  return 0;
+}
+
 int oracle_rk_substep(oracle_t *o, int rk_step, double delta_t, double time,
                       int time_step, int create_new_flow) {
   const dgpu_config *c = &o->c;

@@ -39,6 +39,8 @@ int oracle_download_les(oracle_t *o, double *nu_t, double *kappa_t);
 int oracle_rk_substep(oracle_t *o, int rk_step, double delta_t, double time,
                       int time_step, int create_new_flow);
 int oracle_rem_div(oracle_t *o);
+/* apply_BC_vel_mpi_post + apply_BC_th_mpi_post (for/flow.f90:1161-1360, 1596-1732) */
+int oracle_bc_post(oracle_t *o, double time);
 
 int oracle_fft_xz_to_fourier(oracle_t *o, const double *phys, double *four);
 int oracle_fft_xz_to_physical(oracle_t *o, const double *four, double *phys);

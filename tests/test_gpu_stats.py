@@ -1,0 +1,67 @@
+"""Device energy-budget statistics (dgpu_stats) against the oracle's restatement
+of save_stats_chan, through the C ABI.
+
+Bar: every profile within 1e-10 of the oracle, relative to the profile's largest
+magnitude (the acceptance bar for the budget terms); the state after the call
+equals the oracle's state after the same BC posts (save_stats_chan applies them to
+the flow it saves).  Both averaging modes: plane means (x-homogeneous infinite
+front) and z-line means (finite front, IC type 6).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from oracle import diagnostics
+from diablo3_amd import configs
+from diablo3_amd.problem import Problem
+from diablo3_amd.solver import Solver
+
+pytestmark = pytest.mark.gpu
+
+
+def prepared(cfg, seed=3):
+    p = Problem(cfg)
+    o = oracle.Oracle(p, threads=16)
+    s = p.initial_state(seed=seed, theta_noise=1e-2)
+    rng = np.random.default_rng(seed)
+    sh = (cfg.ny - 3, 4, 4)
+    for a in (s.cu1, s.cu2, s.cu3, s.cth[0]):
+        a[2:cfg.ny - 1, 1:5, 1:5] += 0.02 * (rng.standard_normal(sh) + 1j * rng.standard_normal(sh))
+    o.upload_state(s)
+    o.rem_div()
+    for rk in (1, 2, 3):
+        o.rk_substep(rk, time=0.0, time_step=1)
+    return p, o
+
+
+@pytest.mark.parametrize("which", ["infinite_front_dns", "finite_front"])
+def test_stats_match_oracle(which):
+    if which == "infinite_front_dns":
+        cfg = configs.small_dns(32, 33, 32)
+    else:
+        cfg = configs.front_les(32, 33, 32, 5000.0, 2e-4, 1e-2)
+        cfg.use_les = False  # the state is advanced by the DNS branch here
+    p, o = prepared(cfg)
+    s0 = o.download_state()
+    g = Solver(p)
+    g.upload_state(s0)
+    sg = g.stats(time=0.0)
+    so = diagnostics.save_stats(o, time=0.0)
+    print(which, "homogeneous_x", diagnostics.homogeneous_x(cfg), flush=True)
+    worst = 0.0
+    for name in diagnostics.names(cfg.n_th):
+        a, b = sg[name], so[name]
+        scale = np.abs(b).max()
+        err = np.abs(a - b).max() / (scale if scale > 0 else 1.0)
+        worst = max(worst, err)
+        print(f"  {name:10s} max|ref| {scale:.3e} rel err {err:.2e}", flush=True)
+        assert err <= 1e-10, (name, err)
+    assert abs(sg["u1z_x0"] - so["u1z_x0"]) <= 1e-10 * max(abs(so["u1z_x0"]), 1e-300)
+    # the side effect on the state: the BC posts
+    a, b = g.download_state(), o.download_state()
+    for x, y in ((a.cu1, b.cu1), (a.cu2, b.cu2), (a.cu3, b.cu3), (a.cth[0], b.cth[0])):
+        assert np.array_equal(x, y)
+    # the budget terms are not trivially zero on this state
+    assert so["epsilon"].max() > 0 and np.abs(so["thv01"]).max() > 0 and so["urms"].max() > 0
+    if not diagnostics.homogeneous_x(cfg):
+        assert so["epsilon_m"].max() > 0 and np.abs(so["uu_dudx"]).max() > 0
