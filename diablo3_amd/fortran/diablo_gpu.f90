@@ -42,6 +42,7 @@ module diablo_gpu
   public :: dgpu_upload_rk, dgpu_download_rk, dgpu_rk_substep, dgpu_step, dgpu_rem_div
   public :: dgpu_fft_xz_to_fourier, dgpu_fft_xz_to_physical, dgpu_synchronize
   public :: dgpu_last_step_ms, dgpu_field_doubles, dgpu_last_error_f
+  public :: dgpu_download_les, dgpu_stats, dgpu_stats_doubles
 
   interface
     integer(c_int) function dgpu_create(cfg, h) bind(C, name="dgpu_create")
@@ -104,6 +105,20 @@ module diablo_gpu
       real(c_double), intent(out) :: ms
     end function
     integer(c_int64_t) function dgpu_field_doubles(cfg) bind(C, name="dgpu_field_doubles")
+      import :: c_int64_t, dgpu_config
+      type(dgpu_config), intent(in) :: cfg
+    end function
+    integer(c_int) function dgpu_download_les(h, nu_t, kappa_t) bind(C, name="dgpu_download_les")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h, nu_t, kappa_t
+    end function
+    integer(c_int) function dgpu_stats(h, time, out) bind(C, name="dgpu_stats")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), value :: time
+      type(c_ptr), value :: out
+    end function
+    integer(c_int64_t) function dgpu_stats_doubles(cfg) bind(C, name="dgpu_stats_doubles")
       import :: c_int64_t, dgpu_config
       type(dgpu_config), intent(in) :: cfg
     end function
