@@ -428,7 +428,6 @@ __global__ __launch_bounds__(NT) void k_fwdz_nl(NlArgs a) {
 //     reduction, then the velocity correction and the pressure update
 //     (for/solver.f90:1261-1267).
 // ---------------------------------------------------------------------------
-constexpr int PCR_LMAX = 10;  // up to 1024 rows
 
 struct SolveRows {
   int blo, bhi;      // build range: rows whose cr comes from the assembly
@@ -454,7 +453,7 @@ enum RowCoef {
 
 struct ColArgs {
   Geo g;
-  int rk, do_cn, levels, plevels;
+  int rk, do_cn, plevels;
   int jstart, jend, jstart_th[MAXTH], jend_th[MAXTH];
   int front, f_type;
   double nu, beta, nu_v_scale, ro_inv, delta, grav_x, grav_y, grav_z, ly, px0, omega0, amp_omega0, time;
@@ -467,7 +466,11 @@ struct ColArgs {
   const double *__restrict__ dy, *__restrict__ dyf, *__restrict__ gy, *__restrict__ gyf;
   const double *__restrict__ kx, *__restrict__ kz;
   SolveRows rows[MAXF];
-  const double* __restrict__ pcr;  // [field][2*PCR_LMAX+1][ny2]: alpha_l, gamma_l, 1/b_final
+  // Thomas factorisation of the CN systems, [field][ny2][5]: mu_j (forward
+  // multiplier), beta_j = 1/b'_j, gamma_j = -c_j/b'_j, F_j / G_j (products of mu /
+  // gamma from the chunk start / to the chunk end); rows per chunk
+  const double* __restrict__ thc;
+  int chunk;
   cd* u[MAXF];
   cd* p;
   cd* cf[MAXF];
@@ -583,7 +586,7 @@ __device__ __forceinline__ void assemble(const ColArgs& a, const RowIn<NTH>& r, 
 // Assembly + Crank-Nicolson solves.  T: precision of the nonlinear terms;
 // TC columns x RG row groups per block; RPT >= ceil(ny2 / RG) rows per thread.
 // The CN solution overwrites the state rows [wlo, whi]; k_project follows.
-template <typename T, int TC, int NT, int RPT, int NTH>
+template <typename T, int TC, int NT, int RPT, int NTH, int LMAX>
 __global__ __launch_bounds__(NT) void k_cn(ColArgs a) {
   constexpr int NF = 3 + NTH;
   constexpr int RG = NT / TC;
@@ -597,9 +600,7 @@ __global__ __launch_bounds__(NT) void k_cn(ColArgs a) {
   const int i = c % g.kxp, k = c / g.kxp;
   const bool active = i < g.kxr;
   const long long pc = g.pc;
-  const int lvl = a.levels;
   cd* sR = (cd*)smem;
-  double* sCo = (double*)(sR + NF * TC * ny2);  // one field's PCR table: (2*lvl+1) rows of ny2
   // rows of the TC columns interleaved: lanes of one row group touch consecutive 16-byte slots
   auto R = [&](int f, int j) -> cd& { return sR[(f * ny2 + j) * TC + cc]; };
   const cd* __restrict__ u1 = a.u[0];
@@ -653,42 +654,86 @@ __global__ __launch_bounds__(NT) void k_cn(ColArgs a) {
   if (!a.do_cn) return;  // uniform over the grid
   __syncthreads();
 
-  // ---- 2. PCR with the host's coefficients, scalars first (for/solver.f90:1004-1007)
-  for (int e0 = 0; e0 < NF; ++e0) {
+  // ---- 2. Thomas with the host's factorisation, in chunks.  One wave takes one
+  // field of the TC columns: lane = p*TC + cc, p = chunk of L consecutive rows.
+  // Per chunk: local elimination from zero, a shuffle scan of the chunk-boundary
+  // carries (the affine maps y -> A y + B compose), and the fix-up with the
+  // host's in-chunk products.  Same linear map as thomas_forward/backward_real_mpi
+  // (for/solver.f90:1691-1761, 1816-1852); scalars first (for/solver.f90:1004-1007).
+  constexpr int P = 64 / TC;
+  const int lane = tid & 63, wave = tid / 64;
+  const int p = lane / TC, ccl = lane % TC;
+  const int L = a.chunk, j0 = p * L;
+  const int cl = tile * TC + ccl;
+  const bool act_l = (cl % g.kxp) < g.kxr;
+  for (int e0 = wave; e0 < NF; e0 += NT / 64) {
     const int e = (e0 < NTH) ? 3 + e0 : e0 - NTH;
-    const double* co = a.pcr + (long long)e * (2 * PCR_LMAX + 1) * ny2;
-    for (int q = tid; q < 2 * lvl * ny2; q += NT) sCo[q] = co[q];
-    for (int q = tid; q < ny2; q += NT) sCo[2 * lvl * ny2 + q] = co[2 * PCR_LMAX * ny2 + q];
-    __syncthreads();
-    for (int l = 0; l < lvl; ++l) {
-      const int s = 1 << l;
-      const double* Al = sCo + (2 * l) * ny2;
-      const double* Gl = sCo + (2 * l + 1) * ny2;
-      cd nv[RPT];
+    const double* __restrict__ co = a.thc + (long long)e * ny2 * 5;  // mu, beta, gamma, F, G per row
+    auto Rl = [&](int j) -> cd& { return sR[(e * ny2 + j) * TC + ccl]; };
+    cd y[LMAX];
+    // forward: y_j = g_j + mu_j y_{j-1}, from zero at the chunk start
+    cd prev = {0.0, 0.0};
+    double Af = 1.0;
+    cd Bf = {0.0, 0.0};
 #pragma unroll
-      for (int m = 0; m < RPT; ++m) {
-        const int j = rg + m * RG;
-        if (j < ny2) {
-          const cd dm = (j - s >= 0) ? R(e, j - s) : cd{0.0, 0.0};
-          const cd dp = (j + s < ny2) ? R(e, j + s) : cd{0.0, 0.0};
-          nv[m] = R(e, j) + Al[j] * dm + Gl[j] * dp;
-        }
+    for (int m = 0; m < LMAX; ++m) {
+      const int j = j0 + m;
+      if (m < L && j < ny2) {
+        y[m] = Rl(j) + co[5 * j] * prev;
+        prev = y[m];
+        Af = co[5 * j + 3];
+        Bf = y[m];
       }
-      __syncthreads();
-#pragma unroll
-      for (int m = 0; m < RPT; ++m) {
-        const int j = rg + m * RG;
-        if (j < ny2) R(e, j) = nv[m];
-      }
-      __syncthreads();
     }
+    // inclusive scan of the maps over the chunks below (lanes p - d)
+#pragma unroll
+    for (int d = 1; d < P; d <<= 1) {
+      const double Au = __shfl_up(Af, d * TC);
+      const cd Bu = {__shfl_up(Bf.x, d * TC), __shfl_up(Bf.y, d * TC)};
+      if (p >= d) {
+        Bf = Af * Bu + Bf;
+        Af = Af * Au;
+      }
+    }
+    cd yin = {__shfl_up(Bf.x, TC), __shfl_up(Bf.y, TC)};
+    if (p == 0) yin = {0.0, 0.0};
+#pragma unroll
+    for (int m = 0; m < LMAX; ++m) {
+      const int j = j0 + m;
+      if (m < L && j < ny2) y[m] = y[m] + co[5 * j + 3] * yin;
+    }
+    // backward: x_j = beta_j y_j + gamma_j x_{j+1}, from zero past the chunk end
+    cd nxt = {0.0, 0.0};
+    double Ab = 1.0;
+    cd Bb = {0.0, 0.0};
+#pragma unroll
+    for (int m = LMAX - 1; m >= 0; --m) {
+      const int j = j0 + m;
+      if (m < L && j < ny2) {
+        y[m] = co[5 * j + 1] * y[m] + co[5 * j + 2] * nxt;
+        nxt = y[m];
+        Ab = co[5 * j + 4];
+        Bb = y[m];
+      }
+    }
+#pragma unroll
+    for (int d = 1; d < P; d <<= 1) {
+      const double Ad = __shfl_down(Ab, d * TC);
+      const cd Bd = {__shfl_down(Bb.x, d * TC), __shfl_down(Bb.y, d * TC)};
+      if (p + d < P) {
+        Bb = Ab * Bd + Bb;
+        Ab = Ab * Ad;
+      }
+    }
+    cd xin = {__shfl_down(Bb.x, TC), __shfl_down(Bb.y, TC)};
+    if (p == P - 1) xin = {0.0, 0.0};
     const SolveRows& SR = a.rows[e];
-    const double* Bf = sCo + (2 * lvl) * ny2;
-    for (int m = 0; m < RPT; ++m) {
-      const int j = rg + m * RG;
-      if (active && j < ny2 && j >= SR.wlo && j <= SR.whi) a.u[e][(long long)j * pc + c] = Bf[j] * R(e, j);
+#pragma unroll
+    for (int m = 0; m < LMAX; ++m) {
+      const int j = j0 + m;
+      if (m < L && j < ny2 && act_l && j >= SR.wlo && j <= SR.whi)
+        a.u[e][(long long)j * pc + cl] = y[m] + co[5 * j + 4] * xin;
     }
-    __syncthreads();
   }
 }
 

@@ -178,92 +178,94 @@ int pcr_levels(int n) {
   return l;
 }
 
-// Parallel-cyclic-reduction coefficients of one tridiagonal system (rows 0..n-1,
-// a[0] = c[n-1] = 0): at level l (stride s = 2^l) every row j is replaced by
-// row_j + alpha_l[j] row_{j-s} + gamma_l[j] row_{j+s}; after the last level the
-// system is diagonal and x_j = d_j / b_j.  Computed in long double and stored as
-// double; out = [alpha_0, gamma_0, ..., alpha_{L-1}, gamma_{L-1}, (unused), 1/b].
-int pcr_coefficients(const std::vector<double>& a0, const std::vector<double>& b0,
-                     const std::vector<double>& c0, double* out, int lmax) {
-  const int n = (int)b0.size();
-  const int L = pcr_levels(n);
-  if (L > lmax) return fail(2, "too many rows for the PCR coefficient table");
-  std::vector<long double> a(a0.begin(), a0.end()), b(b0.begin(), b0.end()), c(c0.begin(), c0.end());
-  std::vector<long double> na(n), nb(n), nc(n);
-  for (int l = 0; l < L; ++l) {
-    const int s = 1 << l;
-    double* A = out + (size_t)(2 * l) * n;
-    double* G = out + (size_t)(2 * l + 1) * n;
-    for (int j = 0; j < n; ++j) {
-      long double al = 0, ga = 0, a2 = 0, c2 = 0, b2 = b[j];
-      if (j - s >= 0) {
-        if (b[j - s] == 0) return fail(20, "zero pivot in cyclic reduction");
-        al = -a[j] / b[j - s];
-        a2 = al * a[j - s];
-        b2 += al * c[j - s];
-      }
-      if (j + s < n) {
-        if (b[j + s] == 0) return fail(20, "zero pivot in cyclic reduction");
-        ga = -c[j] / b[j + s];
-        c2 = ga * c[j + s];
-        b2 += ga * a[j + s];
-      }
-      A[j] = (double)al;
-      G[j] = (double)ga;
-      na[j] = a2; nb[j] = b2; nc[j] = c2;
-    }
-    a.swap(na); b.swap(nb); c.swap(nc);
-  }
-  double* B = out + (size_t)(2 * lmax) * n;
-  for (int j = 0; j < n; ++j) {
-    if (b[j] == 0) return fail(20, "singular Crank-Nicolson system");
-    B[j] = (double)(1.0L / b[j]);
-  }
-  // self-check against a long-double Thomas solve of a fixed right-hand side
-  std::vector<long double> g(n), bp(n), x(n);
-  std::vector<double> d(n);
-  for (int j = 0; j < n; ++j) d[j] = std::sin(0.37 * j + 0.1) + 0.5;
-  bp[0] = b0[0];
-  g[0] = d[0];
+struct ColShape;
+int cn_chunk(const dgpu_handle* h);
+
+// Thomas factorisation of one tridiagonal system (rows 0..n-1, a[0] = c[n-1] = 0)
+// for the chunked solve of k_cn: per row j, out[5j..5j+4] = mu_j, beta_j, gamma_j,
+// F_j, G_j with (thomas_forward/backward_real_mpi, for/solver.f90:1691-1852)
+//   b'_0 = b_0, mu_j = -a_j / b'_{j-1}, b'_j = b_j + mu_j c_{j-1},
+//   forward y_j = g_j + mu_j y_{j-1}, backward x_j = beta_j y_j + gamma_j x_{j+1},
+//   beta_j = 1/b'_j, gamma_j = -c_j/b'_j,
+// and, for chunks of L rows, F_j = prod of mu over [chunk start, j] and G_j = prod
+// of gamma over [j, chunk end].  Long double, stored as double; self-checked by
+// running the chunked algorithm against a long-double Thomas solve.
+int thomas_chunk_coefficients(const std::vector<double>& a, const std::vector<double>& b,
+                              const std::vector<double>& c, int L, double* out) {
+  const int n = (int)b.size();
+  std::vector<long double> mu(n), bp(n), be(n), ga(n);
+  bp[0] = b[0];
+  mu[0] = 0;
   for (int j = 1; j < n; ++j) {
-    const long double m = -(long double)a0[j] / bp[j - 1];
-    bp[j] = b0[j] + m * c0[j - 1];
-    g[j] = d[j] + m * g[j - 1];
+    if (bp[j - 1] == 0) return fail(20, "zero pivot in the Crank-Nicolson system");
+    mu[j] = -(long double)a[j] / bp[j - 1];
+    bp[j] = b[j] + mu[j] * c[j - 1];
   }
-  x[n - 1] = g[n - 1] / bp[n - 1];
-  for (int j = n - 2; j >= 0; --j) x[j] = (g[j] - c0[j] * x[j + 1]) / bp[j];
-  std::vector<double> y(d), ny_(n);
-  for (int l = 0; l < L; ++l) {
-    const int s = 1 << l;
-    const double* A = out + (size_t)(2 * l) * n;
-    const double* G = out + (size_t)(2 * l + 1) * n;
-    for (int j = 0; j < n; ++j)
-      ny_[j] = y[j] + A[j] * (j - s >= 0 ? y[j - s] : 0.0) + G[j] * (j + s < n ? y[j + s] : 0.0);
-    y.swap(ny_);
+  for (int j = 0; j < n; ++j) {
+    if (bp[j] == 0) return fail(20, "singular Crank-Nicolson system");
+    be[j] = 1.0L / bp[j];
+    ga[j] = -(long double)c[j] / bp[j];
+  }
+  for (int j = 0; j < n; ++j) {
+    const int s0 = j / L * L, e0 = std::min(s0 + L, n) - 1;
+    long double F = 1, G = 1;
+    for (int i = s0; i <= j; ++i) F *= mu[i];
+    for (int i = j; i <= e0; ++i) G *= ga[i];
+    out[5 * j + 0] = (double)mu[j];
+    out[5 * j + 1] = (double)be[j];
+    out[5 * j + 2] = (double)ga[j];
+    out[5 * j + 3] = (double)F;
+    out[5 * j + 4] = (double)G;
+  }
+  // self-check: the chunked algorithm in double against a long-double Thomas solve
+  std::vector<double> d(n), yl(n);
+  std::vector<long double> g(n), x(n);
+  for (int j = 0; j < n; ++j) d[j] = std::sin(0.37 * j + 0.1) + 0.5;
+  g[0] = d[0];
+  for (int j = 1; j < n; ++j) g[j] = d[j] + mu[j] * g[j - 1];
+  x[n - 1] = g[n - 1] * be[n - 1];
+  for (int j = n - 2; j >= 0; --j) x[j] = be[j] * g[j] + ga[j] * x[j + 1];
+  const int P = (n + L - 1) / L;
+  std::vector<double> yin(P + 1, 0.0), xin(P + 1, 0.0);
+  for (int q = 0; q < P; ++q) {  // forward: local + carries in chunk order
+    double pr = 0.0;
+    for (int j = q * L; j < std::min((q + 1) * L, n); ++j) { yl[j] = d[j] + out[5 * j] * pr; pr = yl[j]; }
+    const int e0 = std::min((q + 1) * L, n) - 1;
+    yin[q + 1] = yl[e0] + out[5 * e0 + 3] * yin[q];
+  }
+  for (int j = 0; j < n; ++j) yl[j] = yl[j] + out[5 * j + 3] * yin[j / L];
+  std::vector<double> xl(n);
+  for (int q = P - 1; q >= 0; --q) {
+    double nx = 0.0;
+    const int s0 = q * L, e0 = std::min((q + 1) * L, n) - 1;
+    for (int j = e0; j >= s0; --j) { xl[j] = out[5 * j + 1] * yl[j] + out[5 * j + 2] * nx; nx = xl[j]; }
+    xin[q] = xl[s0] + out[5 * s0 + 4] * xin[q + 1];
   }
   long double err = 0, nrm = 0;
   for (int j = 0; j < n; ++j) {
-    const long double v = B[j] * y[j];
+    const long double v = xl[j] + out[5 * j + 4] * xin[j / L + 1];
     err = std::max(err, std::fabs(v - x[j]));
     nrm = std::max(nrm, std::fabs(x[j]));
   }
-  if (!(err <= 1e-12L * (nrm > 0 ? nrm : 1.0L))) return fail(21, "cyclic reduction disagrees with Thomas");
+  if (!(err <= 1e-12L * (nrm > 0 ? nrm : 1.0L))) return fail(21, "chunked Thomas disagrees with Thomas");
   return 0;
 }
 
 int ensure_factors(dgpu_handle* h, double delta_t) {
   if (h->fac_dt == delta_t) return 0;
   const int n2 = h->c.ny + 2, nf = 3 + h->nth;
-  const size_t per_field = (size_t)(2 * PCR_LMAX + 1) * n2;
+  const size_t per_field = (size_t)5 * n2;
   std::vector<double> buf((size_t)3 * MAXF * per_field, 0.0), a, b, cc;
   const double hb[3] = {delta_t * (8.0 / 15.0), delta_t * (2.0 / 15.0), delta_t * (5.0 / 15.0)};
+  const int L = cn_chunk(h);
+  if (L <= 0) return fail(2, "ny too large for the column kernels");
   for (int r = 0; r < 3; ++r) {
     const double temp1 = 0.5 * h->c.nu * hb[r];
     for (int F = 0; F < nf; ++F) {
       cn_matrix(h, F, temp1, a, b, cc);
       a[0] = 0.0;
       cc[n2 - 1] = 0.0;
-      int rc = pcr_coefficients(a, b, cc, &buf[((size_t)r * MAXF + F) * per_field], PCR_LMAX);
+      int rc = thomas_chunk_coefficients(a, b, cc, L, &buf[((size_t)r * MAXF + F) * per_field]);
       if (rc) return rc;
     }
   }
@@ -521,11 +523,21 @@ struct ColShape {
 };
 ColShape col_shape(const dgpu_handle* h, bool cn) {
   const int ny2 = h->g.ny2, nf = 3 + h->nth;
-  const int lvl = pcr_levels(ny2);
   ColShape cs{};
+  if (cn) {
+    // k_cn: TC columns x all fields in LDS; one wave solves one field of the TC
+    // columns in 64/TC chunks of L <= 32 rows (LMAX = 4 RPT >= L)
+    for (int tc : {8, 4, 2, 1}) {
+      const size_t lds = (size_t)tc * ny2 * 16 * nf;
+      const int rg = 256 / tc, need = (ny2 + rg - 1) / rg;
+      int rpt = 1;
+      while (rpt < need) rpt *= 2;
+      if (lds <= 80 * 1024 && rpt <= 8) { cs = {tc, rpt, lds}; break; }
+    }
+    return cs;
+  }
   for (int tc : {8, 4, 2}) {
-    const size_t lds = cn ? (size_t)tc * ny2 * 16 * nf + (size_t)(2 * lvl + 1) * ny2 * 8
-                          : (size_t)tc * ny2 * 40;
+    const size_t lds = (size_t)tc * ny2 * 40;
     if (lds <= 80 * 1024 || (tc == 2 && lds <= 160 * 1024)) {
       const int rg = 256 / tc, need = (ny2 + rg - 1) / rg;
       int rpt = 1;
@@ -537,13 +549,23 @@ ColShape col_shape(const dgpu_handle* h, bool cn) {
   return cs;
 }
 
+// rows per chunk of the k_cn solve (0 if no shape fits)
+int cn_chunk(const dgpu_handle* h) {
+  const ColShape cs = col_shape(h, true);
+  if (cs.tc == 0) return 0;
+  const int P = 64 / cs.tc;
+  return (h->g.ny2 + P - 1) / P;
+}
+
 int grid_xcd(long long tiles) { return (int)((tiles + 7) / 8 * 8); }
 
 template <typename T, int NTH, int TC, int RPT>
 int launch_cn_t(dgpu_handle* h, ColArgs& a, size_t lds) {
-  HIPCHK(hipFuncSetAttribute((const void*)&k_cn<T, TC, 256, RPT, NTH>,
+  constexpr int LMAX = 4 * RPT;
+  if (a.chunk > LMAX || a.chunk * (64 / TC) < h->g.ny2) return fail(2, "column chunking mismatch");
+  HIPCHK(hipFuncSetAttribute((const void*)&k_cn<T, TC, 256, RPT, NTH, LMAX>,
                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  k_cn<T, TC, 256, RPT, NTH><<<grid_xcd(h->g.pc / TC), 256, lds, h->st>>>(a);
+  k_cn<T, TC, 256, RPT, NTH, LMAX><<<grid_xcd(h->g.pc / TC), 256, lds, h->st>>>(a);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -560,13 +582,16 @@ int launch_project_t(dgpu_handle* h, ColArgs& a, size_t lds, int update_p) {
 #define DG_SHAPES(X) X(8, 1) X(8, 2) X(8, 4) X(8, 8) X(4, 1) X(4, 2) X(4, 4) X(4, 8) \
                      X(2, 1) X(2, 2) X(2, 4) X(2, 8) X(2, 16)
 
+#define DG_CN_SHAPES(X) X(8, 1) X(8, 2) X(8, 4) X(8, 8) X(4, 1) X(4, 2) X(4, 4) X(4, 8) \
+                        X(2, 1) X(2, 2) X(2, 4) X(2, 8) X(1, 1) X(1, 2) X(1, 4) X(1, 8)
+
 template <typename T, int NTH>
 int launch_cn(dgpu_handle* h, ColArgs& a) {
   const ColShape cs = col_shape(h, true);
   if (cs.tc == 0 || h->g.pc % cs.tc) return fail(2, "ny too large for the column kernels");
 #define DG_X(TC_, RPT_) \
   if (cs.tc == TC_ && cs.rpt == RPT_) return launch_cn_t<T, NTH, TC_, RPT_>(h, a, cs.lds);
-  DG_SHAPES(DG_X)
+  DG_CN_SHAPES(DG_X)
 #undef DG_X
   return fail(2, "ny too large for the column kernels");
 }
@@ -668,7 +693,6 @@ int download_field(dgpu_handle* h, const cd* dev, double* fort) {
 void fill_col_args(dgpu_handle* h, ColArgs& ca, double time) {
   const dgpu_config& c = h->c;
   ca.g = h->g;
-  ca.levels = pcr_levels(h->g.ny2);
   ca.plevels = pcr_levels(h->g.ny);
   ca.jstart = c.jstart; ca.jend = c.jend;
   for (int n = 0; n < MAXTH; ++n) {
@@ -844,7 +868,8 @@ int substep_les(dgpu_handle* h, int rk, double delta_t, double time) {
   ca.temp1 = temp1; ca.temp3 = temp3; ca.temp4 = temp4; ca.temp5 = temp5;
   ca.t1v = temp1 * c.nu_v_scale;
   for (int n = 0; n < NTH; ++n) ca.t1pr[n] = temp1 / c.pr[n] * c.nu_v_scale;
-  ca.pcr = h->d_fac + (size_t)r * MAXF * (2 * PCR_LMAX + 1) * g.ny2;
+  ca.thc = h->d_fac + (size_t)r * MAXF * 5 * g.ny2;
+  ca.chunk = cn_chunk(h);
   for (int f = 0; f < nf; ++f) ca.crout[f] = h->CR[f];
   if ((rc = tbeg(h, 3))) return rc;
   if ((rc = launch_cn<double, NTH>(h, ca))) return rc;
@@ -959,7 +984,8 @@ int substep(dgpu_handle* h, int rk, double delta_t, double time, int time_step, 
   ca.temp1 = temp1; ca.temp3 = temp3; ca.temp4 = temp4; ca.temp5 = temp5;
   ca.t1v = temp1 * c.nu_v_scale;
   for (int n = 0; n < h->nth; ++n) ca.t1pr[n] = temp1 / c.pr[n] * c.nu_v_scale;
-  ca.pcr = h->d_fac + (size_t)r * MAXF * (2 * PCR_LMAX + 1) * g.ny2;
+  ca.thc = h->d_fac + (size_t)r * MAXF * 5 * g.ny2;
+  ca.chunk = cn_chunk(h);
   if ((rc = tbeg(h, 3))) return rc;
   rc = by_nth(h->nth, [&](auto NTHc) {
     constexpr int NTH = decltype(NTHc)::value;
@@ -1032,7 +1058,7 @@ int dgpu_create(const dgpu_config* cfg, dgpu_handle** out) {
   for (int o = 0; o < no; ++o) ALLOC(h->Hout[o], hbytes);
   ALLOC(h->twx64, sizeof(cd) * g.nx); ALLOC(h->twz64, sizeof(cd) * g.nz);
   ALLOC(h->twx32, sizeof(cf) * g.nx); ALLOC(h->twz32, sizeof(cf) * g.nz);
-  ALLOC(h->d_fac, sizeof(double) * 3 * MAXF * (2 * PCR_LMAX + 1) * n2);
+  ALLOC(h->d_fac, sizeof(double) * 3 * MAXF * 5 * n2);
   ALLOC(h->d_rowc, sizeof(double) * RC_COUNT * n2);
   if (cfg->use_les) {
     const size_t rbytes = sizeof(double) * g.pp * n2;
