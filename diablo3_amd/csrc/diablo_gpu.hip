@@ -18,6 +18,7 @@
 #include "dgpu_kernels.hpp"
 #include "dgpu_les.hpp"
 #include "dgpu_stats.hpp"
+#include "dgpu_courant.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -98,6 +99,10 @@ struct dgpu_handle {
   double* Rp[MAXF] = {};
   double* CPs = nullptr;
   double* d_dlt = nullptr;  // [2][ny2]: delta_yf, delta_y of the AMD filter width
+  // variable_dt: courant's CFL limit, reduced per block on the device at rk_step 3
+  int variable_dt = 0, update_dt = 1, cfl_ran = 0;
+  double cfl = 0.5;
+  double* d_cfl = nullptr;  // block minima of the last courant evaluation
   // statistics (allocated on the first dgpu_stats call of a DNS handle)
   double *d_zm = nullptr, *d_zs = nullptr, *d_prof = nullptr;
   double* hstage = nullptr;  // pinned staging, one Fortran array
@@ -793,11 +798,52 @@ int ensure_grad_buffers(dgpu_handle* h) {
   return 0;
 }
 
+constexpr int COURANT_BLOCKS = 1024;
+
+// courant's CFL minimum of the physical velocity u1, u2, u3 (FP64 [j][z][x]);
+// the block minima stay on the device until dgpu_courant
+int launch_courant(dgpu_handle* h, const double* u1, const double* u2, const double* u3) {
+  const dgpu_config& c = h->c;
+  if (!h->d_cfl) {
+    void* p = nullptr;
+    int rc = dalloc(h, &p, sizeof(double) * COURANT_BLOCKS);
+    if (rc) return rc;
+    h->d_cfl = (double*)p;
+  }
+  CourantArgs a{};
+  a.g = h->g; a.nth = h->nth; a.front = c.flavor_front;
+  a.jlo = c.flavor_front ? c.jstart : 1;
+  a.jhi = c.flavor_front ? c.jend : c.ny;
+  a.cfl = h->cfl; a.dx = c.lx / c.nx; a.dz = c.lz / c.nz; a.ly = c.ly;
+  a.f = c.ro_inv / c.delta;
+  for (int n = 0; n < MAXTH; ++n) { a.dthdx[n] = c.dthdx[n]; a.dthdz[n] = c.dthdz[n]; }
+  a.dy = h->d_dy; a.gyf = h->d_gyf; a.u1 = u1; a.u2 = u2; a.u3 = u3; a.part = h->d_cfl;
+  k_courant<<<COURANT_BLOCKS, 256, 0, h->st>>>(a);
+  HIPCHK(hipGetLastError());
+  h->cfl_ran = 1;
+  return 0;
+}
+
+// the DNS path: the CN solution (Fourier rows of the state, before the
+// projection) to physical space, then the reduction
+int courant_dns(dgpu_handle* h) {
+  int rc = ensure_grad_buffers(h);
+  if (rc) return rc;
+  const Geo& g = h->g;
+  void* const* H = (h->prec == DGPU_MIXED) ? h->HinD : h->Hin;
+  const cd* cs[3] = {h->U[0], h->U[1], h->U[2]};
+  void* hs[3] = {H[0], H[1], H[2]};
+  double* rs[3] = {(double*)h->Pg[0], (double*)h->Pg[3], (double*)h->Pg[6]};
+  if ((rc = launch_inv_z<double>(h, 3, cs, hs, 0, g.ny2))) return rc;
+  if ((rc = launch_inv_x<double>(h, 3, hs, rs, 0, g.ny2))) return rc;
+  return launch_courant(h, rs[0], rs[1], rs[2]);
+}
+
 // One substep of the LES branch: rk_chan_1 with les_chan / les_chan_th
 // (for/solver.f90:611-623) and the physical-space Crank–Nicolson solves with the
 // eddy viscosity (for/solver.f90:875-903, 990-1233).
 template <int NTH>
-int substep_les(dgpu_handle* h, int rk, double delta_t, double time) {
+int substep_les(dgpu_handle* h, int rk, double delta_t, double time, int time_step) {
   const dgpu_config& c = h->c;
   int rc = ensure_factors(h, delta_t);
   if (rc) return rc;
@@ -909,6 +955,10 @@ int substep_les(dgpu_handle* h, int rk, double delta_t, double time) {
   k_cn_phys<<<(unsigned)((g.pp + 255) / 256), 256, 0, h->st>>>(pa);
   HIPCHK(hipGetLastError());
   if ((rc = tend(h))) return rc;
+  // courant on the new velocity, before the projection (for/solver.f90:1241-1244)
+  if (h->variable_dt && rk == 3 && time_step % h->update_dt == 0)
+    if ((rc = launch_courant(h, (const double*)h->Pg[0], (const double*)h->Pg[3], (const double*)h->Pg[6])))
+      return rc;
 
   // 8. solved fields back to Fourier space (every plane, as the reference), then
   //    rem_div_chan and the pressure update
@@ -930,7 +980,7 @@ int substep(dgpu_handle* h, int rk, double delta_t, double time, int time_step, 
   if (c.use_les && (!create_new_flow || time_step > 100))
     return by_nth(h->nth, [&](auto NTHc) {
       constexpr int NTH = decltype(NTHc)::value;
-      return substep_les<NTH>(h, rk, delta_t, time);
+      return substep_les<NTH>(h, rk, delta_t, time, time_step);
     });
   int rc = ensure_factors(h, delta_t);
   if (rc) return rc;
@@ -992,6 +1042,9 @@ int substep(dgpu_handle* h, int rk, double delta_t, double time, int time_step, 
     return mixed ? launch_cn<float, NTH>(h, ca) : launch_cn<double, NTH>(h, ca);
   });
   if (rc || (rc = tend(h))) return rc;
+  // courant on the new velocity, before the projection (for/solver.f90:1241-1244)
+  if (h->variable_dt && rk == 3 && time_step % h->update_dt == 0)
+    if ((rc = courant_dns(h))) return rc;
   if ((rc = tbeg(h, 4))) return rc;
   if ((rc = launch_project(h, ca, 1))) return rc;
   return tend(h);
@@ -1208,6 +1261,52 @@ int dgpu_download_les(dgpu_handle* h, double* nu_t, double* kappa_t) {
         std::memcpy(out + ((size_t)j * (g.nz + 2) + z) * (g.nx + 2), &tmp[((size_t)j * g.nz + z) * g.nx],
                     sizeof(double) * g.nx);
   }
+  return 0;
+}
+
+int dgpu_set_variable_dt(dgpu_handle* h, int32_t variable_dt, int32_t update_dt, double cfl) {
+  if (!h) return fail(1, "null handle");
+  if (variable_dt && update_dt < 1) return fail(2, "update_dt must be >= 1");
+  h->variable_dt = variable_dt != 0;
+  h->update_dt = update_dt < 1 ? 1 : update_dt;
+  h->cfl = cfl;
+  h->cfl_ran = 0;
+  return 0;
+}
+
+int dgpu_courant(dgpu_handle* h, double time, double time_limit, double save_stats_time, double save_flow_time,
+                 double* delta_t, int32_t* ran) {
+  if (!h || !delta_t || !ran) return fail(1, "null argument");
+  *ran = h->cfl_ran;
+  *delta_t = 0.0;
+  if (!h->cfl_ran) return 0;
+  h->cfl_ran = 0;
+  std::vector<double> part(COURANT_BLOCKS);
+  HIPCHK(hipMemcpyAsync(part.data(), h->d_cfl, sizeof(double) * COURANT_BLOCKS, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  const dgpu_config& c = h->c;
+  // the limits of courant before its velocity loop (for/flow.f90:757-773)
+  const double pi = 3.1415927410125732;  // pi = 4.*atan(1.0) in real(4), for/fft.f90:147
+  const double dx = c.lx / c.nx, dz = c.lz / c.nz;
+  double dt = 1.0;
+  dt = std::fmin(dt, 0.5 * std::pow(std::fmin(dx, dz), 2.0 * c.beta) / c.nu);
+  for (int n = 0; n < c.n_th; ++n) dt = std::fmin(dt, dt * c.nu / (c.nu / c.pr[n]));
+  if (c.ro_inv != 0.0) dt = std::fmin(dt, 2.0 * pi / std::fabs(c.ro_inv / c.delta) / 20.0);
+  for (int n = 0; n < c.n_th; ++n)
+    dt = std::fmin(dt, (double)0.1f * 2.0 * pi / std::sqrt(std::fabs(c.th_bc_ymin_c1[n])));
+  for (double v : part) dt = std::fmin(dt, v);
+  // caps and the scheduling towards the next event (for/flow.f90:810-851)
+  double dtn;
+  if (dt <= 0) dtn = 0.00001;
+  else if (dt >= (double)1000.f) dtn = 1000.0;
+  else dtn = dt;
+  const double next = std::fmin(time_limit, std::fmin(save_stats_time - time,
+                                std::fmin(save_flow_time - time, time_limit - time)));
+  const double r_max = (double)(1.f - 0.025f);
+  if (next < dt) dtn = next + 1.e-14;
+  else if (next < (1.0 + r_max) * dt) dtn = 1.0 / (1.0 + r_max) * next;
+  else dtn = dt;
+  *delta_t = dtn;
   return 0;
 }
 

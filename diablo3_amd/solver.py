@@ -57,6 +57,9 @@ def load():
                                    ctypes.POINTER(ctypes.c_char_p)]
     L.dgpu_download_les.argtypes = [P, D, D]
     L.dgpu_stats.argtypes = [P, ctypes.c_double, D]
+    L.dgpu_set_variable_dt.argtypes = [P, ctypes.c_int32, ctypes.c_int32, ctypes.c_double]
+    L.dgpu_courant.argtypes = [P, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                               D, ctypes.POINTER(ctypes.c_int32)]
     L.dgpu_stats_doubles.argtypes = [P]
     L.dgpu_stats_doubles.restype = ctypes.c_int64
     L.dgpu_field_doubles.argtypes = [P]
@@ -143,6 +146,16 @@ class Solver:
         self._check(load().dgpu_download_les(self.h, _dp(nt), _dp(kt)))
         return (layout.from_fortran_physical(self.cfg, nt),
                 [layout.from_fortran_physical(self.cfg, kt[n]) for n in range(self.cfg.n_th)])
+
+    def set_variable_dt(self, variable_dt: bool, update_dt: int = 1, cfl: float = 0.5):
+        self._check(load().dgpu_set_variable_dt(self.h, int(variable_dt), update_dt, cfl))
+
+    def courant(self, time, time_limit, save_stats_time, save_flow_time):
+        """(delta_t, ran) of the courant evaluation in the last rk_step 3."""
+        dt, ran = ctypes.c_double(), ctypes.c_int32()
+        self._check(load().dgpu_courant(self.h, time, time_limit, save_stats_time, save_flow_time,
+                                        ctypes.byref(dt), ctypes.byref(ran)))
+        return dt.value, bool(ran.value)
 
     def stats(self, time: float = 0.0):
         """save_stats_chan's energy-budget profiles: {name: (Ny+2,) array}, 'u1z_x0'."""

@@ -161,6 +161,17 @@ int dgpu_timing_enable(dgpu_handle *h, int32_t enable);
 int dgpu_timing_read(dgpu_handle *h, int32_t kernel, double *total_ms, int64_t *launches,
                      const char **name);
 
+/* variable_dt (input.dat VARIABLE_DT, UPDATE_DT, CFL; for/parameters.f90:53-58):
+ * while enabled, rk_step 3 of every update_dt-th time step evaluates courant
+ * (for/flow.f90:746-857) on the new velocity before the projection, as
+ * for/solver.f90:1241-1244 does.  dgpu_courant then returns the delta_t courant
+ * sets (ran = 1), with its event scheduling against time_limit, save_stats_time
+ * and save_flow_time, or ran = 0 if courant has not run since the last call.
+ * The caller passes the returned delta_t to the following steps (h_bar follows it). */
+int dgpu_set_variable_dt(dgpu_handle *h, int32_t variable_dt, int32_t update_dt, double cfl);
+int dgpu_courant(dgpu_handle *h, double time, double time_limit, double save_stats_time,
+                 double save_flow_time, double *delta_t, int32_t *ran);
+
 /* Energy-budget statistics of save_stats_chan (for/diagnostics.f90:3-669, with
  * compute_averages, compute_TKE_diss, compute_MKE_diss, compute_TKE and
  * compute_TKE_Production, 676-1519), computed on the device in FP64 from the

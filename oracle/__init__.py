@@ -43,6 +43,13 @@ def lib():
                                         ctypes.c_int, ctypes.c_int]
         L.oracle_rem_div.argtypes = [P]
         L.oracle_bc_post.argtypes = [P, ctypes.c_double]
+        L.oracle_set_variable_dt.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_double]
+        L.oracle_courant.argtypes = [P, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                     ctypes.c_double, D, ctypes.POINTER(ctypes.c_int)]
+        L.oracle_courant_schedule.argtypes = [ctypes.c_double] * 5
+        L.oracle_courant_schedule.restype = ctypes.c_double
+        L.oracle_courant_dt_state.argtypes = [P, ctypes.c_double]
+        L.oracle_courant_dt_state.restype = ctypes.c_double
         L.oracle_fft_xz_to_fourier.argtypes = [P, D, D]
         L.oracle_fft_xz_to_physical.argtypes = [P, D, D]
         L.oracle_wavenumbers.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double,
@@ -136,6 +143,19 @@ class Oracle:
 
     def rem_div(self):
         self._check(lib().oracle_rem_div(self.h))
+
+    def set_variable_dt(self, variable_dt, update_dt=1, cfl=0.5):
+        self._check(lib().oracle_set_variable_dt(self.h, int(variable_dt), update_dt, cfl))
+
+    def courant(self, time, time_limit, save_stats_time, save_flow_time):
+        """(delta_t, ran) of the courant call inside the last rk_step 3."""
+        dt, ran = ctypes.c_double(), ctypes.c_int()
+        self._check(lib().oracle_courant(self.h, time, time_limit, save_stats_time, save_flow_time,
+                                         ctypes.byref(dt), ctypes.byref(ran)))
+        return dt.value, bool(ran.value)
+
+    def courant_dt_state(self, cfl):
+        return lib().oracle_courant_dt_state(self.h, cfl)
 
     def bc_post(self, time=0.0):
         self._check(lib().oracle_bc_post(self.h, time))

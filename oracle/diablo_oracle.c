@@ -121,6 +121,9 @@ struct oracle {
   double *nu_t, *kappa_t;
   /* LES work arrays (allocated when use_les): physical [j][z][x] and a
    * Fourier scratch; the index names follow les.f90 */
+  /* variable_dt (for/parameters.f90:53, for/solver.f90:1241-1244) */
+  int variable_dt, update_dt, cfl_ran;
+  double cfl, cfl_dt;
   double *les_p;   /* NLESP physical arrays, see les_phys() */
   zc *les_c;       /* one Fourier scratch array */
   fftplan px, pz;
@@ -1128,6 +1131,96 @@ int oracle_bc_post(oracle_t *o, double time) {
   return 0;
 }
 
+/* ------------------------------------------------------------------------- */
+/* courant (for/flow.f90:746-857)                                             */
+/* ------------------------------------------------------------------------- */
+/* The time-step limits of courant up to get_minimum_mpi: viscous, diffusive,
+ * inertial and buoyancy periods, and the CFL limit of the physical velocity
+ * (with the thermal wind for flavour 'Front'). */
+static double courant_dt(const oracle_t *o) {
+  const dgpu_config *c = &o->c;
+  const double pi = 3.1415927410125732;      /* pi = 4.*atan(1.0) in real(4), for/fft.f90:147 */
+  const double dx = c->lx / o->nx, dz = c->lz / o->nz;
+  double dt = 1.0;
+  dt = fmin(dt, 0.5 * pow(fmin(dx, dz), 2.0 * c->beta) / c->nu);
+  for (int n = 0; n < o->nth; n++) dt = fmin(dt, dt * c->nu / (c->nu / c->pr[n]));
+  if (c->ro_inv != 0.0) dt = fmin(dt, 2.0 * pi / fabs(c->ro_inv / c->delta) / 20.0);
+  for (int n = 0; n < o->nth; n++) {
+    const double nmax = sqrt(fabs(c->th_bc_ymin_c1[n]));
+    dt = fmin(dt, (double)0.1f * 2.0 * pi / nmax);  /* the literal 0.1 is real(4) */
+  }
+  const double cfl = o->cfl;
+  if (c->flavor_front) {
+    for (int n = 0; n < o->nth; n++)
+      for (int j = c->jstart; j <= c->jend; j++)
+        for (int z = 0; z < o->nz; z++)
+          for (int x = 0; x < o->nx; x++) {
+            const size_t p = PI(j, z, x);
+            const double dtx = cfl * dx / fabs(o->u1[p] - c->dthdz[n] * o->gyf[j] / (c->ro_inv / c->delta));
+            const double dty = cfl * o->dy[j] / fabs(o->u2[p]);
+            const double dtz = cfl * dz / fabs(o->u3[p] + (1.0 / (c->ro_inv / c->delta)) * c->dthdx[n] *
+                                                           (o->gyf[j] - 0.5 * c->ly));
+            dt = fmin(dt, fmin(dtx, fmin(dty, dtz)));
+          }
+  } else {
+    for (int j = 1; j <= o->ny; j++)
+      for (int z = 0; z < o->nz; z++)
+        for (int x = 0; x < o->nx; x++) {
+          const size_t p = PI(j, z, x);
+          const double dtx = cfl * dx / fabs(o->u1[p]);
+          const double dty = cfl * o->dy[j] / fabs(o->u2[p]);
+          const double dtz = cfl * dz / fabs(o->u3[p]);
+          dt = fmin(dt, fmin(dtx, fmin(dty, dtz)));
+        }
+  }
+  return dt;
+}
+
+/* the rest of courant: caps and the scheduling towards the next event */
+double oracle_courant_schedule(double dt, double time, double time_limit, double save_stats_time,
+                               double save_flow_time) {
+  double delta_t;
+  if (dt <= 0) delta_t = 0.00001;
+  else if (dt >= (double)1000.f) delta_t = 1000.0;
+  else delta_t = dt;
+  double next = fmin(time_limit, fmin(save_stats_time - time, fmin(save_flow_time - time, time_limit - time)));
+  const double r_max = (double)(1.f - 0.025f);
+  if (next < dt) delta_t = next + 1.e-14;
+  else if (next < (1.0 + r_max) * dt) delta_t = 1.0 / (1.0 + r_max) * next;
+  else delta_t = dt;
+  return delta_t;
+}
+
+/* courant's limits for the current Fourier state (transformed to physical space) */
+double oracle_courant_dt_state(oracle_t *o, double cfl) {
+  to_physical(o, o->cu1, o->u1);
+  to_physical(o, o->cu2, o->u2);
+  to_physical(o, o->cu3, o->u3);
+  const double keep = o->cfl;
+  o->cfl = cfl;
+  const double dt = courant_dt(o);
+  o->cfl = keep;
+  return dt;
+}
+
+int oracle_set_variable_dt(oracle_t *o, int variable_dt, int update_dt, double cfl) {
+  if (variable_dt && update_dt < 1) { set_err("update_dt must be >= 1"); return 1; }
+  o->variable_dt = variable_dt;
+  o->update_dt = update_dt;
+  o->cfl = cfl;
+  o->cfl_ran = 0;
+  return 0;
+}
+
+int oracle_courant(oracle_t *o, double time, double time_limit, double save_stats_time,
+                   double save_flow_time, double *delta_t, int *ran) {
+  *ran = o->cfl_ran;
+  *delta_t = o->cfl_ran ? oracle_courant_schedule(o->cfl_dt, time, time_limit, save_stats_time, save_flow_time)
+                        : 0.0;
+  o->cfl_ran = 0;
+  return 0;
+}
+
 int oracle_rk_substep(oracle_t *o, int rk_step, double delta_t, double time,
                       int time_step, int create_new_flow) {
   const dgpu_config *c = &o->c;
@@ -1406,6 +1499,11 @@ int oracle_rk_substep(oracle_t *o, int rk_step, double delta_t, double time,
   cn_solve_phys(o, 1, temp1, temp2, r2, o->u2, 1, ny + 1, c->use_les ? nt : NULL, time);
   cn_solve_phys(o, 0, temp1, temp2, r1, o->u1, js - 1, je + 1, c->use_les ? nt : NULL, time);
   cn_solve_phys(o, 2, temp1, temp2, r3, o->u3, js - 1, je + 1, c->use_les ? nt : NULL, time);
+  /* courant on the new velocity, before the projection (1241-1244) */
+  if (o->variable_dt && rk_step == 3 && time_step % o->update_dt == 0) {
+    o->cfl_dt = courant_dt(o);
+    o->cfl_ran = 1;
+  }
   /* back to Fourier space (1247-1252), projection (1257), pressure (1261-1267) */
   to_fourier(o, o->u1, o->cu1);
   to_fourier(o, o->u2, o->cu2);

@@ -39,6 +39,16 @@ int oracle_download_les(oracle_t *o, double *nu_t, double *kappa_t);
 int oracle_rk_substep(oracle_t *o, int rk_step, double delta_t, double time,
                       int time_step, int create_new_flow);
 int oracle_rem_div(oracle_t *o);
+/* variable_dt: courant (for/flow.f90:746-857) runs inside rk_step 3 of every
+ * update_dt-th step on the velocity before the projection (for/solver.f90:1241);
+ * oracle_courant returns its delta_t (ran = 0 if it did not run since the last call). */
+int oracle_set_variable_dt(oracle_t *o, int variable_dt, int update_dt, double cfl);
+int oracle_courant(oracle_t *o, double time, double time_limit, double save_stats_time,
+                   double save_flow_time, double *delta_t, int *ran);
+double oracle_courant_schedule(double dt, double time, double time_limit, double save_stats_time,
+                               double save_flow_time);
+/* courant's limits (before the caps and scheduling) for the current state */
+double oracle_courant_dt_state(oracle_t *o, double cfl);
 /* apply_BC_vel_mpi_post + apply_BC_th_mpi_post (for/flow.f90:1161-1360, 1596-1732) */
 int oracle_bc_post(oracle_t *o, double time);
 

@@ -370,3 +370,46 @@ def test_ten_steps_stay_divergence_free_and_finite():
     assert all(np.isfinite(a).all() for a in (s.cu1, s.cu2, s.cu3, s.cp, s.cth[0]))
     d = np.abs(_div(p, s))
     assert d[1:-1].max() < 1e-13 * max(1e-30, np.abs(s.cu1).max())
+
+
+# ---------------------------------------------------------------- courant
+def test_courant_limits_closed_form():
+    """courant (for/flow.f90:746-806) on a state whose extrema are known: u1 = A sin(2 pi x / Lx)
+    plus the thermal wind the 'Front' flavour adds to u3."""
+    cfg = configs.small_dns(32, 17, 16)
+    p = Problem(cfg)
+    o = oracle.Oracle(p, threads=2)
+    shp = (cfg.ny2, cfg.nz, cfg.nx)
+    x = np.arange(cfg.nx) * cfg.lx / cfg.nx
+    zero = np.zeros(shp)
+    pi32 = 3.1415927410125732
+    dx, dz = cfg.lx / cfg.nx, cfg.lz / cfg.nz
+    f = cfg.ro_inv / cfg.delta
+    gyf = p.met["gyf"]
+    vg = np.abs((1.0 / f) * cfg.dthdx[0] * (gyf[cfg.jstart:cfg.jend + 1] - 0.5 * cfg.ly)).max()
+    limits = [1.0, 0.5 * min(dx, dz) ** (2 * cfg.beta) / cfg.nu, 2 * pi32 / abs(f) / 20,
+              float(np.float32(0.1)) * 2 * pi32 / np.sqrt(abs(cfg.th_bc_ymin_c1[0]))]
+    for amp in (1e-3, 5.0, 40.0):
+        u1 = np.broadcast_to(amp * np.sin(2 * np.pi * x / cfg.lx), shp).copy()
+        s = FourierState(layout.fft_xz_to_fourier(cfg, u1), layout.compact_zeros(cfg),
+                         layout.compact_zeros(cfg), layout.compact_zeros(cfg), [layout.compact_zeros(cfg)])
+        o.upload_state(s)
+        cfl = 0.5
+        got = o.courant_dt_state(cfl)
+        ref = min(limits + [cfl * dx / amp, cfl * dz / vg])
+        assert abs(got - ref) <= 1e-12 * ref, (amp, got, ref)
+
+
+def test_courant_schedule_cases():
+    """The event scheduling of courant (for/flow.f90:810-851)."""
+    sch = oracle.lib().oracle_courant_schedule
+    r_max = float(np.float32(1.0) - np.float32(0.025))
+    # far from any event: dt itself
+    assert sch(1e-3, 0.0, 10.0, 5.0, 5.0) == 1e-3
+    # the next event is closer than dt: go straight there
+    assert sch(1e-3, 4.9995, 10.0, 5.0, 5.0) == (5.0 - 4.9995) + 1e-14
+    # between one and two steps away: two equal steps
+    nxt = 1.5e-3
+    assert sch(1e-3, 5.0 - nxt, 10.0, 5.0, 5.0) == 1.0 / (1.0 + r_max) * (5.0 - (5.0 - nxt))
+    # the caps are overridden by the scheduling branch, as in the reference
+    assert sch(2000.0, 0.0, 1e9, 1e9, 1e9) == 2000.0
