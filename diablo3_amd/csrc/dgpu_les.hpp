@@ -26,6 +26,7 @@ namespace dgpu {
 
 constexpr int MAXA = 7 + 2 * MAXTH;  // SGS addition fields
 constexpr double C_AMD = 0.2887;     // for/les.f90:25
+constexpr double C_SMAG = 0.17;      // for/les.f90:24
 constexpr double BETA_SGS = 3.0;     // for/les.f90:43
 
 // ---------------------------------------------------------------------------
@@ -215,20 +216,100 @@ __device__ __forceinline__ double coef_gy(const AmdIn<T>& in, const AmdArgs& a, 
                   w(f.o5, fm.o5), p.o6);
 }
 
-// tau_12 (e = 0) or tau_23 (e = 1) at GY j: delta_y(j) coef(j) (S - du/dy / 2).
-// At j = 1 the reference scales the strain itself (its coefficient loop starts
-// at j = 2, for/les.f90:762); that row feeds only rows the BC rows overwrite.
+// The strain S and rotation O of the unscaled gradient (compute_strain_chan,
+// compute_rotation_chan, for/les.f90:1443-1596), per GYF and GY plane, for types 1, 4
+struct Sgyf {
+  double s1, s2, s3, s5, o5;
+};
+struct Sgy {
+  double s4, s6, o4, o6;
+};
 template <typename T>
+__device__ __forceinline__ Sgyf sgyf_plane(const AmdIn<T>& in, const AmdArgs& a, int j, long long q) {
+  const double du1dz = in.at(2, j, q), du3dx = in.at(7, j, q);
+  return {in.at(1, j, q), (in.at(3, j + 1, q) - in.at(3, j, q)) / a.dyf[j], in.at(8, j, q),
+          0.5 * (du1dz + du3dx), 0.5 * (du1dz - du3dx)};
+}
+template <typename T>
+__device__ __forceinline__ Sgy sgy_plane(const AmdIn<T>& in, const AmdArgs& a, int j, long long q) {
+  const double du1dy = (in.at(0, j, q) - in.at(0, j - 1, q)) / a.dy[j];
+  const double du3dy = (in.at(6, j, q) - in.at(6, j - 1, q)) / a.dy[j];
+  const double du2dx = in.at(4, j, q), du2dz = in.at(5, j, q);
+  return {0.5 * (du1dy + du2dx), 0.5 * (du2dz + du3dy), 0.5 * (du1dy - du2dx), 0.5 * (du2dz - du3dy)};
+}
+// type 4 (for/les.f90:276-440): the AMD polynomial with the squared filter widths
+// dX, dY, dZ as weights; at GYF (g6 = false) the last cubic term multiplies
+// a4 a5 a6, at GY a4 a6 a5 (the reference's operand orders)
+__device__ __forceinline__ double amd4_coef(double dX, double dY, double dZ, double a1, double a2, double a3,
+                                            double a4, double a5, double a6, double o4, double o5, double o6,
+                                            bool gy) {
+  double v = dX * (a1 * a1 * a1) + dY * (a2 * a2 * a2) + dZ * (a3 * a3 * a3) + (2.0 * dX + dY) * a1 * (a4 * a4) +
+             (2.0 * dX + dZ) * a1 * (a5 * a5) + (2.0 * dY + dX) * a2 * (a4 * a4) + (2.0 * dY + dZ) * a2 * (a6 * a6) +
+             (2.0 * dZ + dX) * a3 * (a5 * a5) + (2.0 * dZ + dY) * a3 * (a6 * a6) +
+             (gy ? 2.0 * (dX + dY + dZ) * a4 * a6 * a5 : 2.0 * (dX + dY + dZ) * a4 * a5 * a6);
+  v = -v - dY * a1 * (o4 * o4) - dZ * a1 * (o5 * o5) - dX * a2 * (o4 * o4) - dZ * a2 * (o6 * o6) -
+      dX * a3 * (o5 * o5) - dY * a3 * (o6 * o6) - 2.0 * dZ * a4 * o5 * o6 + 2.0 * dY * a5 * o4 * o6 -
+      2.0 * dX * a6 * o4 * o5;
+  if (v <= 0.0) return 0.0;
+  return v / (a1 * a1 + a2 * a2 + a3 * a3 + 2.0 * (a4 * a4) + 2.0 * (o4 * o4) + 2.0 * (a5 * a5) +
+              2.0 * (o5 * o5) + 2.0 * (a6 * a6) + 2.0 * (o6 * o6));
+}
+
+// the model coefficient at GYF j (s1 of les_chan) and at GY j (temp)
+template <int MODEL, typename T>
+__device__ __forceinline__ double model_gyf(const AmdIn<T>& in, const AmdArgs& a, int j, long long q) {
+  if constexpr (MODEL == 5) {
+    return coef_gyf(in, a, j, q);
+  } else {
+    const Sgyf f = sgyf_plane(in, a, j, q);
+    const Sgy p = sgy_plane(in, a, j + 1, q), m = sgy_plane(in, a, j, q);
+    const double s4 = 0.5 * (p.s4 + m.s4), s6 = 0.5 * (p.s6 + m.s6);
+    if constexpr (MODEL == 1) {  // |S| (for/les.f90:99-112)
+      return sqrt(2.0 * (f.s1 * f.s1) + 4.0 * (s4 * s4) + 4.0 * (f.s5 * f.s5) + 2.0 * (f.s2 * f.s2) +
+                  4.0 * (s6 * s6) + 2.0 * (f.s3 * f.s3));
+    } else {
+      const double dY = (a.dy[j] * 2.0) * (a.dy[j] * 2.0);
+      return amd4_coef(a.dxs * a.dxs, dY, a.dzs * a.dzs, f.s1, f.s2, f.s3, s4, f.s5, s6, 0.5 * (p.o4 + m.o4),
+                       f.o5, 0.5 * (p.o6 + m.o6), false);
+    }
+  }
+}
+template <int MODEL, typename T>
+__device__ __forceinline__ double model_gy(const AmdIn<T>& in, const AmdArgs& a, int j, long long q) {
+  if constexpr (MODEL == 5) {
+    return coef_gy(in, a, j, q);
+  } else {
+    const Sgyf f = sgyf_plane(in, a, j, q), fm = sgyf_plane(in, a, j - 1, q);
+    const Sgy p = sgy_plane(in, a, j, q);
+    const double wc = a.dyf[j - 1], wm = a.dyf[j], den = 2.0 * a.dy[j];
+    auto w = [&](double c, double m) { return (c * wc + m * wm) / den; };
+    const double s1 = w(f.s1, fm.s1), s2 = w(f.s2, fm.s2), s3 = w(f.s3, fm.s3), s5 = w(f.s5, fm.s5);
+    if constexpr (MODEL == 1) {  // (for/les.f90:114-132)
+      return sqrt(2.0 * (s1 * s1) + 4.0 * (p.s4 * p.s4) + 4.0 * (s5 * s5) + 2.0 * (s2 * s2) + 4.0 * (p.s6 * p.s6) +
+                  2.0 * (s3 * s3));
+    } else {
+      const double dY = (a.dy[j] * 2.0) * (a.dy[j] * 2.0);
+      return amd4_coef(a.dxs * a.dxs, dY, a.dzs * a.dzs, s1, s2, s3, p.s4, s5, p.s6, p.o4, w(f.o5, fm.o5), p.o6,
+                       true);
+    }
+  }
+}
+
+// tau_12 (e = 0) or tau_23 (e = 1) at GY j: delta_y(j) coef(j) (S - du/dy / 2).
+// At j = 1 types 4 and 5 scale the strain itself (their coefficient loops start
+// at j = 2, for/les.f90:454, 762), type 1 multiplies it (for/les.f90:145); that
+// row feeds only rows the BC rows overwrite.
+template <int MODEL, typename T>
 __device__ __forceinline__ double tau_gy(const AmdIn<T>& in, const AmdArgs& a, int e, int j, long long q) {
   const double dlt = a.dlt[a.g.ny2 + j];
-  if (j == 1) {
+  if (MODEL != 1 && j == 1) {
     const double dvdy = (in.at(e ? 6 : 0, 1, q) - in.at(e ? 6 : 0, 0, q)) / a.dy[1];
     return dlt * 0.5 * (dvdy + in.at(e ? 5 : 4, 1, q));
   }
-  return dlt * (coef_gy(in, a, j, q) * (0.5 * in.at(e ? 5 : 4, j, q)));
+  return dlt * (model_gy<MODEL>(in, a, j, q) * (0.5 * in.at(e ? 5 : 4, j, q)));
 }
 
-template <typename T, int NTH>
+template <typename T, int NTH, int MODEL>
 __global__ __launch_bounds__(256) void k_amd(AmdArgs a) {
   const Geo& g = a.g;
   const long long q = (long long)blockIdx.x * 256 + threadIdx.x;
@@ -242,27 +323,40 @@ __global__ __launch_bounds__(256) void k_amd(AmdArgs a) {
   auto put = [&](void* p, double v) { ((T*)p)[o] = (T)v; };
   // nu_t at GY j (zero on the wall rows of ghost_les_mpi)
   double nut = 0.0;
-  if (j >= 3 && j <= ny - 1) nut = -0.5 * a.dlt[g.ny2 + j] * coef_gy(in, a, j, q);
+  if (j >= 3 && j <= ny - 1) nut = -0.5 * a.dlt[g.ny2 + j] * model_gy<MODEL>(in, a, j, q);
   put(a.nut, nut);
   // GYF stresses, rows j1..j2 = jstart..Nyp-1
   double A0 = 0.0, A1 = 0.0, A2 = 0.0, A3 = 0.0, A4 = 0.0;
   if (j >= a.jstart && j <= ny - 1) {
-    const double dyf = a.dlt[j] * coef_gyf(in, a, j, q);
+    const double dyf = a.dlt[j] * model_gyf<MODEL>(in, a, j, q);
     const double du1dz = in.at(2, j, q), du3dx = in.at(7, j, q);
     A0 = dyf * in.at(1, j, q);
     A1 = dyf * (0.5 * (du1dz + du3dx));
     A2 = dyf * in.at(8, j, q);
-    A3 = (tau_gy(in, a, 0, j + 1, q) - tau_gy(in, a, 0, j, q)) / a.dyf[j];
-    A4 = (tau_gy(in, a, 1, j + 1, q) - tau_gy(in, a, 1, j, q)) / a.dyf[j];
+    A3 = (tau_gy<MODEL>(in, a, 0, j + 1, q) - tau_gy<MODEL>(in, a, 0, j, q)) / a.dyf[j];
+    A4 = (tau_gy<MODEL>(in, a, 1, j + 1, q) - tau_gy<MODEL>(in, a, 1, j, q)) / a.dyf[j];
   }
   put(a.aug[0], A0); put(a.aug[1], A1); put(a.aug[2], A2); put(a.aug[3], A3); put(a.aug[4], A4);
   // GY stresses and the explicit cross terms, rows 2..Nyp
   double A5 = 0.0, A6 = 0.0;
   if (j >= 2 && j <= ny) {
-    A5 = tau_gy(in, a, 0, j, q) - nut * (in.at(0, j, q) - in.at(0, j - 1, q)) / a.dy[j];
-    A6 = tau_gy(in, a, 1, j, q) - nut * (in.at(6, j, q) - in.at(6, j - 1, q)) / a.dy[j];
+    A5 = tau_gy<MODEL>(in, a, 0, j, q) - nut * (in.at(0, j, q) - in.at(0, j - 1, q)) / a.dy[j];
+    A6 = tau_gy<MODEL>(in, a, 1, j, q) - nut * (in.at(6, j, q) - in.at(6, j - 1, q)) / a.dy[j];
   }
   put(a.aug[5], A5); put(a.aug[6], A6);
+  if constexpr (MODEL != 5) {
+    // les_chan_th types 1 and 4 (for/les.f90:1118-1193): kappa_t = nu_t (GY), and
+    // cfth += ikx F(kappa_t dth/dx) + ikz F(kappa_t dth/dz) on rows jstart_th..jend_th
+#pragma unroll
+    for (int n = 0; n < NTH; ++n) {
+      const int ft = 3 * (3 + n);
+      put(a.kap[n], nut);
+      const bool row = (j >= a.jstart_th[n] && j <= a.jend_th[n]);
+      put(a.aug[7 + 2 * n], row ? -(nut * in.at(ft + 1, j, q)) : 0.0);
+      put(a.aug[8 + 2 * n], row ? -(nut * in.at(ft + 2, j, q)) : 0.0);
+    }
+    return;
+  }
   // scalars (les_chan_th type 5)
 #pragma unroll
   for (int n = 0; n < NTH; ++n) {

@@ -637,7 +637,8 @@ int validate(const dgpu_config* c) {
   }
   if (c->delta == 0.0) return fail(2, "delta must be non-zero");
   if (c->use_les) {
-    if (c->les_model_type != 5) return fail(2, "only les_model_type 5 (AMD) is built");
+    if (c->les_model_type != 1 && c->les_model_type != 4 && c->les_model_type != 5)
+      return fail(2, "les_model_type must be 1, 4 or 5 (2 and 3 stop in the reference)");
     if (c->precision != DGPU_FP64) return fail(2, "the LES branch runs in FP64 precision in this build");
     // the Fourier terms that rk_chan_1 forms before les_chan's BC posts differ
     // from those after them only on rows that the BC rows of Dirichlet v overwrite
@@ -882,7 +883,10 @@ int substep_les(dgpu_handle* h, int rk, double delta_t, double time, int time_st
   for (int n = 0; n < NTH; ++n) aa.kap[n] = h->KapT[n];
   for (int m = 0; m < 7 + 2 * NTH; ++m) aa.aug[m] = h->Aug[m];
   if ((rc = tbeg(h, 7))) return rc;
-  k_amd<double, NTH><<<dim3((unsigned)((g.pp + 255) / 256), g.ny2), 256, 0, h->st>>>(aa);
+  const dim3 agrid((unsigned)((g.pp + 255) / 256), g.ny2);
+  if (c.les_model_type == 1) k_amd<double, NTH, 1><<<agrid, 256, 0, h->st>>>(aa);
+  else if (c.les_model_type == 4) k_amd<double, NTH, 4><<<agrid, 256, 0, h->st>>>(aa);
+  else k_amd<double, NTH, 5><<<agrid, 256, 0, h->st>>>(aa);
   HIPCHK(hipGetLastError());
   if ((rc = tend(h))) return rc;
 
@@ -1164,13 +1168,23 @@ int dgpu_create(const dgpu_config* cfg, dgpu_handle** out) {
   ok = ok && up(h->d_rowc, rowc.data(), sizeof(double) * rowc.size());
   std::vector<double> dlt(2 * (size_t)n2, 0.0);
   if (cfg->use_les) {
-    // delta_yf(j), delta_y(j) of les_chan type 5 (for/les.f90:756-780):
-    // -2 C_AMD^2 * 3 / (1/(dx beta)^2 + 1/(2 h)^2 + 1/(dz beta)^2)
+    // the filter-width factors delta_yf(j) (GYF), delta_y(j) (GY) of les_chan
     const double ax = cfg->lx / cfg->nx * BETA_SGS, az = cfg->lz / cfg->nz * BETA_SGS;
     for (int j = 1; j <= cfg->ny + 1; ++j)
       for (int w = 0; w < 2; ++w) {
-        const double b = (w == 0 ? h->dyf[j] : h->dy[j]) * 2.0;
-        dlt[(size_t)w * n2 + j] = -(2.0 * (C_AMD * C_AMD) * 3.0 / (1.0 / (ax * ax) + 1.0 / (b * b) + 1.0 / (az * az)));
+        double v;
+        if (cfg->les_model_type == 1) {
+          // -2 c_smag^2 (dx beta h dz beta)^(2/3), h = dyf(j) * 1 or dy(j) * 2 (for/les.f90:161-170)
+          const double hh = (w == 0) ? h->dyf[j] * 1.0 : h->dy[j] * 2.0;
+          v = -(2.0 * (C_SMAG * C_SMAG) * std::pow(ax * hh * az, 2.0 / 3.0));
+        } else if (cfg->les_model_type == 4) {
+          v = -(2.0 * (C_AMD * C_AMD));  // for/les.f90:464-469
+        } else {
+          // -2 C_AMD^2 * 3 / (1/(dx beta)^2 + 1/(2 h)^2 + 1/(dz beta)^2) (for/les.f90:756-780)
+          const double b = (w == 0 ? h->dyf[j] : h->dy[j]) * 2.0;
+          v = -(2.0 * (C_AMD * C_AMD) * 3.0 / (1.0 / (ax * ax) + 1.0 / (b * b) + 1.0 / (az * az)));
+        }
+        dlt[(size_t)w * n2 + j] = v;
       }
     ok = ok && up(h->d_dlt, dlt.data(), sizeof(double) * dlt.size());
   }

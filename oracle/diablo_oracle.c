@@ -211,8 +211,8 @@ int oracle_create(const dgpu_config *cfg, oracle_t **out) {
   o->rth = xcalloc(o->np * nt, 8); o->s1 = xcalloc(o->np, 8);
   o->nu_t = xcalloc(o->np, 8); o->kappa_t = xcalloc(o->np * nt, 8);
   if (cfg->use_les) {
-    if (cfg->les_model_type != 5) {
-      set_err("oracle: only les_model_type 5 (AMD) is restated");
+    if (cfg->les_model_type != 1 && cfg->les_model_type != 4 && cfg->les_model_type != 5) {
+      set_err("oracle: les_model_type must be 1, 4 or 5 (2 and 3 stop in the reference)");
       oracle_destroy(o);
       return 2;
     }
@@ -796,7 +796,7 @@ static void les_grad(oracle_t *o, int jlo, int jhi, zc (*f)(const oracle_t *, in
 /* Every spectral field of compute_strain_chan, compute_strain_chan_invariant,
  * compute_rotation_chan_invariant and compute_all_gradients_chan, by id. */
 enum { G_S1, G_S2, G_S3, G_S4, G_S5, G_S6, G_SI1, G_SI2, G_SI3, G_SI4, G_SI5, G_SI6,
-       G_O4, G_O5, G_O6, G_DU1DX, G_DU2DY, G_DU3DZ, G_DU2DX, G_DU3DX, G_DU1DZ, G_DU2DZ,
+       G_O4, G_O5, G_O6, G_R4, G_R5, G_R6, G_DU1DX, G_DU2DY, G_DU3DZ, G_DU2DX, G_DU3DX, G_DU1DZ, G_DU2DZ,
        G_DU1DY, G_DU3DY, G_DTHDX, G_DTHDY, G_DTHDZ };
 static zc les_spec(const oracle_t *o, int j, int k, int i, int which) {
   const zc ikx = I * o->kx[i], ikz = I * o->kz[k];
@@ -834,6 +834,10 @@ static zc les_spec(const oracle_t *o, int j, int k, int i, int which) {
       const double dyl = o->dy[j] * 2.0;
       return 0.5 * (ikz * u2[p] * (dz / dyl) - (u3[p] - u3[FI(j - 1, k, i)]) / o->dy[j] * (dyl / dz));
     }
+    /* compute_rotation_chan (1572-1589) */
+    case G_R5: return 0.5 * (ikz * u1[p] - ikx * u3[p]);
+    case G_R4: return 0.5 * ((u1[p] - u1[FI(j - 1, k, i)]) / o->dy[j] - ikx * u2[p]);
+    case G_R6: return 0.5 * (ikz * u2[p] - (u3[p] - u3[FI(j - 1, k, i)]) / o->dy[j]);
     /* compute_all_gradients_chan (1663-1713) */
     case G_DU1DX: return ikx * u1[p];
     case G_DU1DZ: return ikz * u1[p];
@@ -857,11 +861,238 @@ static zc les_spec_th(const oracle_t *o, int j, int k, int i, int which) {
   return (t[p] - t[FI(j - 1, k, i)]) / o->dy[j];
 }
 
-/* les_chan, les_model_type 5 (for/les.f90:3-75, 551-901): AMD eddy viscosity
- * nu_t at GY points and the SGS stress divergence added to cf1..cf3; leaves
- * u1..u3 in physical space.  Then les_chan_th type 5 (for/les.f90:1072-1117,
- * 1201-1395): kappa_t and the SGS flux divergence added to cfth; th in
- * physical space. */
+/* The part of les_chan common to every model (for/les.f90:800-901): tau_ij
+ * (the coefficient times S_ij, in S) to Fourier space, the filter-width factors,
+ * the stress divergence into cf1..cf3 on the one-rank rows, and the cross terms
+ * -nu_t du/dy treated explicitly; temp is scratch. */
+static void les_sgs_cf(oracle_t *o, double **S, double *const *dyfac, const double *u1, const double *u3,
+                       double *temp) {
+  const dgpu_config *c = &o->c;
+  const int ny = o->ny, nx = o->nx, nz = o->nz, nxp = o->nxp, kzr = o->kzr;
+  const int js = c->jstart;
+  const double *dy = o->dy, *dyf = o->dyf;
+  /* tau_ij in Fourier space and its divergence into cf (for/les.f90:800-901) */
+  zc *C1 = o->les_c + o->nf, *C3 = o->les_c + 2 * o->nf, *C4 = o->les_c + 3 * o->nf;
+  zc *C5 = o->les_c + 4 * o->nf, *C6 = o->les_c + 5 * o->nf;
+  to_fourier(o, S[0], C1);
+  to_fourier(o, S[2], C3);
+  to_fourier(o, S[3], C4);
+  to_fourier(o, S[4], C5);
+  to_fourier(o, S[5], C6);
+  LOOP_FX(1, ny) {
+    const size_t p = FI(j, k, i);
+    C1[p] = dyfac[0][j] * C1[p];
+    C5[p] = dyfac[0][j] * C5[p];
+    C3[p] = dyfac[0][j] * C3[p];
+  }
+  LOOP_FX(1, ny + 1) {
+    const size_t p = FI(j, k, i);
+    C4[p] = dyfac[1][j] * C4[p];
+    C6[p] = dyfac[1][j] * C6[p];
+  }
+  /* one rank: rankY == NprocY-1 sets j1 = jstart, j2 = Nyp-1 (for/les.f90:45-57) */
+  LOOP_FX(js, ny - 1) {
+    const size_t p = FI(j, k, i), pu = FI(j + 1, k, i);
+    const zc ikx = I * o->kx[i], ikz = I * o->kz[k];
+    o->cf1[p] = o->cf1[p] - ikx * C1[p] - (C4[pu] - C4[p]) / dyf[j] - ikz * C5[p];
+    o->cf3[p] = o->cf3[p] - ikx * C5[p] - (C6[pu] - C6[p]) / dyf[j] - ikz * C3[p];
+  }
+  LOOP_FX(2, ny) {
+    const size_t p = FI(j, k, i);
+    o->cf2[p] = o->cf2[p] - I * o->kx[i] * C4[p] - I * o->kz[k] * C6[p];
+  }
+  /* the cross terms treated explicitly (for/les.f90:874-901) */
+  for (int m = 0; m < 2; m++) {
+    const double *uu = m == 0 ? u1 : u3;
+    memset(temp, 0, 8 * o->np);
+    LOOP_PX(2, ny) {
+      const size_t p = PI(j, z, x);
+      temp[p] = o->nu_t[p] * (uu[p] - uu[PI(j - 1, z, x)]) / dy[j];
+    }
+    to_fourier(o, temp, C6);
+    LOOP_FX(2, ny) {
+      const size_t p = FI(j, k, i);
+      o->cf2[p] = o->cf2[p] + (m == 0 ? I * o->kx[i] : I * o->kz[k]) * C6[p];
+    }
+  }
+
+}
+
+#define C_SMAG 0.17
+/* les_chan types 1 and 4 after compute_strain_chan (and compute_rotation_chan for
+ * type 4), then les_chan_th types 1/4 (for/les.f90:1118-1193) */
+static void les_chan_14(oracle_t *o, double **S, double **O) {
+  const dgpu_config *c = &o->c;
+  const int ny = o->ny, nx = o->nx, nz = o->nz, nxp = o->nxp, kzr = o->kzr;
+  const double *dy = o->dy, *dyf = o->dyf;
+  const int model = c->les_model_type;
+  to_physical(o, o->cu1, o->u1);
+  to_physical(o, o->cu2, o->u2);
+  to_physical(o, o->cu3, o->u3);
+  const double *u1 = o->u1, *u3 = o->u3;
+  double *s1 = o->s1, *temp = les_phys(o, L_TEMP);
+  memset(s1, 0, 8 * o->np);
+  memset(temp, 0, 8 * o->np);
+  const double bx = c->lx / nx * BETA_SGS, bz = c->lz / nz * BETA_SGS;
+  double *dyfac[2];
+  dyfac[0] = (double *)xcalloc((size_t)o->ny2, 8);
+  dyfac[1] = (double *)xcalloc((size_t)o->ny2, 8);
+  if (model == 1) {
+    /* |S| at GYF (for/les.f90:99-112) and at GY (114-132) */
+#pragma omp parallel for schedule(static)
+    for (int j = 1; j <= ny; j++)
+      for (int z = 0; z < nz; z++)
+        for (int x = 0; x < nx; x++) {
+          const size_t p = PI(j, z, x), q = PI(j + 1, z, x);
+          const double s4 = 0.5 * (S[3][q] + S[3][p]), s6 = 0.5 * (S[5][q] + S[5][p]);
+          s1[p] = sqrt(2.0 * (S[0][p] * S[0][p]) + 4.0 * (s4 * s4) + 4.0 * (S[4][p] * S[4][p]) +
+                       2.0 * (S[1][p] * S[1][p]) + 4.0 * (s6 * s6) + 2.0 * (S[2][p] * S[2][p]));
+        }
+#pragma omp parallel for schedule(static)
+    for (int j = 1; j <= ny + 1; j++) {
+      const double wc = dyf[j - 1], wm = dyf[j], den = 2.0 * dy[j];
+      for (int z = 0; z < nz; z++)
+        for (int x = 0; x < nx; x++) {
+          const size_t p = PI(j, z, x), m = PI(j - 1, z, x);
+          const double a1 = (S[0][p] * wc + S[0][m] * wm) / den, a5 = (S[4][p] * wc + S[4][m] * wm) / den;
+          const double a2 = (S[1][p] * wc + S[1][m] * wm) / den, a3 = (S[2][p] * wc + S[2][m] * wm) / den;
+          temp[p] = sqrt(2.0 * (a1 * a1) + 4.0 * (S[3][p] * S[3][p]) + 4.0 * (a5 * a5) + 2.0 * (a2 * a2) +
+                         4.0 * (S[5][p] * S[5][p]) + 2.0 * (a3 * a3));
+        }
+    }
+    /* (for/les.f90:134-158) */
+    for (int j = 1; j <= ny; j++)
+      for (size_t q = PI(j, 0, 0); q < PI(j + 1, 0, 0); q++) {
+        S[0][q] = s1[q] * S[0][q];
+        S[4][q] = s1[q] * S[4][q];
+        S[1][q] = 0.0;
+        S[2][q] = s1[q] * S[2][q];
+      }
+    for (int j = 1; j <= ny + 1; j++)
+      for (int z = 0; z < nz; z++)
+        for (int x = 0; x < nx; x++) {
+          const size_t p = PI(j, z, x), m = PI(j - 1, z, x);
+          S[3][p] = temp[p] * (S[3][p] - 0.5 * (u1[p] - u1[m]) / dy[j]);
+          S[5][p] = temp[p] * (S[5][p] - 0.5 * (u3[p] - u3[m]) / dy[j]);
+        }
+    /* (for/les.f90:161-170); the GYF width uses dyf(j) * 1, the GY width dy(j) * 2 */
+    for (int j = 1; j <= ny; j++)
+      dyfac[0][j] = -(2.0 * (C_SMAG * C_SMAG) * pow(bx * dyf[j] * 1.0 * bz, 2.0 / 3.0));
+    for (int j = 1; j <= ny + 1; j++)
+      dyfac[1][j] = -(2.0 * (C_SMAG * C_SMAG) * pow(bx * dy[j] * 2.0 * bz, 2.0 / 3.0));
+  } else {
+    /* type 4 at GYF (for/les.f90:276-343): weights (dx beta)^2, (2 dy(j))^2, (dz beta)^2 */
+    const double dX = bx * bx, dZ = bz * bz;
+#pragma omp parallel for schedule(static)
+    for (int j = 1; j <= ny; j++) {
+      const double dY = (dy[j] * 2.0) * (dy[j] * 2.0);
+      for (int z = 0; z < nz; z++)
+        for (int x = 0; x < nx; x++) {
+          const size_t p = PI(j, z, x), q = PI(j + 1, z, x);
+          const double a1 = S[0][p], a2 = S[1][p], a3 = S[2][p], a5 = S[4][p], o5 = O[1][p];
+          const double a4 = 0.5 * (S[3][q] + S[3][p]), a6 = 0.5 * (S[5][q] + S[5][p]);
+          const double o4 = 0.5 * (O[0][q] + O[0][p]), o6 = 0.5 * (O[2][q] + O[2][p]);
+          double v = dX * (a1 * a1 * a1) + dY * (a2 * a2 * a2) + dZ * (a3 * a3 * a3) +
+                     (2.0 * dX + dY) * a1 * (a4 * a4) + (2.0 * dX + dZ) * a1 * (a5 * a5) +
+                     (2.0 * dY + dX) * a2 * (a4 * a4) + (2.0 * dY + dZ) * a2 * (a6 * a6) +
+                     (2.0 * dZ + dX) * a3 * (a5 * a5) + (2.0 * dZ + dY) * a3 * (a6 * a6) +
+                     2.0 * (dX + dY + dZ) * a4 * a5 * a6;
+          v = -v - dY * a1 * (o4 * o4) - dZ * a1 * (o5 * o5) - dX * a2 * (o4 * o4) - dZ * a2 * (o6 * o6) -
+              dX * a3 * (o5 * o5) - dY * a3 * (o6 * o6) - 2.0 * dZ * a4 * o5 * o6 + 2.0 * dY * a5 * o4 * o6 -
+              2.0 * dX * a6 * o4 * o5;
+          if (v <= 0.0) v = 0.0;
+          else
+            v = v / (a1 * a1 + a2 * a2 + a3 * a3 + 2.0 * (a4 * a4) + 2.0 * (o4 * o4) + 2.0 * (a5 * a5) +
+                     2.0 * (o5 * o5) + 2.0 * (a6 * a6) + 2.0 * (o6 * o6));
+          s1[p] = v;
+        }
+    }
+    /* at GY (for/les.f90:345-440) */
+#pragma omp parallel for schedule(static)
+    for (int j = 2; j <= ny + 1; j++) {
+      const double dY = (dy[j] * 2.0) * (dy[j] * 2.0);
+      const double wc = dyf[j - 1], wm = dyf[j], den = 2.0 * dy[j];
+      for (int z = 0; z < nz; z++)
+        for (int x = 0; x < nx; x++) {
+          const size_t p = PI(j, z, x), m = PI(j - 1, z, x);
+          const double a1 = (S[0][p] * wc + S[0][m] * wm) / den, a2 = (S[1][p] * wc + S[1][m] * wm) / den;
+          const double a3 = (S[2][p] * wc + S[2][m] * wm) / den, a5 = (S[4][p] * wc + S[4][m] * wm) / den;
+          const double o5 = (O[1][p] * wc + O[1][m] * wm) / den;
+          const double a4 = S[3][p], a6 = S[5][p], o4 = O[0][p], o6 = O[2][p];
+          double v = dX * (a1 * a1 * a1) + dY * (a2 * a2 * a2) + dZ * (a3 * a3 * a3) +
+                     (2.0 * dX + dY) * a1 * (a4 * a4) + (2.0 * dX + dZ) * a1 * (a5 * a5) +
+                     (2.0 * dY + dX) * a2 * (a4 * a4) + (2.0 * dY + dZ) * a2 * (a6 * a6) +
+                     (2.0 * dZ + dX) * a3 * (a5 * a5) + (2.0 * dZ + dY) * a3 * (a6 * a6) +
+                     2.0 * (dX + dY + dZ) * a4 * a6 * a5;
+          v = -v - dY * a1 * (o4 * o4) - dZ * a1 * (o5 * o5) - dX * a2 * (o4 * o4) - dZ * a2 * (o6 * o6) -
+              dX * a3 * (o5 * o5) - dY * a3 * (o6 * o6) - 2.0 * dZ * a4 * o5 * o6 + 2.0 * dY * a5 * o4 * o6 -
+              2.0 * dX * a6 * o4 * o5;
+          if (v <= 0.0) v = 0.0;
+          else
+            v = v / (a1 * a1 + a2 * a2 + a3 * a3 + 2.0 * (a4 * a4) + 2.0 * (o4 * o4) + 2.0 * (a5 * a5) +
+                     2.0 * (o5 * o5) + 2.0 * (a6 * a6) + 2.0 * (o6 * o6));
+          temp[p] = v;
+        }
+    }
+    /* (for/les.f90:444-463); s1(Nyp+1) is never set by the reference (0 here) */
+    for (int j = 1; j <= ny + 1; j++)
+      for (size_t q = PI(j, 0, 0); q < PI(j + 1, 0, 0); q++) {
+        S[0][q] = s1[q] * S[0][q];
+        S[4][q] = s1[q] * S[4][q];
+        S[1][q] = 0.0;
+        S[2][q] = s1[q] * S[2][q];
+      }
+    for (int j = 2; j <= ny + 1; j++)
+      for (int z = 0; z < nz; z++)
+        for (int x = 0; x < nx; x++) {
+          const size_t p = PI(j, z, x), m = PI(j - 1, z, x);
+          S[3][p] = temp[p] * (S[3][p] - 0.5 * (u1[p] - u1[m]) / dy[j]);
+          S[5][p] = temp[p] * (S[5][p] - 0.5 * (u3[p] - u3[m]) / dy[j]);
+        }
+    /* (for/les.f90:464-469): a constant factor */
+    for (int j = 1; j <= ny + 1; j++) dyfac[0][j] = dyfac[1][j] = -(2.0 * (C_AMD * C_AMD));
+  }
+  /* nu_t = -delta_y/2 * coefficient at GY on rows 2..Nyp, zero at 1, 2, Nyp, Nyp+1 */
+  for (int j = 2; j <= ny; j++)
+    for (size_t q = PI(j, 0, 0); q < PI(j + 1, 0, 0); q++) o->nu_t[q] = -0.5 * dyfac[1][j] * temp[q];
+  for (size_t q = PI(1, 0, 0); q < PI(3, 0, 0); q++) o->nu_t[q] = 0.0;
+  for (size_t q = PI(ny, 0, 0); q < PI(ny + 2, 0, 0); q++) o->nu_t[q] = 0.0;
+  les_sgs_cf(o, S, dyfac, u1, u3, temp);
+  free(dyfac[0]);
+  free(dyfac[1]);
+  /* les_chan_th types 1 and 4: kappa_t = nu_t, flux kappa_t grad(th) (for/les.f90:1118-1193) */
+  zc *cs = o->les_c;
+  double *f = o->s1;
+  for (int n = 0; n < o->nth; n++) {
+    double *kt = o->kappa_t + n * o->np;
+    zc *cth = o->cth + n * o->nf, *cfth = o->cfth + n * o->nf;
+    for (int j = 1; j <= ny + 1; j++)
+      for (size_t q = PI(j, 0, 0); q < PI(j + 1, 0, 0); q++) kt[q] = 1.0 * o->nu_t[q];
+    for (int d = 0; d < 2; d++) {
+      memset(cs, 0, sizeof(zc) * o->nf);
+      for (int j = 1; j <= ny + 1; j++)
+        for (int k = 0; k < kzr; k++)
+          for (int i = 0; i < nxp; i++)
+            cs[FI(j, k, i)] = (d == 0 ? I * o->kx[i] : I * o->kz[k]) * cth[FI(j, k, i)];
+      to_physical(o, cs, f);
+      for (int j = 1; j <= ny + 1; j++)
+        for (size_t q = PI(j, 0, 0); q < PI(j + 1, 0, 0); q++) f[q] = kt[q] * f[q];
+      to_fourier(o, f, cs);
+      for (int j = c->jstart_th[n]; j <= c->jend_th[n]; j++)
+        for (int k = 0; k < kzr; k++)
+          for (int i = 0; i < nxp; i++)
+            cfth[FI(j, k, i)] = cfth[FI(j, k, i)] + (d == 0 ? I * o->kx[i] : I * o->kz[k]) * cs[FI(j, k, i)];
+    }
+  }
+  for (int n = 0; n < o->nth; n++) to_physical(o, o->cth + n * o->nf, o->th + n * o->np);
+}
+
+/* les_chan (for/les.f90:3-75, 77-901): eddy viscosity nu_t at GY points and
+ * the SGS stress divergence added to cf1..cf3, u1..u3 left in physical space, for
+ * les_model_type 1 (constant Smagorinsky, 77-254), 4 (AMD with the filter widths
+ * as weights, 260-550) and 5 (AMD of the scaled gradient, 551-797).  Then
+ * les_chan_th (1072-1395): type 5 its own AMD diffusivity, types 1 and 4
+ * kappa_t = nu_t; the SGS flux divergence added to cfth; th in physical space. */
 static void les_amd(oracle_t *o, double time) {
   const dgpu_config *c = &o->c;
   const int ny = o->ny, nx = o->nx, nz = o->nz, nxp = o->nxp, kzr = o->kzr;
@@ -870,9 +1101,9 @@ static void les_amd(oracle_t *o, double time) {
   /* for/les.f90:63-75 */
   bc_post(o, time);
   bc_les(o);
+  const int model = c->les_model_type;
   /* compute_strain_chan: Sij1..Sij6.  CSij2(Nyp+1) holds the -10000 sentinel of
-   * ghost_CS2_mpi (for/les.f90:2509-2513); Sij2 is not read by type 5, so it is
-   * not transformed. */
+   * ghost_CS2_mpi (for/les.f90:2509-2513); Sij2 is read by types 1 and 4 only. */
   double *S[6], *SI[6], *O[3];
   for (int m = 0; m < 6; m++) { S[m] = les_phys(o, L_S1 + m); SI[m] = les_phys(o, L_SI1 + m); }
   for (int m = 0; m < 3; m++) O[m] = les_phys(o, L_O4 + m);
@@ -881,6 +1112,25 @@ static void les_amd(oracle_t *o, double time) {
   les_grad(o, 0, ny + 1, les_spec, G_S5, S[4]);
   les_grad(o, 1, ny + 1, les_spec, G_S4, S[3]);
   les_grad(o, 1, ny + 1, les_spec, G_S6, S[5]);
+  if (model != 5) {
+    zc *cs = o->les_c;
+    memset(cs, 0, sizeof(zc) * o->nf);
+    for (int j = 0; j <= ny; j++)
+      for (int k = 0; k < kzr; k++)
+        for (int i = 0; i < nxp; i++) cs[FI(j, k, i)] = les_spec(o, j, k, i, G_S2);
+    for (int k = 0; k < kzr; k++)
+      for (int i = 0; i < nxp; i++) cs[FI(ny + 1, k, i)] = -10000.0;
+    to_physical(o, cs, S[1]);
+  }
+  if (model == 4) {
+    les_grad(o, 1, ny + 1, les_spec, G_R4, O[0]);
+    les_grad(o, 0, ny + 1, les_spec, G_R5, O[1]);
+    les_grad(o, 1, ny + 1, les_spec, G_R6, O[2]);
+  }
+  if (model == 1 || model == 4) {
+    les_chan_14(o, S, O);
+    return;
+  }
   /* compute_strain_chan_invariant, with the sentinel plane of CSIij2 */
   les_grad(o, 0, ny + 1, les_spec, G_SI1, SI[0]);
   {
@@ -995,52 +1245,10 @@ static void les_amd(oracle_t *o, double time) {
   LOOP_PX(2, ny + 1) o->nu_t[PI(j, z, x)] = -0.5 * dyfac[1][j] * temp[PI(j, z, x)];
   LOOP_PX(1, 2) o->nu_t[PI(j, z, x)] = 0.0;
   LOOP_PX(ny, ny + 1) o->nu_t[PI(j, z, x)] = 0.0;
-  /* tau_ij in Fourier space and its divergence into cf (for/les.f90:800-901) */
-  zc *C1 = o->les_c + o->nf, *C3 = o->les_c + 2 * o->nf, *C4 = o->les_c + 3 * o->nf;
-  zc *C5 = o->les_c + 4 * o->nf, *C6 = o->les_c + 5 * o->nf;
-  to_fourier(o, S[0], C1);
-  to_fourier(o, S[2], C3);
-  to_fourier(o, S[3], C4);
-  to_fourier(o, S[4], C5);
-  to_fourier(o, S[5], C6);
-  LOOP_FX(1, ny) {
-    const size_t p = FI(j, k, i);
-    C1[p] = dyfac[0][j] * C1[p];
-    C5[p] = dyfac[0][j] * C5[p];
-    C3[p] = dyfac[0][j] * C3[p];
-  }
-  LOOP_FX(1, ny + 1) {
-    const size_t p = FI(j, k, i);
-    C4[p] = dyfac[1][j] * C4[p];
-    C6[p] = dyfac[1][j] * C6[p];
-  }
-  /* one rank: rankY == NprocY-1 sets j1 = jstart, j2 = Nyp-1 (for/les.f90:45-57) */
-  LOOP_FX(js, ny - 1) {
-    const size_t p = FI(j, k, i), pu = FI(j + 1, k, i);
-    const zc ikx = I * o->kx[i], ikz = I * o->kz[k];
-    o->cf1[p] = o->cf1[p] - ikx * C1[p] - (C4[pu] - C4[p]) / dyf[j] - ikz * C5[p];
-    o->cf3[p] = o->cf3[p] - ikx * C5[p] - (C6[pu] - C6[p]) / dyf[j] - ikz * C3[p];
-  }
-  LOOP_FX(2, ny) {
-    const size_t p = FI(j, k, i);
-    o->cf2[p] = o->cf2[p] - I * o->kx[i] * C4[p] - I * o->kz[k] * C6[p];
-  }
-  /* the cross terms treated explicitly (for/les.f90:874-901) */
-  for (int m = 0; m < 2; m++) {
-    const double *uu = m == 0 ? u1 : u3;
-    memset(temp, 0, 8 * o->np);
-    LOOP_PX(2, ny) {
-      const size_t p = PI(j, z, x);
-      temp[p] = o->nu_t[p] * (uu[p] - uu[PI(j - 1, z, x)]) / dy[j];
-    }
-    to_fourier(o, temp, C6);
-    LOOP_FX(2, ny) {
-      const size_t p = FI(j, k, i);
-      o->cf2[p] = o->cf2[p] + (m == 0 ? I * o->kx[i] : I * o->kz[k]) * C6[p];
-    }
-  }
+  les_sgs_cf(o, S, dyfac, u1, u3, temp);
 
   /* ---- les_chan_th, type 5 ---- */
+  zc *C1 = o->les_c + o->nf, *C5 = o->les_c + 4 * o->nf;
   const double dxs = c->lx / nx * BETA_SGS, dzs = c->lz / nz * BETA_SGS;
   double *s1th = les_phys(o, L_S1TH), *tth = les_phys(o, L_TEMPTH);
   for (int n = 0; n < o->nth; n++) to_physical(o, o->cth + n * o->nf, o->th + n * o->np);

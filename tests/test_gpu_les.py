@@ -1,5 +1,5 @@
-"""Parity of the HIP LES branch (AMD closure, les_model_type 5) with the CPU
-oracle, through the C ABI.
+"""Parity of the HIP LES branch (les_model_type 1 constant Smagorinsky, 4 and 5
+the two AMD forms) with the CPU oracle, through the C ABI.
 
 Bars: after 3 RKW3 steps past the spin-up gate, u1, u2, u3 and theta match the
 oracle to relative L2 <= 1e-11; nu_t and kappa_t of the last substep to 1e-11;
@@ -26,10 +26,10 @@ def fields(s):
     return {"cu1": s.cu1, "cu2": s.cu2, "cu3": s.cu3, "cp": s.cp, **{f"cth{n}": t for n, t in enumerate(s.cth)}}
 
 
-def les_problem(nx, ny, nz, amp=0.05, seed=11):
+def les_problem(nx, ny, nz, amp=0.05, seed=11, model=5):
     cfg = configs.small_dns(nx, ny, nz)
     cfg.use_les = True
-    cfg.les_model_type = 5
+    cfg.les_model_type = model
     p = Problem(cfg)
     s = p.initial_state(seed=seed, theta_noise=1e-3)
     rng = np.random.default_rng(seed)
@@ -63,11 +63,11 @@ def run_oracle(p, s0, nsteps, variant=0, perturb=False):
         oracle.lib().oracle_set_variant(0)
 
 
-@pytest.mark.parametrize("nx,ny,nz", [(32, 33, 32), (64, 33, 16)])
-def test_les_steps_match_oracle(nx, ny, nz):
-    p, s0 = les_problem(nx, ny, nz)
+@pytest.mark.parametrize("nx,ny,nz,model", [(32, 33, 32, 5), (64, 33, 16, 5), (32, 33, 32, 1), (32, 33, 32, 4)])
+def test_les_steps_match_oracle(nx, ny, nz, model):
+    p, s0 = les_problem(nx, ny, nz, model=model)
     nsteps = 3
-    print(f"LES parity {nx}x{ny}x{nz}, {nsteps} steps", flush=True)
+    print(f"LES parity type {model} {nx}x{ny}x{nz}, {nsteps} steps", flush=True)
     g = Solver(p)
     g.upload_state(s0)
     t = 0.0

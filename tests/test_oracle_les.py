@@ -1,5 +1,6 @@
-"""Pins for the oracle's LES branch: the AMD closure of les_chan / les_chan_th,
-les_model_type 5 (for/les.f90:551-901, 1201-1395).
+"""Pins for the oracle's LES branch: les_chan / les_chan_th for les_model_type 5
+(AMD of the scaled gradient, for/les.f90:551-901, 1201-1395), 1 (constant
+Smagorinsky, 77-254) and 4 (AMD weighted by the filter widths, 260-550).
 
 The reference ships no LES test data, so each check compares the oracle against
 something of a different form:
@@ -201,3 +202,112 @@ def test_spin_up_gate():
         eq = all(np.array_equal(x, y) for x, y in ((a.cu1, b.cu1), (a.cu2, b.cu2), (a.cu3, b.cu3),
                                                    (a.cth[0], b.cth[0])))
         assert eq == same, ts
+
+
+# ---------------------------------------------------------------- types 1 and 4
+C_SMAG = 0.17
+
+
+def les_cfg_model(model, nx=16, ny=17, nz=16):
+    cfg = les_cfg(nx, ny, nz)
+    cfg.les_model_type = model
+    return cfg
+
+
+def test_type1_smagorinsky_closed_forms():
+    """Constant Smagorinsky (for/les.f90:77-254): nu_t = c^2 (Dx Dy Dz)^(2/3) |S| at GY,
+    |S| = |U'| for a pure shear U(y), sqrt(2 (g1^2 + g3^2)) for a diagonal gradient;
+    kappa_t = nu_t (for/les.f90:1118-1126)."""
+    cfg = les_cfg_model(1, 32, 17, 16)
+    p = Problem(cfg)
+    dy, gyf = p.met["dy"], p.met["gyf"]
+    bx, bz = cfg.lx / cfg.nx * BETA_SGS, cfg.lz / cfg.nz * BETA_SGS
+    fac = lambda j: C_SMAG ** 2 * (bx * dy[j] * 2.0 * bz) ** (2.0 / 3.0)
+    shp = (cfg.ny2, cfg.nz, cfg.nx)
+    y = gyf[:, None, None]
+    u1 = np.broadcast_to(0.3 * y ** 2 + 0.1 * y, shp).copy()
+    zero = np.zeros(shp)
+    o = run_substep(p, state_from_physical(p, u1, zero, zero, zero))
+    nt, kt = o.download_les()
+    for j in range(3, cfg.ny - 1):
+        dudy = (u1[j, 0, 0] - u1[j - 1, 0, 0]) / dy[j]
+        np.testing.assert_allclose(nt[j], fac(j) * abs(dudy), rtol=1e-12)
+        np.testing.assert_array_equal(kt[0][j], nt[j])
+    x, z = grids(cfg)
+    a, b = 0.8, -0.5
+    u1 = np.broadcast_to(a * np.sin(2 * math.pi * x / cfg.lx)[None, None, :], shp).copy()
+    u3 = np.broadcast_to(b * np.sin(2 * math.pi * z / cfg.lz)[None, :, None], shp).copy()
+    o = run_substep(p, state_from_physical(p, u1, zero, u3, zero))
+    nt, _ = o.download_les()
+    g1 = (a * p.kx[1] * np.cos(2 * math.pi * x / cfg.lx))[None, :]
+    g3 = (b * p.kz[1] * np.cos(2 * math.pi * z / cfg.lz))[:, None]
+    for j in range(3, cfg.ny - 1):
+        np.testing.assert_allclose(nt[j], fac(j) * np.sqrt(2 * (g1 ** 2 + g3 ** 2)), rtol=1e-12, atol=1e-16)
+
+
+def test_type4_shear_and_diagonal_closed_forms():
+    cfg = les_cfg_model(4, 32, 17, 16)
+    p = Problem(cfg)
+    shp = (cfg.ny2, cfg.nz, cfg.nx)
+    y = p.met["gyf"][:, None, None]
+    zero = np.zeros(shp)
+    u1 = np.broadcast_to(np.sin(1.3 * y), shp).copy()
+    nt, kt = run_substep(p, state_from_physical(p, u1, zero, zero, zero)).download_les()
+    assert np.all(nt == 0.0) and np.all(kt[0] == 0.0)
+    x, z = grids(cfg)
+    a, b = 0.8, -0.5
+    u1 = np.broadcast_to(a * np.sin(2 * math.pi * x / cfg.lx)[None, None, :], shp).copy()
+    u3 = np.broadcast_to(b * np.sin(2 * math.pi * z / cfg.lz)[None, :, None], shp).copy()
+    nt, kt = run_substep(p, state_from_physical(p, u1, zero, u3, zero)).download_les()
+    dX = (cfg.lx / cfg.nx * BETA_SGS) ** 2
+    dZ = (cfg.lz / cfg.nz * BETA_SGS) ** 2
+    g1 = (a * p.kx[1] * np.cos(2 * math.pi * x / cfg.lx))[None, :]
+    g3 = (b * p.kz[1] * np.cos(2 * math.pi * z / cfg.lz))[:, None]
+    num = -(dX * g1 ** 3 + dZ * g3 ** 3)
+    ref = C_AMD ** 2 * np.where(num > 0, num / (g1 ** 2 + g3 ** 2), 0.0)
+    for j in range(3, cfg.ny - 1):
+        np.testing.assert_allclose(nt[j], ref, rtol=1e-12, atol=1e-16)
+        np.testing.assert_array_equal(kt[0][j], nt[j])
+    assert nt.max() > 0
+
+
+def test_type4_random_field_matches_tensor_form():
+    """Type 4 is an anisotropic AMD of the unscaled gradient G = grad u:
+    -(1/2) sum_k D_k^2 (G_ik G_jk + G_ki G_kj) S_ij / |G|^2, D = (dx beta, 2 dy(j), dz beta)."""
+    cfg = les_cfg_model(4, 16, 17, 16)
+    p = Problem(cfg)
+    s = p.initial_state(seed=3, theta_noise=1e-2)
+    rng = np.random.default_rng(7)
+    for arr in (s.cu1, s.cu2, s.cu3):
+        arr[2:cfg.ny - 1, 1:4, 1:4] += 0.05 * (rng.standard_normal((cfg.ny - 3, 3, 3)) +
+                                               1j * rng.standard_normal((cfg.ny - 3, 3, 3)))
+    o = run_substep(p, s)
+    nt, _ = o.download_les()
+    phys = lambda c: layout.fft_xz_to_physical(cfg, c)
+    ikx = 1j * p.kx[:cfg.kxr][None, None, :]
+    ikz = 1j * p.kz[None, :, None]
+    dy, dyf = p.met["dy"], p.met["dyf"]
+    u = [s.cu1, s.cu2, s.cu3]
+    dx_ = [phys(ikx * c) for c in u]
+    dz_ = [phys(ikz * c) for c in u]
+    up = [phys(c) for c in u]
+    active = 0
+    for j in range(3, cfg.ny - 1):
+        w = lambda f: (f[j] * dyf[j - 1] + f[j - 1] * dyf[j]) / (2 * dy[j])
+        du2dy = lambda jj: (up[1][jj + 1] - up[1][jj]) / dyf[jj]
+        G = np.zeros((cfg.nz, cfg.nx, 3, 3))
+        G[..., 0, 0] = w(dx_[0]); G[..., 0, 1] = (up[0][j] - up[0][j - 1]) / dy[j]; G[..., 0, 2] = w(dz_[0])
+        G[..., 1, 0] = dx_[1][j]
+        G[..., 1, 1] = (du2dy(j) * dyf[j - 1] + du2dy(j - 1) * dyf[j]) / (2 * dy[j])
+        G[..., 1, 2] = dz_[1][j]
+        G[..., 2, 0] = w(dx_[2]); G[..., 2, 1] = (up[2][j] - up[2][j - 1]) / dy[j]; G[..., 2, 2] = w(dz_[2])
+        D2 = np.array([(cfg.lx / cfg.nx * BETA_SGS) ** 2, (2 * dy[j]) ** 2, (cfg.lz / cfg.nz * BETA_SGS) ** 2])
+        S = 0.5 * (G + np.swapaxes(G, -1, -2))
+        # the expansion of for/les.f90:276-343 is the mean of the two contractions
+        num = -0.5 * (np.einsum("k,...ik,...jk,...ij->...", D2, G, G, S) +
+                      np.einsum("k,...ki,...kj,...ij->...", D2, G, G, S))
+        den = np.einsum("...ij,...ij->...", G, G)
+        ref = C_AMD ** 2 * np.where(num > 0, num / den, 0.0)
+        np.testing.assert_allclose(nt[j], ref, rtol=1e-11, atol=1e-18)
+        active += int((ref > 0).sum())
+    assert active > 100
