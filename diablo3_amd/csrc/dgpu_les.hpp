@@ -424,6 +424,7 @@ __global__ __launch_bounds__(256) void k_amd(AmdArgs a) {
 // for/solver.f90:1691-1761, 1816-1852) writes rows [wlo, whi] of the field.
 // One thread per column; c' is kept in scratch, d' overwrites r.
 // ---------------------------------------------------------------------------
+template <typename TV>
 struct PhysCnArgs {
   Geo g;
   int nf;
@@ -437,11 +438,12 @@ struct PhysCnArgs {
   const double* __restrict__ dyf;
   double* u[MAXF];                    // physical field, solution written in place
   double* r[MAXF];                    // right-hand side; becomes d'
-  const double* visc[MAXF];           // nu_t (velocity) or kappa_t_n (scalar)
+  const TV* visc[MAXF];               // nu_t (velocity) or kappa_t_n (scalar)
   double* cp;                         // scratch, 2 * ny2 planes
 };
 
-__global__ __launch_bounds__(256) void k_cn_phys(PhysCnArgs a) {
+template <typename TV>
+__global__ __launch_bounds__(256) void k_cn_phys(PhysCnArgs<TV> a) {
   const Geo& g = a.g;
   const long long q = (long long)blockIdx.x * 256 + threadIdx.x;
   if (q >= g.pp) return;
@@ -453,7 +455,7 @@ __global__ __launch_bounds__(256) void k_cn_phys(PhysCnArgs a) {
   for (int e = 0; e < a.nf; ++e) {
     double* __restrict__ u = a.u[e];
     double* __restrict__ r = a.r[e];
-    const double* __restrict__ nt = a.visc[e];
+    const TV* __restrict__ nt = a.visc[e];
     const double T = a.tcn[e], t2 = a.temp2;
     const int fld = a.field[e];
     const bool v2 = (fld == 1);
@@ -463,7 +465,7 @@ __global__ __launch_bounds__(256) void k_cn_phys(PhysCnArgs a) {
       if (j >= a.blo[e] && j <= a.bhi[e]) {
         const long long o = (long long)j * pp + q;
         const double um = u[o - pp], uc = u[o], up = u[o + pp];
-        const double nm = nt[o - pp], nc = nt[o], np = nt[o + pp];
+        const double nm = (double)nt[o - pp], nc = (double)nt[o], np = (double)nt[o + pp];
         rhs = r[o];
         if (!v2) {
           l = -T / (dy[j] * dyf[j]);

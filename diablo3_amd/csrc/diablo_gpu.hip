@@ -78,7 +78,12 @@ struct dgpu_handle {
   cd* CF[MAXF] = {};
   void* NL[MAXF] = {};     // nonlinear tendencies (complex T), packed layout
   void* Hin[MAXF] = {};
-  void* HinD[MAXF] = {};  // FP64 x-spectra of a mixed-precision handle (statistics only)
+  // FP64 x-spectra, z-derivative x-spectra and physical v, dv/dx, dv/dz: the
+  // physical-space CN stage of LES, statistics and courant.  On an FP64 handle
+  // these alias Hin / HinZ / Pg; on a mixed one they are separate buffers.
+  void* HinD[MAXF] = {};
+  void* HinZD[MAXF] = {};
+  void* PgD[3 * MAXF] = {};
   void* Hout[MAXO] = {};
   void *twx64 = nullptr, *twz64 = nullptr, *twx32 = nullptr, *twz32 = nullptr;
   double* d_fac = nullptr;
@@ -90,8 +95,8 @@ struct dgpu_handle {
   // LES branch (use_les): physical fields of u, du/dx, du/dz per transported
   // variable, z-derivative x-spectra, eddy viscosity/diffusivity, SGS additions,
   // physical-space CN right-hand sides and scratch
-  void* HinZ[MAXF] = {};
-  void* Pg[3 * MAXF] = {};
+  void* HinZ[MAXF] = {};    // complex T
+  void* Pg[3 * MAXF] = {};  // T
   void* NuT = nullptr;
   void* KapT[MAXTH] = {};
   void* Aug[MAXA] = {};
@@ -639,7 +644,6 @@ int validate(const dgpu_config* c) {
   if (c->use_les) {
     if (c->les_model_type != 1 && c->les_model_type != 4 && c->les_model_type != 5)
       return fail(2, "les_model_type must be 1, 4 or 5 (2 and 3 stop in the reference)");
-    if (c->precision != DGPU_FP64) return fail(2, "the LES branch runs in FP64 precision in this build");
     // the Fourier terms that rk_chan_1 forms before les_chan's BC posts differ
     // from those after them only on rows that the BC rows of Dirichlet v overwrite
     if (c->v_bc_ymin != DGPU_BC_DIRICHLET || c->v_bc_ymax != DGPU_BC_DIRICHLET)
@@ -754,48 +758,51 @@ int launch_bc_post(dgpu_handle* h, double time, int les) {
   return 0;
 }
 
-// physical v, dv/dx, dv/dz of every transported field on all planes (FP64), into
-// the LES gradient buffers
-int physical_gradients(dgpu_handle* h) {
+// physical v, dv/dx, dv/dz of every transported field on all planes, in
+// precision T: x-spectra into H, z-derivative x-spectra into HZ, fields into P
+template <typename T>
+int physical_gradients(dgpu_handle* h, void* const* H, void* const* HZ, void* const* P) {
   const Geo& g = h->g;
   const int nf = 3 + h->nth;
   const cd* zs[2 * MAXF];
   void* zd[2 * MAXF];
-  void* const* H = (h->prec == DGPU_MIXED) ? h->HinD : h->Hin;  // FP64 x-spectra
   for (int f = 0; f < nf; ++f) {
     zs[f] = h->U[f]; zd[f] = H[f];
-    zs[nf + f] = h->U[f]; zd[nf + f] = h->HinZ[f];
+    zs[nf + f] = h->U[f]; zd[nf + f] = HZ[f];
   }
   int rc;
-  if ((rc = launch_inv_z<double>(h, 2 * nf, zs, zd, 0, g.ny2, ((1u << nf) - 1u) << nf))) return rc;
+  if ((rc = launch_inv_z<T>(h, 2 * nf, zs, zd, 0, g.ny2, ((1u << nf) - 1u) << nf))) return rc;
   void* xs[3 * MAXF];
-  double* xd[3 * MAXF];
+  T* xd[3 * MAXF];
   unsigned dxmask = 0;
   for (int f = 0; f < nf; ++f) {
-    xs[3 * f] = H[f]; xd[3 * f] = (double*)h->Pg[3 * f];
-    xs[3 * f + 1] = H[f]; xd[3 * f + 1] = (double*)h->Pg[3 * f + 1];
-    xs[3 * f + 2] = h->HinZ[f]; xd[3 * f + 2] = (double*)h->Pg[3 * f + 2];
+    xs[3 * f] = H[f]; xd[3 * f] = (T*)P[3 * f];
+    xs[3 * f + 1] = H[f]; xd[3 * f + 1] = (T*)P[3 * f + 1];
+    xs[3 * f + 2] = HZ[f]; xd[3 * f + 2] = (T*)P[3 * f + 2];
     dxmask |= 1u << (3 * f + 1);
   }
-  return launch_inv_x<double>(h, 3 * nf, xs, xd, 0, g.ny2, dxmask);
+  return launch_inv_x<T>(h, 3 * nf, xs, xd, 0, g.ny2, dxmask);
 }
 
-// the physical-gradient buffers of the LES branch, for a handle built without it
+// the FP64 gradient buffers (aliases on an FP64 handle that has the LES buffers)
 int ensure_grad_buffers(dgpu_handle* h) {
   const Geo& g = h->g;
   const int nf = 3 + h->nth;
   const size_t rbytes = sizeof(double) * g.pp * g.ny2;
   const size_t hbytes = sizeof(cd) * g.ph * g.ny2;
+  const bool fp64 = h->prec == DGPU_FP64;
   void* p = nullptr;
   int rc;
   for (int f = 0; f < nf; ++f) {
-    if (!h->HinZ[f]) { if ((rc = dalloc(h, &p, hbytes))) return rc; h->HinZ[f] = p; }
-    for (int m = 0; m < 3; ++m)
-      if (!h->Pg[3 * f + m]) { if ((rc = dalloc(h, &p, rbytes))) return rc; h->Pg[3 * f + m] = p; }
+    if (fp64) h->HinD[f] = h->Hin[f];
+    else if (!h->HinD[f]) { if ((rc = dalloc(h, &p, hbytes))) return rc; h->HinD[f] = p; }
+    if (fp64 && h->HinZ[f]) h->HinZD[f] = h->HinZ[f];
+    if (!h->HinZD[f]) { if ((rc = dalloc(h, &p, hbytes))) return rc; h->HinZD[f] = p; }
+    for (int m = 0; m < 3; ++m) {
+      if (fp64 && h->Pg[3 * f + m]) h->PgD[3 * f + m] = h->Pg[3 * f + m];
+      if (!h->PgD[3 * f + m]) { if ((rc = dalloc(h, &p, rbytes))) return rc; h->PgD[3 * f + m] = p; }
+    }
   }
-  // FP64 x-spectra for the transforms (the mixed handle keeps float ones in Hin)
-  if (h->prec == DGPU_MIXED && !h->HinD[0])
-    for (int f = 0; f < nf; ++f) { if ((rc = dalloc(h, &p, hbytes))) return rc; h->HinD[f] = p; }
   return 0;
 }
 
@@ -831,10 +838,9 @@ int courant_dns(dgpu_handle* h) {
   int rc = ensure_grad_buffers(h);
   if (rc) return rc;
   const Geo& g = h->g;
-  void* const* H = (h->prec == DGPU_MIXED) ? h->HinD : h->Hin;
   const cd* cs[3] = {h->U[0], h->U[1], h->U[2]};
-  void* hs[3] = {H[0], H[1], H[2]};
-  double* rs[3] = {(double*)h->Pg[0], (double*)h->Pg[3], (double*)h->Pg[6]};
+  void* hs[3] = {h->HinD[0], h->HinD[1], h->HinD[2]};
+  double* rs[3] = {(double*)h->PgD[0], (double*)h->PgD[3], (double*)h->PgD[6]};
   if ((rc = launch_inv_z<double>(h, 3, cs, hs, 0, g.ny2))) return rc;
   if ((rc = launch_inv_x<double>(h, 3, hs, rs, 0, g.ny2))) return rc;
   return launch_courant(h, rs[0], rs[1], rs[2]);
@@ -843,11 +849,12 @@ int courant_dns(dgpu_handle* h) {
 // One substep of the LES branch: rk_chan_1 with les_chan / les_chan_th
 // (for/solver.f90:611-623) and the physical-space Crank–Nicolson solves with the
 // eddy viscosity (for/solver.f90:875-903, 990-1233).
-template <int NTH>
+template <typename T, int NTH>
 int substep_les(dgpu_handle* h, int rk, double delta_t, double time, int time_step) {
   const dgpu_config& c = h->c;
   int rc = ensure_factors(h, delta_t);
   if (rc) return rc;
+  constexpr bool mixed = sizeof(T) == 4;
   const double hb[3] = {delta_t * (8.0 / 15.0), delta_t * (2.0 / 15.0), delta_t * (5.0 / 15.0)};
   const double bb[3] = {1.0, 25.0 / 8.0, 9.0 / 4.0};
   const double zb[3] = {0.0, -17.0 / 8.0, -5.0 / 4.0};
@@ -867,9 +874,10 @@ int substep_les(dgpu_handle* h, int rk, double delta_t, double time, int time_st
   if ((rc = launch_bc_post(h, time, 1))) return rc;
   if ((rc = tend(h))) return rc;
 
-  // 1. x-spectra of v and dv/dz, then physical v, dv/dx, dv/dz on all planes
+  // 1. x-spectra of v and dv/dz, then physical v, dv/dx, dv/dz on all planes (T);
+  //    a mixed handle also needs v in FP64 for the CN stage
   if ((rc = tbeg(h, 6))) return rc;
-  if ((rc = physical_gradients(h))) return rc;
+  if ((rc = physical_gradients<T>(h, h->Hin, h->HinZ, h->Pg))) return rc;
   if ((rc = tend(h))) return rc;
 
   // 2. AMD: nu_t, kappa_t and the SGS additions to the products
@@ -884,9 +892,9 @@ int substep_les(dgpu_handle* h, int rk, double delta_t, double time, int time_st
   for (int m = 0; m < 7 + 2 * NTH; ++m) aa.aug[m] = h->Aug[m];
   if ((rc = tbeg(h, 7))) return rc;
   const dim3 agrid((unsigned)((g.pp + 255) / 256), g.ny2);
-  if (c.les_model_type == 1) k_amd<double, NTH, 1><<<agrid, 256, 0, h->st>>>(aa);
-  else if (c.les_model_type == 4) k_amd<double, NTH, 4><<<agrid, 256, 0, h->st>>>(aa);
-  else k_amd<double, NTH, 5><<<agrid, 256, 0, h->st>>>(aa);
+  if (c.les_model_type == 1) k_amd<T, NTH, 1><<<agrid, 256, 0, h->st>>>(aa);
+  else if (c.les_model_type == 4) k_amd<T, NTH, 4><<<agrid, 256, 0, h->st>>>(aa);
+  else k_amd<T, NTH, 5><<<agrid, 256, 0, h->st>>>(aa);
   HIPCHK(hipGetLastError());
   if ((rc = tend(h))) return rc;
 
@@ -899,7 +907,7 @@ int substep_les(dgpu_handle* h, int rk, double delta_t, double time, int time_st
   for (int o = 0; o < 6 + 2 * NTH; ++o) xa.hout[o] = h->Hout[o];
   for (int m = 0; m < 7 + 2 * NTH; ++m) xa.aug[m] = h->Aug[m];
   if ((rc = tbeg(h, 1))) return rc;
-  if ((rc = launch_xprod<double, NTH, true>(h, xa, nj))) return rc;
+  if ((rc = launch_xprod<T, NTH, true>(h, xa, nj))) return rc;
   if ((rc = tend(h))) return rc;
   NlArgs na{};
   na.g = g; na.jlo = jlo; na.jstart = c.jstart; na.jend = c.jend;
@@ -908,7 +916,7 @@ int substep_les(dgpu_handle* h, int rk, double delta_t, double time, int time_st
   for (int o = 0; o < 6 + 2 * NTH; ++o) na.h[o] = h->Hout[o];
   for (int f = 0; f < nf; ++f) na.nl[f] = h->NL[f];
   if ((rc = tbeg(h, 2))) return rc;
-  if ((rc = launch_fwdz_nl<double>(h, na, nj))) return rc;
+  if ((rc = launch_fwdz_nl<T>(h, na, nj))) return rc;
   if ((rc = tend(h))) return rc;
 
   // 5. Fourier assembly of cf and of the right-hand side (molecular explicit half included)
@@ -922,28 +930,37 @@ int substep_les(dgpu_handle* h, int rk, double delta_t, double time, int time_st
   ca.chunk = cn_chunk(h);
   for (int f = 0; f < nf; ++f) ca.crout[f] = h->CR[f];
   if ((rc = tbeg(h, 3))) return rc;
-  if ((rc = launch_cn<double, NTH>(h, ca))) return rc;
+  if ((rc = launch_cn<T, NTH>(h, ca))) return rc;
   if ((rc = tend(h))) return rc;
 
-  // 6. right-hand sides to physical space
+  // 6. right-hand sides to physical space (FP64), and on a mixed handle the FP64
+  //    velocity and scalars the physical-space solve starts from
   const cd* cs[MAXF];
   double* rs[MAXF];
   for (int f = 0; f < nf; ++f) { cs[f] = h->CR[f]; rs[f] = h->Rp[f]; }
   if ((rc = tbeg(h, 9))) return rc;
-  if ((rc = launch_inv_z<double>(h, nf, cs, h->Hin, 0, g.ny2))) return rc;
-  if ((rc = launch_inv_x<double>(h, nf, h->Hin, rs, 0, g.ny2))) return rc;
+  if ((rc = launch_inv_z<double>(h, nf, cs, h->HinD, 0, g.ny2))) return rc;
+  if ((rc = launch_inv_x<double>(h, nf, h->HinD, rs, 0, g.ny2))) return rc;
+  double* ucn[MAXF];
+  for (int f = 0; f < nf; ++f) ucn[f] = (double*)h->PgD[3 * f];
+  if (mixed) {
+    const cd* us0[MAXF];
+    for (int f = 0; f < nf; ++f) us0[f] = h->U[f];
+    if ((rc = launch_inv_z<double>(h, nf, us0, h->HinD, 0, g.ny2))) return rc;
+    if ((rc = launch_inv_x<double>(h, nf, h->HinD, ucn, 0, g.ny2))) return rc;
+  }
   if ((rc = tend(h))) return rc;
 
   // 7. per-column CN with nu_t / kappa_t, scalars first (for/solver.f90:1004-1007)
-  PhysCnArgs pa{};
+  PhysCnArgs<T> pa{};
   pa.g = g; pa.nf = nf; pa.temp2 = temp2; pa.dy = h->d_dy; pa.dyf = h->d_dyf; pa.cp = h->CPs;
   for (int e0 = 0; e0 < nf; ++e0) {
     const int e = e0 < NTH ? 3 + e0 : e0 - NTH;  // slot e0 solves field e
     const SolveRows& R = ca.rows[e];
     pa.blo[e0] = R.blo; pa.bhi[e0] = R.bhi; pa.wlo[e0] = R.wlo; pa.whi[e0] = R.whi;
-    pa.u[e0] = (double*)h->Pg[3 * e];
+    pa.u[e0] = ucn[e];
     pa.r[e0] = h->Rp[e];
-    pa.visc[e0] = (const double*)(e < 3 ? h->NuT : h->KapT[e - 3]);
+    pa.visc[e0] = (const T*)(e < 3 ? h->NuT : h->KapT[e - 3]);
     pa.field[e0] = e;
     if (e == 0) { pa.lo_kind[e0] = c.u_bc_ymin; pa.hi_kind[e0] = c.u_bc_ymax; pa.lo_c1[e0] = c.u_bc_ymin_c1; pa.hi_c1[e0] = c.u_bc_ymax_c1; }
     else if (e == 1) { pa.lo_kind[e0] = c.v_bc_ymin; pa.hi_kind[e0] = c.v_bc_ymax; pa.lo_c1[e0] = c.v_bc_ymin_c1; pa.hi_c1[e0] = c.v_bc_ymax_c1; }
@@ -956,22 +973,21 @@ int substep_les(dgpu_handle* h, int rk, double delta_t, double time, int time_st
     pa.tcn[e0] = e < 3 ? temp1 * c.nu_v_scale : temp1 / c.pr[e - 3] * c.nu_v_scale;
   }
   if ((rc = tbeg(h, 8))) return rc;
-  k_cn_phys<<<(unsigned)((g.pp + 255) / 256), 256, 0, h->st>>>(pa);
+  k_cn_phys<T><<<(unsigned)((g.pp + 255) / 256), 256, 0, h->st>>>(pa);
   HIPCHK(hipGetLastError());
   if ((rc = tend(h))) return rc;
   // courant on the new velocity, before the projection (for/solver.f90:1241-1244)
   if (h->variable_dt && rk == 3 && time_step % h->update_dt == 0)
-    if ((rc = launch_courant(h, (const double*)h->Pg[0], (const double*)h->Pg[3], (const double*)h->Pg[6])))
-      return rc;
+    if ((rc = launch_courant(h, ucn[0], ucn[1], ucn[2]))) return rc;
 
   // 8. solved fields back to Fourier space (every plane, as the reference), then
   //    rem_div_chan and the pressure update
   double* us[MAXF];
   cd* ud[MAXF];
-  for (int f = 0; f < nf; ++f) { us[f] = (double*)h->Pg[3 * f]; ud[f] = h->U[f]; }
+  for (int f = 0; f < nf; ++f) { us[f] = ucn[f]; ud[f] = h->U[f]; }
   if ((rc = tbeg(h, 9))) return rc;
-  if ((rc = launch_fwd_x<double>(h, nf, us, h->Hin, 0, g.ny2))) return rc;
-  if ((rc = launch_fwd_z<double>(h, nf, h->Hin, ud, 0, g.ny2))) return rc;
+  if ((rc = launch_fwd_x<double>(h, nf, us, h->HinD, 0, g.ny2))) return rc;
+  if ((rc = launch_fwd_z<double>(h, nf, h->HinD, ud, 0, g.ny2))) return rc;
   if ((rc = tend(h))) return rc;
   if ((rc = tbeg(h, 4))) return rc;
   if ((rc = launch_project(h, ca, 1))) return rc;
@@ -984,7 +1000,8 @@ int substep(dgpu_handle* h, int rk, double delta_t, double time, int time_step, 
   if (c.use_les && (!create_new_flow || time_step > 100))
     return by_nth(h->nth, [&](auto NTHc) {
       constexpr int NTH = decltype(NTHc)::value;
-      return substep_les<NTH>(h, rk, delta_t, time, time_step);
+      return h->prec == DGPU_MIXED ? substep_les<float, NTH>(h, rk, delta_t, time, time_step)
+                                   : substep_les<double, NTH>(h, rk, delta_t, time, time_step);
     });
   int rc = ensure_factors(h, delta_t);
   if (rc) return rc;
@@ -1118,18 +1135,22 @@ int dgpu_create(const dgpu_config* cfg, dgpu_handle** out) {
   ALLOC(h->d_fac, sizeof(double) * 3 * MAXF * 5 * n2);
   ALLOC(h->d_rowc, sizeof(double) * RC_COUNT * n2);
   if (cfg->use_les) {
+    // gradients, eddy viscosity and SGS fields in the transform precision; the
+    // right-hand sides and the solve scratch in FP64
     const size_t rbytes = sizeof(double) * g.pp * n2;
+    const size_t tbytes = (h->prec == DGPU_MIXED ? sizeof(float) : sizeof(double)) * g.pp * n2;
     for (int f = 0; f < nf; ++f) {
       ALLOC(h->HinZ[f], hbytes);
       ALLOC(h->CR[f], cbytes);
       ALLOC(h->Rp[f], rbytes);
-      for (int m = 0; m < 3; ++m) ALLOC(h->Pg[3 * f + m], rbytes);
+      for (int m = 0; m < 3; ++m) ALLOC(h->Pg[3 * f + m], tbytes);
     }
-    ALLOC(h->NuT, rbytes);
-    for (int n = 0; n < h->nth; ++n) ALLOC(h->KapT[n], rbytes);
-    for (int m = 0; m < 7 + 2 * h->nth; ++m) ALLOC(h->Aug[m], rbytes);
+    ALLOC(h->NuT, tbytes);
+    for (int n = 0; n < h->nth; ++n) ALLOC(h->KapT[n], tbytes);
+    for (int m = 0; m < 7 + 2 * h->nth; ++m) ALLOC(h->Aug[m], tbytes);
     ALLOC(h->CPs, 2 * rbytes);
     ALLOC(h->d_dlt, sizeof(double) * 2 * n2);
+    if ((rc = ensure_grad_buffers(h))) return bail(rc);
   }
 #undef ALLOC
   h->fort_doubles = fortran_doubles(cfg);
@@ -1347,7 +1368,7 @@ int dgpu_stats(dgpu_handle* h, double time, double* out) {
   }
   // save_stats_chan starts with the BC posts on the state (for/diagnostics.f90:26-27)
   if ((rc = launch_bc_post(h, time, 0))) return rc;
-  if ((rc = physical_gradients(h))) return rc;
+  if ((rc = physical_gradients<double>(h, h->HinD, h->HinZD, h->PgD))) return rc;
   StatArgs a{};
   a.g = g; a.nth = nth;
   a.hx = c.homogeneous_x != 0;
@@ -1356,7 +1377,7 @@ int dgpu_stats(dgpu_handle* h, double time, double* out) {
   a.twf = (nth > 0 && c.ro_inv != 0.0) ? (1.0 / (c.ro_inv / c.delta)) * c.dthdx[0] : 0.0;
   for (int n = 0; n < MAXTH; ++n) { a.dthdx[n] = c.dthdx[n]; a.dthdz[n] = c.dthdz[n]; }
   a.dy = h->d_dy; a.dyf = h->d_dyf; a.gy = h->d_gy; a.gyf = h->d_gyf;
-  for (int s = 0; s < 3 * (3 + nth); ++s) a.f[s] = (const double*)h->Pg[s];
+  for (int s = 0; s < 3 * (3 + nth); ++s) a.f[s] = (const double*)h->PgD[s];
   for (int f = 0; f < 3 + nth; ++f) a.u[f] = h->U[f];
   a.p = h->P;
   a.zm = h->d_zm; a.zs = h->d_zs; a.prof = h->d_prof;

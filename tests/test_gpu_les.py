@@ -127,3 +127,29 @@ def test_les_gate_runs_dns_before_spin_up():
         out.append(fields(g.download_state()))
     for k in out[0]:
         assert np.array_equal(out[0][k], out[1][k]), k
+
+
+def test_les_mixed_precision_error():
+    """DGPU_MIXED LES (float gradients, model, products; FP64 state, solves,
+    projection) against the FP64 oracle after 3 steps: the error a float
+    transform carries, reported and bounded (u, theta <= 1e-6; nu_t <= 1e-5)."""
+    from diablo3_amd.config import MIXED
+    p64, s0 = les_problem(32, 33, 32)
+    cfg = p64.cfg.copy(precision=MIXED)
+    p = Problem(cfg)
+    g = Solver(p)
+    g.upload_state(s0)
+    t = 0.0
+    for n in range(3):
+        for rk in (1, 2, 3):
+            g.rk_substep(rk, time=t, time_step=n + 1, create_new_flow=False)
+        t += cfg.delta_t
+    o = run_oracle(p64, s0, 3)
+    fg, fo = fields(g.download_state()), fields(o.download_state())
+    errs = {k: rel(fg[k], fo[k]) for k in fo}
+    (ntg, _), (nto, _) = g.download_les(), o.download_les()
+    errs["nu_t"] = rel(ntg, nto)
+    print("LES mixed x3", {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+    for k in ("cu1", "cu2", "cu3", "cth0"):
+        assert errs[k] <= 1e-6, (k, errs[k])
+    assert errs["nu_t"] <= 1e-5
