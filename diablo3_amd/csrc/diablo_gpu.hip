@@ -1279,12 +1279,19 @@ int dgpu_download_les(dgpu_handle* h, double* nu_t, double* kappa_t) {
   const Geo& g = h->g;
   const size_t fd = fortran_doubles(&h->c);
   const int nfld = 1 + h->nth;
+  // nu_t and kappa_t are held in the handle's working precision
+  const bool f32 = h->prec == DGPU_MIXED;
   std::vector<double> tmp((size_t)g.pp * g.ny2);
+  std::vector<float> tmpf(f32 ? tmp.size() : 0);
   for (int m = 0; m < nfld; ++m) {
     double* out = m == 0 ? nu_t : (kappa_t ? kappa_t + (size_t)(m - 1) * fd : nullptr);
     if (!out) return fail(1, "null kappa_t");
     const void* src = m == 0 ? h->NuT : h->KapT[m - 1];
-    if (src) {
+    if (src && f32) {
+      HIPCHK(hipMemcpyAsync(tmpf.data(), src, tmpf.size() * 4, hipMemcpyDeviceToHost, h->st));
+      HIPCHK(hipStreamSynchronize(h->st));
+      std::copy(tmpf.begin(), tmpf.end(), tmp.begin());
+    } else if (src) {
       HIPCHK(hipMemcpyAsync(tmp.data(), src, tmp.size() * 8, hipMemcpyDeviceToHost, h->st));
       HIPCHK(hipStreamSynchronize(h->st));
     } else {
