@@ -140,6 +140,11 @@ struct AmdArgs {
   void* nut;
   void* kap[MAXTH];
   void* aug[MAXA];
+  // les_model_type 6: the learned closure's coefficients, per point (k_closure)
+  const float* cgyf;  // velocity, at GYF j
+  const float* cgy;   // velocity, at GY j
+  const float* kgyf;  // scalar, at GYF j
+  const float* kgy;   // scalar, at GY j
 };
 
 // the invariant-scaled strain/rotation entries of one GYF plane
@@ -260,6 +265,8 @@ template <int MODEL, typename T>
 __device__ __forceinline__ double model_gyf(const AmdIn<T>& in, const AmdArgs& a, int j, long long q) {
   if constexpr (MODEL == 5) {
     return coef_gyf(in, a, j, q);
+  } else if constexpr (MODEL == 6) {
+    return (double)a.cgyf[(long long)j * in.pp + q];
   } else {
     const Sgyf f = sgyf_plane(in, a, j, q);
     const Sgy p = sgy_plane(in, a, j + 1, q), m = sgy_plane(in, a, j, q);
@@ -278,6 +285,8 @@ template <int MODEL, typename T>
 __device__ __forceinline__ double model_gy(const AmdIn<T>& in, const AmdArgs& a, int j, long long q) {
   if constexpr (MODEL == 5) {
     return coef_gy(in, a, j, q);
+  } else if constexpr (MODEL == 6) {
+    return (double)a.cgy[(long long)j * in.pp + q];
   } else {
     const Sgyf f = sgyf_plane(in, a, j, q), fm = sgyf_plane(in, a, j - 1, q);
     const Sgy p = sgy_plane(in, a, j, q);
@@ -293,6 +302,54 @@ __device__ __forceinline__ double model_gy(const AmdIn<T>& in, const AmdArgs& a,
                        true);
     }
   }
+}
+
+// The type-5 scalar coefficient max{0, v}/|b'|^2 (les_chan_th), with the scaled
+// scalar gradient b' = (delta_x th_x, delta_y th_y, delta_z th_z) and the
+// unscaled th_y it was formed from
+struct ThCoef {
+  double c, bx, by, bz, ty;
+};
+// at GY j (for/les.f90:1259-1306)
+template <typename T>
+__device__ __forceinline__ ThCoef amd_th_gy(const AmdIn<T>& in, const AmdArgs& a, int ft, int j, long long q) {
+  const double dxs2 = a.dxs * a.dxs, dzs2 = a.dzs * a.dzs;
+  const double dyl = a.dy[j] * 2.0, dyl2 = dyl * dyl;
+  const double wc = a.dyf[j - 1], wm = a.dyf[j], den = 2.0 * a.dy[j];
+  auto w = [&](int s) { return (in.at(s, j, q) * wc + in.at(s, j - 1, q) * wm) / den; };
+  const double du1dx = w(1), du3dz = w(8), du1dz = w(2), du3dx = w(7);
+  const double du2dy = ((in.at(3, j + 1, q) - in.at(3, j, q)) / a.dyf[j] * wc +
+                        (in.at(3, j, q) - in.at(3, j - 1, q)) / a.dyf[j - 1] * wm) / den;
+  const double tx = w(ft + 1), tz = w(ft + 2);
+  const double ty = (in.at(ft, j, q) - in.at(ft, j - 1, q)) / a.dy[j];
+  const double du1dy = (in.at(0, j, q) - in.at(0, j - 1, q)) / a.dy[j];
+  const double du3dy = (in.at(6, j, q) - in.at(6, j - 1, q)) / a.dy[j];
+  const double du2dx = in.at(4, j, q), du2dz = in.at(5, j, q);
+  const double v = -(dxs2 * du1dx * (tx * tx) + dyl2 * du2dy * (ty * ty) + dzs2 * du3dz * (tz * tz) +
+                     (dyl2 * du1dy + dxs2 * du2dx) * tx * ty + (dzs2 * du1dz + dxs2 * du3dx) * tx * tz +
+                     (dzs2 * du2dz + dyl2 * du3dy) * ty * tz);
+  const double ax = a.dxs * tx, ay = dyl * ty, az = a.dzs * tz;
+  return {v > 0.0 ? v / (ax * ax + ay * ay + az * az) : 0.0, ax, ay, az, ty};
+}
+// at GYF j (for/les.f90:1221-1256, 1310-1318)
+template <typename T>
+__device__ __forceinline__ ThCoef amd_th_gyf(const AmdIn<T>& in, const AmdArgs& a, int ft, int j, long long q) {
+  const double dxs2 = a.dxs * a.dxs, dzs2 = a.dzs * a.dzs;
+  const double dyl = a.dyf[j] * 2.0, dyl2 = dyl * dyl;
+  auto gyv = [&](int s, int jj) { return (in.at(s, jj, q) - in.at(s, jj - 1, q)) / a.dy[jj]; };
+  const double tx = in.at(ft + 1, j, q), tz = in.at(ft + 2, j, q);
+  const double ty = 0.5 * (gyv(ft, j + 1) + gyv(ft, j));
+  const double du1dx = in.at(1, j, q), du3dz = in.at(8, j, q);
+  const double du2dy = (in.at(3, j + 1, q) - in.at(3, j, q)) / a.dyf[j];
+  const double du1dy = 0.5 * (gyv(0, j + 1) + gyv(0, j)), du3dy = 0.5 * (gyv(6, j + 1) + gyv(6, j));
+  const double du2dx = 0.5 * (in.at(4, j + 1, q) + in.at(4, j, q));
+  const double du2dz = 0.5 * (in.at(5, j + 1, q) + in.at(5, j, q));
+  const double du1dz = in.at(2, j, q), du3dx = in.at(7, j, q);
+  const double v = -(dxs2 * du1dx * (tx * tx) + dyl2 * du2dy * (ty * ty) + dzs2 * du3dz * (tz * tz) +
+                     (dyl2 * du1dy + dxs2 * du2dx) * tx * ty + (dzs2 * du1dz + dxs2 * du3dx) * tx * tz +
+                     (dzs2 * du2dz + dyl2 * du3dy) * ty * tz);
+  const double ax = a.dxs * tx, ay = dyl * ty, az = a.dzs * tz;
+  return {v > 0.0 ? v / (ax * ax + ay * ay + az * az) : 0.0, ax, ay, az, ty};
 }
 
 // tau_12 (e = 0) or tau_23 (e = 1) at GY j: delta_y(j) coef(j) (S - du/dy / 2).
@@ -344,7 +401,7 @@ __global__ __launch_bounds__(256) void k_amd(AmdArgs a) {
     A6 = tau_gy<MODEL>(in, a, 1, j, q) - nut * (in.at(6, j, q) - in.at(6, j - 1, q)) / a.dy[j];
   }
   put(a.aug[5], A5); put(a.aug[6], A6);
-  if constexpr (MODEL != 5) {
+  if constexpr (MODEL == 1 || MODEL == 4) {
     // les_chan_th types 1 and 4 (for/les.f90:1118-1193): kappa_t = nu_t (GY), and
     // cfth += ikx F(kappa_t dth/dx) + ikz F(kappa_t dth/dz) on rows jstart_th..jend_th
 #pragma unroll
@@ -357,57 +414,27 @@ __global__ __launch_bounds__(256) void k_amd(AmdArgs a) {
     }
     return;
   }
-  // scalars (les_chan_th type 5)
+  // scalars (les_chan_th type 5; type 6 reads the learned coefficients)
 #pragma unroll
   for (int n = 0; n < NTH; ++n) {
     const int ft = 3 * (3 + n);
     double kap = 0.0, fx = 0.0, fz = 0.0;
-    const double dxs2 = a.dxs * a.dxs, dzs2 = a.dzs * a.dzs;
     // kappa_t at GY j (for/les.f90:1259-1306, 1344-1348)
     if (j >= 3 && j <= ny - 1) {
-      const double dyl = a.dy[j] * 2.0, dyl2 = dyl * dyl;
-      const double wc = a.dyf[j - 1], wm = a.dyf[j], den = 2.0 * a.dy[j];
-      auto w = [&](int s) { return (in.at(s, j, q) * wc + in.at(s, j - 1, q) * wm) / den; };
-      const double du1dx = w(1), du3dz = w(8), du1dz = w(2), du3dx = w(7);
-      const double du2dy = ((in.at(3, j + 1, q) - in.at(3, j, q)) / a.dyf[j] * wc +
-                            (in.at(3, j, q) - in.at(3, j - 1, q)) / a.dyf[j - 1] * wm) / den;
-      const double tx = w(ft + 1), tz = w(ft + 2);
-      const double ty = (in.at(ft, j, q) - in.at(ft, j - 1, q)) / a.dy[j];
-      const double du1dy = (in.at(0, j, q) - in.at(0, j - 1, q)) / a.dy[j];
-      const double du3dy = (in.at(6, j, q) - in.at(6, j - 1, q)) / a.dy[j];
-      const double du2dx = in.at(4, j, q), du2dz = in.at(5, j, q);
-      double v = -(dxs2 * du1dx * (tx * tx) + dyl2 * du2dy * (ty * ty) + dzs2 * du3dz * (tz * tz) +
-                   (dyl2 * du1dy + dxs2 * du2dx) * tx * ty + (dzs2 * du1dz + dxs2 * du3dx) * tx * tz +
-                   (dzs2 * du2dz + dyl2 * du3dy) * ty * tz);
-      if (v > 0.0) {
-        const double ax = a.dxs * tx, ay = dyl * ty, az = a.dzs * tz;
-        kap = -0.5 * a.dlt[g.ny2 + j] * (v / (ax * ax + ay * ay + az * az));
-      }
+      double c;
+      if constexpr (MODEL == 6) c = (double)a.kgy[o];
+      else c = amd_th_gy(in, a, ft, j, q).c;
+      kap = -0.5 * a.dlt[g.ny2 + j] * c;
     }
     put(a.kap[n], kap);
     // SGS flux at GYF j (for/les.f90:1221-1256, 1310-1318, 1372-1393)
     if (j >= a.jstart_th[n] && j <= a.jend_th[n]) {
-      const double dyl = a.dyf[j] * 2.0, dyl2 = dyl * dyl;
-      auto gyv = [&](int s, int jj) { return (in.at(s, jj, q) - in.at(s, jj - 1, q)) / a.dy[jj]; };
-      const double tx = in.at(ft + 1, j, q), tz = in.at(ft + 2, j, q);
-      const double ty = 0.5 * (gyv(ft, j + 1) + gyv(ft, j));
-      const double du1dx = in.at(1, j, q), du3dz = in.at(8, j, q);
-      const double du2dy = (in.at(3, j + 1, q) - in.at(3, j, q)) / a.dyf[j];
-      const double du1dy = 0.5 * (gyv(0, j + 1) + gyv(0, j)), du3dy = 0.5 * (gyv(6, j + 1) + gyv(6, j));
-      const double du2dx = 0.5 * (in.at(4, j + 1, q) + in.at(4, j, q));
-      const double du2dz = 0.5 * (in.at(5, j + 1, q) + in.at(5, j, q));
-      const double du1dz = in.at(2, j, q), du3dx = in.at(7, j, q);
-      double v = -(dxs2 * du1dx * (tx * tx) + dyl2 * du2dy * (ty * ty) + dzs2 * du3dz * (tz * tz) +
-                   (dyl2 * du1dy + dxs2 * du2dx) * tx * ty + (dzs2 * du1dz + dxs2 * du3dx) * tx * tz +
-                   (dzs2 * du2dz + dyl2 * du3dy) * ty * tz);
-      double s1 = 0.0;
-      if (v > 0.0) {
-        const double ax = a.dxs * tx, ay = dyl * ty, az = a.dzs * tz;
-        s1 = v / (ax * ax + ay * ay + az * az);
-      }
+      double s1;
+      if constexpr (MODEL == 6) s1 = (double)a.kgyf[o];
+      else s1 = amd_th_gyf(in, a, ft, j, q).c;
       const double f = 0.5 * a.dlt[j] * s1;
-      fx = f * tx;
-      fz = f * tz;
+      fx = f * in.at(ft + 1, j, q);
+      fz = f * in.at(ft + 2, j, q);
     }
     put(a.aug[7 + 2 * n], fx);
     put(a.aug[8 + 2 * n], fz);

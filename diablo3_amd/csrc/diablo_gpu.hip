@@ -15,10 +15,13 @@
 // The LES substep (use_LES past the spin-up gate) adds the AMD closure and solves
 // CN per physical column (substep_les, kernels in dgpu_les.hpp).
 #include "../../include/diablo_gpu.h"
+#include "../../include/dgpu_closure.h"
 #include "dgpu_kernels.hpp"
 #include "dgpu_les.hpp"
+#include "dgpu_closure.hpp"
 #include "dgpu_stats.hpp"
 #include "dgpu_courant.hpp"
+#include "dgpu_error.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -64,6 +67,8 @@ constexpr int ti_for(int nz, int sz) {
 
 }  // namespace
 
+int dgpu::set_error(int code, const std::string& msg) { return fail(code, msg); }
+
 struct dgpu_handle {
   dgpu_config c{};
   std::vector<double> gy, gyf, dy, dyf, kx, kz;
@@ -104,6 +109,13 @@ struct dgpu_handle {
   double* Rp[MAXF] = {};
   double* CPs = nullptr;
   double* d_dlt = nullptr;  // [2][ny2]: delta_yf, delta_y of the AMD filter width
+  // les_model_type 6: the network (BF16 forward weights w1t, w2t, w3t; FP32
+  // parameter block for the biases; normaliser) and the per-point coefficients
+  unsigned short* d_netw = nullptr;
+  float* d_netp = nullptr;
+  float* d_netnorm = nullptr;
+  int have_net = 0;
+  float *Cgyf = nullptr, *Cgy = nullptr, *Kgyf = nullptr, *Kgy = nullptr;
   // variable_dt: courant's CFL limit, reduced per block on the device at rk_step 3
   int variable_dt = 0, update_dt = 1, cfl_ran = 0;
   double cfl = 0.5;
@@ -359,9 +371,9 @@ int ensure_fft_scratch(dgpu_handle* h) {
 // --------------------------------------------------------------------------------
 // per-kernel timing
 // --------------------------------------------------------------------------------
-constexpr int NKERN = 10;
-const char* const kKernelNames[NKERN] = {"k_inv_z",   "k_xprod", "k_fwdz_nl", "k_cn",      "k_project",
-                                         "k_bc_post", "les_grad", "k_amd",    "k_cn_phys", "les_cn_fft"};
+constexpr int NKERN = 11;
+const char* const kKernelNames[NKERN] = {"k_inv_z",   "k_xprod",  "k_fwdz_nl", "k_cn",      "k_project", "k_bc_post",
+                                         "les_grad", "k_amd",    "k_cn_phys", "les_cn_fft", "k_closure"};
 
 hipEvent_t pool_event(dgpu_handle* h) {
   if (!h->ev_pool.empty()) {
@@ -642,8 +654,9 @@ int validate(const dgpu_config* c) {
   }
   if (c->delta == 0.0) return fail(2, "delta must be non-zero");
   if (c->use_les) {
-    if (c->les_model_type != 1 && c->les_model_type != 4 && c->les_model_type != 5)
-      return fail(2, "les_model_type must be 1, 4 or 5 (2 and 3 stop in the reference)");
+    if (c->les_model_type != 1 && c->les_model_type != 4 && c->les_model_type != 5 && c->les_model_type != 6)
+      return fail(2, "les_model_type must be 1, 4, 5 or 6 (learned); 2 and 3 stop in the reference");
+    if (c->les_model_type == 6 && c->n_th != 1) return fail(2, "les_model_type 6 needs exactly one scalar");
     // the Fourier terms that rk_chan_1 forms before les_chan's BC posts differ
     // from those after them only on rows that the BC rows of Dirichlet v overwrite
     if (c->v_bc_ymin != DGPU_BC_DIRICHLET || c->v_bc_ymax != DGPU_BC_DIRICHLET)
@@ -849,6 +862,39 @@ int courant_dns(dgpu_handle* h) {
 // One substep of the LES branch: rk_chan_1 with les_chan / les_chan_th
 // (for/solver.f90:611-623) and the physical-space Crank–Nicolson solves with the
 // eddy viscosity (for/solver.f90:875-903, 990-1233).
+// arguments of k_closure from the handle's network and an AmdArgs already filled
+ClosureArgs closure_args(dgpu_handle* h, const AmdArgs& aa) {
+  ClosureArgs k{};
+  k.a = aa;
+  MlpW& w = k.w;
+  w.w1t = h->d_netw;
+  w.w2t = h->d_netw + 64 * 16;
+  w.w3t = h->d_netw + 64 * 16 + 64 * 64;
+  w.w2 = nullptr;
+  w.w3 = nullptr;
+  w.b1 = h->d_netp + MLP_OB1;
+  w.b2 = h->d_netp + MLP_OB2;
+  w.b3 = h->d_netp + MLP_OB3;
+  w.norm = h->d_netnorm;
+  k.have_net = h->have_net;
+  k.nu = h->c.nu;
+  k.gyb = h->c.grav_y;
+  k.cgyf = h->Cgyf; k.cgy = h->Cgy; k.kgyf = h->Kgyf; k.kgy = h->Kgy;
+  return k;
+}
+
+// the AmdArgs of the handle's current gradient fields (outputs left unset)
+AmdArgs amd_args(dgpu_handle* h) {
+  const dgpu_config& c = h->c;
+  AmdArgs aa{};
+  aa.g = h->g; aa.nth = h->nth; aa.jstart = c.jstart; aa.jend = c.jend;
+  for (int n = 0; n < MAXTH; ++n) { aa.jstart_th[n] = c.jstart_th[n]; aa.jend_th[n] = c.jend_th[n]; }
+  aa.dxs = c.lx / c.nx * BETA_SGS; aa.dzs = c.lz / c.nz * BETA_SGS;
+  aa.dy = h->d_dy; aa.dyf = h->d_dyf; aa.dlt = h->d_dlt;
+  for (int s = 0; s < 3 * (3 + h->nth); ++s) aa.f[s] = h->Pg[s];
+  return aa;
+}
+
 template <typename T, int NTH>
 int substep_les(dgpu_handle* h, int rk, double delta_t, double time, int time_step) {
   const dgpu_config& c = h->c;
@@ -890,10 +936,22 @@ int substep_les(dgpu_handle* h, int rk, double delta_t, double time, int time_st
   aa.nut = h->NuT;
   for (int n = 0; n < NTH; ++n) aa.kap[n] = h->KapT[n];
   for (int m = 0; m < 7 + 2 * NTH; ++m) aa.aug[m] = h->Aug[m];
-  if ((rc = tbeg(h, 7))) return rc;
   const dim3 agrid((unsigned)((g.pp + 255) / 256), g.ny2);
+  if constexpr (NTH == 1) {
+    if (c.les_model_type == 6) {
+      // the learned coefficients (AMD where a point leaves the envelope)
+      ClosureArgs ka = closure_args(h, aa);
+      if ((rc = tbeg(h, 10))) return rc;
+      k_closure<T, NTH><<<agrid, 256, 0, h->st>>>(ka);
+      HIPCHK(hipGetLastError());
+      if ((rc = tend(h))) return rc;
+      aa.cgyf = h->Cgyf; aa.cgy = h->Cgy; aa.kgyf = h->Kgyf; aa.kgy = h->Kgy;
+    }
+  }
+  if ((rc = tbeg(h, 7))) return rc;
   if (c.les_model_type == 1) k_amd<T, NTH, 1><<<agrid, 256, 0, h->st>>>(aa);
   else if (c.les_model_type == 4) k_amd<T, NTH, 4><<<agrid, 256, 0, h->st>>>(aa);
+  else if (c.les_model_type == 6 && NTH == 1) k_amd<T, NTH, 6><<<agrid, 256, 0, h->st>>>(aa);
   else k_amd<T, NTH, 5><<<agrid, 256, 0, h->st>>>(aa);
   HIPCHK(hipGetLastError());
   if ((rc = tend(h))) return rc;
@@ -1150,6 +1208,16 @@ int dgpu_create(const dgpu_config* cfg, dgpu_handle** out) {
     for (int m = 0; m < 7 + 2 * h->nth; ++m) ALLOC(h->Aug[m], tbytes);
     ALLOC(h->CPs, 2 * rbytes);
     ALLOC(h->d_dlt, sizeof(double) * 2 * n2);
+    if (cfg->les_model_type == 6) {
+      const size_t fbytes = sizeof(float) * g.pp * n2;
+      ALLOC(h->Cgyf, fbytes);
+      ALLOC(h->Cgy, fbytes);
+      ALLOC(h->Kgyf, fbytes);
+      ALLOC(h->Kgy, fbytes);
+      ALLOC(h->d_netw, sizeof(unsigned short) * (64 * 16 + 64 * 64 + 16 * 64));
+      ALLOC(h->d_netp, sizeof(float) * MLP_NP);
+      ALLOC(h->d_netnorm, sizeof(float) * MLP_NNORM);
+    }
     if ((rc = ensure_grad_buffers(h))) return bail(rc);
   }
 #undef ALLOC
@@ -1303,6 +1371,102 @@ int dgpu_download_les(dgpu_handle* h, double* nu_t, double* kappa_t) {
         std::memcpy(out + ((size_t)j * (g.nz + 2) + z) * (g.nx + 2), &tmp[((size_t)j * g.nz + z) * g.nx],
                     sizeof(double) * g.nx);
   }
+  return 0;
+}
+
+int dgpu_set_closure(dgpu_handle* h, const float* params, const float* norm) {
+  if (!h || !params || !norm) return fail(1, "null argument");
+  if (!h->c.use_les || h->c.les_model_type != 6) return fail(2, "dgpu_set_closure needs an LES handle of type 6");
+  // BF16 copies of the forward weights: w1t [64][16], w2t [64][64], w3t [16][64]
+  // (round to nearest even, as the device's own conversion)
+  auto bf = [](float x) -> unsigned short {
+    uint32_t u;
+    std::memcpy(&u, &x, 4);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+  };
+  std::vector<float> p(params, params + MLP_NP);
+  for (int i = 0; i < MLP_NP; ++i) {
+    const bool dead = (i < MLP_OB1 && (i - MLP_OW1) / MLP_H >= MLP_IN) ||
+                      (i >= MLP_OW3 && i < MLP_OB3 && (i - MLP_OW3) % MLP_OUTP >= MLP_OUT) ||
+                      (i >= MLP_OB3 && i - MLP_OB3 >= MLP_OUT);
+    if (dead) p[i] = 0.f;
+  }
+  std::vector<unsigned short> w(64 * 16 + 64 * 64 + 16 * 64);
+  for (int in = 0; in < MLP_INP; ++in)
+    for (int out = 0; out < MLP_H; ++out) w[out * MLP_INP + in] = bf(p[MLP_OW1 + in * MLP_H + out]);
+  for (int in = 0; in < MLP_H; ++in)
+    for (int out = 0; out < MLP_H; ++out) w[64 * 16 + out * MLP_H + in] = bf(p[MLP_OW2 + in * MLP_H + out]);
+  for (int in = 0; in < MLP_H; ++in)
+    for (int out = 0; out < MLP_OUTP; ++out)
+      w[64 * 16 + 64 * 64 + out * MLP_H + in] = bf(p[MLP_OW3 + in * MLP_OUTP + out]);
+  HIPCHK(hipMemcpyAsync(h->d_netw, w.data(), sizeof(unsigned short) * w.size(), hipMemcpyHostToDevice, h->st));
+  HIPCHK(hipMemcpyAsync(h->d_netp, p.data(), sizeof(float) * MLP_NP, hipMemcpyHostToDevice, h->st));
+  HIPCHK(hipMemcpyAsync(h->d_netnorm, norm, sizeof(float) * MLP_NNORM, hipMemcpyHostToDevice, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  h->have_net = 1;
+  return 0;
+}
+
+int dgpu_les_features(dgpu_handle* h, double time, float* X, float* scale, float* amd, int32_t* inside,
+                      int64_t* n_out) {
+  if (!h || !n_out) return fail(1, "null argument");
+  if (!h->c.use_les || h->nth < 1) return fail(2, "dgpu_les_features needs an LES handle with a scalar");
+  const Geo& g = h->g;
+  const int jf0 = h->c.jstart, ny = g.ny;
+  const long long n = (long long)(ny - jf0) * g.pp;
+  *n_out = n;
+  if (!X && !scale && !amd && !inside) return 0;
+  // the BC posts and gradient fields of an LES substep, then the features
+  int rc;
+  if ((rc = launch_bc_post(h, time, 1))) return rc;
+  const bool mixed = h->prec == DGPU_MIXED;
+  if ((rc = mixed ? physical_gradients<float>(h, h->Hin, h->HinZ, h->Pg)
+                  : physical_gradients<double>(h, h->Hin, h->HinZ, h->Pg)))
+    return rc;
+  float *dX = nullptr, *dS = nullptr, *dA = nullptr;
+  int* dI = nullptr;
+  auto cleanup = [&]() {
+    for (void* q : {(void*)dX, (void*)dS, (void*)dA, (void*)dI})
+      if (q) (void)hipFree(q);
+  };
+  if (hipMalloc((void**)&dX, sizeof(float) * MLP_INP * (size_t)n) != hipSuccess ||
+      hipMalloc((void**)&dS, sizeof(float) * (size_t)n) != hipSuccess ||
+      hipMalloc((void**)&dA, sizeof(float) * (size_t)n) != hipSuccess ||
+      hipMalloc((void**)&dI, sizeof(int) * (size_t)n) != hipSuccess) {
+    cleanup();
+    return fail(11, "hipMalloc failed (feature buffers)");
+  }
+  const AmdArgs aa = amd_args(h);
+  ClosureArgs k{};
+  if (h->c.les_model_type == 6) {
+    k = closure_args(h, aa);
+  } else {
+    k.a = aa;
+    k.have_net = 0;
+    k.nu = h->c.nu;
+    k.gyb = h->c.grav_y;
+  }
+  k.feat = 1;
+  k.jf0 = jf0;
+  k.X = dX; k.scale = dS; k.amd = dA; k.inside = dI;
+  const dim3 grid((unsigned)((g.pp + 255) / 256), g.ny2);
+  bool ok = true;
+  if (h->nth == 1) {
+    if (mixed) k_closure<float, 1><<<grid, 256, 0, h->st>>>(k);
+    else k_closure<double, 1><<<grid, 256, 0, h->st>>>(k);
+  } else {
+    cleanup();
+    return fail(2, "dgpu_les_features supports one scalar");
+  }
+  ok = hipGetLastError() == hipSuccess;
+  if (ok && X) ok = hipMemcpyAsync(X, dX, sizeof(float) * MLP_INP * (size_t)n, hipMemcpyDeviceToHost, h->st) == hipSuccess;
+  if (ok && scale) ok = hipMemcpyAsync(scale, dS, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost, h->st) == hipSuccess;
+  if (ok && amd) ok = hipMemcpyAsync(amd, dA, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost, h->st) == hipSuccess;
+  if (ok && inside) ok = hipMemcpyAsync(inside, dI, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost, h->st) == hipSuccess;
+  ok = ok && hipStreamSynchronize(h->st) == hipSuccess;
+  cleanup();
+  if (!ok) return fail(10, "dgpu_les_features: kernel or download failed");
   return 0;
 }
 

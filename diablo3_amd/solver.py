@@ -64,15 +64,33 @@ def load():
     L.dgpu_stats_doubles.restype = ctypes.c_int64
     L.dgpu_field_doubles.argtypes = [P]
     L.dgpu_field_doubles.restype = ctypes.c_int64
+    F = ctypes.POINTER(ctypes.c_float)
+    I64 = ctypes.c_int64
+    L.dgpu_mlp_create.argtypes = [F, ctypes.c_uint64, ctypes.POINTER(P)]
+    L.dgpu_mlp_destroy.argtypes = [P]
+    L.dgpu_mlp_set_params.argtypes = [P, F]
+    L.dgpu_mlp_get_params.argtypes = [P, F]
+    L.dgpu_mlp_set_norm.argtypes = [P, F]
+    L.dgpu_mlp_set_data.argtypes = [P, F, F, I64]
+    L.dgpu_mlp_grad.argtypes = [P, I64, F, F, F]
+    L.dgpu_mlp_adam_step.argtypes = [P, F, ctypes.c_float]
+    L.dgpu_mlp_train.argtypes = [P, ctypes.c_int32, ctypes.c_int32, ctypes.c_float, ctypes.c_uint64, F, F]
+    L.dgpu_mlp_predict.argtypes = [P, F, I64, F]
+    L.dgpu_set_closure.argtypes = [P, F, F]
+    L.dgpu_les_features.argtypes = [P, ctypes.c_double, F, F, F, ctypes.POINTER(ctypes.c_int32),
+                                    ctypes.POINTER(I64)]
     _lib = L
     return L
 
 
 def exported_symbols() -> List[str]:
-    """Names of the functions include/diablo_gpu.h declares."""
+    """Names of the functions the headers under include/ declare."""
+    import glob
     import re
-    txt = open(HEADER_PATH).read()
-    return sorted(set(re.findall(r"\b(dgpu_[a-z0-9_]+)\s*\(", txt)))
+    names = set()
+    for path in glob.glob(os.path.join(os.path.dirname(HEADER_PATH), "*.h")):
+        names |= set(re.findall(r"\b(dgpu_[a-z0-9_]+)\s*\(", open(path).read()))
+    return sorted(names)
 
 
 def _dp(a: np.ndarray):
@@ -198,7 +216,7 @@ class Solver:
     def timing_read(self):
         """{kernel name: (total device ms, launches)} since timing(True), launched kernels only."""
         out = {}
-        for k in range(10):
+        for k in range(11):
             ms, n, name = ctypes.c_double(), ctypes.c_int64(), ctypes.c_char_p()
             self._check(load().dgpu_timing_read(self.h, k, ctypes.byref(ms), ctypes.byref(n),
                                                 ctypes.byref(name)))

@@ -154,7 +154,8 @@ int dgpu_last_step_ms(dgpu_handle *h, double *ms);
  * state, 1 = fused x pass (products), 2 = z forward transform of the products,
  * 3 = Fourier assembly + Crank-Nicolson solves, 4 = projection + pressure update;
  * LES substeps add 5 = BC posts, 6 = gradient fields, 7 = AMD model,
- * 8 = physical-column CN solves, 9 = right-hand-side and solution transforms.
+ * 8 = physical-column CN solves, 9 = right-hand-side and solution transforms,
+ * 10 = the learned closure's coefficients (les_model_type 6, dgpu_closure.h).
  * dgpu_timing_read returns the
  * device time and launch count accumulated since the last enable, and the name. */
 int dgpu_timing_enable(dgpu_handle *h, int32_t enable);

@@ -1,0 +1,91 @@
+"""les_model_type 6 (the learned closure) through the C ABI.
+
+* With no network set, or with an empty training envelope, every point takes
+  the AMD coefficient: the step must match the oracle's type 5 to the LES
+  parity bar (1e-11; cp at 30x the oracle's own round-off disagreement).
+* dgpu_les_features: the AMD column times the scale is the type-5 eddy
+  viscosity at GYF, and the features are finite and bounded where defined.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle
+from diablo3_amd import closure, configs
+from diablo3_amd.problem import Problem
+from diablo3_amd.solver import Solver, load
+
+from test_gpu_les import fields, les_problem, rel, run_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def type6(p5):
+    cfg = p5.cfg.copy(les_model_type=6)
+    return Problem(cfg)
+
+
+def features(g, time=0.0):
+    L = load()
+    n = ctypes.c_int64()
+    closure._check(L.dgpu_les_features(g.h, time, None, None, None, None, ctypes.byref(n)))
+    X = np.zeros((n.value, 16), np.float32)
+    sc = np.zeros(n.value, np.float32)
+    am = np.zeros(n.value, np.float32)
+    ins = np.zeros(n.value, np.int32)
+    closure._check(L.dgpu_les_features(g.h, time, closure._fp(X), closure._fp(sc), closure._fp(am),
+                                       ins.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), ctypes.byref(n)))
+    return X, sc, am, ins
+
+
+@pytest.mark.parametrize("envelope", ["none", "empty"])
+def test_type6_fallback_matches_amd_oracle(envelope):
+    p5, s0 = les_problem(32, 33, 32)
+    p6 = type6(p5)
+    g = Solver(p6)
+    if envelope == "empty":
+        norm = np.zeros(64, np.float32)
+        norm[16:29] = 1.0
+        norm[32:48] = 1.0   # lo > hi: no point is inside
+        norm[48:64] = -1.0
+        m = closure.Mlp(seed=1)
+        closure._check(load().dgpu_set_closure(g.h, closure._fp(m.params), closure._fp(norm)))
+    g.upload_state(s0)
+    t = 0.0
+    for n in range(3):
+        for rk in (1, 2, 3):
+            g.rk_substep(rk, time=t, time_step=n + 1, create_new_flow=False)
+        t += p5.cfg.delta_t
+    o = run_oracle(p5, s0, 3)
+    fg, fo = fields(g.download_state()), fields(o.download_state())
+    fa = fields(run_oracle(p5, s0, 3, variant=1).download_state())
+    cp_floor = rel(fa["cp"], fo["cp"])
+    errs = {k: rel(fg[k], fo[k]) for k in fo}
+    (ntg, ktg), (nto, kto) = g.download_les(), o.download_les()
+    errs["nu_t"] = rel(ntg, nto)
+    errs["kappa_t"] = rel(ktg[0], kto[0])
+    print(envelope, {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+    for k, e in errs.items():
+        assert e <= (max(1e-11, 30 * cp_floor) if k == "cp" else 1e-11), (k, e)
+
+
+def test_features_amd_column_is_type5_viscosity():
+    p5, s0 = les_problem(32, 33, 32)
+    g = Solver(type6(p5))
+    g.upload_state(s0)
+    X, sc, am, ins = features(g)
+    cfg = p5.cfg
+    n = (cfg.ny - cfg.jstart) * cfg.nz * cfg.nx
+    assert X.shape == (n, 16) and np.all(ins == 1)
+    assert np.all(np.isfinite(X)) and np.all(X[:, 13:] == 0)
+    # bounded features: ratios in [-1, 1] (0, 5, 7, 8, 9, 10), others by construction
+    for k in (0, 5, 9, 10):
+        assert np.all(np.abs(X[:, k]) <= 1.0 + 1e-6), k
+    for k in (4, 7, 8):
+        assert np.all((X[:, k] >= -1e-6) & (X[:, k] <= 1.0 + 1e-6)), k
+    # the AMD column: coef / sqrt(2 s2) = max(0, f3) / sqrt(2) on every point with s2 > 0
+    live = sc > 0
+    assert live.mean() > 0.9
+    assert np.allclose(am[live], np.maximum(0.0, X[live, 3]) / np.sqrt(2.0), rtol=1e-5, atol=1e-6)
+    print("features: s2>0 on", f"{live.mean():.3f}", "AMD active on", f"{(am > 0).mean():.3f}", flush=True)
