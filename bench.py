@@ -49,21 +49,34 @@ def kernel_bytes(cfg, mixed):
         "k_project": 4 * C + 4 * C,
     }
     if cfg.use_les:
-        # physical field R (FP64: the LES branch runs in FP64); na SGS addition fields
-        R = cfg.nx * cfg.nz * 8 * ny2
-        Rj = cfg.nx * cfg.nz * 8 * nj
+        # physical fields: R8 FP64, Rt in the transform precision; na SGS addition fields
+        R8 = cfg.nx * cfg.nz * 8 * ny2
+        Rt = cfg.nx * cfg.nz * s * ny2
+        Rtj = cfg.nx * cfg.nz * s * nj
+        Hf = cfg.kxr * cfg.nz * 2 * 4 * ny2        # float x-spectrum (increment transforms)
+        Rf = cfg.nx * cfg.nz * 4 * ny2
         na = 7 + 2 * cfg.n_th
+        learned = cfg.les_model_type == 6
         out["k_bc_post"] = nf * 6 * C / ny2
         # z inverse of v and dv/dz (2 nf), x inverse of v, dv/dx, dv/dz (3 nf)
-        out["les_grad"] = 2 * nf * C + 2 * nf * H + 3 * nf * H + 3 * nf * R
-        out["k_amd"] = 3 * nf * R + (1 + cfg.n_th + na) * R
-        out["k_xprod"] += na * Rj
+        out["les_grad"] = 2 * nf * C + 2 * nf * H + 3 * nf * H + 3 * nf * Rt
+        # gradient fields in; nu_t, kappa_t, SGS fields out (the learned model also
+        # reads its four coefficient planes)
+        out["k_amd"] = 3 * nf * Rt + (1 + cfg.n_th + na) * Rt + (4 * Rt if learned else 0)
+        if learned:
+            out["k_closure"] = 3 * nf * Rt + 4 * Rt
+        out["k_xprod"] += na * Rtj
         # assembly only: the right-hand side goes out instead of the solve
         out["k_cn"] = nf * Cj_t + (nf + 1) * C + (2.0 / 3.0) * nf * Cj + nf * Cj + nf * C
-        # per field: r, u, nu_t in, c', b', g' out and back, u out
-        out["k_cn_phys"] = nf * 10 * R
-        # rhs: C -> H -> R; solution: R -> H -> C
-        out["les_cn_fft"] = 2 * nf * C + 4 * nf * H + 2 * nf * R
+        npts = cfg.nx * cfg.nz * ny2
+        if mixed:
+            # increments: u (float), r - u (float), nu in (twice); 1/b', g' out and back; delta out
+            out["k_cn_phys"] = nf * npts * (4 + 4 + 2 * s + 16 + 16 + 4)
+            # per launch (two per substep): C -> Hf -> Rf, then Rf -> Hf -> C (read + write)
+            out["les_cn_fft"] = 0.5 * nf * ((C + 2 * Hf + Rf) + (Rf + 2 * Hf + 2 * C))
+        else:
+            out["k_cn_phys"] = nf * npts * (8 + 8 + 2 * s + 16 + 16 + 8)
+            out["les_cn_fft"] = 0.5 * nf * ((C + 2 * H + R8) + (R8 + 2 * H + C))
     return out
 
 
@@ -123,6 +136,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="dns_256", choices=["dns_256", "dns_64", "les_512", "les_128"])
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--closure", default="amd", choices=["amd", "learned"],
+                    help="LES workloads: the AMD closure (type 5) or the learned one (type 6)")
+    ap.add_argument("--closure-file", default=None,
+                    help="learned closure parameters (.npz with params, norm); default: the "
+                         "committed front closure if present, else a seeded network")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
     args = ap.parse_args()
@@ -138,8 +156,19 @@ def main():
 
     mixed = args.precision == "mixed"
     cfg = configs.NAMED[args.workload](MIXED if mixed else FP64)
+    learned = args.closure == "learned"
+    if learned:
+        if not cfg.use_les:
+            print("bench.py: --closure learned needs an LES workload", file=sys.stderr)
+            sys.exit(2)
+        cfg = cfg.copy(les_model_type=6)
     p = Problem(cfg)
     g = Solver(p)
+    closure_src = None
+    if learned:
+        from diablo3_amd import closure
+        params, norm, closure_src = closure.load_params(args.closure_file)
+        g.set_closure(params, norm)
     g.upload_state(p.initial_state(seed=0, theta_noise=1e-3))
     g.rem_div()
     # LES: a restart-style run (create_new_flow false), so the SGS model is on from step 1
@@ -183,11 +212,12 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64" if not mixed else "f32+f64",
         "data": "synthetic (seeded front IC, projected divergence-free)",
-        "config": {"workload": f"front {'LES (AMD closure)' if cfg.use_les else 'DNS'} "
+        "config": {"workload": f"front {('LES (learned closure)' if learned else 'LES (AMD closure)') if cfg.use_les else 'DNS'} "
                                f"{cfg.nx}x{cfg.ny}x{cfg.nz} (x,y,z), "
                                f"{'FP64' if not mixed else 'mixed FP32 transforms / FP64 solves'}, "
                                f"dt {cfg.delta_t}, Re {cfg.re}",
-                   "nx": cfg.nx, "ny": cfg.ny, "nz": cfg.nz, "precision": args.precision},
+                   "nx": cfg.nx, "ny": cfg.ny, "nz": cfg.nz, "precision": args.precision,
+                   **({"closure": closure_src} if learned else {})},
         "roofline": roof,
         "step_hbm_frac_model_v1": round(step_model / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "kernels": per_kernel,

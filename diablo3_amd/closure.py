@@ -10,6 +10,7 @@ There is no CPU fallback: without the HIP library construction raises.
 from __future__ import annotations
 
 import ctypes
+import os
 from typing import Optional, Sequence
 
 import numpy as np
@@ -51,6 +52,25 @@ def make_norm(X: np.ndarray, envelope_quantile: float = 0.0) -> np.ndarray:
     n[32:32 + NFEAT] = lo
     n[48:48 + NFEAT] = hi
     return n
+
+
+DEFAULT_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "closure_front.npz")
+
+
+def load_params(path: Optional[str] = None):
+    """(params, norm, source) from a .npz written by train_closure, the committed
+    front closure, or (neither present) a seeded network with an unbounded
+    envelope."""
+    path = path or (DEFAULT_FILE if os.path.exists(DEFAULT_FILE) else None)
+    if path:
+        z = np.load(path, allow_pickle=False)
+        return z["params"].astype(np.float32), z["norm"].astype(np.float32), os.path.basename(path)
+    m = Mlp(seed=0)
+    n = np.zeros(NNORM, dtype=np.float32)
+    n[16:16 + NFEAT] = 1.0
+    n[32:48] = -np.inf
+    n[48:64] = np.inf
+    return m.params, n, "seeded network (untrained)"
 
 
 class Mlp:

@@ -4,15 +4,17 @@
 // network's prediction, and keeps the AMD coefficient wherever the point's
 // features leave the training envelope.
 //
-// k_closure: per (j, x, z) point, the features at GY j and at GYF j from the
-// same interpolated scaled-gradient entries the type-5 coefficients use
-// (coef_gy, coef_gyf) and the type-5 scalar gradients (amd_th_gy, amd_th_gyf);
-// each wave stages its 64 points x 2 locations in LDS and evaluates them as 8
-// MFMA tiles of 16 samples.  It writes the velocity and scalar coefficients at
-// GY and GYF (1/time; nu_t = C^2 D2 coefficient), which k_amd<..., 6> turns
-// into nu_t, kappa_t and the SGS stresses and fluxes exactly as type 5 does.
-// In feature mode it writes the features of the GYF points of rows jf0..Nyp-1
-// instead (training data, dgpu_les_features).
+// k_closure: per (j, x, z) point, the features at GY j from the same
+// interpolated scaled-gradient entries coef_gy gives amd_coef and the type-5
+// scalar gradient of amd_th_gy; each wave stages its 64 points in LDS and
+// evaluates them as 4 MFMA tiles of 16 samples.  It writes the velocity and
+// scalar coefficients at GY (1/time; nu_t = C^2 D2 coefficient) with the sign
+// bit marking a fallback point (the AMD value, stored negated).  k_amd<..., 6>
+// takes |coefficient| at GY, and at GYF j the mean of GY j and j+1 when both
+// are network points, else the type-5 GYF coefficient, so that a handle with
+// no network (or an empty envelope) steps exactly as type 5.
+// Feature mode (dgpu_les_features) evaluates the features at the GYF points of
+// rows jf0..Nyp-1 instead (the training data).
 #pragma once
 #include "dgpu_les.hpp"
 #include "dgpu_mlp.hpp"
@@ -24,34 +26,35 @@ struct ClosureArgs {
   MlpW w;
   int have_net;   // 0: no network set (every point takes the AMD coefficient)
   double nu, gyb; // molecular viscosity, grav_y
-  float* cgyf;    // [ny2][pp] outputs (step mode)
-  float* cgy;
-  float* kgyf;
-  float* kgy;
+  void* cgy;      // [ny2][pp] outputs at GY (step mode), precision T; sign bit set
+  void* kgy;      // where the point took the AMD value
   int feat, jf0;  // feature mode and its first row
   float* X;       // [(ny - jf0) pp][16]
   float* scale;   // C^2 D2 sqrt(2 s2)
-  float* amd;     // type-5 coefficient / sqrt(2 s2)
+  float* amd;     // [n][2]: type-5 velocity and scalar coefficients / sqrt(2 s2)
   int* inside;
 };
 
 // the 13 features (raw) of one point from the scaled strain (a1..a6), rotation
 // (o4..o6), the scaled scalar gradient b, the unscaled th_y and |grad th|, the
-// filter width D2 and the y width delta_y; returns s2 = S:S
-__device__ __forceinline__ double closure_features(double a1, double a2, double a3, double a4, double a5, double a6,
-                                                   double o4, double o5, double o6, double bx, double by, double bz,
-                                                   double ty, double gth, double D2, double dyl, double dxs,
-                                                   double dzs, double nu, double gyb, float* f) {
-  const double s2 = a1 * a1 + a2 * a2 + a3 * a3 + 2.0 * (a4 * a4 + a5 * a5 + a6 * a6);
-  const double w2 = 2.0 * (o4 * o4 + o5 * o5 + o6 * o6);
-  const double e = s2 + w2;
+// filter width D2 and the y width delta_y, in float; returns s2 = S:S
+__device__ __forceinline__ float closure_features(float a1, float a2, float a3, float a4, float a5, float a6,
+                                                  float o4, float o5, float o6, float bx, float by, float bz,
+                                                  float ty, float gth, float D2, float dyl, float dxs, float dzs,
+                                                  float nu, float gyb, float* f) {
+  const float s2 = a1 * a1 + a2 * a2 + a3 * a3 + 2.f * (a4 * a4 + a5 * a5 + a6 * a6);
+  const float w2 = 2.f * (o4 * o4 + o5 * o5 + o6 * o6);
+  const float e = s2 + w2;
   // S = [[a1 a4 a5][a4 a2 a6][a5 a6 a3]], O = [[0 o4 o5][-o4 0 o6][-o5 -o6 0]], P = O O^T
-  const double S[3][3] = {{a1, a4, a5}, {a4, a2, a6}, {a5, a6, a3}};
-  const double O[3][3] = {{0.0, o4, o5}, {-o4, 0.0, o6}, {-o5, -o6, 0.0}};
-  double S2[3][3], P[3][3];
+  const float S[3][3] = {{a1, a4, a5}, {a4, a2, a6}, {a5, a6, a3}};
+  const float O[3][3] = {{0.f, o4, o5}, {-o4, 0.f, o6}, {-o5, -o6, 0.f}};
+  float S2[3][3], P[3][3];
+#pragma unroll
   for (int i = 0; i < 3; ++i)
+#pragma unroll
     for (int k = 0; k < 3; ++k) {
-      double s = 0.0, p = 0.0;
+      float s = 0.f, p = 0.f;
+#pragma unroll
       for (int m = 0; m < 3; ++m) {
         s += S[i][m] * S[m][k];
         p += O[i][m] * O[k][m];
@@ -59,37 +62,36 @@ __device__ __forceinline__ double closure_features(double a1, double a2, double 
       S2[i][k] = s;
       P[i][k] = p;
     }
-  double trS3 = 0.0, trSP = 0.0, trS2P = 0.0;
+  float trS3 = 0.f, trSP = 0.f, trS2P = 0.f;
+  const float b[3] = {bx, by, bz};
+  const float b2 = bx * bx + by * by + bz * bz;
+  float bSb = 0.f, bS2b = 0.f, bPb = 0.f;
+#pragma unroll
   for (int i = 0; i < 3; ++i)
+#pragma unroll
     for (int k = 0; k < 3; ++k) {
       trS3 += S2[i][k] * S[k][i];
       trSP += S[i][k] * P[k][i];
       trS2P += S2[i][k] * P[k][i];
-    }
-  const double b[3] = {bx, by, bz};
-  const double b2 = bx * bx + by * by + bz * bz;
-  double bSb = 0.0, bS2b = 0.0, bPb = 0.0;
-  for (int i = 0; i < 3; ++i)
-    for (int k = 0; k < 3; ++k) {
       bSb += b[i] * S[i][k] * b[k];
       bS2b += b[i] * S2[i][k] * b[k];
       bPb += b[i] * P[i][k] * b[k];
     }
-  const double rs = sqrt(s2);
-  auto q = [](double n, double d) { return d > 0.0 ? n / d : 0.0; };
-  f[0] = (float)q(w2 - s2, e);
-  f[1] = (float)q(trS3, s2 * rs);
-  f[2] = (float)q(trSP, rs * w2);
-  f[3] = (float)q(-(trS3 + trSP), rs * e);
-  f[4] = (float)q(trS2P, s2 * w2);
-  f[5] = (float)q(by * by, b2);
-  f[6] = (float)q(bSb, b2 * rs);
-  f[7] = (float)q(bS2b, b2 * s2);
-  f[8] = (float)q(bPb, b2 * w2);
-  f[9] = (float)tanh(q(gyb * ty, s2));
-  f[10] = (float)tanh(q(fabs(gyb) * gth, s2));
-  f[11] = (float)log1p(D2 * sqrt(2.0 * s2) / nu);
-  f[12] = (float)log(dyl * dyl / (dxs * dzs));
+  const float rs = sqrtf(s2);
+  auto q = [](float n, float d) { return d > 0.f ? n / d : 0.f; };
+  f[0] = q(w2 - s2, e);
+  f[1] = q(trS3, s2 * rs);
+  f[2] = q(trSP, rs * w2);
+  f[3] = q(-(trS3 + trSP), rs * e);
+  f[4] = q(trS2P, s2 * w2);
+  f[5] = q(by * by, b2);
+  f[6] = q(bSb, b2 * rs);
+  f[7] = q(bS2b, b2 * s2);
+  f[8] = q(bPb, b2 * w2);
+  f[9] = tanhf(q(gyb * ty, s2));
+  f[10] = tanhf(q(fabsf(gyb) * gth, s2));
+  f[11] = log1pf(D2 * sqrtf(2.f * s2) / nu);
+  f[12] = logf(dyl * dyl / (dxs * dzs));
   f[13] = f[14] = f[15] = 0.f;
   return s2;
 }
@@ -101,10 +103,24 @@ __device__ __forceinline__ bool closure_inside(const float* norm, const float* f
   return in;
 }
 
+// the entries coef_gy passes to amd_coef at GY j
+template <typename T>
+struct GyEntries {
+  double a1, a2, a3, a4, a5, a6, o4, o5, o6;
+};
+template <typename T>
+__device__ __forceinline__ GyEntries<T> gy_entries(const AmdIn<T>& in, const AmdArgs& a, int j, long long q) {
+  const Gyf f = gyf_plane(in, a, j, q), fm = gyf_plane(in, a, j - 1, q);
+  const Gy p = gy_plane(in, a, j, q);
+  const double wc = a.dyf[j - 1], wm = a.dyf[j], den = 2.0 * a.dy[j];
+  auto w = [&](double x, double y) { return (x * wc + y * wm) / den; };
+  return {w(f.a1, fm.a1), w(f.a2, fm.a2), w(f.a3, fm.a3), p.a4, w(f.a5, fm.a5), p.a6, p.o4, w(f.o5, fm.o5), p.o6};
+}
+
 template <typename T, int NTH>
 __global__ __launch_bounds__(256) void k_closure(ClosureArgs c) {
-  __shared__ float sF[4][128][MLP_INP + 1];  // per wave: 64 GY + 64 GYF samples
-  __shared__ float sY[4][128][2];
+  __shared__ float sF[4][64][MLP_INP + 1];  // per wave: 64 samples
+  __shared__ float sY[4][64][2];
   const AmdArgs& a = c.a;
   const Geo& g = a.g;
   const long long q0 = (long long)blockIdx.x * 256 + threadIdx.x;
@@ -116,68 +132,46 @@ __global__ __launch_bounds__(256) void k_closure(ClosureArgs c) {
   in.pp = g.pp;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const double C2 = C_AMD * C_AMD;
-  // GYF rows: the velocity stresses jstart..Nyp-1, the scalar fluxes jstart_th..jend_th
-  int jg0 = a.jstart, jg1 = ny - 1;
-  for (int n = 0; n < NTH; ++n) {
-    jg0 = a.jstart_th[n] < jg0 ? a.jstart_th[n] : jg0;
-    jg1 = a.jend_th[n] > jg1 ? a.jend_th[n] : jg1;
-  }
-  const bool gy_on = live && !c.feat && j >= 2 && j <= ny;
-  const bool gyf_on = live && (c.feat ? (j >= c.jf0 && j <= ny - 1) : (j >= jg0 && j <= jg1));
-  float fa[MLP_INP], fb[MLP_INP];
-  double s2a = 0.0, s2b = 0.0, amda = 0.0, amdb = 0.0, ka = 0.0, kb = 0.0;
+  // step mode: GY rows 2..Nyp; feature mode: GYF rows jf0..Nyp-1
+  const bool on = live && (c.feat ? (j >= c.jf0 && j <= ny - 1) : (j >= 2 && j <= ny));
+  float fe[MLP_INP];
 #pragma unroll
-  for (int k = 0; k < MLP_INP; ++k) fa[k] = fb[k] = 0.f;
-  if (gy_on) {
-    // the entries coef_gy passes to amd_coef
-    const Gyf f = gyf_plane(in, a, j, q), fm = gyf_plane(in, a, j - 1, q);
-    const Gy p = gy_plane(in, a, j, q);
-    const double wc = a.dyf[j - 1], wm = a.dyf[j], den = 2.0 * a.dy[j];
-    auto w = [&](double x, double y) { return (x * wc + y * wm) / den; };
-    const double a1 = w(f.a1, fm.a1), a2 = w(f.a2, fm.a2), a3 = w(f.a3, fm.a3), a5 = w(f.a5, fm.a5),
-                 o5 = w(f.o5, fm.o5);
-    amda = amd_coef(a1, a2, a3, p.a4, a5, p.a6, p.o4, o5, p.o6);
-    double bx = 0.0, by = 0.0, bz = 0.0, ty = 0.0, gth = 0.0;
-    if constexpr (NTH > 0) {
-      const ThCoef t = amd_th_gy(in, a, 9, j, q);
-      bx = t.bx; by = t.by; bz = t.bz; ty = t.ty;
-      ka = t.c;
-      const double tx = bx / a.dxs, tz = bz / a.dzs;
-      gth = sqrt(tx * tx + ty * ty + tz * tz);
+  for (int k = 0; k < MLP_INP; ++k) fe[k] = 0.f;
+  double s2 = 0.0, amd_v = 0.0, amd_t = 0.0;
+  if (on) {
+    double a1, a2, a3, a4, a5, a6, o4, o5, o6, dyl, D2;
+    ThCoef t{0.0, 0.0, 0.0, 0.0, 0.0};
+    if (c.feat) {
+      // the entries coef_gyf passes to amd_coef at GYF j
+      const Gyf f = gyf_plane(in, a, j, q);
+      const Gy p = gy_plane(in, a, j + 1, q), m = gy_plane(in, a, j, q);
+      a1 = f.a1; a2 = f.a2; a3 = f.a3; a5 = f.a5; o5 = f.o5;
+      a4 = 0.5 * (p.a4 + m.a4); a6 = 0.5 * (p.a6 + m.a6); o4 = 0.5 * (p.o4 + m.o4); o6 = 0.5 * (p.o6 + m.o6);
+      if constexpr (NTH > 0) t = amd_th_gyf(in, a, 9, j, q);
+      dyl = a.dyf[j] * 2.0;
+      D2 = -a.dlt[j] / (2.0 * C2);
+    } else {
+      const GyEntries<T> E = gy_entries(in, a, j, q);
+      a1 = E.a1; a2 = E.a2; a3 = E.a3; a4 = E.a4; a5 = E.a5; a6 = E.a6; o4 = E.o4; o5 = E.o5; o6 = E.o6;
+      if constexpr (NTH > 0) t = amd_th_gy(in, a, 9, j, q);
+      dyl = a.dy[j] * 2.0;
+      D2 = -a.dlt[g.ny2 + j] / (2.0 * C2);
     }
-    const double D2 = -a.dlt[g.ny2 + j] / (2.0 * C2);
-    s2a = closure_features(a1, a2, a3, p.a4, a5, p.a6, p.o4, o5, p.o6, bx, by, bz, ty, gth, D2, a.dy[j] * 2.0,
-                           a.dxs, a.dzs, c.nu, c.gyb, fa);
+    amd_v = amd_coef(a1, a2, a3, a4, a5, a6, o4, o5, o6);
+    amd_t = t.c;
+    const double tx = t.bx / a.dxs, tz = t.bz / a.dzs;
+    const double gth = sqrt(tx * tx + t.ty * t.ty + tz * tz);
+    s2 = closure_features((float)a1, (float)a2, (float)a3, (float)a4, (float)a5, (float)a6, (float)o4, (float)o5,
+                          (float)o6, (float)t.bx, (float)t.by, (float)t.bz, (float)t.ty, (float)gth, (float)D2,
+                          (float)dyl, (float)a.dxs, (float)a.dzs, (float)c.nu, (float)c.gyb, fe);
   }
-  if (gyf_on) {
-    // the entries coef_gyf passes to amd_coef
-    const Gyf f = gyf_plane(in, a, j, q);
-    const Gy p = gy_plane(in, a, j + 1, q), m = gy_plane(in, a, j, q);
-    const double a4 = 0.5 * (p.a4 + m.a4), a6 = 0.5 * (p.a6 + m.a6), o4 = 0.5 * (p.o4 + m.o4),
-                 o6 = 0.5 * (p.o6 + m.o6);
-    amdb = amd_coef(f.a1, f.a2, f.a3, a4, f.a5, a6, o4, f.o5, o6);
-    double bx = 0.0, by = 0.0, bz = 0.0, ty = 0.0, gth = 0.0;
-    if constexpr (NTH > 0) {
-      const ThCoef t = amd_th_gyf(in, a, 9, j, q);
-      bx = t.bx; by = t.by; bz = t.bz; ty = t.ty;
-      kb = t.c;
-      const double tx = bx / a.dxs, tz = bz / a.dzs;
-      gth = sqrt(tx * tx + ty * ty + tz * tz);
-    }
-    const double D2 = -a.dlt[j] / (2.0 * C2);
-    s2b = closure_features(f.a1, f.a2, f.a3, a4, f.a5, a6, o4, f.o5, o6, bx, by, bz, ty, gth, D2, a.dyf[j] * 2.0,
-                           a.dxs, a.dzs, c.nu, c.gyb, fb);
-  }
-  // network: 8 tiles of 16 samples per wave (GY samples 0..63, GYF 64..127)
+  // network: 4 tiles of 16 samples per wave
 #pragma unroll
-  for (int k = 0; k < MLP_INP; ++k) {
-    sF[wave][lane][k] = fa[k];
-    sF[wave][64 + lane][k] = fb[k];
-  }
+  for (int k = 0; k < MLP_INP; ++k) sF[wave][lane][k] = fe[k];
   __syncthreads();
-  if (c.have_net) {
+  if (c.have_net && !c.feat) {
     const int r = lane & 15, gq = lane >> 4;
-    for (int t = 0; t < 8; ++t) {
+    for (int t = 0; t < 4; ++t) {
       const float* row = &sF[wave][16 * t + r][4 * gq];
       bf16x4 xb;
 #pragma unroll
@@ -194,28 +188,28 @@ __global__ __launch_bounds__(256) void k_closure(ClosureArgs c) {
     }
   }
   __syncthreads();
-  const bool ina = c.have_net && s2a > 0.0 && closure_inside(c.w.norm, fa);
-  const bool inb = c.have_net && s2b > 0.0 && closure_inside(c.w.norm, fb);
-  const long long o = (long long)j * g.pp + q;
+  const bool inside = c.have_net && s2 > 0.0 && closure_inside(c.w.norm, fe);
   if (c.feat) {
-    if (gyf_on) {
+    if (on) {
       const long long s = (long long)(j - c.jf0) * g.pp + q;
 #pragma unroll
-      for (int k = 0; k < MLP_INP; ++k) c.X[s * MLP_INP + k] = fb[k];
-      const double r2 = sqrt(2.0 * s2b);
+      for (int k = 0; k < MLP_INP; ++k) c.X[s * MLP_INP + k] = fe[k];
+      const double r2 = sqrt(2.0 * s2);
       c.scale[s] = (float)(-0.5 * a.dlt[j] * r2);
-      c.amd[s] = (float)(r2 > 0.0 ? amdb / r2 : 0.0);
-      c.inside[s] = c.have_net ? (int)inb : 1;
+      c.amd[2 * s] = (float)(r2 > 0.0 ? amd_v / r2 : 0.0);
+      c.amd[2 * s + 1] = (float)(r2 > 0.0 ? amd_t / r2 : 0.0);
+      c.inside[s] = c.have_net ? (int)inside : 1;
     }
     return;
   }
   if (!live) return;
-  // coefficients (1/time): the network's sqrt(2 s2) max(0, y), else AMD's
-  const double ra = sqrt(2.0 * s2a), rb = sqrt(2.0 * s2b);
-  c.cgy[o] = (float)(ina ? ra * fmax(0.0, (double)sY[wave][lane][0]) : amda);
-  c.kgy[o] = (float)(ina ? ra * fmax(0.0, (double)sY[wave][lane][1]) : ka);
-  c.cgyf[o] = (float)(inb ? rb * fmax(0.0, (double)sY[wave][64 + lane][0]) : amdb);
-  c.kgyf[o] = (float)(inb ? rb * fmax(0.0, (double)sY[wave][64 + lane][1]) : kb);
+  // coefficients (1/time) at GY: the network's sqrt(2 s2) max(0, y), or AMD's stored negated
+  const long long o = (long long)j * g.pp + q;
+  const double r2 = sqrt(2.0 * s2);
+  const double cv = inside ? r2 * fmax(0.0, (double)sY[wave][lane][0]) : -amd_v;
+  const double ct = inside ? r2 * fmax(0.0, (double)sY[wave][lane][1]) : -amd_t;
+  ((T*)c.cgy)[o] = inside ? (T)cv : (T)copysign(fabs(cv), -1.0);
+  ((T*)c.kgy)[o] = inside ? (T)ct : (T)copysign(fabs(ct), -1.0);
 }
 
 }  // namespace dgpu

@@ -68,7 +68,8 @@ __global__ __launch_bounds__(NT) void k_inv_z(Ptrs<const cd*, MAXO> src, Ptrs<cx
   }
 }
 
-template <typename T, int NZ, int TI, int NT>
+// ACC: add the transform into the destination (an increment of the state)
+template <typename T, int NZ, int TI, int NT, bool ACC = false>
 __global__ __launch_bounds__(NT) void k_fwd_z(Ptrs<const cx<T>*, MAXO> src, Ptrs<cd*, MAXO> dst,
                                               Geo g, int j0, const cx<T>* __restrict__ tw) {
   constexpr int LD = NZ + 1;
@@ -87,7 +88,8 @@ __global__ __launch_bounds__(NT) void k_fwd_z(Ptrs<const cx<T>*, MAXO> src, Ptrs
     const int ii = idx % TI, p = idx / TI, i = i0 + ii;
     if (i < g.kxr) {
       cd v = cvt<double>(s[ii * LD + swz<T>(zpos(g, p))]);
-      C[(long long)p * g.kxp + i] = rnz * v;
+      if constexpr (ACC) C[(long long)p * g.kxp + i] = C[(long long)p * g.kxp + i] + rnz * v;
+      else C[(long long)p * g.kxp + i] = rnz * v;
     }
   }
 }
@@ -454,6 +456,7 @@ enum RowCoef {
 struct ColArgs {
   Geo g;
   int rk, do_cn, plevels;
+  int cr_minus_u;   // do_cn == 0: write the right-hand side minus the state (increment solves)
   int jstart, jend, jstart_th[MAXTH], jend_th[MAXTH];
   int front, f_type;
   double nu, beta, nu_v_scale, ro_inv, delta, grav_x, grav_y, grav_z, ly, px0, omega0, amp_omega0, time;
@@ -643,9 +646,14 @@ __global__ __launch_bounds__(NT) void k_cn(ColArgs a) {
       crv[e] = v;
     }
     if (!a.do_cn) {
-      if (active)
+      if (active) {
 #pragma unroll
-        for (int e = 0; e < NF; ++e) a.crout[e][o] = crv[e];
+        for (int e = 0; e < NF; ++e) {
+          cd v = crv[e];
+          if (a.cr_minus_u && inner && j >= a.rows[e].blo && j <= a.rows[e].bhi) v = v - a.u[e][o];
+          a.crout[e][o] = v;
+        }
+      }
       continue;
     }
 #pragma unroll

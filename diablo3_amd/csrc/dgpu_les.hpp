@@ -140,11 +140,10 @@ struct AmdArgs {
   void* nut;
   void* kap[MAXTH];
   void* aug[MAXA];
-  // les_model_type 6: the learned closure's coefficients, per point (k_closure)
-  const float* cgyf;  // velocity, at GYF j
-  const float* cgy;   // velocity, at GY j
-  const float* kgyf;  // scalar, at GYF j
-  const float* kgy;   // scalar, at GY j
+  // les_model_type 6: the learned closure's coefficients at GY (k_closure), in
+  // the transform precision T, sign bit set where the point took the AMD value
+  const void* cgy;   // velocity
+  const void* kgy;   // scalar
 };
 
 // the invariant-scaled strain/rotation entries of one GYF plane
@@ -266,7 +265,10 @@ __device__ __forceinline__ double model_gyf(const AmdIn<T>& in, const AmdArgs& a
   if constexpr (MODEL == 5) {
     return coef_gyf(in, a, j, q);
   } else if constexpr (MODEL == 6) {
-    return (double)a.cgyf[(long long)j * in.pp + q];
+    // the mean of the network's GY j and j+1 values, else the type-5 GYF coefficient
+    const double c0 = (double)((const T*)a.cgy)[(long long)j * in.pp + q];
+    const double c1 = (double)((const T*)a.cgy)[(long long)(j + 1) * in.pp + q];
+    return (!signbit(c0) && !signbit(c1)) ? 0.5 * (c0 + c1) : coef_gyf(in, a, j, q);
   } else {
     const Sgyf f = sgyf_plane(in, a, j, q);
     const Sgy p = sgy_plane(in, a, j + 1, q), m = sgy_plane(in, a, j, q);
@@ -286,7 +288,7 @@ __device__ __forceinline__ double model_gy(const AmdIn<T>& in, const AmdArgs& a,
   if constexpr (MODEL == 5) {
     return coef_gy(in, a, j, q);
   } else if constexpr (MODEL == 6) {
-    return (double)a.cgy[(long long)j * in.pp + q];
+    return fabs((double)((const T*)a.cgy)[(long long)j * in.pp + q]);
   } else {
     const Sgyf f = sgyf_plane(in, a, j, q), fm = sgyf_plane(in, a, j - 1, q);
     const Sgy p = sgy_plane(in, a, j, q);
@@ -422,7 +424,7 @@ __global__ __launch_bounds__(256) void k_amd(AmdArgs a) {
     // kappa_t at GY j (for/les.f90:1259-1306, 1344-1348)
     if (j >= 3 && j <= ny - 1) {
       double c;
-      if constexpr (MODEL == 6) c = (double)a.kgy[o];
+      if constexpr (MODEL == 6) c = fabs((double)((const T*)a.kgy)[o]);
       else c = amd_th_gy(in, a, ft, j, q).c;
       kap = -0.5 * a.dlt[g.ny2 + j] * c;
     }
@@ -430,8 +432,12 @@ __global__ __launch_bounds__(256) void k_amd(AmdArgs a) {
     // SGS flux at GYF j (for/les.f90:1221-1256, 1310-1318, 1372-1393)
     if (j >= a.jstart_th[n] && j <= a.jend_th[n]) {
       double s1;
-      if constexpr (MODEL == 6) s1 = (double)a.kgyf[o];
-      else s1 = amd_th_gyf(in, a, ft, j, q).c;
+      if constexpr (MODEL == 6) {
+        const double k0 = (double)((const T*)a.kgy)[o], k1 = (double)((const T*)a.kgy)[o + g.pp];
+        s1 = (!signbit(k0) && !signbit(k1)) ? 0.5 * (k0 + k1) : amd_th_gyf(in, a, ft, j, q).c;
+      } else {
+        s1 = amd_th_gyf(in, a, ft, j, q).c;
+      }
       const double f = 0.5 * a.dlt[j] * s1;
       fx = f * in.at(ft + 1, j, q);
       fz = f * in.at(ft + 2, j, q);
@@ -448,10 +454,23 @@ __global__ __launch_bounds__(256) void k_amd(AmdArgs a) {
 // the Fourier right-hand side, which already holds the molecular part), the
 // matrix gets the implicit half, the BC rows of apply_BC_* are set for every
 // column, and Thomas (thomas_forward/backward_real_mpi at one rank,
-// for/solver.f90:1691-1761, 1816-1852) writes rows [wlo, whi] of the field.
-// One thread per column; c' is kept in scratch, d' overwrites r.
+// for/solver.f90:1691-1761, 1816-1852) solves rows 0..Nyp+1.
+// One thread per (column, field): the fields' systems are independent, so
+// blockIdx.y picks the field.  The thread keeps a register window of u and nu_t
+// over rows j-1..j+1 and issues the loads of rows j+2, j+3 while row j is
+// eliminated (the elimination chain is serial in j; the loads are not).
+// The reciprocal pivot 1/b' and g' go to scratch; the back substitution reads
+// them in reverse with the same look-ahead and recomputes c from nu_t (the
+// same row function, so the same value).
+//
+// INC (the mixed-precision path): u is the state at the substep's start in
+// float and r holds r - u (float).  The solve is then for the increment
+// delta = u_new - u: right-hand side rhs - (l u_{j-1} + d u_j + c u_{j+1}) of the
+// same rows, in FP64, and delta is written back to r (0 outside [wlo, whi]),
+// so the state moves by a float transform of an O(dt) quantity.  Otherwise the
+// solution overwrites u on rows [wlo, whi].
 // ---------------------------------------------------------------------------
-template <typename TV>
+template <typename TU, typename TV>
 struct PhysCnArgs {
   Geo g;
   int nf;
@@ -463,110 +482,131 @@ struct PhysCnArgs {
   double temp2;
   const double* __restrict__ dy;
   const double* __restrict__ dyf;
-  double* u[MAXF];                    // physical field, solution written in place
-  double* r[MAXF];                    // right-hand side; becomes d'
+  TU* u[MAXF];                        // physical field
+  TU* r[MAXF];                        // right-hand side (INC: r - u, then delta)
   const TV* visc[MAXF];               // nu_t (velocity) or kappa_t_n (scalar)
-  double* cp;                         // scratch, 2 * ny2 planes
+  double* cp;                         // scratch, nf * 2 * ny2 planes: 1/b', g'
 };
 
-template <typename TV>
-__global__ __launch_bounds__(256) void k_cn_phys(PhysCnArgs<TV> a) {
+template <typename TU, typename TV, bool INC>
+__global__ __launch_bounds__(256) void k_cn_phys(PhysCnArgs<TU, TV> a) {
   const Geo& g = a.g;
   const long long q = (long long)blockIdx.x * 256 + threadIdx.x;
-  if (q >= g.pp) return;
-  const int ny = g.ny, n2 = g.ny2;
+  const int e = blockIdx.y;
+  if (q >= g.pp || e >= a.nf) return;
+  const int ny = g.ny, n2 = g.ny2, last = ny + 1;
   const long long pp = g.pp;
   const double* __restrict__ dy = a.dy;
   const double* __restrict__ dyf = a.dyf;
-  double* __restrict__ cp = a.cp;
-  for (int e = 0; e < a.nf; ++e) {
-    double* __restrict__ u = a.u[e];
-    double* __restrict__ r = a.r[e];
-    const TV* __restrict__ nt = a.visc[e];
-    const double T = a.tcn[e], t2 = a.temp2;
-    const int fld = a.field[e];
-    const bool v2 = (fld == 1);
-    // row j of the system: (l, d, c | rhs)
-    auto row = [&](int j, double& l, double& d, double& c, double& rhs) {
-      l = 0.0; d = 1.0; c = 0.0; rhs = 0.0;
-      if (j >= a.blo[e] && j <= a.bhi[e]) {
-        const long long o = (long long)j * pp + q;
-        const double um = u[o - pp], uc = u[o], up = u[o + pp];
-        const double nm = (double)nt[o - pp], nc = (double)nt[o], np = (double)nt[o + pp];
-        rhs = r[o];
-        if (!v2) {
-          l = -T / (dy[j] * dyf[j]);
-          // the u1/u3 builds write the diagonal as 1 - T(-a - b) (for/solver.f90:1136-1145)
-          d = (fld >= 3) ? 1. + T / (dy[j + 1] * dyf[j]) + T / (dy[j] * dyf[j])
-                       : 1. - T * (-1. / (dy[j + 1] * dyf[j]) - 1. / (dy[j] * dyf[j]));
-          c = -T / (dy[j + 1] * dyf[j]);
-          rhs = rhs + t2 * ((np * (up - uc) / dy[j + 1] - nc * (uc - um) / dy[j]) / dyf[j]);
-          l = l - t2 * nc / (dy[j] * dyf[j]);
-          d = d + t2 * np / (dy[j + 1] * dyf[j]) + t2 * nc / (dy[j] * dyf[j]);
-          c = c - t2 * np / (dy[j + 1] * dyf[j]);
-        } else {
-          l = -T / (dyf[j - 1] * dy[j]);
-          d = 1. + T / (dyf[j] * dy[j]) + T / (dyf[j - 1] * dy[j]);
-          c = -T / (dyf[j] * dy[j]);
-          rhs = rhs + t2 * ((0.5 * (nc + np) * (up - uc) / dyf[j] - 0.5 * (nc + nm) * (uc - um) / dyf[j - 1]) / dy[j]);
-          l = l - t2 * 0.5 * (nc + nm) / (dyf[j - 1] * dy[j]);
-          d = d + t2 * 0.5 * (nc + np) / (dyf[j] * dy[j]) + t2 * 0.5 * (nc + nm) / (dyf[j - 1] * dy[j]);
-          c = c - t2 * 0.5 * (nc + np) / (dyf[j] * dy[j]);
-        }
-      }
-      // BC rows (apply_BC_* of the reference, for/solver.f90:1901-2232)
+  TU* __restrict__ u = a.u[e];
+  TU* __restrict__ r = a.r[e];
+  const TV* __restrict__ nt = a.visc[e];
+  double* __restrict__ sb = a.cp + (long long)e * 2 * n2 * pp;  // 1/b'
+  double* __restrict__ sg = sb + (long long)n2 * pp;            // g'
+  const double T = a.tcn[e], t2 = a.temp2;
+  const int fld = a.field[e];
+  const bool v2 = (fld == 1);
+  const int blo = a.blo[e], bhi = a.bhi[e], wlo = a.wlo[e], whi = a.whi[e];
+  auto U = [&](int j) -> double { return (j >= 0 && j <= last) ? (double)u[(long long)j * pp + q] : 0.0; };
+  auto V = [&](int j) -> double { return (j >= 0 && j <= last) ? (double)nt[(long long)j * pp + q] : 0.0; };
+  auto Rr = [&](int j) -> double { return (j >= 0 && j <= last) ? (double)r[(long long)j * pp + q] : 0.0; };
+  // row j of the system: (l, d, c | rhs) from the window values
+  auto row = [&](int j, double um, double uc, double up, double nm, double nc, double np, double rv, double& l,
+                 double& d, double& c, double& rhs) {
+    l = 0.0; d = 1.0; c = 0.0; rhs = 0.0;
+    if (j >= blo && j <= bhi) {
+      rhs = INC ? rv + uc : rv;
       if (!v2) {
-        if (j == 0) { l = 0.; d = 1.; c = 0.; rhs = 0.; }
-        if (j == 1) {
-          if (a.lo_kind[e] == DGPU_BC_DIRICHLET) { l = 0.; d = 1.; c = 0.; rhs = a.lo_c1[e]; }
-          else { l = 0.; d = -1.; c = 1.; rhs = dy[2] * a.lo_c1[e]; }
-        }
-        if (j == ny) {
-          if (a.hi_kind[e] == DGPU_BC_DIRICHLET) { l = 0.; d = 1.; c = 0.; rhs = a.hi_c1[e]; }
-          else { l = -1.; d = 1.; c = 0.; rhs = dy[ny] * a.hi_c1[e]; }
-        }
-        if (j == ny + 1) { l = 0.; d = 1.; c = 0.; rhs = 0.; }
+        l = -T / (dy[j] * dyf[j]);
+        // the u1/u3 builds write the diagonal as 1 - T(-a - b) (for/solver.f90:1136-1145)
+        d = (fld >= 3) ? 1. + T / (dy[j + 1] * dyf[j]) + T / (dy[j] * dyf[j])
+                       : 1. - T * (-1. / (dy[j + 1] * dyf[j]) - 1. / (dy[j] * dyf[j]));
+        c = -T / (dy[j + 1] * dyf[j]);
+        rhs = rhs + t2 * ((np * (up - uc) / dy[j + 1] - nc * (uc - um) / dy[j]) / dyf[j]);
+        l = l - t2 * nc / (dy[j] * dyf[j]);
+        d = d + t2 * np / (dy[j + 1] * dyf[j]) + t2 * nc / (dy[j] * dyf[j]);
+        c = c - t2 * np / (dy[j + 1] * dyf[j]);
       } else {
-        if (a.lo_kind[e] == DGPU_BC_DIRICHLET) {
-          if (j == 1 || j == 2) { l = 0.; d = 1.; c = 0.; rhs = a.lo_c1[e]; }
-        } else if (a.lo_kind[e] == DGPU_BC_NEUMANN) {
-          if (j == 1) { l = 0.; d = -1.; c = 1.; rhs = dyf[1] * a.lo_c1[e]; }
-        }
-        if (j == 0) { l = 0.; d = 1.; c = 0.; rhs = 0.; }
-        if (a.hi_kind[e] == DGPU_BC_DIRICHLET) {
-          if (j == ny + 1 || j == ny) { l = 0.; d = 1.; c = 0.; rhs = a.hi_c1[e]; }
-        } else if (a.hi_kind[e] == DGPU_BC_NEUMANN) {
-          if (j == ny + 1) { l = -1.; d = 1.; c = 0.; rhs = dyf[ny] * a.hi_c1[e]; }
-        }
+        l = -T / (dyf[j - 1] * dy[j]);
+        d = 1. + T / (dyf[j] * dy[j]) + T / (dyf[j - 1] * dy[j]);
+        c = -T / (dyf[j] * dy[j]);
+        rhs = rhs + t2 * ((0.5 * (nc + np) * (up - uc) / dyf[j] - 0.5 * (nc + nm) * (uc - um) / dyf[j - 1]) / dy[j]);
+        l = l - t2 * 0.5 * (nc + nm) / (dyf[j - 1] * dy[j]);
+        d = d + t2 * 0.5 * (nc + np) / (dyf[j] * dy[j]) + t2 * 0.5 * (nc + nm) / (dyf[j - 1] * dy[j]);
+        c = c - t2 * 0.5 * (nc + np) / (dyf[j] * dy[j]);
       }
-    };
-    // forward elimination, rows 0..ny+1; c_j and the eliminated b'_j are kept in
-    // the scratch (planes 0..ny+1 and ny2..2*ny2-1), g'_j overwrites r
+    }
+    // BC rows (apply_BC_* of the reference, for/solver.f90:1901-2232)
+    if (!v2) {
+      if (j == 0) { l = 0.; d = 1.; c = 0.; rhs = 0.; }
+      if (j == 1) {
+        if (a.lo_kind[e] == DGPU_BC_DIRICHLET) { l = 0.; d = 1.; c = 0.; rhs = a.lo_c1[e]; }
+        else { l = 0.; d = -1.; c = 1.; rhs = dy[2] * a.lo_c1[e]; }
+      }
+      if (j == ny) {
+        if (a.hi_kind[e] == DGPU_BC_DIRICHLET) { l = 0.; d = 1.; c = 0.; rhs = a.hi_c1[e]; }
+        else { l = -1.; d = 1.; c = 0.; rhs = dy[ny] * a.hi_c1[e]; }
+      }
+      if (j == ny + 1) { l = 0.; d = 1.; c = 0.; rhs = 0.; }
+    } else {
+      if (a.lo_kind[e] == DGPU_BC_DIRICHLET) {
+        if (j == 1 || j == 2) { l = 0.; d = 1.; c = 0.; rhs = a.lo_c1[e]; }
+      } else if (a.lo_kind[e] == DGPU_BC_NEUMANN) {
+        if (j == 1) { l = 0.; d = -1.; c = 1.; rhs = dyf[1] * a.lo_c1[e]; }
+      }
+      if (j == 0) { l = 0.; d = 1.; c = 0.; rhs = 0.; }
+      if (a.hi_kind[e] == DGPU_BC_DIRICHLET) {
+        if (j == ny + 1 || j == ny) { l = 0.; d = 1.; c = 0.; rhs = a.hi_c1[e]; }
+      } else if (a.hi_kind[e] == DGPU_BC_NEUMANN) {
+        if (j == ny + 1) { l = -1.; d = 1.; c = 0.; rhs = dyf[ny] * a.hi_c1[e]; }
+      }
+    }
+    if (INC) rhs = rhs - (l * um + d * uc + c * up);
+  };
+  // forward elimination with a two-row look-ahead on the loads
+  double um = 0.0, uc = U(0), u1 = U(1), u2 = U(2);
+  double nm = 0.0, nc = V(0), n1 = V(1), n2v = V(2);
+  double r0 = Rr(0), r1 = Rr(1);
+  double binv = 0.0, cprev = 0.0, gprev = 0.0;
+  for (int j = 0; j <= last; ++j) {
+    const double u3 = U(j + 3), n3 = V(j + 3), r2 = Rr(j + 2);
     double l, d, c, rhs;
-    row(0, l, d, c, rhs);
-    cp[q] = c;
-    cp[(long long)n2 * pp + q] = d;
-    r[q] = rhs;
-    double bprev = d, cprev = c, gprev = rhs;
-    for (int j = 1; j <= ny + 1; ++j) {
-      row(j, l, d, c, rhs);
-      const double m = -l / bprev;
+    row(j, um, uc, u1, nm, nc, n1, r0, l, d, c, rhs);
+    if (j > 0) {
+      const double m = -l * binv;
       d = d + m * cprev;
       rhs = rhs + m * gprev;
-      const long long o = (long long)j * pp + q;
-      cp[o] = c;
-      cp[(long long)(n2 + j) * pp + q] = d;
-      r[o] = rhs;
-      bprev = d; cprev = c; gprev = rhs;
     }
-    // backward substitution; rows outside [wlo, whi] keep their values
-    double x = gprev / bprev;
-    if (ny + 1 >= a.wlo[e] && ny + 1 <= a.whi[e]) u[(long long)(ny + 1) * pp + q] = x;
-    for (int j = ny; j >= 0; --j) {
-      const long long o = (long long)j * pp + q;
-      x = (r[o] - cp[o] * x) / cp[(long long)(n2 + j) * pp + q];
-      if (j >= a.wlo[e] && j <= a.whi[e]) u[o] = x;
-    }
+    binv = 1.0 / d;
+    const long long o = (long long)j * pp + q;
+    sb[o] = binv;
+    sg[o] = rhs;
+    cprev = c; gprev = rhs;
+    um = uc; uc = u1; u1 = u2; u2 = u3;
+    nm = nc; nc = n1; n1 = n2v; n2v = n3;
+    r0 = r1; r1 = r2;
+  }
+  // back substitution, rows last..0; c of row j from nu_t at j-1, j, j+1
+  auto S = [&](double* p, int j) -> double { return j >= 0 ? p[(long long)j * pp + q] : 0.0; };
+  double x = gprev * binv;
+  double b0 = S(sb, last - 1), g0 = S(sg, last - 1);
+  double b1 = S(sb, last - 2), g1 = S(sg, last - 2);
+  double vp = V(last), vc = V(last - 1), vm = V(last - 2), vmm = V(last - 3);
+  auto put = [&](int j, double v) {
+    const long long o = (long long)j * pp + q;
+    if (INC) r[o] = (TU)((j >= wlo && j <= whi) ? v : 0.0);
+    else if (j >= wlo && j <= whi) u[o] = (TU)v;
+  };
+  put(last, x);
+  for (int j = last - 1; j >= 0; --j) {
+    const double b2 = S(sb, j - 2), g2 = S(sg, j - 2), v3 = V(j - 3);
+    double l, d, c, rhs;
+    row(j, 0.0, 0.0, 0.0, vm, vc, vp, 0.0, l, d, c, rhs);
+    x = (g0 - c * x) * b0;
+    put(j, x);
+    b0 = b1; g0 = g1;
+    b1 = b2; g1 = g2;
+    vp = vc; vc = vm; vm = vmm; vmm = v3;
   }
 }
 

@@ -115,7 +115,7 @@ struct dgpu_handle {
   float* d_netp = nullptr;
   float* d_netnorm = nullptr;
   int have_net = 0;
-  float *Cgyf = nullptr, *Cgy = nullptr, *Kgyf = nullptr, *Kgy = nullptr;
+  void *Cgy = nullptr, *Kgy = nullptr;  // coefficients at GY, precision T
   // variable_dt: courant's CFL limit, reduced per block on the device at rk_step 3
   int variable_dt = 0, update_dt = 1, cfl_ran = 0;
   double cfl = 0.5;
@@ -444,7 +444,7 @@ int launch_inv_z(dgpu_handle* h, int nf, const cd* const* src, void* const* dst,
   return 0;
 }
 
-template <typename T>
+template <typename T, bool ACC = false>
 int launch_fwd_z(dgpu_handle* h, int nf, void* const* src, cd* const* dst, int j0, int nj) {
   Ptrs<const cx<T>*, MAXO> s{};
   Ptrs<cd*, MAXO> d{};
@@ -453,7 +453,7 @@ int launch_fwd_z(dgpu_handle* h, int nf, void* const* src, cd* const* dst, int j
   DG_SWITCH_N(h->g.nz, {
     constexpr int TI = ti_for(N_, sizeof(cx<T>));
     dim3 grid((h->g.kxr + TI - 1) / TI, nj, nf);
-    k_fwd_z<T, N_, TI, 256><<<grid, 256, 0, h->st>>>(s, d, h->g, j0, tw);
+    k_fwd_z<T, N_, TI, 256, ACC><<<grid, 256, 0, h->st>>>(s, d, h->g, j0, tw);
   });
   HIPCHK(hipGetLastError());
   return 0;
@@ -879,7 +879,7 @@ ClosureArgs closure_args(dgpu_handle* h, const AmdArgs& aa) {
   k.have_net = h->have_net;
   k.nu = h->c.nu;
   k.gyb = h->c.grav_y;
-  k.cgyf = h->Cgyf; k.cgy = h->Cgy; k.kgyf = h->Kgyf; k.kgy = h->Kgy;
+  k.cgy = h->Cgy; k.kgy = h->Kgy;
   return k;
 }
 
@@ -945,7 +945,7 @@ int substep_les(dgpu_handle* h, int rk, double delta_t, double time, int time_st
       k_closure<T, NTH><<<agrid, 256, 0, h->st>>>(ka);
       HIPCHK(hipGetLastError());
       if ((rc = tend(h))) return rc;
-      aa.cgyf = h->Cgyf; aa.cgy = h->Cgy; aa.kgyf = h->Kgyf; aa.kgy = h->Kgy;
+      aa.cgy = h->Cgy; aa.kgy = h->Kgy;
     }
   }
   if ((rc = tbeg(h, 7))) return rc;
@@ -986,31 +986,45 @@ int substep_les(dgpu_handle* h, int rk, double delta_t, double time, int time_st
   for (int n = 0; n < NTH; ++n) ca.t1pr[n] = temp1 / c.pr[n] * c.nu_v_scale;
   ca.thc = h->d_fac + (size_t)r * MAXF * 5 * g.ny2;
   ca.chunk = cn_chunk(h);
+  // the mixed handle solves for the increment of the state in float transforms,
+  // except where courant needs the new velocity in FP64 (rk 3 of an update step)
+  const bool courant_now = h->variable_dt && rk == 3 && time_step % h->update_dt == 0;
+  const bool inc = mixed && !courant_now;
+  ca.cr_minus_u = inc ? 1 : 0;
   for (int f = 0; f < nf; ++f) ca.crout[f] = h->CR[f];
   if ((rc = tbeg(h, 3))) return rc;
   if ((rc = launch_cn<T, NTH>(h, ca))) return rc;
   if ((rc = tend(h))) return rc;
 
-  // 6. right-hand sides to physical space (FP64), and on a mixed handle the FP64
-  //    velocity and scalars the physical-space solve starts from
+  // 6. right-hand sides to physical space: FP64, and on a mixed handle without
+  //    increments also the FP64 state the solve starts from; with increments the
+  //    float transform of r - u (the float state is the gradient stage's)
   const cd* cs[MAXF];
   double* rs[MAXF];
   for (int f = 0; f < nf; ++f) { cs[f] = h->CR[f]; rs[f] = h->Rp[f]; }
-  if ((rc = tbeg(h, 9))) return rc;
-  if ((rc = launch_inv_z<double>(h, nf, cs, h->HinD, 0, g.ny2))) return rc;
-  if ((rc = launch_inv_x<double>(h, nf, h->HinD, rs, 0, g.ny2))) return rc;
   double* ucn[MAXF];
   for (int f = 0; f < nf; ++f) ucn[f] = (double*)h->PgD[3 * f];
-  if (mixed) {
-    const cd* us0[MAXF];
-    for (int f = 0; f < nf; ++f) us0[f] = h->U[f];
-    if ((rc = launch_inv_z<double>(h, nf, us0, h->HinD, 0, g.ny2))) return rc;
-    if ((rc = launch_inv_x<double>(h, nf, h->HinD, ucn, 0, g.ny2))) return rc;
+  float* rinc[MAXF];
+  for (int f = 0; f < nf; ++f) rinc[f] = (float*)h->Rp[f];
+  if ((rc = tbeg(h, 9))) return rc;
+  if (inc) {
+    if ((rc = launch_inv_z<float>(h, nf, cs, h->Hin, 0, g.ny2))) return rc;
+    if ((rc = launch_inv_x<float>(h, nf, h->Hin, rinc, 0, g.ny2))) return rc;
+  } else {
+    if ((rc = launch_inv_z<double>(h, nf, cs, h->HinD, 0, g.ny2))) return rc;
+    if ((rc = launch_inv_x<double>(h, nf, h->HinD, rs, 0, g.ny2))) return rc;
+    if (mixed) {
+      const cd* us0[MAXF];
+      for (int f = 0; f < nf; ++f) us0[f] = h->U[f];
+      if ((rc = launch_inv_z<double>(h, nf, us0, h->HinD, 0, g.ny2))) return rc;
+      if ((rc = launch_inv_x<double>(h, nf, h->HinD, ucn, 0, g.ny2))) return rc;
+    }
   }
   if ((rc = tend(h))) return rc;
 
-  // 7. per-column CN with nu_t / kappa_t, scalars first (for/solver.f90:1004-1007)
-  PhysCnArgs<T> pa{};
+  // 7. per-column CN with nu_t / kappa_t (the field systems are independent; the
+  //    reference solves the scalars first, for/solver.f90:1004-1007)
+  PhysCnArgs<double, T> pa{};
   pa.g = g; pa.nf = nf; pa.temp2 = temp2; pa.dy = h->d_dy; pa.dyf = h->d_dyf; pa.cp = h->CPs;
   for (int e0 = 0; e0 < nf; ++e0) {
     const int e = e0 < NTH ? 3 + e0 : e0 - NTH;  // slot e0 solves field e
@@ -1031,21 +1045,42 @@ int substep_les(dgpu_handle* h, int rk, double delta_t, double time, int time_st
     pa.tcn[e0] = e < 3 ? temp1 * c.nu_v_scale : temp1 / c.pr[e - 3] * c.nu_v_scale;
   }
   if ((rc = tbeg(h, 8))) return rc;
-  k_cn_phys<T><<<(unsigned)((g.pp + 255) / 256), 256, 0, h->st>>>(pa);
+  const dim3 pgrid((unsigned)((g.pp + 255) / 256), nf);
+  if (inc) {
+    PhysCnArgs<float, T> pf{};
+    pf.g = pa.g; pf.nf = pa.nf; pf.temp2 = pa.temp2; pf.dy = pa.dy; pf.dyf = pa.dyf; pf.cp = pa.cp;
+    for (int e0 = 0; e0 < nf; ++e0) {
+      pf.field[e0] = pa.field[e0]; pf.blo[e0] = pa.blo[e0]; pf.bhi[e0] = pa.bhi[e0];
+      pf.wlo[e0] = pa.wlo[e0]; pf.whi[e0] = pa.whi[e0]; pf.lo_kind[e0] = pa.lo_kind[e0];
+      pf.hi_kind[e0] = pa.hi_kind[e0]; pf.lo_c1[e0] = pa.lo_c1[e0]; pf.hi_c1[e0] = pa.hi_c1[e0];
+      pf.tcn[e0] = pa.tcn[e0]; pf.visc[e0] = pa.visc[e0];
+      // slot e0 holds field pa.field[e0]
+      pf.u[e0] = (float*)h->Pg[3 * pa.field[e0]];
+      pf.r[e0] = rinc[pa.field[e0]];
+    }
+    k_cn_phys<float, T, true><<<pgrid, 256, 0, h->st>>>(pf);
+  } else {
+    k_cn_phys<double, T, false><<<pgrid, 256, 0, h->st>>>(pa);
+  }
   HIPCHK(hipGetLastError());
   if ((rc = tend(h))) return rc;
   // courant on the new velocity, before the projection (for/solver.f90:1241-1244)
-  if (h->variable_dt && rk == 3 && time_step % h->update_dt == 0)
+  if (courant_now)
     if ((rc = launch_courant(h, ucn[0], ucn[1], ucn[2]))) return rc;
 
-  // 8. solved fields back to Fourier space (every plane, as the reference), then
-  //    rem_div_chan and the pressure update
+  // 8. solved fields (or increments) back to Fourier space on every plane, as the
+  //    reference, then rem_div_chan and the pressure update
   double* us[MAXF];
   cd* ud[MAXF];
   for (int f = 0; f < nf; ++f) { us[f] = ucn[f]; ud[f] = h->U[f]; }
   if ((rc = tbeg(h, 9))) return rc;
-  if ((rc = launch_fwd_x<double>(h, nf, us, h->HinD, 0, g.ny2))) return rc;
-  if ((rc = launch_fwd_z<double>(h, nf, h->HinD, ud, 0, g.ny2))) return rc;
+  if (inc) {
+    if ((rc = launch_fwd_x<float>(h, nf, rinc, h->Hin, 0, g.ny2))) return rc;
+    if ((rc = launch_fwd_z<float, true>(h, nf, h->Hin, ud, 0, g.ny2))) return rc;
+  } else {
+    if ((rc = launch_fwd_x<double>(h, nf, us, h->HinD, 0, g.ny2))) return rc;
+    if ((rc = launch_fwd_z<double>(h, nf, h->HinD, ud, 0, g.ny2))) return rc;
+  }
   if ((rc = tend(h))) return rc;
   if ((rc = tbeg(h, 4))) return rc;
   if ((rc = launch_project(h, ca, 1))) return rc;
@@ -1206,14 +1241,11 @@ int dgpu_create(const dgpu_config* cfg, dgpu_handle** out) {
     ALLOC(h->NuT, tbytes);
     for (int n = 0; n < h->nth; ++n) ALLOC(h->KapT[n], tbytes);
     for (int m = 0; m < 7 + 2 * h->nth; ++m) ALLOC(h->Aug[m], tbytes);
-    ALLOC(h->CPs, 2 * rbytes);
+    ALLOC(h->CPs, 2 * (size_t)nf * rbytes);
     ALLOC(h->d_dlt, sizeof(double) * 2 * n2);
     if (cfg->les_model_type == 6) {
-      const size_t fbytes = sizeof(float) * g.pp * n2;
-      ALLOC(h->Cgyf, fbytes);
-      ALLOC(h->Cgy, fbytes);
-      ALLOC(h->Kgyf, fbytes);
-      ALLOC(h->Kgy, fbytes);
+      ALLOC(h->Cgy, tbytes);
+      ALLOC(h->Kgy, tbytes);
       ALLOC(h->d_netw, sizeof(unsigned short) * (64 * 16 + 64 * 64 + 16 * 64));
       ALLOC(h->d_netp, sizeof(float) * MLP_NP);
       ALLOC(h->d_netnorm, sizeof(float) * MLP_NNORM);
@@ -1432,7 +1464,7 @@ int dgpu_les_features(dgpu_handle* h, double time, float* X, float* scale, float
   };
   if (hipMalloc((void**)&dX, sizeof(float) * MLP_INP * (size_t)n) != hipSuccess ||
       hipMalloc((void**)&dS, sizeof(float) * (size_t)n) != hipSuccess ||
-      hipMalloc((void**)&dA, sizeof(float) * (size_t)n) != hipSuccess ||
+      hipMalloc((void**)&dA, sizeof(float) * 2 * (size_t)n) != hipSuccess ||
       hipMalloc((void**)&dI, sizeof(int) * (size_t)n) != hipSuccess) {
     cleanup();
     return fail(11, "hipMalloc failed (feature buffers)");
@@ -1462,7 +1494,7 @@ int dgpu_les_features(dgpu_handle* h, double time, float* X, float* scale, float
   ok = hipGetLastError() == hipSuccess;
   if (ok && X) ok = hipMemcpyAsync(X, dX, sizeof(float) * MLP_INP * (size_t)n, hipMemcpyDeviceToHost, h->st) == hipSuccess;
   if (ok && scale) ok = hipMemcpyAsync(scale, dS, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost, h->st) == hipSuccess;
-  if (ok && amd) ok = hipMemcpyAsync(amd, dA, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost, h->st) == hipSuccess;
+  if (ok && amd) ok = hipMemcpyAsync(amd, dA, sizeof(float) * 2 * (size_t)n, hipMemcpyDeviceToHost, h->st) == hipSuccess;
   if (ok && inside) ok = hipMemcpyAsync(inside, dI, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost, h->st) == hipSuccess;
   ok = ok && hipStreamSynchronize(h->st) == hipSuccess;
   cleanup();

@@ -165,6 +165,28 @@ class Solver:
         return (layout.from_fortran_physical(self.cfg, nt),
                 [layout.from_fortran_physical(self.cfg, kt[n]) for n in range(self.cfg.n_th)])
 
+    def set_closure(self, params: np.ndarray, norm: np.ndarray):
+        """The learned closure's network for les_model_type 6 (dgpu_set_closure)."""
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        n = np.ascontiguousarray(norm, dtype=np.float32)
+        F = ctypes.POINTER(ctypes.c_float)
+        self._check(load().dgpu_set_closure(self.h, p.ctypes.data_as(F), n.ctypes.data_as(F)))
+
+    def les_features(self, time: float = 0.0):
+        """dgpu_les_features: (X [n][16], scale [n], amd [n][2], inside [n])."""
+        L = load()
+        F = ctypes.POINTER(ctypes.c_float)
+        n = ctypes.c_int64()
+        self._check(L.dgpu_les_features(self.h, time, None, None, None, None, ctypes.byref(n)))
+        X = np.zeros((n.value, 16), np.float32)
+        sc = np.zeros(n.value, np.float32)
+        am = np.zeros((n.value, 2), np.float32)
+        ins = np.zeros(n.value, np.int32)
+        self._check(L.dgpu_les_features(self.h, time, X.ctypes.data_as(F), sc.ctypes.data_as(F),
+                                        am.ctypes.data_as(F), ins.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                        ctypes.byref(n)))
+        return X, sc, am, ins
+
     def set_variable_dt(self, variable_dt: bool, update_dt: int = 1, cfl: float = 0.5):
         self._check(load().dgpu_set_variable_dt(self.h, int(variable_dt), update_dt, cfl))
 

@@ -7,9 +7,11 @@
  *
  * It has no counterpart in the reference.  It is a further LES model beside the
  * reference's les_model_type 1, 4, 5 (les_chan / les_chan_th, for/les.f90:3-1435):
- * les_model_type 6 evaluates the network where type 5 (AMD) evaluates its
- * coefficient, and falls back to AMD wherever a point's features leave the
- * training envelope.  Conventions as in diablo_gpu.h (int status, synchronous
+ * les_model_type 6 evaluates the network at the GY points, where type 5 (AMD)
+ * evaluates its eddy viscosity and diffusivity, takes the mean of two GY
+ * neighbours at the GYF points between them, and falls back to the type-5
+ * coefficient wherever a point's features leave the training envelope (at a GYF
+ * point: where either neighbour does).  Conventions as in diablo_gpu.h (int status, synchronous
  * calls, the library owns device memory, dgpu_last_error for the message).
  *
  * Network:  y = W3^T g(W2^T g(W1^T x' + b1) + b2) + b3,  g = GELU in its tanh form
@@ -95,8 +97,9 @@ int dgpu_set_closure(dgpu_handle *h, const float *params, const float *norm);
 
 /* Features of the current state at the GYF points of rows jstart..Nyp-1 (the
  * rows where les_chan forms the GYF stresses), after the BC posts of an LES
- * substep at `time`: X [n][16], scale [n] = C^2 D2 sqrt(2 s2), amd [n] = the type-5
- * coefficient at the point divided by sqrt(2 s2) (so nu_t(AMD) = scale * amd),
+ * substep at `time`: X [n][16], scale [n] = C^2 D2 sqrt(2 s2), amd [n][2] = the
+ * type-5 velocity and scalar coefficients at the point divided by sqrt(2 s2) (so
+ * nu_t(AMD) = scale * amd[.][0], kappa_t(AMD) = scale * amd[.][1]),
  * inside [n] = 1 where every feature lies in the handle's envelope (needs a
  * closure set; else all 1).  n = (Nyp - jstart) * Nz * Nx points, ordered
  * [j][z][x].  *n_out receives n; any output pointer but n_out may be NULL. */

@@ -6,8 +6,6 @@
 * dgpu_les_features: the AMD column times the scale is the type-5 eddy
   viscosity at GYF, and the features are finite and bounded where defined.
 """
-import ctypes
-
 import numpy as np
 import pytest
 
@@ -26,19 +24,6 @@ def type6(p5):
     return Problem(cfg)
 
 
-def features(g, time=0.0):
-    L = load()
-    n = ctypes.c_int64()
-    closure._check(L.dgpu_les_features(g.h, time, None, None, None, None, ctypes.byref(n)))
-    X = np.zeros((n.value, 16), np.float32)
-    sc = np.zeros(n.value, np.float32)
-    am = np.zeros(n.value, np.float32)
-    ins = np.zeros(n.value, np.int32)
-    closure._check(L.dgpu_les_features(g.h, time, closure._fp(X), closure._fp(sc), closure._fp(am),
-                                       ins.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), ctypes.byref(n)))
-    return X, sc, am, ins
-
-
 @pytest.mark.parametrize("envelope", ["none", "empty"])
 def test_type6_fallback_matches_amd_oracle(envelope):
     p5, s0 = les_problem(32, 33, 32)
@@ -50,7 +35,7 @@ def test_type6_fallback_matches_amd_oracle(envelope):
         norm[32:48] = 1.0   # lo > hi: no point is inside
         norm[48:64] = -1.0
         m = closure.Mlp(seed=1)
-        closure._check(load().dgpu_set_closure(g.h, closure._fp(m.params), closure._fp(norm)))
+        g.set_closure(m.params, norm)
     g.upload_state(s0)
     t = 0.0
     for n in range(3):
@@ -74,7 +59,7 @@ def test_features_amd_column_is_type5_viscosity():
     p5, s0 = les_problem(32, 33, 32)
     g = Solver(type6(p5))
     g.upload_state(s0)
-    X, sc, am, ins = features(g)
+    X, sc, am, ins = g.les_features()
     cfg = p5.cfg
     n = (cfg.ny - cfg.jstart) * cfg.nz * cfg.nx
     assert X.shape == (n, 16) and np.all(ins == 1)
@@ -87,5 +72,6 @@ def test_features_amd_column_is_type5_viscosity():
     # the AMD column: coef / sqrt(2 s2) = max(0, f3) / sqrt(2) on every point with s2 > 0
     live = sc > 0
     assert live.mean() > 0.9
-    assert np.allclose(am[live], np.maximum(0.0, X[live, 3]) / np.sqrt(2.0), rtol=1e-5, atol=1e-6)
-    print("features: s2>0 on", f"{live.mean():.3f}", "AMD active on", f"{(am > 0).mean():.3f}", flush=True)
+    assert np.allclose(am[live, 0], np.maximum(0.0, X[live, 3]) / np.sqrt(2.0), rtol=1e-5, atol=1e-6)
+    assert np.all(np.isfinite(am)) and np.all(am >= 0) and (am[:, 1] > 0).mean() > 0.05
+    print("features: s2>0 on", f"{live.mean():.3f}", "AMD active on", f"{(am[:, 0] > 0).mean():.3f}", flush=True)
