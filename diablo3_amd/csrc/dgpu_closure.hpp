@@ -106,14 +106,15 @@ __device__ __forceinline__ bool closure_inside(const float* norm, const float* f
 // the entries coef_gy passes to amd_coef at GY j
 template <typename T>
 struct GyEntries {
-  double a1, a2, a3, a4, a5, a6, o4, o5, o6;
+  T a1, a2, a3, a4, a5, a6, o4, o5, o6;
 };
 template <typename T>
 __device__ __forceinline__ GyEntries<T> gy_entries(const AmdIn<T>& in, const AmdArgs& a, int j, long long q) {
-  const Gyf f = gyf_plane(in, a, j, q), fm = gyf_plane(in, a, j - 1, q);
-  const Gy p = gy_plane(in, a, j, q);
-  const double wc = a.dyf[j - 1], wm = a.dyf[j], den = 2.0 * a.dy[j];
-  auto w = [&](double x, double y) { return (x * wc + y * wm) / den; };
+  using R = T;
+  const Gyf<T> f = gyf_plane(in, a, j, q), fm = gyf_plane(in, a, j - 1, q);
+  const Gy<T> p = gy_plane(in, a, j, q);
+  const R wc = R(a.dyf[j - 1]), wm = R(a.dyf[j]), den = R(2.0) * R(a.dy[j]);
+  auto w = [&](R x, R y) { return (x * wc + y * wm) / den; };
   return {w(f.a1, fm.a1), w(f.a2, fm.a2), w(f.a3, fm.a3), p.a4, w(f.a5, fm.a5), p.a6, p.o4, w(f.o5, fm.o5), p.o6};
 }
 
@@ -137,16 +138,20 @@ __global__ __launch_bounds__(256) void k_closure(ClosureArgs c) {
   float fe[MLP_INP];
 #pragma unroll
   for (int k = 0; k < MLP_INP; ++k) fe[k] = 0.f;
-  double s2 = 0.0, amd_v = 0.0, amd_t = 0.0;
+  // the AMD values in the transform precision T (as k_amd's type 5), the features in float
+  T amd_v = T(0), amd_t = T(0);
+  float s2 = 0.f;
   if (on) {
-    double a1, a2, a3, a4, a5, a6, o4, o5, o6, dyl, D2;
-    ThCoef t{0.0, 0.0, 0.0, 0.0, 0.0};
+    T a1, a2, a3, a4, a5, a6, o4, o5, o6;
+    double dyl, D2;
+    ThCoef<T> t{T(0), T(0), T(0), T(0), T(0)};
     if (c.feat) {
       // the entries coef_gyf passes to amd_coef at GYF j
-      const Gyf f = gyf_plane(in, a, j, q);
-      const Gy p = gy_plane(in, a, j + 1, q), m = gy_plane(in, a, j, q);
+      const Gyf<T> f = gyf_plane(in, a, j, q);
+      const Gy<T> p = gy_plane(in, a, j + 1, q), m = gy_plane(in, a, j, q);
       a1 = f.a1; a2 = f.a2; a3 = f.a3; a5 = f.a5; o5 = f.o5;
-      a4 = 0.5 * (p.a4 + m.a4); a6 = 0.5 * (p.a6 + m.a6); o4 = 0.5 * (p.o4 + m.o4); o6 = 0.5 * (p.o6 + m.o6);
+      a4 = T(0.5) * (p.a4 + m.a4); a6 = T(0.5) * (p.a6 + m.a6); o4 = T(0.5) * (p.o4 + m.o4);
+      o6 = T(0.5) * (p.o6 + m.o6);
       if constexpr (NTH > 0) t = amd_th_gyf(in, a, 9, j, q);
       dyl = a.dyf[j] * 2.0;
       D2 = -a.dlt[j] / (2.0 * C2);
@@ -159,11 +164,11 @@ __global__ __launch_bounds__(256) void k_closure(ClosureArgs c) {
     }
     amd_v = amd_coef(a1, a2, a3, a4, a5, a6, o4, o5, o6);
     amd_t = t.c;
-    const double tx = t.bx / a.dxs, tz = t.bz / a.dzs;
-    const double gth = sqrt(tx * tx + t.ty * t.ty + tz * tz);
+    const float tx = (float)t.bx / (float)a.dxs, tz = (float)t.bz / (float)a.dzs, ty = (float)t.ty;
+    const float gth = sqrtf(tx * tx + ty * ty + tz * tz);
     s2 = closure_features((float)a1, (float)a2, (float)a3, (float)a4, (float)a5, (float)a6, (float)o4, (float)o5,
-                          (float)o6, (float)t.bx, (float)t.by, (float)t.bz, (float)t.ty, (float)gth, (float)D2,
-                          (float)dyl, (float)a.dxs, (float)a.dzs, (float)c.nu, (float)c.gyb, fe);
+                          (float)o6, (float)t.bx, (float)t.by, (float)t.bz, ty, gth, (float)D2, (float)dyl,
+                          (float)a.dxs, (float)a.dzs, (float)c.nu, (float)c.gyb, fe);
   }
   // network: 4 tiles of 16 samples per wave
 #pragma unroll
@@ -171,33 +176,28 @@ __global__ __launch_bounds__(256) void k_closure(ClosureArgs c) {
   __syncthreads();
   if (c.have_net && !c.feat) {
     const int r = lane & 15, gq = lane >> 4;
-    for (int t = 0; t < 4; ++t) {
-      const float* row = &sF[wave][16 * t + r][4 * gq];
-      bf16x4 xb;
+    MlpReg m;
+    mlp_load(c.w, m);
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int fi = 4 * gq + v;
-        xb[v] = (short)f2bf((row[v] - c.w.norm[fi]) * c.w.norm[MLP_INP + fi]);
-      }
-      MlpAct act;
-      mlp_forward(c.w, xb, act);
+    for (int t = 0; t < 4; ++t) {
+      const f32x4 y = mlp_infer(m, &sF[wave][16 * t + r][4 * gq]);
       if (gq == 0) {
-        sY[wave][16 * t + r][0] = act.y[0];
-        sY[wave][16 * t + r][1] = act.y[1];
+        sY[wave][16 * t + r][0] = y[0];
+        sY[wave][16 * t + r][1] = y[1];
       }
     }
   }
   __syncthreads();
-  const bool inside = c.have_net && s2 > 0.0 && closure_inside(c.w.norm, fe);
+  const bool inside = c.have_net && s2 > 0.f && closure_inside(c.w.norm, fe);
   if (c.feat) {
     if (on) {
       const long long s = (long long)(j - c.jf0) * g.pp + q;
 #pragma unroll
       for (int k = 0; k < MLP_INP; ++k) c.X[s * MLP_INP + k] = fe[k];
-      const double r2 = sqrt(2.0 * s2);
+      const double r2 = sqrt(2.0 * (double)s2);
       c.scale[s] = (float)(-0.5 * a.dlt[j] * r2);
-      c.amd[2 * s] = (float)(r2 > 0.0 ? amd_v / r2 : 0.0);
-      c.amd[2 * s + 1] = (float)(r2 > 0.0 ? amd_t / r2 : 0.0);
+      c.amd[2 * s] = (float)(r2 > 0.0 ? (double)amd_v / r2 : 0.0);
+      c.amd[2 * s + 1] = (float)(r2 > 0.0 ? (double)amd_t / r2 : 0.0);
       c.inside[s] = c.have_net ? (int)inside : 1;
     }
     return;
@@ -205,11 +205,9 @@ __global__ __launch_bounds__(256) void k_closure(ClosureArgs c) {
   if (!live) return;
   // coefficients (1/time) at GY: the network's sqrt(2 s2) max(0, y), or AMD's stored negated
   const long long o = (long long)j * g.pp + q;
-  const double r2 = sqrt(2.0 * s2);
-  const double cv = inside ? r2 * fmax(0.0, (double)sY[wave][lane][0]) : -amd_v;
-  const double ct = inside ? r2 * fmax(0.0, (double)sY[wave][lane][1]) : -amd_t;
-  ((T*)c.cgy)[o] = inside ? (T)cv : (T)copysign(fabs(cv), -1.0);
-  ((T*)c.kgy)[o] = inside ? (T)ct : (T)copysign(fabs(ct), -1.0);
+  const float r2 = sqrtf(2.f * s2);
+  ((T*)c.cgy)[o] = inside ? (T)(r2 * fmaxf(0.f, sY[wave][lane][0])) : -amd_v;
+  ((T*)c.kgy)[o] = inside ? (T)(r2 * fmaxf(0.f, sY[wave][lane][1])) : -amd_t;
 }
 
 }  // namespace dgpu

@@ -147,160 +147,174 @@ struct AmdArgs {
 };
 
 // the invariant-scaled strain/rotation entries of one GYF plane
+template <typename R>
 struct Gyf {
-  double a1, a2, a3, a5, o5;
+  R a1, a2, a3, a5, o5;
 };
 // ... and of one GY plane
+template <typename R>
 struct Gy {
-  double a4, o4, a6, o6;
+  R a4, o4, a6, o6;
 };
 
 template <typename T>
 struct AmdIn {
   const T* __restrict__ v[3 * MAXF];
   long long pp;
-  __device__ double at(int f, int j, long long q) const { return (double)v[f][(long long)j * pp + q]; }
+  __device__ T at(int f, int j, long long q) const { return v[f][(long long)j * pp + q]; }
 };
 
 // field slots: 3*f + {0: v, 1: dv/dx, 2: dv/dz}, f = 0 u1, 1 u2, 2 u3, 3+n th_n
 template <typename T>
-__device__ __forceinline__ Gyf gyf_plane(const AmdIn<T>& in, const AmdArgs& a, int j, long long q) {
-  Gyf r;
-  const double du1dz = in.at(2, j, q), du3dx = in.at(7, j, q);
+__device__ __forceinline__ Gyf<T> gyf_plane(const AmdIn<T>& in, const AmdArgs& a, int j, long long q) {
+  using R = T;
+  Gyf<T> r;
+  const R du1dz = in.at(2, j, q), du3dx = in.at(7, j, q);
   r.a1 = in.at(1, j, q);                                              // SI11 = du1/dx
-  r.a2 = (in.at(3, j + 1, q) - in.at(3, j, q)) / a.dyf[j];             // SI22 = du2/dy
+  r.a2 = (in.at(3, j + 1, q) - in.at(3, j, q)) / R(a.dyf[j]);             // SI22 = du2/dy
   r.a3 = in.at(8, j, q);                                              // SI33 = du3/dz
-  r.a5 = 0.5 * (du1dz * (a.dzs / a.dxs) + du3dx * (a.dxs / a.dzs));   // SI13
-  r.o5 = 0.5 * (du1dz * (a.dzs / a.dxs) - du3dx * (a.dxs / a.dzs));   // O13
+  r.a5 = R(0.5) * (du1dz * (R(a.dzs) / R(a.dxs)) + du3dx * (R(a.dxs) / R(a.dzs)));   // SI13
+  r.o5 = R(0.5) * (du1dz * (R(a.dzs) / R(a.dxs)) - du3dx * (R(a.dxs) / R(a.dzs)));   // O13
   return r;
 }
 template <typename T>
-__device__ __forceinline__ Gy gy_plane(const AmdIn<T>& in, const AmdArgs& a, int j, long long q) {
-  Gy r;
-  const double dyl = a.dy[j] * 2.0;
-  const double du1dy = (in.at(0, j, q) - in.at(0, j - 1, q)) / a.dy[j];
-  const double du3dy = (in.at(6, j, q) - in.at(6, j - 1, q)) / a.dy[j];
-  const double du2dx = in.at(4, j, q), du2dz = in.at(5, j, q);
-  r.a4 = 0.5 * (du1dy * (dyl / a.dxs) + du2dx * (a.dxs / dyl));
-  r.o4 = 0.5 * (du1dy * (dyl / a.dxs) - du2dx * (a.dxs / dyl));
-  r.a6 = 0.5 * (du2dz * (a.dzs / dyl) + du3dy * (dyl / a.dzs));
-  r.o6 = 0.5 * (du2dz * (a.dzs / dyl) - du3dy * (dyl / a.dzs));
+__device__ __forceinline__ Gy<T> gy_plane(const AmdIn<T>& in, const AmdArgs& a, int j, long long q) {
+  using R = T;
+  Gy<T> r;
+  const R dyl = R(a.dy[j]) * R(2.0);
+  const R du1dy = (in.at(0, j, q) - in.at(0, j - 1, q)) / R(a.dy[j]);
+  const R du3dy = (in.at(6, j, q) - in.at(6, j - 1, q)) / R(a.dy[j]);
+  const R du2dx = in.at(4, j, q), du2dz = in.at(5, j, q);
+  r.a4 = R(0.5) * (du1dy * (dyl / R(a.dxs)) + du2dx * (R(a.dxs) / dyl));
+  r.o4 = R(0.5) * (du1dy * (dyl / R(a.dxs)) - du2dx * (R(a.dxs) / dyl));
+  r.a6 = R(0.5) * (du2dz * (R(a.dzs) / dyl) + du3dy * (dyl / R(a.dzs)));
+  r.o6 = R(0.5) * (du2dz * (R(a.dzs) / dyl) - du3dy * (dyl / R(a.dzs)));
   return r;
 }
 
 // max{0, -(I3 - I4)} / (I1 - I2) from the six strain and three rotation entries
-__device__ __forceinline__ double amd_coef(double a1, double a2, double a3, double a4, double a5, double a6,
-                                           double o4, double o5, double o6) {
-  double v = a1 * a1 * a1 + a2 * a2 * a2 + a3 * a3 * a3 + 3.0 * a1 * (a4 * a4) + 3.0 * a1 * (a5 * a5) +
-             3.0 * a2 * (a4 * a4) + 3.0 * a2 * (a6 * a6) + 3.0 * a3 * (a5 * a5) + 3.0 * a3 * (a6 * a6) +
-             6.0 * a4 * a5 * a6;
+template <typename R>
+__device__ __forceinline__ R amd_coef(R a1, R a2, R a3, R a4, R a5, R a6,
+                                           R o4, R o5, R o6) {
+  R v = a1 * a1 * a1 + a2 * a2 * a2 + a3 * a3 * a3 + R(3.0) * a1 * (a4 * a4) + R(3.0) * a1 * (a5 * a5) +
+             R(3.0) * a2 * (a4 * a4) + R(3.0) * a2 * (a6 * a6) + R(3.0) * a3 * (a5 * a5) + R(3.0) * a3 * (a6 * a6) +
+             R(6.0) * a4 * a5 * a6;
   v = -v - a1 * (o4 * o4) - a1 * (o5 * o5) - a2 * (o4 * o4) - a2 * (o6 * o6) - a3 * (o5 * o5) -
-      a3 * (o6 * o6) - 2.0 * a4 * o5 * o6 + 2.0 * a5 * o4 * o6 - 2.0 * a6 * o4 * o5;
-  if (v <= 0.0) return 0.0;
-  return v / (a1 * a1 + a2 * a2 + a3 * a3 + 2.0 * (a4 * a4) + 2.0 * (o4 * o4) + 2.0 * (a5 * a5) +
-              2.0 * (o5 * o5) + 2.0 * (a6 * a6) + 2.0 * (o6 * o6));
+      a3 * (o6 * o6) - R(2.0) * a4 * o5 * o6 + R(2.0) * a5 * o4 * o6 - R(2.0) * a6 * o4 * o5;
+  if (v <= R(0.0)) return R(0.0);
+  return v / (a1 * a1 + a2 * a2 + a3 * a3 + R(2.0) * (a4 * a4) + R(2.0) * (o4 * o4) + R(2.0) * (a5 * a5) +
+              R(2.0) * (o5 * o5) + R(2.0) * (a6 * a6) + R(2.0) * (o6 * o6));
 }
 
 // coefficient at GYF j (GY entries averaged from j and j+1)
 template <typename T>
-__device__ __forceinline__ double coef_gyf(const AmdIn<T>& in, const AmdArgs& a, int j, long long q) {
-  const Gyf f = gyf_plane(in, a, j, q);
-  const Gy p = gy_plane(in, a, j + 1, q), m = gy_plane(in, a, j, q);
-  return amd_coef(f.a1, f.a2, f.a3, 0.5 * (p.a4 + m.a4), f.a5, 0.5 * (p.a6 + m.a6), 0.5 * (p.o4 + m.o4), f.o5,
-                  0.5 * (p.o6 + m.o6));
+__device__ __forceinline__ T coef_gyf(const AmdIn<T>& in, const AmdArgs& a, int j, long long q) {
+  using R = T;
+  const Gyf<T> f = gyf_plane(in, a, j, q);
+  const Gy<T> p = gy_plane(in, a, j + 1, q), m = gy_plane(in, a, j, q);
+  return amd_coef(f.a1, f.a2, f.a3, R(0.5) * (p.a4 + m.a4), f.a5, R(0.5) * (p.a6 + m.a6), R(0.5) * (p.o4 + m.o4), f.o5,
+                  R(0.5) * (p.o6 + m.o6));
 }
 // coefficient at GY j (GYF entries interpolated from j-1 and j)
 template <typename T>
-__device__ __forceinline__ double coef_gy(const AmdIn<T>& in, const AmdArgs& a, int j, long long q) {
-  const Gyf f = gyf_plane(in, a, j, q), fm = gyf_plane(in, a, j - 1, q);
-  const Gy p = gy_plane(in, a, j, q);
-  const double wc = a.dyf[j - 1], wm = a.dyf[j], den = 2.0 * a.dy[j];
-  auto w = [&](double c, double m) { return (c * wc + m * wm) / den; };
+__device__ __forceinline__ T coef_gy(const AmdIn<T>& in, const AmdArgs& a, int j, long long q) {
+  using R = T;
+  const Gyf<T> f = gyf_plane(in, a, j, q), fm = gyf_plane(in, a, j - 1, q);
+  const Gy<T> p = gy_plane(in, a, j, q);
+  const R wc = R(a.dyf[j - 1]), wm = R(a.dyf[j]), den = R(2.0) * R(a.dy[j]);
+  auto w = [&](R c, R m) { return (c * wc + m * wm) / den; };
   return amd_coef(w(f.a1, fm.a1), w(f.a2, fm.a2), w(f.a3, fm.a3), p.a4, w(f.a5, fm.a5), p.a6, p.o4,
                   w(f.o5, fm.o5), p.o6);
 }
 
 // The strain S and rotation O of the unscaled gradient (compute_strain_chan,
 // compute_rotation_chan, for/les.f90:1443-1596), per GYF and GY plane, for types 1, 4
+template <typename R>
 struct Sgyf {
-  double s1, s2, s3, s5, o5;
+  R s1, s2, s3, s5, o5;
 };
+template <typename R>
 struct Sgy {
-  double s4, s6, o4, o6;
+  R s4, s6, o4, o6;
 };
 template <typename T>
-__device__ __forceinline__ Sgyf sgyf_plane(const AmdIn<T>& in, const AmdArgs& a, int j, long long q) {
-  const double du1dz = in.at(2, j, q), du3dx = in.at(7, j, q);
-  return {in.at(1, j, q), (in.at(3, j + 1, q) - in.at(3, j, q)) / a.dyf[j], in.at(8, j, q),
-          0.5 * (du1dz + du3dx), 0.5 * (du1dz - du3dx)};
+__device__ __forceinline__ Sgyf<T> sgyf_plane(const AmdIn<T>& in, const AmdArgs& a, int j, long long q) {
+  using R = T;
+  const R du1dz = in.at(2, j, q), du3dx = in.at(7, j, q);
+  return {in.at(1, j, q), (in.at(3, j + 1, q) - in.at(3, j, q)) / R(a.dyf[j]), in.at(8, j, q),
+          R(0.5) * (du1dz + du3dx), R(0.5) * (du1dz - du3dx)};
 }
 template <typename T>
-__device__ __forceinline__ Sgy sgy_plane(const AmdIn<T>& in, const AmdArgs& a, int j, long long q) {
-  const double du1dy = (in.at(0, j, q) - in.at(0, j - 1, q)) / a.dy[j];
-  const double du3dy = (in.at(6, j, q) - in.at(6, j - 1, q)) / a.dy[j];
-  const double du2dx = in.at(4, j, q), du2dz = in.at(5, j, q);
-  return {0.5 * (du1dy + du2dx), 0.5 * (du2dz + du3dy), 0.5 * (du1dy - du2dx), 0.5 * (du2dz - du3dy)};
+__device__ __forceinline__ Sgy<T> sgy_plane(const AmdIn<T>& in, const AmdArgs& a, int j, long long q) {
+  using R = T;
+  const R du1dy = (in.at(0, j, q) - in.at(0, j - 1, q)) / R(a.dy[j]);
+  const R du3dy = (in.at(6, j, q) - in.at(6, j - 1, q)) / R(a.dy[j]);
+  const R du2dx = in.at(4, j, q), du2dz = in.at(5, j, q);
+  return {R(0.5) * (du1dy + du2dx), R(0.5) * (du2dz + du3dy), R(0.5) * (du1dy - du2dx), R(0.5) * (du2dz - du3dy)};
 }
 // type 4 (for/les.f90:276-440): the AMD polynomial with the squared filter widths
 // dX, dY, dZ as weights; at GYF (g6 = false) the last cubic term multiplies
 // a4 a5 a6, at GY a4 a6 a5 (the reference's operand orders)
-__device__ __forceinline__ double amd4_coef(double dX, double dY, double dZ, double a1, double a2, double a3,
-                                            double a4, double a5, double a6, double o4, double o5, double o6,
+template <typename R>
+__device__ __forceinline__ R amd4_coef(R dX, R dY, R dZ, R a1, R a2, R a3,
+                                            R a4, R a5, R a6, R o4, R o5, R o6,
                                             bool gy) {
-  double v = dX * (a1 * a1 * a1) + dY * (a2 * a2 * a2) + dZ * (a3 * a3 * a3) + (2.0 * dX + dY) * a1 * (a4 * a4) +
-             (2.0 * dX + dZ) * a1 * (a5 * a5) + (2.0 * dY + dX) * a2 * (a4 * a4) + (2.0 * dY + dZ) * a2 * (a6 * a6) +
-             (2.0 * dZ + dX) * a3 * (a5 * a5) + (2.0 * dZ + dY) * a3 * (a6 * a6) +
-             (gy ? 2.0 * (dX + dY + dZ) * a4 * a6 * a5 : 2.0 * (dX + dY + dZ) * a4 * a5 * a6);
+  R v = dX * (a1 * a1 * a1) + dY * (a2 * a2 * a2) + dZ * (a3 * a3 * a3) + (R(2.0) * dX + dY) * a1 * (a4 * a4) +
+             (R(2.0) * dX + dZ) * a1 * (a5 * a5) + (R(2.0) * dY + dX) * a2 * (a4 * a4) + (R(2.0) * dY + dZ) * a2 * (a6 * a6) +
+             (R(2.0) * dZ + dX) * a3 * (a5 * a5) + (R(2.0) * dZ + dY) * a3 * (a6 * a6) +
+             (gy ? R(2.0) * (dX + dY + dZ) * a4 * a6 * a5 : R(2.0) * (dX + dY + dZ) * a4 * a5 * a6);
   v = -v - dY * a1 * (o4 * o4) - dZ * a1 * (o5 * o5) - dX * a2 * (o4 * o4) - dZ * a2 * (o6 * o6) -
-      dX * a3 * (o5 * o5) - dY * a3 * (o6 * o6) - 2.0 * dZ * a4 * o5 * o6 + 2.0 * dY * a5 * o4 * o6 -
-      2.0 * dX * a6 * o4 * o5;
-  if (v <= 0.0) return 0.0;
-  return v / (a1 * a1 + a2 * a2 + a3 * a3 + 2.0 * (a4 * a4) + 2.0 * (o4 * o4) + 2.0 * (a5 * a5) +
-              2.0 * (o5 * o5) + 2.0 * (a6 * a6) + 2.0 * (o6 * o6));
+      dX * a3 * (o5 * o5) - dY * a3 * (o6 * o6) - R(2.0) * dZ * a4 * o5 * o6 + R(2.0) * dY * a5 * o4 * o6 -
+      R(2.0) * dX * a6 * o4 * o5;
+  if (v <= R(0.0)) return R(0.0);
+  return v / (a1 * a1 + a2 * a2 + a3 * a3 + R(2.0) * (a4 * a4) + R(2.0) * (o4 * o4) + R(2.0) * (a5 * a5) +
+              R(2.0) * (o5 * o5) + R(2.0) * (a6 * a6) + R(2.0) * (o6 * o6));
 }
 
 // the model coefficient at GYF j (s1 of les_chan) and at GY j (temp)
 template <int MODEL, typename T>
-__device__ __forceinline__ double model_gyf(const AmdIn<T>& in, const AmdArgs& a, int j, long long q) {
+__device__ __forceinline__ T model_gyf(const AmdIn<T>& in, const AmdArgs& a, int j, long long q) {
+  using R = T;
   if constexpr (MODEL == 5) {
     return coef_gyf(in, a, j, q);
   } else if constexpr (MODEL == 6) {
     // the mean of the network's GY j and j+1 values, else the type-5 GYF coefficient
-    const double c0 = (double)((const T*)a.cgy)[(long long)j * in.pp + q];
-    const double c1 = (double)((const T*)a.cgy)[(long long)(j + 1) * in.pp + q];
-    return (!signbit(c0) && !signbit(c1)) ? 0.5 * (c0 + c1) : coef_gyf(in, a, j, q);
+    const R c0 = (R)((const T*)a.cgy)[(long long)j * in.pp + q];
+    const R c1 = (R)((const T*)a.cgy)[(long long)(j + 1) * in.pp + q];
+    return (!signbit(c0) && !signbit(c1)) ? R(0.5) * (c0 + c1) : coef_gyf(in, a, j, q);
   } else {
-    const Sgyf f = sgyf_plane(in, a, j, q);
-    const Sgy p = sgy_plane(in, a, j + 1, q), m = sgy_plane(in, a, j, q);
-    const double s4 = 0.5 * (p.s4 + m.s4), s6 = 0.5 * (p.s6 + m.s6);
+    const Sgyf<T> f = sgyf_plane(in, a, j, q);
+    const Sgy<T> p = sgy_plane(in, a, j + 1, q), m = sgy_plane(in, a, j, q);
+    const R s4 = R(0.5) * (p.s4 + m.s4), s6 = R(0.5) * (p.s6 + m.s6);
     if constexpr (MODEL == 1) {  // |S| (for/les.f90:99-112)
-      return sqrt(2.0 * (f.s1 * f.s1) + 4.0 * (s4 * s4) + 4.0 * (f.s5 * f.s5) + 2.0 * (f.s2 * f.s2) +
-                  4.0 * (s6 * s6) + 2.0 * (f.s3 * f.s3));
+      return sqrt(R(2.0) * (f.s1 * f.s1) + R(4.0) * (s4 * s4) + R(4.0) * (f.s5 * f.s5) + R(2.0) * (f.s2 * f.s2) +
+                  R(4.0) * (s6 * s6) + R(2.0) * (f.s3 * f.s3));
     } else {
-      const double dY = (a.dy[j] * 2.0) * (a.dy[j] * 2.0);
-      return amd4_coef(a.dxs * a.dxs, dY, a.dzs * a.dzs, f.s1, f.s2, f.s3, s4, f.s5, s6, 0.5 * (p.o4 + m.o4),
-                       f.o5, 0.5 * (p.o6 + m.o6), false);
+      const R dY = (R(a.dy[j]) * R(2.0)) * (R(a.dy[j]) * R(2.0));
+      return amd4_coef(R(a.dxs) * R(a.dxs), dY, R(a.dzs) * R(a.dzs), f.s1, f.s2, f.s3, s4, f.s5, s6, R(0.5) * (p.o4 + m.o4),
+                       f.o5, R(0.5) * (p.o6 + m.o6), false);
     }
   }
 }
 template <int MODEL, typename T>
-__device__ __forceinline__ double model_gy(const AmdIn<T>& in, const AmdArgs& a, int j, long long q) {
+__device__ __forceinline__ T model_gy(const AmdIn<T>& in, const AmdArgs& a, int j, long long q) {
+  using R = T;
   if constexpr (MODEL == 5) {
     return coef_gy(in, a, j, q);
   } else if constexpr (MODEL == 6) {
-    return fabs((double)((const T*)a.cgy)[(long long)j * in.pp + q]);
+    return fabs((R)((const T*)a.cgy)[(long long)j * in.pp + q]);
   } else {
-    const Sgyf f = sgyf_plane(in, a, j, q), fm = sgyf_plane(in, a, j - 1, q);
-    const Sgy p = sgy_plane(in, a, j, q);
-    const double wc = a.dyf[j - 1], wm = a.dyf[j], den = 2.0 * a.dy[j];
-    auto w = [&](double c, double m) { return (c * wc + m * wm) / den; };
-    const double s1 = w(f.s1, fm.s1), s2 = w(f.s2, fm.s2), s3 = w(f.s3, fm.s3), s5 = w(f.s5, fm.s5);
+    const Sgyf<T> f = sgyf_plane(in, a, j, q), fm = sgyf_plane(in, a, j - 1, q);
+    const Sgy<T> p = sgy_plane(in, a, j, q);
+    const R wc = R(a.dyf[j - 1]), wm = R(a.dyf[j]), den = R(2.0) * R(a.dy[j]);
+    auto w = [&](R c, R m) { return (c * wc + m * wm) / den; };
+    const R s1 = w(f.s1, fm.s1), s2 = w(f.s2, fm.s2), s3 = w(f.s3, fm.s3), s5 = w(f.s5, fm.s5);
     if constexpr (MODEL == 1) {  // (for/les.f90:114-132)
-      return sqrt(2.0 * (s1 * s1) + 4.0 * (p.s4 * p.s4) + 4.0 * (s5 * s5) + 2.0 * (s2 * s2) + 4.0 * (p.s6 * p.s6) +
-                  2.0 * (s3 * s3));
+      return sqrt(R(2.0) * (s1 * s1) + R(4.0) * (p.s4 * p.s4) + R(4.0) * (s5 * s5) + R(2.0) * (s2 * s2) + R(4.0) * (p.s6 * p.s6) +
+                  R(2.0) * (s3 * s3));
     } else {
-      const double dY = (a.dy[j] * 2.0) * (a.dy[j] * 2.0);
-      return amd4_coef(a.dxs * a.dxs, dY, a.dzs * a.dzs, s1, s2, s3, p.s4, s5, p.s6, p.o4, w(f.o5, fm.o5), p.o6,
+      const R dY = (R(a.dy[j]) * R(2.0)) * (R(a.dy[j]) * R(2.0));
+      return amd4_coef(R(a.dxs) * R(a.dxs), dY, R(a.dzs) * R(a.dzs), s1, s2, s3, p.s4, s5, p.s6, p.o4, w(f.o5, fm.o5), p.o6,
                        true);
     }
   }
@@ -309,49 +323,52 @@ __device__ __forceinline__ double model_gy(const AmdIn<T>& in, const AmdArgs& a,
 // The type-5 scalar coefficient max{0, v}/|b'|^2 (les_chan_th), with the scaled
 // scalar gradient b' = (delta_x th_x, delta_y th_y, delta_z th_z) and the
 // unscaled th_y it was formed from
+template <typename R>
 struct ThCoef {
-  double c, bx, by, bz, ty;
+  R c, bx, by, bz, ty;
 };
 // at GY j (for/les.f90:1259-1306)
 template <typename T>
-__device__ __forceinline__ ThCoef amd_th_gy(const AmdIn<T>& in, const AmdArgs& a, int ft, int j, long long q) {
-  const double dxs2 = a.dxs * a.dxs, dzs2 = a.dzs * a.dzs;
-  const double dyl = a.dy[j] * 2.0, dyl2 = dyl * dyl;
-  const double wc = a.dyf[j - 1], wm = a.dyf[j], den = 2.0 * a.dy[j];
+__device__ __forceinline__ ThCoef<T> amd_th_gy(const AmdIn<T>& in, const AmdArgs& a, int ft, int j, long long q) {
+  using R = T;
+  const R dxs2 = R(a.dxs) * R(a.dxs), dzs2 = R(a.dzs) * R(a.dzs);
+  const R dyl = R(a.dy[j]) * R(2.0), dyl2 = dyl * dyl;
+  const R wc = R(a.dyf[j - 1]), wm = R(a.dyf[j]), den = R(2.0) * R(a.dy[j]);
   auto w = [&](int s) { return (in.at(s, j, q) * wc + in.at(s, j - 1, q) * wm) / den; };
-  const double du1dx = w(1), du3dz = w(8), du1dz = w(2), du3dx = w(7);
-  const double du2dy = ((in.at(3, j + 1, q) - in.at(3, j, q)) / a.dyf[j] * wc +
-                        (in.at(3, j, q) - in.at(3, j - 1, q)) / a.dyf[j - 1] * wm) / den;
-  const double tx = w(ft + 1), tz = w(ft + 2);
-  const double ty = (in.at(ft, j, q) - in.at(ft, j - 1, q)) / a.dy[j];
-  const double du1dy = (in.at(0, j, q) - in.at(0, j - 1, q)) / a.dy[j];
-  const double du3dy = (in.at(6, j, q) - in.at(6, j - 1, q)) / a.dy[j];
-  const double du2dx = in.at(4, j, q), du2dz = in.at(5, j, q);
-  const double v = -(dxs2 * du1dx * (tx * tx) + dyl2 * du2dy * (ty * ty) + dzs2 * du3dz * (tz * tz) +
+  const R du1dx = w(1), du3dz = w(8), du1dz = w(2), du3dx = w(7);
+  const R du2dy = ((in.at(3, j + 1, q) - in.at(3, j, q)) / R(a.dyf[j]) * wc +
+                        (in.at(3, j, q) - in.at(3, j - 1, q)) / R(a.dyf[j - 1]) * wm) / den;
+  const R tx = w(ft + 1), tz = w(ft + 2);
+  const R ty = (in.at(ft, j, q) - in.at(ft, j - 1, q)) / R(a.dy[j]);
+  const R du1dy = (in.at(0, j, q) - in.at(0, j - 1, q)) / R(a.dy[j]);
+  const R du3dy = (in.at(6, j, q) - in.at(6, j - 1, q)) / R(a.dy[j]);
+  const R du2dx = in.at(4, j, q), du2dz = in.at(5, j, q);
+  const R v = -(dxs2 * du1dx * (tx * tx) + dyl2 * du2dy * (ty * ty) + dzs2 * du3dz * (tz * tz) +
                      (dyl2 * du1dy + dxs2 * du2dx) * tx * ty + (dzs2 * du1dz + dxs2 * du3dx) * tx * tz +
                      (dzs2 * du2dz + dyl2 * du3dy) * ty * tz);
-  const double ax = a.dxs * tx, ay = dyl * ty, az = a.dzs * tz;
-  return {v > 0.0 ? v / (ax * ax + ay * ay + az * az) : 0.0, ax, ay, az, ty};
+  const R ax = R(a.dxs) * tx, ay = dyl * ty, az = R(a.dzs) * tz;
+  return {v > R(0.0) ? v / (ax * ax + ay * ay + az * az) : R(0.0), ax, ay, az, ty};
 }
 // at GYF j (for/les.f90:1221-1256, 1310-1318)
 template <typename T>
-__device__ __forceinline__ ThCoef amd_th_gyf(const AmdIn<T>& in, const AmdArgs& a, int ft, int j, long long q) {
-  const double dxs2 = a.dxs * a.dxs, dzs2 = a.dzs * a.dzs;
-  const double dyl = a.dyf[j] * 2.0, dyl2 = dyl * dyl;
-  auto gyv = [&](int s, int jj) { return (in.at(s, jj, q) - in.at(s, jj - 1, q)) / a.dy[jj]; };
-  const double tx = in.at(ft + 1, j, q), tz = in.at(ft + 2, j, q);
-  const double ty = 0.5 * (gyv(ft, j + 1) + gyv(ft, j));
-  const double du1dx = in.at(1, j, q), du3dz = in.at(8, j, q);
-  const double du2dy = (in.at(3, j + 1, q) - in.at(3, j, q)) / a.dyf[j];
-  const double du1dy = 0.5 * (gyv(0, j + 1) + gyv(0, j)), du3dy = 0.5 * (gyv(6, j + 1) + gyv(6, j));
-  const double du2dx = 0.5 * (in.at(4, j + 1, q) + in.at(4, j, q));
-  const double du2dz = 0.5 * (in.at(5, j + 1, q) + in.at(5, j, q));
-  const double du1dz = in.at(2, j, q), du3dx = in.at(7, j, q);
-  const double v = -(dxs2 * du1dx * (tx * tx) + dyl2 * du2dy * (ty * ty) + dzs2 * du3dz * (tz * tz) +
+__device__ __forceinline__ ThCoef<T> amd_th_gyf(const AmdIn<T>& in, const AmdArgs& a, int ft, int j, long long q) {
+  using R = T;
+  const R dxs2 = R(a.dxs) * R(a.dxs), dzs2 = R(a.dzs) * R(a.dzs);
+  const R dyl = R(a.dyf[j]) * R(2.0), dyl2 = dyl * dyl;
+  auto gyv = [&](int s, int jj) { return (in.at(s, jj, q) - in.at(s, jj - 1, q)) / R(a.dy[jj]); };
+  const R tx = in.at(ft + 1, j, q), tz = in.at(ft + 2, j, q);
+  const R ty = R(0.5) * (gyv(ft, j + 1) + gyv(ft, j));
+  const R du1dx = in.at(1, j, q), du3dz = in.at(8, j, q);
+  const R du2dy = (in.at(3, j + 1, q) - in.at(3, j, q)) / R(a.dyf[j]);
+  const R du1dy = R(0.5) * (gyv(0, j + 1) + gyv(0, j)), du3dy = R(0.5) * (gyv(6, j + 1) + gyv(6, j));
+  const R du2dx = R(0.5) * (in.at(4, j + 1, q) + in.at(4, j, q));
+  const R du2dz = R(0.5) * (in.at(5, j + 1, q) + in.at(5, j, q));
+  const R du1dz = in.at(2, j, q), du3dx = in.at(7, j, q);
+  const R v = -(dxs2 * du1dx * (tx * tx) + dyl2 * du2dy * (ty * ty) + dzs2 * du3dz * (tz * tz) +
                      (dyl2 * du1dy + dxs2 * du2dx) * tx * ty + (dzs2 * du1dz + dxs2 * du3dx) * tx * tz +
                      (dzs2 * du2dz + dyl2 * du3dy) * ty * tz);
-  const double ax = a.dxs * tx, ay = dyl * ty, az = a.dzs * tz;
-  return {v > 0.0 ? v / (ax * ax + ay * ay + az * az) : 0.0, ax, ay, az, ty};
+  const R ax = R(a.dxs) * tx, ay = dyl * ty, az = R(a.dzs) * tz;
+  return {v > R(0.0) ? v / (ax * ax + ay * ay + az * az) : R(0.0), ax, ay, az, ty};
 }
 
 // tau_12 (e = 0) or tau_23 (e = 1) at GY j: delta_y(j) coef(j) (S - du/dy / 2).
@@ -359,17 +376,19 @@ __device__ __forceinline__ ThCoef amd_th_gyf(const AmdIn<T>& in, const AmdArgs& 
 // at j = 2, for/les.f90:454, 762), type 1 multiplies it (for/les.f90:145); that
 // row feeds only rows the BC rows overwrite.
 template <int MODEL, typename T>
-__device__ __forceinline__ double tau_gy(const AmdIn<T>& in, const AmdArgs& a, int e, int j, long long q) {
-  const double dlt = a.dlt[a.g.ny2 + j];
+__device__ __forceinline__ T tau_gy(const AmdIn<T>& in, const AmdArgs& a, int e, int j, long long q) {
+  using R = T;
+  const R dlt = R(a.dlt[a.g.ny2 + j]);
   if (MODEL != 1 && j == 1) {
-    const double dvdy = (in.at(e ? 6 : 0, 1, q) - in.at(e ? 6 : 0, 0, q)) / a.dy[1];
-    return dlt * 0.5 * (dvdy + in.at(e ? 5 : 4, 1, q));
+    const R dvdy = (in.at(e ? 6 : 0, 1, q) - in.at(e ? 6 : 0, 0, q)) / R(a.dy[1]);
+    return dlt * R(0.5) * (dvdy + in.at(e ? 5 : 4, 1, q));
   }
-  return dlt * (model_gy<MODEL>(in, a, j, q) * (0.5 * in.at(e ? 5 : 4, j, q)));
+  return dlt * (model_gy<MODEL>(in, a, j, q) * (R(0.5) * in.at(e ? 5 : 4, j, q)));
 }
 
 template <typename T, int NTH, int MODEL>
 __global__ __launch_bounds__(256) void k_amd(AmdArgs a) {
+  using R = T;
   const Geo& g = a.g;
   const long long q = (long long)blockIdx.x * 256 + threadIdx.x;
   const int j = blockIdx.y;
@@ -379,28 +398,28 @@ __global__ __launch_bounds__(256) void k_amd(AmdArgs a) {
   in.pp = g.pp;
   const int ny = g.ny;
   const long long o = (long long)j * g.pp + q;
-  auto put = [&](void* p, double v) { ((T*)p)[o] = (T)v; };
+  auto put = [&](void* p, R v) { ((T*)p)[o] = (T)v; };
   // nu_t at GY j (zero on the wall rows of ghost_les_mpi)
-  double nut = 0.0;
-  if (j >= 3 && j <= ny - 1) nut = -0.5 * a.dlt[g.ny2 + j] * model_gy<MODEL>(in, a, j, q);
+  R nut = R(0.0);
+  if (j >= 3 && j <= ny - 1) nut = -R(0.5) * R(a.dlt[g.ny2 + j]) * model_gy<MODEL>(in, a, j, q);
   put(a.nut, nut);
   // GYF stresses, rows j1..j2 = jstart..Nyp-1
-  double A0 = 0.0, A1 = 0.0, A2 = 0.0, A3 = 0.0, A4 = 0.0;
+  R A0 = R(0.0), A1 = R(0.0), A2 = R(0.0), A3 = R(0.0), A4 = R(0.0);
   if (j >= a.jstart && j <= ny - 1) {
-    const double dyf = a.dlt[j] * model_gyf<MODEL>(in, a, j, q);
-    const double du1dz = in.at(2, j, q), du3dx = in.at(7, j, q);
+    const R dyf = R(a.dlt[j]) * model_gyf<MODEL>(in, a, j, q);
+    const R du1dz = in.at(2, j, q), du3dx = in.at(7, j, q);
     A0 = dyf * in.at(1, j, q);
-    A1 = dyf * (0.5 * (du1dz + du3dx));
+    A1 = dyf * (R(0.5) * (du1dz + du3dx));
     A2 = dyf * in.at(8, j, q);
-    A3 = (tau_gy<MODEL>(in, a, 0, j + 1, q) - tau_gy<MODEL>(in, a, 0, j, q)) / a.dyf[j];
-    A4 = (tau_gy<MODEL>(in, a, 1, j + 1, q) - tau_gy<MODEL>(in, a, 1, j, q)) / a.dyf[j];
+    A3 = (tau_gy<MODEL>(in, a, 0, j + 1, q) - tau_gy<MODEL>(in, a, 0, j, q)) / R(a.dyf[j]);
+    A4 = (tau_gy<MODEL>(in, a, 1, j + 1, q) - tau_gy<MODEL>(in, a, 1, j, q)) / R(a.dyf[j]);
   }
   put(a.aug[0], A0); put(a.aug[1], A1); put(a.aug[2], A2); put(a.aug[3], A3); put(a.aug[4], A4);
   // GY stresses and the explicit cross terms, rows 2..Nyp
-  double A5 = 0.0, A6 = 0.0;
+  R A5 = R(0.0), A6 = R(0.0);
   if (j >= 2 && j <= ny) {
-    A5 = tau_gy<MODEL>(in, a, 0, j, q) - nut * (in.at(0, j, q) - in.at(0, j - 1, q)) / a.dy[j];
-    A6 = tau_gy<MODEL>(in, a, 1, j, q) - nut * (in.at(6, j, q) - in.at(6, j - 1, q)) / a.dy[j];
+    A5 = tau_gy<MODEL>(in, a, 0, j, q) - nut * (in.at(0, j, q) - in.at(0, j - 1, q)) / R(a.dy[j]);
+    A6 = tau_gy<MODEL>(in, a, 1, j, q) - nut * (in.at(6, j, q) - in.at(6, j - 1, q)) / R(a.dy[j]);
   }
   put(a.aug[5], A5); put(a.aug[6], A6);
   if constexpr (MODEL == 1 || MODEL == 4) {
@@ -411,8 +430,8 @@ __global__ __launch_bounds__(256) void k_amd(AmdArgs a) {
       const int ft = 3 * (3 + n);
       put(a.kap[n], nut);
       const bool row = (j >= a.jstart_th[n] && j <= a.jend_th[n]);
-      put(a.aug[7 + 2 * n], row ? -(nut * in.at(ft + 1, j, q)) : 0.0);
-      put(a.aug[8 + 2 * n], row ? -(nut * in.at(ft + 2, j, q)) : 0.0);
+      put(a.aug[7 + 2 * n], row ? -(nut * in.at(ft + 1, j, q)) : R(0.0));
+      put(a.aug[8 + 2 * n], row ? -(nut * in.at(ft + 2, j, q)) : R(0.0));
     }
     return;
   }
@@ -420,25 +439,25 @@ __global__ __launch_bounds__(256) void k_amd(AmdArgs a) {
 #pragma unroll
   for (int n = 0; n < NTH; ++n) {
     const int ft = 3 * (3 + n);
-    double kap = 0.0, fx = 0.0, fz = 0.0;
+    R kap = R(0.0), fx = R(0.0), fz = R(0.0);
     // kappa_t at GY j (for/les.f90:1259-1306, 1344-1348)
     if (j >= 3 && j <= ny - 1) {
-      double c;
-      if constexpr (MODEL == 6) c = fabs((double)((const T*)a.kgy)[o]);
+      R c;
+      if constexpr (MODEL == 6) c = fabs((R)((const T*)a.kgy)[o]);
       else c = amd_th_gy(in, a, ft, j, q).c;
-      kap = -0.5 * a.dlt[g.ny2 + j] * c;
+      kap = -R(0.5) * R(a.dlt[g.ny2 + j]) * c;
     }
     put(a.kap[n], kap);
     // SGS flux at GYF j (for/les.f90:1221-1256, 1310-1318, 1372-1393)
     if (j >= a.jstart_th[n] && j <= a.jend_th[n]) {
-      double s1;
+      R s1;
       if constexpr (MODEL == 6) {
-        const double k0 = (double)((const T*)a.kgy)[o], k1 = (double)((const T*)a.kgy)[o + g.pp];
-        s1 = (!signbit(k0) && !signbit(k1)) ? 0.5 * (k0 + k1) : amd_th_gyf(in, a, ft, j, q).c;
+        const R k0 = (R)((const T*)a.kgy)[o], k1 = (R)((const T*)a.kgy)[o + g.pp];
+        s1 = (!signbit(k0) && !signbit(k1)) ? R(0.5) * (k0 + k1) : amd_th_gyf(in, a, ft, j, q).c;
       } else {
         s1 = amd_th_gyf(in, a, ft, j, q).c;
       }
-      const double f = 0.5 * a.dlt[j] * s1;
+      const R f = R(0.5) * R(a.dlt[j]) * s1;
       fx = f * in.at(ft + 1, j, q);
       fz = f * in.at(ft + 2, j, q);
     }

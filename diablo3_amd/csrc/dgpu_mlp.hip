@@ -286,20 +286,23 @@ __global__ __launch_bounds__(256) void k_mlp_cast(MlpAdamArgs a) {
   mlp_store_bf16(a, i, w);
 }
 
-// inference on a batch of raw feature rows: Y [n][2]
+// inference on a batch of raw feature rows: Y [n][2] (weights held in registers
+// across the wave's tiles)
 __global__ __launch_bounds__(256) void k_mlp_predict(MlpW w, const float* X, long long n, float* Y) {
   const int wave = threadIdx.x >> 6, l = threadIdx.x & 63, r = l & 15, g = l >> 4;
   const long long ntile = (n + 15) / 16;
+  MlpReg m;
+  mlp_load(w, m);
   for (long long t = (long long)blockIdx.x * 4 + wave; t < ntile; t += (long long)gridDim.x * 4) {
     const long long s = t * 16 + r;
     const bool live = s < n;
-    bf16x4 xb = norm_x(w.norm, X + (live ? s : 0) * MLP_INP + 4 * g, g);
-    if (!live) xb = bf16x4{0, 0, 0, 0};
-    MlpAct act;
-    mlp_forward(w, xb, act);
+    float x[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) x[v] = live ? X[s * MLP_INP + 4 * g + v] : m.shift[v];
+    const f32x4 y = mlp_infer(m, x);
     if (g == 0 && live) {
-      Y[s * 2] = act.y[0];
-      Y[s * 2 + 1] = act.y[1];
+      Y[s * 2] = y[0];
+      Y[s * 2 + 1] = y[1];
     }
   }
 }

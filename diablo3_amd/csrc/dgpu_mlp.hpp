@@ -50,7 +50,7 @@ __device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
 constexpr float GELU_K0 = 0.79788456080286536f, GELU_K1 = 0.044715f;
 __device__ __forceinline__ float gelu_s(float x) {
   const float u = GELU_K0 * (x + GELU_K1 * x * x * x);
-  return 1.0f / (1.0f + __expf(-2.0f * u));
+  return __builtin_amdgcn_rcpf(1.0f + __expf(-2.0f * u));
 }
 __device__ __forceinline__ float gelu(float x) { return x * gelu_s(x); }
 __device__ __forceinline__ float dgelu(float x) {
@@ -127,6 +127,69 @@ __device__ __forceinline__ void mlp_forward(const MlpW& w, bf16x4 xb, MlpAct& a)
 #pragma unroll
   for (int kb = 0; kb < 4; ++kb) c = mfma16(ld4(w.w3t + r * MLP_H + kb * 16 + 4 * g), a.h2[kb], c);
   a.y = c;
+}
+
+// The forward weights' operand fragments and the biases and normaliser entries
+// one lane needs, held in registers across the tiles of a wave (inference).
+struct MlpReg {
+  bf16x4 w1[4], w2[4][4], w3[4];
+  f32x4 b1[4], b2[4], b3;
+  float shift[4], scale[4];
+};
+
+__device__ __forceinline__ void mlp_load(const MlpW& w, MlpReg& m) {
+  const int l = threadIdx.x & 63, r = l & 15, g = l >> 4;
+#pragma unroll
+  for (int ob = 0; ob < 4; ++ob) {
+    m.w1[ob] = ld4(w.w1t + (ob * 16 + r) * MLP_INP + 4 * g);
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb) m.w2[ob][kb] = ld4(w.w2t + (ob * 16 + r) * MLP_H + kb * 16 + 4 * g);
+    m.w3[ob] = ld4(w.w3t + r * MLP_H + ob * 16 + 4 * g);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      m.b1[ob][v] = w.b1[ob * 16 + 4 * g + v];
+      m.b2[ob][v] = w.b2[ob * 16 + 4 * g + v];
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    m.b3[v] = w.b3[4 * g + v];
+    m.shift[v] = w.norm[4 * g + v];
+    m.scale[v] = w.norm[MLP_INP + 4 * g + v];
+  }
+}
+
+// the outputs of one tile (lanes 0..15: outputs 0, 1 of sample lane) from
+// register-resident weights; x holds the lane's 4 raw features 4g..4g+3
+__device__ __forceinline__ f32x4 mlp_infer(const MlpReg& m, const float* x) {
+  f32x4 xn;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) xn[v] = (x[v] - m.shift[v]) * m.scale[v];
+  const bf16x4 xb = tobf(xn);
+  bf16x4 h1[4];
+#pragma unroll
+  for (int ob = 0; ob < 4; ++ob) {
+    const f32x4 c = mfma16(m.w1[ob], xb, m.b1[ob]);
+    f32x4 h;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) h[v] = gelu(c[v]);
+    h1[ob] = tobf(h);
+  }
+  bf16x4 h2[4];
+#pragma unroll
+  for (int ob = 0; ob < 4; ++ob) {
+    f32x4 c = m.b2[ob];
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb) c = mfma16(m.w2[ob][kb], h1[kb], c);
+    f32x4 h;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) h[v] = gelu(c[v]);
+    h2[ob] = tobf(h);
+  }
+  f32x4 y = m.b3;
+#pragma unroll
+  for (int kb = 0; kb < 4; ++kb) y = mfma16(m.w3[kb], h2[kb], y);
+  return y;
 }
 
 // features 4g..4g+3 of one raw sample (p points at feature 4g), normalised and
