@@ -43,6 +43,7 @@ module diablo_gpu
   public :: dgpu_fft_xz_to_fourier, dgpu_fft_xz_to_physical, dgpu_synchronize
   public :: dgpu_last_step_ms, dgpu_field_doubles, dgpu_last_error_f
   public :: dgpu_download_les, dgpu_stats, dgpu_stats_doubles
+  public :: dgpu_set_variable_dt, dgpu_courant, dgpu_set_closure
 
   interface
     integer(c_int) function dgpu_create(cfg, h) bind(C, name="dgpu_create")
@@ -121,6 +122,26 @@ module diablo_gpu
     integer(c_int64_t) function dgpu_stats_doubles(cfg) bind(C, name="dgpu_stats_doubles")
       import :: c_int64_t, dgpu_config
       type(dgpu_config), intent(in) :: cfg
+    end function
+    integer(c_int) function dgpu_set_variable_dt(h, variable_dt, update_dt, cfl) bind(C, name="dgpu_set_variable_dt")
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: variable_dt, update_dt
+      real(c_double), value :: cfl
+    end function
+    integer(c_int) function dgpu_courant(h, time, time_limit, save_stats_time, save_flow_time, delta_t, ran) &
+        bind(C, name="dgpu_courant")
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), value :: time, time_limit, save_stats_time, save_flow_time
+      real(c_double), intent(out) :: delta_t
+      integer(c_int32_t), intent(out) :: ran
+    end function
+    ! les_model_type 6 (the learned closure, include/dgpu_closure.h): parameters
+    ! and normaliser as written by diablo3_amd.train_closure
+    integer(c_int) function dgpu_set_closure(h, params, norm) bind(C, name="dgpu_set_closure")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h, params, norm
     end function
     type(c_ptr) function dgpu_last_error_c() bind(C, name="dgpu_last_error")
       import :: c_ptr
