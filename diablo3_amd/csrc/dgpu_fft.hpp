@@ -132,10 +132,12 @@ __device__ __forceinline__ int swz(int n) {
   return n ^ ((n >> SH) & MASK);
 }
 
-// One Stockham stage of radix R over LINES lines of length N.
-template <typename T, int N, int LINES, int NT, int DIR, int R>
+// One Stockham stage of radix R over LINES lines of length N; Ns (the length of
+// the sub-transforms already done) is a compile-time constant, so the first stage
+// (Ns = 1, every twiddle 1) has no twiddle loads or products.
+template <typename T, int N, int LINES, int NT, int DIR, int R, int Ns>
 __device__ __forceinline__ void fft_stage(cx<T>* __restrict__ s, int ld,
-                                          const cx<T>* __restrict__ tw, int Ns, int tid) {
+                                          const cx<T>* __restrict__ tw, int tid) {
   constexpr int NB = LINES * (N / R);
   constexpr int PER = (NB + NT - 1) / NT;
   cx<T> v[PER][R];
@@ -152,12 +154,14 @@ __device__ __forceinline__ void fft_stage(cx<T>* __restrict__ s, int ld,
 #pragma unroll
       for (int r = 0; r < R; ++r) v[p][r] = line[swz<T>(j + r * (N / R))];
       const int k = j & (Ns - 1);
-      const int tstep = N / (Ns * R);
+      constexpr int tstep = N / (Ns * R);
+      if constexpr (Ns > 1) {
 #pragma unroll
-      for (int r = 1; r < R; ++r) {
-        cx<T> w = tw[r * k * tstep];
-        if constexpr (DIR > 0) w = conjg(w);
-        v[p][r] = v[p][r] * w;
+        for (int r = 1; r < R; ++r) {
+          cx<T> w = tw[r * k * tstep];
+          if constexpr (DIR > 0) w = conjg(w);
+          v[p][r] = v[p][r] * w;
+        }
       }
       if constexpr (R == 16) {
         dft16<DIR>(v[p]);
@@ -201,23 +205,31 @@ constexpr int stage_radix(int L, int st) {
 }
 constexpr int stage_count(int L) { return L % 4 == 0 ? L / 4 : L / 4 + 1; }
 
-template <typename T, int N, int LINES, int NT, int DIR, int ST, int NST>
+template <typename T, int N, int LINES, int NT, int DIR, int ST, int NST, int Ns>
 __device__ __forceinline__ void fft_stages(cx<T>* __restrict__ s, int ld, const cx<T>* __restrict__ tw,
-                                           int Ns, int tid) {
+                                           int tid) {
   if constexpr (ST < NST) {
     constexpr int R = stage_radix(ilog2(N), ST);
-    fft_stage<T, N, LINES, NT, DIR, R>(s, ld, tw, Ns, tid);
-    fft_stages<T, N, LINES, NT, DIR, ST + 1, NST>(s, ld, tw, Ns * R, tid);
+    fft_stage<T, N, LINES, NT, DIR, R, Ns>(s, ld, tw, tid);
+    fft_stages<T, N, LINES, NT, DIR, ST + 1, NST, Ns * R>(s, ld, tw, tid);
   }
 }
 
 // Batched in-place FFT of LINES lines of length N (power of two, >= 2) in LDS,
-// element n of line l at s[l*ld + swz<T>(n)].  Caller must __syncthreads() after
-// filling s; the routine ends with a barrier.
+// element n of line l at s[l*ld + swz<T>(n)], with the N twiddles tw (in LDS:
+// see stage_twiddles).  Caller must __syncthreads() after filling s and tw; the
+// routine ends with a barrier.
 template <typename T, int N, int LINES, int NT, int DIR>
 __device__ __forceinline__ void fft_lds(cx<T>* __restrict__ s, int ld,
                                         const cx<T>* __restrict__ tw, int tid) {
-  fft_stages<T, N, LINES, NT, DIR, 0, stage_count(ilog2(N))>(s, ld, tw, 1, tid);
+  fft_stages<T, N, LINES, NT, DIR, 0, stage_count(ilog2(N)), 1>(s, ld, tw, tid);
+}
+
+// copy the N twiddles exp(-2 pi i m / N) from global memory into the kernel's LDS
+// table (before the caller's first barrier)
+template <typename T, int N, int NT>
+__device__ __forceinline__ void stage_twiddles(cx<T>* __restrict__ stw, const cx<T>* __restrict__ tw, int tid) {
+  for (int m = tid; m < N; m += NT) stw[m] = tw[m];
 }
 
 }  // namespace dgpu

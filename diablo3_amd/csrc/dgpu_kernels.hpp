@@ -47,6 +47,8 @@ __global__ __launch_bounds__(NT) void k_inv_z(Ptrs<const cd*, MAXO> src, Ptrs<cx
                                               const double* __restrict__ kz, unsigned dzmask) {
   constexpr int LD = NZ + 1;
   __shared__ cx<T> s[TI * LD];
+  __shared__ cx<T> stw[NZ];  // twiddles
+  stage_twiddles<T, NZ, NT>(stw, tw, threadIdx.x);
   const int f = blockIdx.z, j = j0 + blockIdx.y, i0 = blockIdx.x * TI, tid = threadIdx.x;
   const cd* __restrict__ C = src.p[f] + (long long)j * g.pc;
   const bool dz = (dzmask >> f) & 1u;
@@ -60,7 +62,7 @@ __global__ __launch_bounds__(NT) void k_inv_z(Ptrs<const cd*, MAXO> src, Ptrs<cx
     s[ii * LD + swz<T>(z)] = v;
   }
   __syncthreads();
-  fft_lds<T, NZ, TI, NT, +1>(s, LD, tw, tid);
+  fft_lds<T, NZ, TI, NT, +1>(s, LD, stw, tid);
   cx<T>* __restrict__ H = dst.p[f] + (long long)j * g.ph;
   for (int idx = tid; idx < TI * NZ; idx += NT) {
     const int ii = idx % TI, z = idx / TI, i = i0 + ii;
@@ -74,6 +76,8 @@ __global__ __launch_bounds__(NT) void k_fwd_z(Ptrs<const cx<T>*, MAXO> src, Ptrs
                                               Geo g, int j0, const cx<T>* __restrict__ tw) {
   constexpr int LD = NZ + 1;
   __shared__ cx<T> s[TI * LD];
+  __shared__ cx<T> stw[NZ];  // twiddles
+  stage_twiddles<T, NZ, NT>(stw, tw, threadIdx.x);
   const int f = blockIdx.z, j = j0 + blockIdx.y, i0 = blockIdx.x * TI, tid = threadIdx.x;
   const cx<T>* __restrict__ H = src.p[f] + (long long)j * g.ph;
   for (int idx = tid; idx < TI * NZ; idx += NT) {
@@ -81,7 +85,7 @@ __global__ __launch_bounds__(NT) void k_fwd_z(Ptrs<const cx<T>*, MAXO> src, Ptrs
     s[ii * LD + swz<T>(z)] = (i < g.kxr) ? H[(long long)z * g.kxp + i] : cx<T>{T(0), T(0)};
   }
   __syncthreads();
-  fft_lds<T, NZ, TI, NT, -1>(s, LD, tw, tid);
+  fft_lds<T, NZ, TI, NT, -1>(s, LD, stw, tid);
   cd* __restrict__ C = dst.p[f] + (long long)j * g.pc;
   const double rnz = 1.0 / NZ;
   for (int idx = tid; idx < TI * g.kzr; idx += NT) {
@@ -147,6 +151,8 @@ __global__ __launch_bounds__(NT) void k_inv_x(Ptrs<const cx<T>*, MAXO> src, Ptrs
                                               Geo g, int j0, const cx<T>* __restrict__ tw,
                                               const double* __restrict__ kx, unsigned dxmask) {
   __shared__ cx<T> s[ZP * NX];
+  __shared__ cx<T> stw[NX];  // twiddles
+  stage_twiddles<T, NX, NT>(stw, tw, threadIdx.x);
   const int f = blockIdx.z, j = j0 + blockIdx.y, tid = threadIdx.x;
   const int zb = blockIdx.x * ZP * 2;
   const cx<T>* __restrict__ H = src.p[f] + (long long)j * g.ph;
@@ -158,7 +164,7 @@ __global__ __launch_bounds__(NT) void k_inv_x(Ptrs<const cx<T>*, MAXO> src, Ptrs
     s[q * NX + swz<T>(n)] = dx ? pair_spectrum_dx(A, B, n, NX, g.nkx, kx) : pair_spectrum(A, B, n, NX, g.nkx);
   }
   __syncthreads();
-  fft_lds<T, NX, ZP, NT, +1>(s, NX, tw, tid);
+  fft_lds<T, NX, ZP, NT, +1>(s, NX, stw, tid);
   T* __restrict__ R = dst.p[f] + (long long)j * g.pp;
   for (int idx = tid; idx < ZP * NX; idx += NT) {
     const int q = idx / NX, n = idx % NX, z = zb + 2 * q;
@@ -172,6 +178,8 @@ template <typename T, int NX, int ZP, int NT>
 __global__ __launch_bounds__(NT) void k_fwd_x(Ptrs<const T*, MAXO> src, Ptrs<cx<T>*, MAXO> dst,
                                               Geo g, int j0, const cx<T>* __restrict__ tw) {
   __shared__ cx<T> s[ZP * NX];
+  __shared__ cx<T> stw[NX];  // twiddles
+  stage_twiddles<T, NX, NT>(stw, tw, threadIdx.x);
   const int f = blockIdx.z, j = j0 + blockIdx.y, tid = threadIdx.x;
   const int zb = blockIdx.x * ZP * 2;
   const T* __restrict__ R = src.p[f] + (long long)j * g.pp;
@@ -180,7 +188,7 @@ __global__ __launch_bounds__(NT) void k_fwd_x(Ptrs<const T*, MAXO> src, Ptrs<cx<
     s[q * NX + swz<T>(n)] = {R[(long long)z * NX + n], R[(long long)(z + 1) * NX + n]};
   }
   __syncthreads();
-  fft_lds<T, NX, ZP, NT, -1>(s, NX, tw, tid);
+  fft_lds<T, NX, ZP, NT, -1>(s, NX, stw, tid);
   cx<T>* __restrict__ H = dst.p[f] + (long long)j * g.ph;
   const T rnx = T(1) / T(NX);
   for (int idx = tid; idx < ZP * g.kxr; idx += NT) {
@@ -224,6 +232,8 @@ __global__ __launch_bounds__(NT) void k_xprod(XProdArgs a) {
   constexpr int NF = 3 + NTH, NL = 3 * NF, NP = 8 + 3 * NTH, NO = 6 + 2 * NTH;
   static_assert(NP <= NL, "product lines overlay input lines");
   __shared__ cx<T> s[NL * NX];
+  __shared__ cx<T> stw[NX];  // twiddles
+  stage_twiddles<T, NX, NT>(stw, (const cx<T>*)a.tw, threadIdx.x);
   const Geo& g = a.g;
   const int tid = threadIdx.x, z = 2 * blockIdx.x, j = a.jlo + blockIdx.y;
   const cx<T>* __restrict__ tw = (const cx<T>*)a.tw;
@@ -234,7 +244,7 @@ __global__ __launch_bounds__(NT) void k_xprod(XProdArgs a) {
     s[l * NX + swz<T>(n)] = pair_spectrum(H + (long long)z * g.kxp, H + (long long)(z + 1) * g.kxp, n, NX, g.nkx);
   }
   __syncthreads();
-  fft_lds<T, NX, NL, NT, +1>(s, NX, tw, tid);
+  fft_lds<T, NX, NL, NT, +1>(s, NX, stw, tid);
   const bool vrow = (j >= a.jstart && j <= a.jend);
   const bool v2row = (j >= 2 && j <= g.ny);
   const T dyj = (T)a.dy[j];
@@ -327,7 +337,7 @@ __global__ __launch_bounds__(NT) void k_xprod(XProdArgs a) {
     for (int p = 0; p < NP; ++p) s[p * NX + swz<T>(x)] = P[p];
   }
   __syncthreads();
-  fft_lds<T, NX, NP, NT, -1>(s, NX, tw, tid);
+  fft_lds<T, NX, NP, NT, -1>(s, NX, stw, tid);
   const T rnx = T(1) / T(NX);
   for (int idx = tid; idx < NO * g.kxr; idx += NT) {
     const int o = idx / g.kxr, i = idx % g.kxr;
@@ -388,6 +398,8 @@ template <typename T, int NZ, int TI, int NT>
 __global__ __launch_bounds__(NT) void k_fwdz_nl(NlArgs a) {
   constexpr int LD = NZ + 1;
   __shared__ cx<T> s[2 * TI * LD];
+  __shared__ cx<T> stw[NZ];  // twiddles
+  stage_twiddles<T, NZ, NT>(stw, (const cx<T>*)a.tw, threadIdx.x);
   const Geo& g = a.g;
   const int tid = threadIdx.x, j = a.jlo + blockIdx.y, i0 = blockIdx.x * TI, e = blockIdx.z;
   const cx<T>* __restrict__ tw = (const cx<T>*)a.tw;
@@ -405,7 +417,7 @@ __global__ __launch_bounds__(NT) void k_fwdz_nl(NlArgs a) {
     s[(which * TI + ii) * LD + swz<T>(z)] = (i < g.kxr) ? Hs[(long long)z * g.kxp + i] : cx<T>{T(0), T(0)};
   }
   __syncthreads();
-  fft_lds<T, NZ, 2 * TI, NT, -1>(s, LD, tw, tid);
+  fft_lds<T, NZ, 2 * TI, NT, -1>(s, LD, stw, tid);
   cx<T>* NL = (cx<T>*)a.nl[e] + (long long)j * g.pc;
   for (int idx = tid; idx < TI * g.kzr; idx += NT) {
     const int ii = idx % TI, k = idx / TI, i = i0 + ii;

@@ -512,7 +512,7 @@ template <typename T, int NTH, bool LES = false>
 int launch_xprod(dgpu_handle* h, XProdArgs& a, int nj) {
   a.tw = sizeof(T) == 8 ? h->twx64 : h->twx32;
   DG_SWITCH_N(h->g.nx, {
-    constexpr size_t lds = sizeof(cx<T>) * 3 * (3 + NTH) * N_;
+    constexpr size_t lds = sizeof(cx<T>) * (3 * (3 + NTH) + 1) * N_;  // lines + twiddle table
     if constexpr (lds > 163840) {
       return fail(2, "this nx needs the mixed-precision mode (x pass exceeds LDS in FP64)");
     } else {
