@@ -40,9 +40,14 @@ def kernel_bytes(cfg, mixed):
     H = cfg.kxr * cfg.nz * 2 * s * ny2
     Hj = cfg.kxr * cfg.nz * 2 * s * nj
     Cj_t = cfg.kxr * cfg.kzr * 2 * s * nj   # nonlinear terms, complex T
+    Rt = cfg.nx * cfg.nz * s * ny2
+    Rtj = cfg.nx * cfg.nz * s * nj
     out = {
         "k_inv_z": nf * C + nf * H,
-        "k_xprod": nf * H + no * Hj,
+        # x pass in two kernels: inverse x transforms (x-spectra in, physical
+        # fields out), then the products reading the physical fields once
+        "k_inv_x": nf * H + nf * Rt,
+        "k_xprod": nf * Rtj + no * Hj,
         "k_fwdz_nl": no * Hj + nf * Cj_t,
         # nonlinear terms, state, cp and (rk 2, 3: 2 of 3 launches) old cf in; cf and state out
         "k_cn": nf * Cj_t + (nf + 1) * C + (2.0 / 3.0) * nf * Cj + nf * Cj + nf * C,
@@ -51,8 +56,6 @@ def kernel_bytes(cfg, mixed):
     if cfg.use_les:
         # physical fields: R8 FP64, Rt in the transform precision; na SGS addition fields
         R8 = cfg.nx * cfg.nz * 8 * ny2
-        Rt = cfg.nx * cfg.nz * s * ny2
-        Rtj = cfg.nx * cfg.nz * s * nj
         Hf = cfg.kxr * cfg.nz * 2 * 4 * ny2        # float x-spectrum (increment transforms)
         Rf = cfg.nx * cfg.nz * 4 * ny2
         na = 7 + 2 * cfg.n_th

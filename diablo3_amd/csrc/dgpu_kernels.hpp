@@ -225,26 +225,46 @@ struct XProdArgs {
   // (tau_11, tau_13, tau_33, the two stress fluxes, the two GY stresses, the
   // scalar fluxes); null for DNS
   const void* aug[7 + 2 * MAXTH];
+  // SPLIT: the fields in physical space ([j][z][x], T), all planes; the kernel
+  // then reads them instead of inverse-transforming hin
+  const void* phys[MAXF];
 };
 
-template <typename T, int NX, int NT, int NTH, bool LES>
+// SPLIT = false: each block inverse-transforms planes j-1, j, j+1 of every field
+// itself (no physical-space traffic).  SPLIT = true: a separate inverse x
+// transform has written the physical fields once per plane and the block reads
+// the three planes of its x columns from them; only the product lines need LDS.
+template <typename T, int NX, int NT, int NTH, bool LES, bool SPLIT = false>
 __global__ __launch_bounds__(NT) void k_xprod(XProdArgs a) {
   constexpr int NF = 3 + NTH, NL = 3 * NF, NP = 8 + 3 * NTH, NO = 6 + 2 * NTH;
   static_assert(NP <= NL, "product lines overlay input lines");
-  __shared__ cx<T> s[NL * NX];
+  __shared__ cx<T> s[(SPLIT ? NP : NL) * NX];
   __shared__ cx<T> stw[NX];  // twiddles
   stage_twiddles<T, NX, NT>(stw, (const cx<T>*)a.tw, threadIdx.x);
   const Geo& g = a.g;
   const int tid = threadIdx.x, z = 2 * blockIdx.x, j = a.jlo + blockIdx.y;
   const cx<T>* __restrict__ tw = (const cx<T>*)a.tw;
-  // inverse x transforms of (field f, plane j-1+d) into line 3f+d
-  for (int idx = tid; idx < NL * NX; idx += NT) {
-    const int l = idx / NX, n = idx % NX, f = l / 3, jj = j - 1 + (l - 3 * f);
-    const cx<T>* H = (const cx<T>*)a.hin[f] + (long long)jj * g.ph;
-    s[l * NX + swz<T>(n)] = pair_spectrum(H + (long long)z * g.kxp, H + (long long)(z + 1) * g.kxp, n, NX, g.nkx);
+  if constexpr (!SPLIT) {
+    // inverse x transforms of (field f, plane j-1+d) into line 3f+d
+    for (int idx = tid; idx < NL * NX; idx += NT) {
+      const int l = idx / NX, n = idx % NX, f = l / 3, jj = j - 1 + (l - 3 * f);
+      const cx<T>* H = (const cx<T>*)a.hin[f] + (long long)jj * g.ph;
+      s[l * NX + swz<T>(n)] = pair_spectrum(H + (long long)z * g.kxp, H + (long long)(z + 1) * g.kxp, n, NX, g.nkx);
+    }
+    __syncthreads();
+    fft_lds<T, NX, NL, NT, +1>(s, NX, stw, tid);
+  } else {
+    __syncthreads();  // the twiddle table
   }
-  __syncthreads();
-  fft_lds<T, NX, NL, NT, +1>(s, NX, stw, tid);
+  // (field f, plane j-1+d) of the line pair (z, z+1) at column x
+  auto field = [&](int f, int d, int x) -> cx<T> {
+    if constexpr (SPLIT) {
+      const T* R = (const T*)a.phys[f] + (long long)(j - 1 + d) * g.pp + (long long)z * NX + x;
+      return cx<T>{R[0], R[NX]};
+    } else {
+      return s[(f * 3 + d) * NX + swz<T>(x)];
+    }
+  };
   const bool vrow = (j >= a.jstart && j <= a.jend);
   const bool v2row = (j >= 2 && j <= g.ny);
   const T dyj = (T)a.dy[j];
@@ -254,15 +274,15 @@ __global__ __launch_bounds__(NT) void k_xprod(XProdArgs a) {
     cx<T> U1[3], U2[3], U3[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      U1[d] = s[(0 * 3 + d) * NX + swz<T>(x)];
-      U2[d] = s[(1 * 3 + d) * NX + swz<T>(x)];
-      U3[d] = s[(2 * 3 + d) * NX + swz<T>(x)];
+      U1[d] = field(0, d, x);
+      U2[d] = field(1, d, x);
+      U3[d] = field(2, d, x);
     }
     cx<T> TH[NTH > 0 ? NTH : 1][3];
 #pragma unroll
     for (int n = 0; n < NTH; ++n)
 #pragma unroll
-      for (int d = 0; d < 3; ++d) TH[n][d] = s[((3 + n) * 3 + d) * NX + swz<T>(x)];
+      for (int d = 0; d < 3; ++d) TH[n][d] = field(3 + n, d, x);
     cx<T> P[NP];
     // the two real lines of the pair are the .x and .y components
 #define DG_PROD(expr_x, expr_y) cx<T>{expr_x, expr_y}
@@ -332,7 +352,8 @@ __global__ __launch_bounds__(NT) void k_xprod(XProdArgs a) {
         P[9 + 3 * n] = P[9 + 3 * n] + A(8 + 2 * n);
       }
     }
-    // each thread rewrites only column x, which only it has read
+    // each thread rewrites only column x, which only it has read (SPLIT: the
+    // lines are free)
 #pragma unroll
     for (int p = 0; p < NP; ++p) s[p * NX + swz<T>(x)] = P[p];
   }

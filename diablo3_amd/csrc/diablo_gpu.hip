@@ -24,6 +24,7 @@
 #include "dgpu_error.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <type_traits>
 #include <cstdio>
@@ -90,6 +91,11 @@ struct dgpu_handle {
   void* HinZD[MAXF] = {};
   void* PgD[3 * MAXF] = {};
   void* Hout[MAXO] = {};
+  // the x pass in two kernels (inverse x transforms, then products reading the
+  // physical fields) or fused (each block inverse-transforms its three planes);
+  // DNS handles keep the physical fields in Px (T), LES handles use Pg
+  int xsplit = 1;
+  void* Px[MAXF] = {};
   void *twx64 = nullptr, *twz64 = nullptr, *twx32 = nullptr, *twz32 = nullptr;
   double* d_fac = nullptr;
   double fac_dt = NAN;
@@ -371,9 +377,9 @@ int ensure_fft_scratch(dgpu_handle* h) {
 // --------------------------------------------------------------------------------
 // per-kernel timing
 // --------------------------------------------------------------------------------
-constexpr int NKERN = 11;
+constexpr int NKERN = 12;
 const char* const kKernelNames[NKERN] = {"k_inv_z",   "k_xprod",  "k_fwdz_nl", "k_cn",      "k_project", "k_bc_post",
-                                         "les_grad", "k_amd",    "k_cn_phys", "les_cn_fft", "k_closure"};
+                                         "les_grad", "k_amd",    "k_cn_phys", "les_cn_fft", "k_closure", "k_inv_x"};
 
 hipEvent_t pool_event(dgpu_handle* h) {
   if (!h->ev_pool.empty()) {
@@ -508,17 +514,18 @@ int launch_fwd_x(dgpu_handle* h, int nf, T* const* src, void* const* dst, int j0
 
 constexpr int xprod_nt(int nx) { return nx <= 32 ? 64 : (nx == 64 ? 128 : 256); }
 
-template <typename T, int NTH, bool LES = false>
+template <typename T, int NTH, bool LES = false, bool SPLIT = false>
 int launch_xprod(dgpu_handle* h, XProdArgs& a, int nj) {
   a.tw = sizeof(T) == 8 ? h->twx64 : h->twx32;
   DG_SWITCH_N(h->g.nx, {
-    constexpr size_t lds = sizeof(cx<T>) * (3 * (3 + NTH) + 1) * N_;  // lines + twiddle table
+    // input (or product) lines + twiddle table
+    constexpr size_t lds = sizeof(cx<T>) * ((SPLIT ? 8 + 3 * NTH : 3 * (3 + NTH)) + 1) * N_;
     if constexpr (lds > 163840) {
       return fail(2, "this nx needs the mixed-precision mode (x pass exceeds LDS in FP64)");
     } else {
       constexpr int NT = xprod_nt(N_);
       dim3 grid(h->g.nz / 2, nj, 1);
-      k_xprod<T, N_, NT, NTH, LES><<<grid, NT, 0, h->st>>>(a);
+      k_xprod<T, N_, NT, NTH, LES, SPLIT><<<grid, NT, 0, h->st>>>(a);
     }
   });
   HIPCHK(hipGetLastError());
@@ -964,8 +971,13 @@ int substep_les(dgpu_handle* h, int rk, double delta_t, double time, int time_st
   for (int f = 0; f < nf; ++f) xa.hin[f] = h->Hin[f];
   for (int o = 0; o < 6 + 2 * NTH; ++o) xa.hout[o] = h->Hout[o];
   for (int m = 0; m < 7 + 2 * NTH; ++m) xa.aug[m] = h->Aug[m];
+  for (int f = 0; f < nf; ++f) {
+    xa.phys[f] = h->Pg[3 * f];  // the gradient stage's physical fields
+    if (!xa.phys[f]) return fail(12, "x pass: physical field buffer missing");
+  }
   if ((rc = tbeg(h, 1))) return rc;
-  if ((rc = launch_xprod<T, NTH, true>(h, xa, nj))) return rc;
+  if ((rc = h->xsplit ? launch_xprod<T, NTH, true, true>(h, xa, nj) : launch_xprod<T, NTH, true>(h, xa, nj)))
+    return rc;
   if ((rc = tend(h))) return rc;
   NlArgs na{};
   na.g = g; na.jlo = jlo; na.jstart = c.jstart; na.jend = c.jend;
@@ -1125,9 +1137,21 @@ int substep(dgpu_handle* h, int rk, double delta_t, double time, int time_step, 
   xa.dy = h->d_dy; xa.dyf = h->d_dyf; xa.kx = h->d_kx;
   for (int f = 0; f < nf; ++f) xa.hin[f] = h->Hin[f];
   for (int o = 0; o < 6 + 2 * h->nth; ++o) xa.hout[o] = h->Hout[o];
+  if (h->xsplit) {
+    // the physical fields once per plane, then the products read them
+    for (int f = 0; f < nf; ++f)
+      if (!h->Px[f]) return fail(12, "x pass: physical field buffer missing");
+    if ((rc = tbeg(h, 11))) return rc;
+    rc = mixed ? launch_inv_x<float>(h, nf, h->Hin, (float* const*)h->Px, 0, g.ny2)
+               : launch_inv_x<double>(h, nf, h->Hin, (double* const*)h->Px, 0, g.ny2);
+    if (rc || (rc = tend(h))) return rc;
+    for (int f = 0; f < nf; ++f) xa.phys[f] = h->Px[f];
+  }
   if ((rc = tbeg(h, 1))) return rc;
   rc = by_nth(h->nth, [&](auto NTHc) {
     constexpr int NTH = decltype(NTHc)::value;
+    if (h->xsplit)
+      return mixed ? launch_xprod<float, NTH, false, true>(h, xa, nj) : launch_xprod<double, NTH, false, true>(h, xa, nj);
     return mixed ? launch_xprod<float, NTH>(h, xa, nj) : launch_xprod<double, NTH>(h, xa, nj);
   });
   if (rc || (rc = tend(h))) return rc;
@@ -1223,6 +1247,12 @@ int dgpu_create(const dgpu_config* cfg, dgpu_handle** out) {
   ALLOC(h->P, cbytes);
   for (int f = 0; f < nf; ++f) ALLOC(h->Hin[f], hbytes);
   for (int o = 0; o < no; ++o) ALLOC(h->Hout[o], hbytes);
+  // DGPU_XPASS=fused selects the single-kernel x pass (measurement of the two forms)
+  if (const char* xp = std::getenv("DGPU_XPASS")) h->xsplit = std::strcmp(xp, "fused") != 0;
+  if (h->xsplit && !cfg->use_les) {
+    const size_t pbytes = (h->prec == DGPU_MIXED ? sizeof(float) : sizeof(double)) * g.pp * n2;
+    for (int f = 0; f < nf; ++f) ALLOC(h->Px[f], pbytes);
+  }
   ALLOC(h->twx64, sizeof(cd) * g.nx); ALLOC(h->twz64, sizeof(cd) * g.nz);
   ALLOC(h->twx32, sizeof(cf) * g.nx); ALLOC(h->twz32, sizeof(cf) * g.nz);
   ALLOC(h->d_fac, sizeof(double) * 3 * MAXF * 5 * n2);
@@ -1243,6 +1273,9 @@ int dgpu_create(const dgpu_config* cfg, dgpu_handle** out) {
     for (int m = 0; m < 7 + 2 * h->nth; ++m) ALLOC(h->Aug[m], tbytes);
     ALLOC(h->CPs, 2 * (size_t)nf * rbytes);
     ALLOC(h->d_dlt, sizeof(double) * 2 * n2);
+    // the DNS branch of an LES handle (before the spin-up gate) keeps its physical
+    // fields in the gradient stage's buffers
+    for (int f = 0; f < nf; ++f) h->Px[f] = h->Pg[3 * f];
     if (cfg->les_model_type == 6) {
       ALLOC(h->Cgy, tbytes);
       ALLOC(h->Kgy, tbytes);

@@ -238,7 +238,7 @@ class Solver:
     def timing_read(self):
         """{kernel name: (total device ms, launches)} since timing(True), launched kernels only."""
         out = {}
-        for k in range(11):
+        for k in range(12):
             ms, n, name = ctypes.c_double(), ctypes.c_int64(), ctypes.c_char_p()
             self._check(load().dgpu_timing_read(self.h, k, ctypes.byref(ms), ctypes.byref(n),
                                                 ctypes.byref(name)))
