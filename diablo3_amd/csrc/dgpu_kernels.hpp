@@ -52,14 +52,25 @@ __global__ __launch_bounds__(NT) void k_inv_z(Ptrs<const cd*, MAXO> src, Ptrs<cx
   const int f = blockIdx.z, j = j0 + blockIdx.y, i0 = blockIdx.x * TI, tid = threadIdx.x;
   const cd* __restrict__ C = src.p[f] + (long long)j * g.pc;
   const bool dz = (dzmask >> f) & 1u;
-  for (int idx = tid; idx < TI * NZ; idx += NT) {
-    const int ii = idx % TI, z = idx / TI, i = i0 + ii, p = zpack(g, z);
-    cx<T> v = {T(0), T(0)};
-    if (i < g.kxr && p >= 0) {
-      const cd c = C[(long long)p * g.kxp + i];
-      v = cvt<T>(dz ? ik(kz[p], c) : c);
+  // every load of the thread first, then the LDS stores (the loads overlap)
+  constexpr int PER = (TI * NZ + NT - 1) / NT;
+  cx<T> v[PER];
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int idx = tid + q * NT;
+    v[q] = {T(0), T(0)};
+    if (PER * NT == TI * NZ || idx < TI * NZ) {
+      const int ii = idx % TI, z = idx / TI, i = i0 + ii, p = zpack(g, z);
+      if (i < g.kxr && p >= 0) {
+        const cd c = C[(long long)p * g.kxp + i];
+        v[q] = cvt<T>(dz ? ik(kz[p], c) : c);
+      }
     }
-    s[ii * LD + swz<T>(z)] = v;
+  }
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int idx = tid + q * NT;
+    if (PER * NT == TI * NZ || idx < TI * NZ) s[(idx % TI) * LD + swz<T>(idx / TI)] = v[q];
   }
   __syncthreads();
   fft_lds<T, NZ, TI, NT, +1>(s, LD, stw, tid);
@@ -157,11 +168,23 @@ __global__ __launch_bounds__(NT) void k_inv_x(Ptrs<const cx<T>*, MAXO> src, Ptrs
   const int zb = blockIdx.x * ZP * 2;
   const cx<T>* __restrict__ H = src.p[f] + (long long)j * g.ph;
   const bool dx = (dxmask >> f) & 1u;
-  for (int idx = tid; idx < ZP * NX; idx += NT) {
-    const int q = idx / NX, n = idx % NX, z = zb + 2 * q;
-    const cx<T>* A = H + (long long)z * g.kxp;
-    const cx<T>* B = H + (long long)(z + 1) * g.kxp;
-    s[q * NX + swz<T>(n)] = dx ? pair_spectrum_dx(A, B, n, NX, g.nkx, kx) : pair_spectrum(A, B, n, NX, g.nkx);
+  // every load of the thread first, then the LDS stores (the loads overlap)
+  constexpr int PER = (ZP * NX + NT - 1) / NT;
+  cx<T> v[PER];
+#pragma unroll
+  for (int p = 0; p < PER; ++p) {
+    const int idx = tid + p * NT;
+    if (PER * NT == ZP * NX || idx < ZP * NX) {
+      const int q = idx / NX, n = idx % NX, z = zb + 2 * q;
+      const cx<T>* A = H + (long long)z * g.kxp;
+      const cx<T>* B = H + (long long)(z + 1) * g.kxp;
+      v[p] = dx ? pair_spectrum_dx(A, B, n, NX, g.nkx, kx) : pair_spectrum(A, B, n, NX, g.nkx);
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < PER; ++p) {
+    const int idx = tid + p * NT;
+    if (PER * NT == ZP * NX || idx < ZP * NX) s[(idx / NX) * NX + swz<T>(idx % NX)] = v[p];
   }
   __syncthreads();
   fft_lds<T, NX, ZP, NT, +1>(s, NX, stw, tid);
@@ -415,37 +438,77 @@ __device__ __forceinline__ void eq_rows(int e, int jstart, int jend, const int* 
   else { lo = jst[e - 3]; hi = jet[e - 3]; }
 }
 
-template <typename T, int NZ, int TI, int NT>
-__global__ __launch_bounds__(NT) void k_fwdz_nl(NlArgs a) {
+// Persistent blocks: tile t = (column block, plane, equation) runs t = blockIdx.x,
+// blockIdx.x + gridDim.x, ...; the next tile's spectra are loaded into registers
+// while the current tile is transformed, so the loads overlap the FFT.
+template <typename T, int NZ, int TI, int NT, bool PERSIST>
+__global__ __launch_bounds__(NT) void k_fwdz_nl(NlArgs a, int ntiles) {
   constexpr int LD = NZ + 1;
+  constexpr int NE = 2 * TI * NZ;              // elements per tile
+  constexpr int PER = (NE + NT - 1) / NT;      // per thread
   __shared__ cx<T> s[2 * TI * LD];
   __shared__ cx<T> stw[NZ];  // twiddles
   stage_twiddles<T, NZ, NT>(stw, (const cx<T>*)a.tw, threadIdx.x);
   const Geo& g = a.g;
-  const int tid = threadIdx.x, j = a.jlo + blockIdx.y, i0 = blockIdx.x * TI, e = blockIdx.z;
-  const cx<T>* __restrict__ tw = (const cx<T>*)a.tw;
+  const int tid = threadIdx.x;
+  const int nib = (g.kxr + TI - 1) / TI, nj = g.ny - a.jlo + 1;
   const T rnz = T(1) / T(NZ);
-  int lo, hi;
-  eq_rows(e, a.jstart, a.jend, a.jstart_th, a.jend_th, g.ny, lo, hi);
-  if (j < lo || j > hi) return;  // uniform over the block
-  const int oN = e < 3 ? e : 6 + 2 * (e - 3);
-  const int oZ = e < 3 ? 3 + e : 7 + 2 * (e - 3);
-  const cx<T>* HN = (const cx<T>*)a.h[oN] + (long long)j * g.ph;
-  const cx<T>* HZ = (const cx<T>*)a.h[oZ] + (long long)j * g.ph;
-  for (int idx = tid; idx < 2 * TI * NZ; idx += NT) {
-    const int ii = idx % TI, r = idx / TI, which = r / NZ, z = r - which * NZ, i = i0 + ii;
-    const cx<T>* Hs = which ? HZ : HN;
-    s[(which * TI + ii) * LD + swz<T>(z)] = (i < g.kxr) ? Hs[(long long)z * g.kxp + i] : cx<T>{T(0), T(0)};
-  }
-  __syncthreads();
-  fft_lds<T, NZ, 2 * TI, NT, -1>(s, LD, stw, tid);
-  cx<T>* NL = (cx<T>*)a.nl[e] + (long long)j * g.pc;
-  for (int idx = tid; idx < TI * g.kzr; idx += NT) {
-    const int ii = idx % TI, k = idx / TI, i = i0 + ii;
-    if (i >= g.kxr) continue;
-    const int zp = zpos(g, k);
-    const cx<T> fn = s[ii * LD + swz<T>(zp)], fz = s[(TI + ii) * LD + swz<T>(zp)];
-    NL[(long long)k * g.kxp + i] = rnz * (fn - ik((T)a.kz[k], fz));
+  // tile -> (equation e, plane j, first column i0); live: j is one of e's rows
+  auto coords = [&](int t, int& e, int& j, int& i0) -> bool {
+    const int ib = t % nib, r = t / nib;
+    j = a.jlo + r % nj;
+    e = r / nj;
+    i0 = ib * TI;
+    int lo, hi;
+    eq_rows(e, a.jstart, a.jend, a.jstart_th, a.jend_th, g.ny, lo, hi);
+    return j >= lo && j <= hi;
+  };
+  cx<T> buf[PER];
+  auto load = [&](int t) {
+    if (t >= ntiles) return;
+    int e, j, i0;
+    const bool live = coords(t, e, j, i0);
+    const int oN = e < 3 ? e : 6 + 2 * (e - 3);
+    const int oZ = e < 3 ? 3 + e : 7 + 2 * (e - 3);
+    const cx<T>* HN = (const cx<T>*)a.h[oN] + (long long)j * g.ph;
+    const cx<T>* HZ = (const cx<T>*)a.h[oZ] + (long long)j * g.ph;
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+      const int idx = tid + p * NT;
+      const int ii = idx % TI, r = idx / TI, which = r / NZ, z = r - which * NZ, i = i0 + ii;
+      const cx<T>* Hs = which ? HZ : HN;
+      buf[p] = (live && idx < NE && i < g.kxr) ? Hs[(long long)z * g.kxp + i] : cx<T>{T(0), T(0)};
+    }
+  };
+  const int step = PERSIST ? (int)gridDim.x : ntiles;  // one tile per block unless persistent
+  int t = blockIdx.x;
+  load(t);
+  __syncthreads();  // the twiddle table
+  for (; t < ntiles; t += step) {
+    int e, j, i0;
+    const bool live = coords(t, e, j, i0);
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+      const int idx = tid + p * NT;
+      if (idx < NE) {
+        const int ii = idx % TI, r = idx / TI, which = r / NZ, z = r - which * NZ;
+        s[(which * TI + ii) * LD + swz<T>(z)] = buf[p];
+      }
+    }
+    __syncthreads();
+    if (PERSIST) load(t + step);
+    fft_lds<T, NZ, 2 * TI, NT, -1>(s, LD, stw, tid);
+    if (live) {
+      cx<T>* NL = (cx<T>*)a.nl[e] + (long long)j * g.pc;
+      for (int idx = tid; idx < TI * g.kzr; idx += NT) {
+        const int ii = idx % TI, k = idx / TI, i = i0 + ii;
+        if (i >= g.kxr) continue;
+        const int zp = zpos(g, k);
+        const cx<T> fn = s[ii * LD + swz<T>(zp)], fz = s[(TI + ii) * LD + swz<T>(zp)];
+        NL[(long long)k * g.kxp + i] = rnz * (fn - ik((T)a.kz[k], fz));
+      }
+    }
+    __syncthreads();  // s is rewritten by the next tile
   }
 }
 

@@ -95,6 +95,11 @@ struct dgpu_handle {
   // physical fields) or fused (each block inverse-transforms its three planes);
   // DNS handles keep the physical fields in Px (T), LES handles use Pg
   int xsplit = 1;
+  // persistent kernels: blocks to launch (a few per CU)
+  int resident_blocks = 1024;
+  // k_fwdz_nl launch variant (DGPU_FWDZ: 0 columns per block TI, 1 TI/2,
+  // 2 TI persistent, 3 TI/2 persistent; -1 the measured default per precision)
+  int fwdz_variant = -1;
   void* Px[MAXF] = {};
   void *twx64 = nullptr, *twz64 = nullptr, *twx32 = nullptr, *twz32 = nullptr;
   double* d_fac = nullptr;
@@ -536,9 +541,26 @@ template <typename T>
 int launch_fwdz_nl(dgpu_handle* h, NlArgs& a, int nj) {
   a.tw = sizeof(T) == 8 ? h->twz64 : h->twz32;
   DG_SWITCH_N(h->g.nz, {
-    constexpr int TI = ti_for(N_, sizeof(cx<T>));
-    dim3 grid((h->g.kxr + TI - 1) / TI, nj, 3 + h->nth);
-    k_fwdz_nl<T, N_, TI, 256><<<grid, 256, 0, h->st>>>(a);
+    constexpr int TI0 = ti_for(N_, sizeof(cx<T>));
+    auto go = [&](auto TIc, auto Pc) {
+      constexpr int TI = decltype(TIc)::value;
+      constexpr bool PERSIST = decltype(Pc)::value;
+      const int ntiles = (h->g.kxr + TI - 1) / TI * nj * (3 + h->nth);
+      const int nblk = PERSIST ? std::min(ntiles, h->resident_blocks) : ntiles;
+      k_fwdz_nl<T, N_, TI, 256, PERSIST><<<nblk, 256, 0, h->st>>>(a, ntiles);
+    };
+    using I0 = std::integral_constant<int, TI0>;
+    using I1 = std::integral_constant<int, (TI0 > 1 ? TI0 / 2 : 1)>;
+    using P0 = std::false_type;
+    using P1 = std::true_type;
+    // measured at 256^3: FP64 best persistent with TI columns, float with TI/2 per block
+    const int v = h->fwdz_variant >= 0 ? h->fwdz_variant : (sizeof(T) == 8 ? 2 : 1);
+    switch (v) {
+      case 1: go(I1{}, P0{}); break;
+      case 2: go(I0{}, P1{}); break;
+      case 3: go(I1{}, P1{}); break;
+      default: go(I0{}, P0{}); break;
+    }
   });
   HIPCHK(hipGetLastError());
   return 0;
@@ -1247,6 +1269,13 @@ int dgpu_create(const dgpu_config* cfg, dgpu_handle** out) {
   ALLOC(h->P, cbytes);
   for (int f = 0; f < nf; ++f) ALLOC(h->Hin[f], hbytes);
   for (int o = 0; o < no; ++o) ALLOC(h->Hout[o], hbytes);
+  {
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && ncu > 0)
+      h->resident_blocks = 2 * ncu;
+  }
+  if (const char* fv = std::getenv("DGPU_FWDZ")) h->fwdz_variant = std::atoi(fv);
   // DGPU_XPASS=fused selects the single-kernel x pass (measurement of the two forms)
   if (const char* xp = std::getenv("DGPU_XPASS")) h->xsplit = std::strcmp(xp, "fused") != 0;
   if (h->xsplit && !cfg->use_les) {
