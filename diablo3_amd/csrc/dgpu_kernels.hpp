@@ -513,6 +513,20 @@ __global__ __launch_bounds__(NT) void k_fwdz_nl(NlArgs a, int ntiles) {
 }
 
 // ---------------------------------------------------------------------------
+// filter_chan (for/solver.f90:1597-1685): the horizontal filter of one scalar,
+// cth *= sigma(kx, kz) on rows [jlo, jhi]; sigma [kzr][kxp] from the host
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_filter_th(Geo g, cd* __restrict__ th, const double* __restrict__ sigma,
+                                                     int jlo, int jhi) {
+  const long long c = (long long)blockIdx.x * 256 + threadIdx.x;
+  const int j = jlo + blockIdx.y;
+  if (c >= g.pc || j > jhi) return;
+  if (c % g.kxp >= g.kxr) return;
+  const long long o = (long long)j * g.pc + c;
+  th[o] = sigma[c] * th[o];
+}
+
+// ---------------------------------------------------------------------------
 // Column kernel.  Per (kx, kz) column and all rows j = 0..Nyp+1:
 //  1. Fourier-space assembly of rk_chan_1 (for/solver.f90:385-838, 909-973, and
 //     user_rhs_chan_fourier 35-111): new RK memory cf and the right-hand side cr,

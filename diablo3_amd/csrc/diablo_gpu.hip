@@ -100,6 +100,8 @@ struct dgpu_handle {
   // k_fwdz_nl launch variant (DGPU_FWDZ: 0 columns per block TI, 1 TI/2,
   // 2 TI persistent, 3 TI/2 persistent; -1 the measured default per precision)
   int fwdz_variant = -1;
+  // k_cn column tile (DGPU_CN_TC = 8, 4, 2 or 1; 0: the widest that fits)
+  int cn_tc = 0;
   void* Px[MAXF] = {};
   void *twx64 = nullptr, *twz64 = nullptr, *twx32 = nullptr, *twz32 = nullptr;
   double* d_fac = nullptr;
@@ -120,6 +122,7 @@ struct dgpu_handle {
   double* Rp[MAXF] = {};
   double* CPs = nullptr;
   double* d_dlt = nullptr;  // [2][ny2]: delta_yf, delta_y of the AMD filter width
+  double* d_sigma = nullptr;  // filter_chan's horizontal filter, packed [kzr][kxp]
   // les_model_type 6: the network (BF16 forward weights w1t, w2t, w3t; FP32
   // parameter block for the biases; normaliser) and the per-point coefficients
   unsigned short* d_netw = nullptr;
@@ -577,8 +580,10 @@ ColShape col_shape(const dgpu_handle* h, bool cn) {
   ColShape cs{};
   if (cn) {
     // k_cn: TC columns x all fields in LDS; one wave solves one field of the TC
-    // columns in 64/TC chunks of L <= 32 rows (LMAX = 4 RPT >= L)
-    for (int tc : {8, 4, 2, 1}) {
+    // columns in 64/TC chunks of L <= 32 rows (LMAX = 4 RPT >= L); the widest TC
+    // that fits, or h->cn_tc when it fits
+    for (int tc : {h->cn_tc, 8, 4, 2, 1}) {
+      if (tc <= 0) continue;
       const size_t lds = (size_t)tc * ny2 * 16 * nf;
       const int rg = 256 / tc, need = (ny2 + rg - 1) / rg;
       int rpt = 1;
@@ -1276,6 +1281,10 @@ int dgpu_create(const dgpu_config* cfg, dgpu_handle** out) {
       h->resident_blocks = 2 * ncu;
   }
   if (const char* fv = std::getenv("DGPU_FWDZ")) h->fwdz_variant = std::atoi(fv);
+  if (const char* ct = std::getenv("DGPU_CN_TC")) {
+    const int v = std::atoi(ct);
+    h->cn_tc = (v == 8 || v == 4 || v == 2 || v == 1) ? v : 0;
+  }
   // DGPU_XPASS=fused selects the single-kernel x pass (measurement of the two forms)
   if (const char* xp = std::getenv("DGPU_XPASS")) h->xsplit = std::strcmp(xp, "fused") != 0;
   if (h->xsplit && !cfg->use_les) {
@@ -1691,6 +1700,38 @@ int dgpu_rem_div(dgpu_handle* h) {
   fill_col_args(h, ca, 0.0);
   int rc = launch_project(h, ca, 0);
   if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(h->st));
+  return 0;
+}
+
+int dgpu_filter_chan(dgpu_handle* h, int32_t n) {
+  if (!h) return fail(1, "null handle");
+  if (n < 0 || n >= h->nth) return fail(2, "dgpu_filter_chan: no such scalar");
+  const dgpu_config& c = h->c;
+  const Geo& g = h->g;
+  if (!h->d_sigma) {
+    // sigma(i, k) of filter_chan (for/solver.f90:1617-1626) in the packed layout
+    std::vector<double> sg((size_t)g.pc, 0.0);
+    for (int k = 0; k < g.kzr; ++k)
+      for (int i = 0; i < g.kxr; ++i) {
+        const double ax = h->kx[i] * c.lx * 1.0 / (double)(float)c.nx;
+        const double az = h->kz[k] * c.lz * 1.0 / (double)(float)c.nz;
+        const double s0 = 0.5 * (1.0 + std::cos(std::sqrt(std::pow(ax, 2.0) + std::pow(az, 2.0))));
+        sg[(size_t)k * g.kxp + i] =
+            std::pow(s0, 4.0) * (35.0 - 84.0 * s0 + 70.0 * std::pow(s0, 2.0) - 20.0 * std::pow(s0, 3.0));
+      }
+    void* p = nullptr;
+    int rc = dalloc(h, &p, sizeof(double) * sg.size());
+    if (rc) return rc;
+    h->d_sigma = (double*)p;
+    HIPCHK(hipMemcpyAsync(h->d_sigma, sg.data(), sizeof(double) * sg.size(), hipMemcpyHostToDevice, h->st));
+  }
+  const int jlo = c.jstart_th[n], jhi = c.jend_th[n];
+  if (jhi >= jlo) {
+    const dim3 grid((unsigned)((g.pc + 255) / 256), jhi - jlo + 1);
+    k_filter_th<<<grid, 256, 0, h->st>>>(g, h->U[3 + n], h->d_sigma, jlo, jhi);
+    HIPCHK(hipGetLastError());
+  }
   HIPCHK(hipStreamSynchronize(h->st));
   return 0;
 }

@@ -43,7 +43,7 @@ module diablo_gpu
   public :: dgpu_fft_xz_to_fourier, dgpu_fft_xz_to_physical, dgpu_synchronize
   public :: dgpu_last_step_ms, dgpu_field_doubles, dgpu_last_error_f
   public :: dgpu_download_les, dgpu_stats, dgpu_stats_doubles
-  public :: dgpu_set_variable_dt, dgpu_courant, dgpu_set_closure
+  public :: dgpu_set_variable_dt, dgpu_courant, dgpu_set_closure, dgpu_filter_chan
 
   interface
     integer(c_int) function dgpu_create(cfg, h) bind(C, name="dgpu_create")
@@ -136,6 +136,11 @@ module diablo_gpu
       real(c_double), value :: time, time_limit, save_stats_time, save_flow_time
       real(c_double), intent(out) :: delta_t
       integer(c_int32_t), intent(out) :: ran
+    end function
+    integer(c_int) function dgpu_filter_chan(h, n) bind(C, name="dgpu_filter_chan")
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: n
     end function
     ! les_model_type 6 (the learned closure, include/dgpu_closure.h): parameters
     ! and normaliser as written by diablo3_amd.train_closure

@@ -47,6 +47,7 @@ def load():
                                   ctypes.c_int32, ctypes.c_int32]
     L.dgpu_step.argtypes = [P, ctypes.c_double, ctypes.c_double, ctypes.c_int32, ctypes.c_int32]
     L.dgpu_rem_div.argtypes = [P]
+    L.dgpu_filter_chan.argtypes = [P, ctypes.c_int32]
     L.dgpu_fft_xz_to_fourier.argtypes = [P, D, D]
     L.dgpu_fft_xz_to_physical.argtypes = [P, D, D]
     L.dgpu_synchronize.argtypes = [P]
@@ -219,6 +220,10 @@ class Solver:
              create_new_flow: bool = True):
         dt = self.cfg.delta_t if delta_t is None else delta_t
         self._check(load().dgpu_step(self.h, dt, time, time_step, int(create_new_flow)))
+
+    def filter_chan(self, n: int = 0):
+        """filter_chan(n+1) of the reference on the device state (dgpu_filter_chan)."""
+        self._check(load().dgpu_filter_chan(self.h, n))
 
     def rem_div(self):
         self._check(load().dgpu_rem_div(self.h))

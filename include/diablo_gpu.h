@@ -194,6 +194,12 @@ int dgpu_courant(dgpu_handle *h, double time, double time_limit, double save_sta
 int dgpu_stats(dgpu_handle *h, double time, double *out);
 int64_t dgpu_stats_doubles(const dgpu_config *cfg);
 
+/* filter_chan(n) (for/solver.f90:1597-1685), as the driver calls it every
+ * filter_int(n) steps when filter_th(n) (for/diablo.f90:76-81): scalar n
+ * (0-based) times the sharpened raised-cosine filter sigma(kx, kz) on rows
+ * jstart_th..jend_th. */
+int dgpu_filter_chan(dgpu_handle *h, int32_t n);
+
 /* Text of the last error on this thread ("" if none). */
 const char *dgpu_last_error(void);
 

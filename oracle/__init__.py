@@ -42,6 +42,7 @@ def lib():
         L.oracle_rk_substep.argtypes = [P, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_int, ctypes.c_int]
         L.oracle_rem_div.argtypes = [P]
+        L.oracle_filter_chan.argtypes = [P, ctypes.c_int]
         L.oracle_bc_post.argtypes = [P, ctypes.c_double]
         L.oracle_set_variable_dt.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_double]
         L.oracle_courant.argtypes = [P, ctypes.c_double, ctypes.c_double, ctypes.c_double,
@@ -143,6 +144,9 @@ class Oracle:
 
     def rem_div(self):
         self._check(lib().oracle_rem_div(self.h))
+
+    def filter_chan(self, n=0):
+        self._check(lib().oracle_filter_chan(self.h, n))
 
     def set_variable_dt(self, variable_dt, update_dt=1, cfl=0.5):
         self._check(lib().oracle_set_variable_dt(self.h, int(variable_dt), update_dt, cfl))

@@ -624,6 +624,25 @@ int oracle_rem_div(oracle_t *o) {
   return 0;
 }
 
+/* filter_chan (for/solver.f90:1597-1685): the sharpened raised-cosine filter of
+ * scalar n in the horizontal, sigma = s^4 (35 - 84 s + 70 s^2 - 20 s^3) with
+ * s = (1 + cos(sqrt((kx Lx/Nx)^2 + (kz Lz/Nz)^2)))/2, on rows jstart_th..jend_th
+ * (the vertical compact filter of the reference is commented out there). */
+int oracle_filter_chan(oracle_t *o, int n) {
+  const dgpu_config *c = &o->c;
+  if (n < 0 || n >= c->n_th) return 2;
+  zc *cth = o->cth + n * o->nf;
+  for (int k = 0; k < o->kzr; k++)
+    for (int i = 0; i < o->nxp; i++) {
+      const double ax = o->kx[i] * c->lx * 1.0 / (double)(float)c->nx;
+      const double az = o->kz[k] * c->lz * 1.0 / (double)(float)c->nz;
+      const double s0 = 0.5 * (1.0 + cos(sqrt(pow(ax, 2.0) + pow(az, 2.0))));
+      const double sg = pow(s0, 4.0) * (35.0 - 84.0 * s0 + 70.0 * pow(s0, 2.0) - 20.0 * pow(s0, 3.0));
+      for (int j = c->jstart_th[n]; j <= c->jend_th[n]; j++) cth[FI(j, k, i)] = cth[FI(j, k, i)] * sg;
+    }
+  return 0;
+}
+
 /* ------------------------------------------------------------------------- */
 /* rk_chan_1 (for/solver.f90:366-1273)                                        */
 /* ------------------------------------------------------------------------- */

@@ -39,6 +39,7 @@ int oracle_download_les(oracle_t *o, double *nu_t, double *kappa_t);
 int oracle_rk_substep(oracle_t *o, int rk_step, double delta_t, double time,
                       int time_step, int create_new_flow);
 int oracle_rem_div(oracle_t *o);
+int oracle_filter_chan(oracle_t *o, int n);
 /* variable_dt: courant (for/flow.f90:746-857) runs inside rk_step 3 of every
  * update_dt-th step on the velocity before the projection (for/solver.f90:1241);
  * oracle_courant returns its delta_t (ran = 0 if it did not run since the last call). */

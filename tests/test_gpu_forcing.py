@@ -1,0 +1,44 @@
+"""The driver-side hooks and the forcing types on the device against the oracle,
+through the C ABI: filter_chan (for/solver.f90:1597-1685) on a random scalar,
+and 3 RKW3 steps with f_type 1 (constant pressure gradient), 2 (oscillatory
+pressure gradient; for/solver.f90:464-476) and 4 (oscillatory surface forcing of
+w at the top; for/flow.f90:1327-1330).  Bars as the DNS parity tests: 1e-11 on
+u, v, w, theta; cp at 30x the oracle's own round-off disagreement."""
+import numpy as np
+import pytest
+
+import oracle
+from diablo3_amd import configs
+from diablo3_amd.problem import Problem
+from diablo3_amd.solver import Solver
+
+from test_gpu_parity import _run_pair, compare, rel
+
+pytestmark = pytest.mark.gpu
+
+
+def test_filter_chan_matches_oracle():
+    cfg = configs.small_dns(64, 33, 32)
+    p = Problem(cfg)
+    s = p.initial_state(seed=4, theta_noise=1e-2)
+    rng = np.random.default_rng(4)
+    s.cth[0][:] = rng.standard_normal(s.cth[0].shape) + 1j * rng.standard_normal(s.cth[0].shape)
+    g, o = Solver(p), oracle.Oracle(p)
+    g.upload_state(s)
+    o.upload_state(s)
+    g.filter_chan(0)
+    o.filter_chan(0)
+    a, b = g.download_state().cth[0], o.download_state().cth[0]
+    e = rel(a, b)
+    print(f"filter_chan rel {e:.2e}", flush=True)
+    assert e <= 1e-15
+    with pytest.raises(RuntimeError):
+        g.filter_chan(1)
+
+
+@pytest.mark.parametrize("f_type", [1, 2, 4])
+def test_forcing_types_match_oracle(f_type):
+    cfg = configs.small_dns(32, 33, 32).copy(f_type=f_type, px0=-0.3, omega0=2.0, amp_omega0=0.2,
+                                              force_start=0.0)
+    p, g, o, fl = _run_pair(cfg, 3)
+    compare(g.download_state(), o.download_state(), 1e-11, f"f_type {f_type} x3", fl)
