@@ -48,9 +48,12 @@ __device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
 //   gelu(x) = x (1 + tanh u)/2 = x s,  s = 1/(1 + exp(-2u)),  u = sqrt(2/pi) (x + 0.044715 x^3)
 // and its derivative s + x s (1 - s) 2 du/dx
 constexpr float GELU_K0 = 0.79788456080286536f, GELU_K1 = 0.044715f;
+// exp(-2u) = exp2(x (G1 + G3 x^2)) with G1 = -2 K0 log2(e), G3 = G1 K1
+constexpr float GELU_G1 = (float)(-2.0 * 0.79788456080286536 * 1.4426950408889634);
+constexpr float GELU_G3 = (float)(-2.0 * 0.79788456080286536 * 1.4426950408889634 * 0.044715);
 __device__ __forceinline__ float gelu_s(float x) {
-  const float u = GELU_K0 * (x + GELU_K1 * x * x * x);
-  return __builtin_amdgcn_rcpf(1.0f + __expf(-2.0f * u));
+  const float z = x * __builtin_fmaf(GELU_G3, x * x, GELU_G1);
+  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(z));
 }
 __device__ __forceinline__ float gelu(float x) { return x * gelu_s(x); }
 __device__ __forceinline__ float dgelu(float x) {
@@ -79,11 +82,10 @@ __device__ __forceinline__ bf16x4 ld4(const unsigned short* p) {
   r[2] = (short)(u.y & 0xffffu); r[3] = (short)(u.y >> 16);
   return r;
 }
-__device__ __forceinline__ bf16x4 tobf(f32x4 v) {
-  bf16x4 r;
-  r[0] = (short)f2bf(v[0]); r[1] = (short)f2bf(v[1]); r[2] = (short)f2bf(v[2]); r[3] = (short)f2bf(v[3]);
-  return r;
-}
+typedef __bf16 bfv4 __attribute__((ext_vector_type(4)));
+// four values at once with gfx950's v_cvt_pk_bf16_f32 (round to nearest even,
+// the same result as f2bf)
+__device__ __forceinline__ bf16x4 tobf(f32x4 v) { return __builtin_bit_cast(bf16x4, __builtin_convertvector(v, bfv4)); }
 
 // activations of one 16-sample tile in the accumulator layout
 struct MlpAct {
