@@ -113,8 +113,7 @@ __device__ __forceinline__ GyEntries<T> gy_entries(const AmdIn<T>& in, const Amd
   using R = T;
   const Gyf<T> f = gyf_plane(in, a, j, q), fm = gyf_plane(in, a, j - 1, q);
   const Gy<T> p = gy_plane(in, a, j, q);
-  const R wc = R(a.dyf[j - 1]), wm = R(a.dyf[j]), den = R(2.0) * R(a.dy[j]);
-  auto w = [&](R x, R y) { return (x * wc + y * wm) / den; };
+  auto w = [&](R x, R y) { return gy_interp(a, j, x, y); };
   return {w(f.a1, fm.a1), w(f.a2, fm.a2), w(f.a3, fm.a3), p.a4, w(f.a5, fm.a5), p.a6, p.o4, w(f.o5, fm.o5), p.o6};
 }
 
@@ -124,10 +123,12 @@ __global__ __launch_bounds__(256) void k_closure(ClosureArgs c) {
   __shared__ float sY[4][64][2];
   const AmdArgs& a = c.a;
   const Geo& g = a.g;
-  const long long q0 = (long long)blockIdx.x * 256 + threadIdx.x;
+  int qb, j;
+  xcd_rows(blockIdx.x, (int)((g.pp + 255) / 256), g.ny2, qb, j);
+  const long long q0 = (long long)qb * 256 + threadIdx.x;
   const bool live = q0 < g.pp;
   const long long q = live ? q0 : g.pp - 1;  // every lane takes part in the MFMA tiles
-  const int j = blockIdx.y, ny = g.ny;
+  const int ny = g.ny;
   AmdIn<T> in;
   for (int s = 0; s < 3 * (3 + NTH); ++s) in.v[s] = (const T*)a.f[s];
   in.pp = g.pp;

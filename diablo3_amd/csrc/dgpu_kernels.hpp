@@ -557,6 +557,14 @@ enum RowCoef {
   RC_RDYFM,      // 1/dyf(j-1)
   RC_WTH,        // dyf(j-1)/(2 dy(j)): weight of th(j) in the GY interpolation
   RC_WTHM,       // dyf(j)/(2 dy(j)):   weight of th(j-1)
+  RC_FLO,        // 1/(dy(j) dyf(j)):    lower coupling of a GYF row (u1, u3, scalars)
+  RC_FHI,        // 1/(dy(j+1) dyf(j)):  upper coupling of a GYF row
+  RC_GLO,        // 1/(dyf(j-1) dy(j)):  lower coupling of a GY row (u2)
+  RC_GHI,        // 1/(dyf(j) dy(j)):    upper coupling of a GY row
+  RC_YX,         // 2 dy(j) / (dx beta_sgs):  the AMD closure's width ratios at GY j
+  RC_XY,         // (dx beta_sgs) / (2 dy(j))
+  RC_YZ,         // 2 dy(j) / (dz beta_sgs)
+  RC_ZY,         // (dz beta_sgs) / (2 dy(j))
   RC_TW0,        // per scalar n, 4 entries at RC_TW0 + 4n:
   RC_COUNT = RC_TW0 + 4 * MAXTH
   // +0 dthdx (gyf-Ly/2) delta/Ro_inv, +1 -dthdz (gyf-Ly/2) delta/Ro_inv,

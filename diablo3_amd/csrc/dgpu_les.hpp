@@ -133,6 +133,8 @@ struct AmdArgs {
   Geo g;
   int nth, jstart, jend, jstart_th[MAXTH], jend_th[MAXTH];
   double dxs, dzs;                      // dx * beta_sgs, dz * beta_sgs
+  double zx, xz;                        // dzs / dxs, dxs / dzs
+  const double* __restrict__ rowc;      // [ny2][RC_COUNT] per-row grid quotients
   const double* __restrict__ dy;
   const double* __restrict__ dyf;
   const double* __restrict__ dlt;       // [2][ny2]: delta_yf(j), delta_y(j)
@@ -164,6 +166,18 @@ struct AmdIn {
   __device__ T at(int f, int j, long long q) const { return v[f][(long long)j * pp + q]; }
 };
 
+// The reference divides by dy(j), dyf(j) and the filter widths at every point;
+// the row table holds those quotients (RowCoef), so the kernels multiply (the
+// results agree to rounding).
+__device__ __forceinline__ const double* rq(const AmdArgs& a, int j) { return a.rowc + (long long)j * RC_COUNT; }
+// the GY interpolation of GYF values c (row j) and m (row j-1):
+// (c dyf(j-1) + m dyf(j)) / (2 dy(j))
+template <typename R>
+__device__ __forceinline__ R gy_interp(const AmdArgs& a, int j, R c, R m) {
+  const double* rc = rq(a, j);
+  return c * R(rc[RC_WTH]) + m * R(rc[RC_WTHM]);
+}
+
 // field slots: 3*f + {0: v, 1: dv/dx, 2: dv/dz}, f = 0 u1, 1 u2, 2 u3, 3+n th_n
 template <typename T>
 __device__ __forceinline__ Gyf<T> gyf_plane(const AmdIn<T>& in, const AmdArgs& a, int j, long long q) {
@@ -171,24 +185,26 @@ __device__ __forceinline__ Gyf<T> gyf_plane(const AmdIn<T>& in, const AmdArgs& a
   Gyf<T> r;
   const R du1dz = in.at(2, j, q), du3dx = in.at(7, j, q);
   r.a1 = in.at(1, j, q);                                              // SI11 = du1/dx
-  r.a2 = (in.at(3, j + 1, q) - in.at(3, j, q)) / R(a.dyf[j]);             // SI22 = du2/dy
+  r.a2 = (in.at(3, j + 1, q) - in.at(3, j, q)) * R(rq(a, j)[RC_RDYF]);    // SI22 = du2/dy
   r.a3 = in.at(8, j, q);                                              // SI33 = du3/dz
-  r.a5 = R(0.5) * (du1dz * (R(a.dzs) / R(a.dxs)) + du3dx * (R(a.dxs) / R(a.dzs)));   // SI13
-  r.o5 = R(0.5) * (du1dz * (R(a.dzs) / R(a.dxs)) - du3dx * (R(a.dxs) / R(a.dzs)));   // O13
+  r.a5 = R(0.5) * (du1dz * R(a.zx) + du3dx * R(a.xz));                // SI13
+  r.o5 = R(0.5) * (du1dz * R(a.zx) - du3dx * R(a.xz));                // O13
   return r;
 }
 template <typename T>
 __device__ __forceinline__ Gy<T> gy_plane(const AmdIn<T>& in, const AmdArgs& a, int j, long long q) {
   using R = T;
   Gy<T> r;
-  const R dyl = R(a.dy[j]) * R(2.0);
-  const R du1dy = (in.at(0, j, q) - in.at(0, j - 1, q)) / R(a.dy[j]);
-  const R du3dy = (in.at(6, j, q) - in.at(6, j - 1, q)) / R(a.dy[j]);
+  // the width ratios 2 dy / (dx beta), ... of the scaled gradient, from the row table
+  const double* rc = rq(a, j);
+  const R rdy = R(rc[RC_RDY]), yx = R(rc[RC_YX]), xy = R(rc[RC_XY]), yz = R(rc[RC_YZ]), zy = R(rc[RC_ZY]);
+  const R du1dy = (in.at(0, j, q) - in.at(0, j - 1, q)) * rdy;
+  const R du3dy = (in.at(6, j, q) - in.at(6, j - 1, q)) * rdy;
   const R du2dx = in.at(4, j, q), du2dz = in.at(5, j, q);
-  r.a4 = R(0.5) * (du1dy * (dyl / R(a.dxs)) + du2dx * (R(a.dxs) / dyl));
-  r.o4 = R(0.5) * (du1dy * (dyl / R(a.dxs)) - du2dx * (R(a.dxs) / dyl));
-  r.a6 = R(0.5) * (du2dz * (R(a.dzs) / dyl) + du3dy * (dyl / R(a.dzs)));
-  r.o6 = R(0.5) * (du2dz * (R(a.dzs) / dyl) - du3dy * (dyl / R(a.dzs)));
+  r.a4 = R(0.5) * (du1dy * yx + du2dx * xy);
+  r.o4 = R(0.5) * (du1dy * yx - du2dx * xy);
+  r.a6 = R(0.5) * (du2dz * zy + du3dy * yz);
+  r.o6 = R(0.5) * (du2dz * zy - du3dy * yz);
   return r;
 }
 
@@ -221,8 +237,7 @@ __device__ __forceinline__ T coef_gy(const AmdIn<T>& in, const AmdArgs& a, int j
   using R = T;
   const Gyf<T> f = gyf_plane(in, a, j, q), fm = gyf_plane(in, a, j - 1, q);
   const Gy<T> p = gy_plane(in, a, j, q);
-  const R wc = R(a.dyf[j - 1]), wm = R(a.dyf[j]), den = R(2.0) * R(a.dy[j]);
-  auto w = [&](R c, R m) { return (c * wc + m * wm) / den; };
+  auto w = [&](R c, R m) { return gy_interp(a, j, c, m); };
   return amd_coef(w(f.a1, fm.a1), w(f.a2, fm.a2), w(f.a3, fm.a3), p.a4, w(f.a5, fm.a5), p.a6, p.o4,
                   w(f.o5, fm.o5), p.o6);
 }
@@ -241,14 +256,15 @@ template <typename T>
 __device__ __forceinline__ Sgyf<T> sgyf_plane(const AmdIn<T>& in, const AmdArgs& a, int j, long long q) {
   using R = T;
   const R du1dz = in.at(2, j, q), du3dx = in.at(7, j, q);
-  return {in.at(1, j, q), (in.at(3, j + 1, q) - in.at(3, j, q)) / R(a.dyf[j]), in.at(8, j, q),
+  return {in.at(1, j, q), (in.at(3, j + 1, q) - in.at(3, j, q)) * R(rq(a, j)[RC_RDYF]), in.at(8, j, q),
           R(0.5) * (du1dz + du3dx), R(0.5) * (du1dz - du3dx)};
 }
 template <typename T>
 __device__ __forceinline__ Sgy<T> sgy_plane(const AmdIn<T>& in, const AmdArgs& a, int j, long long q) {
   using R = T;
-  const R du1dy = (in.at(0, j, q) - in.at(0, j - 1, q)) / R(a.dy[j]);
-  const R du3dy = (in.at(6, j, q) - in.at(6, j - 1, q)) / R(a.dy[j]);
+  const R rdy = R(rq(a, j)[RC_RDY]);
+  const R du1dy = (in.at(0, j, q) - in.at(0, j - 1, q)) * rdy;
+  const R du3dy = (in.at(6, j, q) - in.at(6, j - 1, q)) * rdy;
   const R du2dx = in.at(4, j, q), du2dz = in.at(5, j, q);
   return {R(0.5) * (du1dy + du2dx), R(0.5) * (du2dz + du3dy), R(0.5) * (du1dy - du2dx), R(0.5) * (du2dz - du3dy)};
 }
@@ -306,8 +322,7 @@ __device__ __forceinline__ T model_gy(const AmdIn<T>& in, const AmdArgs& a, int 
   } else {
     const Sgyf<T> f = sgyf_plane(in, a, j, q), fm = sgyf_plane(in, a, j - 1, q);
     const Sgy<T> p = sgy_plane(in, a, j, q);
-    const R wc = R(a.dyf[j - 1]), wm = R(a.dyf[j]), den = R(2.0) * R(a.dy[j]);
-    auto w = [&](R c, R m) { return (c * wc + m * wm) / den; };
+    auto w = [&](R c, R m) { return gy_interp(a, j, c, m); };
     const R s1 = w(f.s1, fm.s1), s2 = w(f.s2, fm.s2), s3 = w(f.s3, fm.s3), s5 = w(f.s5, fm.s5);
     if constexpr (MODEL == 1) {  // (for/les.f90:114-132)
       return sqrt(R(2.0) * (s1 * s1) + R(4.0) * (p.s4 * p.s4) + R(4.0) * (s5 * s5) + R(2.0) * (s2 * s2) + R(4.0) * (p.s6 * p.s6) +
@@ -333,15 +348,16 @@ __device__ __forceinline__ ThCoef<T> amd_th_gy(const AmdIn<T>& in, const AmdArgs
   using R = T;
   const R dxs2 = R(a.dxs) * R(a.dxs), dzs2 = R(a.dzs) * R(a.dzs);
   const R dyl = R(a.dy[j]) * R(2.0), dyl2 = dyl * dyl;
-  const R wc = R(a.dyf[j - 1]), wm = R(a.dyf[j]), den = R(2.0) * R(a.dy[j]);
-  auto w = [&](int s) { return (in.at(s, j, q) * wc + in.at(s, j - 1, q) * wm) / den; };
+  const double* rc = rq(a, j);
+  const R rdy = R(rc[RC_RDY]);
+  auto w = [&](int s) { return gy_interp(a, j, in.at(s, j, q), in.at(s, j - 1, q)); };
   const R du1dx = w(1), du3dz = w(8), du1dz = w(2), du3dx = w(7);
-  const R du2dy = ((in.at(3, j + 1, q) - in.at(3, j, q)) / R(a.dyf[j]) * wc +
-                        (in.at(3, j, q) - in.at(3, j - 1, q)) / R(a.dyf[j - 1]) * wm) / den;
+  const R du2dy = gy_interp(a, j, (in.at(3, j + 1, q) - in.at(3, j, q)) * R(rc[RC_RDYF]),
+                            (in.at(3, j, q) - in.at(3, j - 1, q)) * R(rc[RC_RDYFM]));
   const R tx = w(ft + 1), tz = w(ft + 2);
-  const R ty = (in.at(ft, j, q) - in.at(ft, j - 1, q)) / R(a.dy[j]);
-  const R du1dy = (in.at(0, j, q) - in.at(0, j - 1, q)) / R(a.dy[j]);
-  const R du3dy = (in.at(6, j, q) - in.at(6, j - 1, q)) / R(a.dy[j]);
+  const R ty = (in.at(ft, j, q) - in.at(ft, j - 1, q)) * rdy;
+  const R du1dy = (in.at(0, j, q) - in.at(0, j - 1, q)) * rdy;
+  const R du3dy = (in.at(6, j, q) - in.at(6, j - 1, q)) * rdy;
   const R du2dx = in.at(4, j, q), du2dz = in.at(5, j, q);
   const R v = -(dxs2 * du1dx * (tx * tx) + dyl2 * du2dy * (ty * ty) + dzs2 * du3dz * (tz * tz) +
                      (dyl2 * du1dy + dxs2 * du2dx) * tx * ty + (dzs2 * du1dz + dxs2 * du3dx) * tx * tz +
@@ -355,11 +371,11 @@ __device__ __forceinline__ ThCoef<T> amd_th_gyf(const AmdIn<T>& in, const AmdArg
   using R = T;
   const R dxs2 = R(a.dxs) * R(a.dxs), dzs2 = R(a.dzs) * R(a.dzs);
   const R dyl = R(a.dyf[j]) * R(2.0), dyl2 = dyl * dyl;
-  auto gyv = [&](int s, int jj) { return (in.at(s, jj, q) - in.at(s, jj - 1, q)) / R(a.dy[jj]); };
+  auto gyv = [&](int s, int jj) { return (in.at(s, jj, q) - in.at(s, jj - 1, q)) * R(rq(a, jj)[RC_RDY]); };
   const R tx = in.at(ft + 1, j, q), tz = in.at(ft + 2, j, q);
   const R ty = R(0.5) * (gyv(ft, j + 1) + gyv(ft, j));
   const R du1dx = in.at(1, j, q), du3dz = in.at(8, j, q);
-  const R du2dy = (in.at(3, j + 1, q) - in.at(3, j, q)) / R(a.dyf[j]);
+  const R du2dy = (in.at(3, j + 1, q) - in.at(3, j, q)) * R(rq(a, j)[RC_RDYF]);
   const R du1dy = R(0.5) * (gyv(0, j + 1) + gyv(0, j)), du3dy = R(0.5) * (gyv(6, j + 1) + gyv(6, j));
   const R du2dx = R(0.5) * (in.at(4, j + 1, q) + in.at(4, j, q));
   const R du2dz = R(0.5) * (in.at(5, j + 1, q) + in.at(5, j, q));
@@ -375,23 +391,41 @@ __device__ __forceinline__ ThCoef<T> amd_th_gyf(const AmdIn<T>& in, const AmdArg
 // At j = 1 types 4 and 5 scale the strain itself (their coefficient loops start
 // at j = 2, for/les.f90:454, 762), type 1 multiplies it (for/les.f90:145); that
 // row feeds only rows the BC rows overwrite.
+// coef: model_gy at row j (unused on the row-1 special case).
 template <int MODEL, typename T>
-__device__ __forceinline__ T tau_gy(const AmdIn<T>& in, const AmdArgs& a, int e, int j, long long q) {
+__device__ __forceinline__ T tau_gy(const AmdIn<T>& in, const AmdArgs& a, int e, int j, long long q, T coef) {
   using R = T;
   const R dlt = R(a.dlt[a.g.ny2 + j]);
   if (MODEL != 1 && j == 1) {
-    const R dvdy = (in.at(e ? 6 : 0, 1, q) - in.at(e ? 6 : 0, 0, q)) / R(a.dy[1]);
+    const R dvdy = (in.at(e ? 6 : 0, 1, q) - in.at(e ? 6 : 0, 0, q)) * R(rq(a, 1)[RC_RDY]);
     return dlt * R(0.5) * (dvdy + in.at(e ? 5 : 4, 1, q));
   }
-  return dlt * (model_gy<MODEL>(in, a, j, q) * (R(0.5) * in.at(e ? 5 : 4, j, q)));
+  return dlt * (coef * (R(0.5) * in.at(e ? 5 : 4, j, q)));
+}
+
+// Block b of a 1D grid over (256-point block qb, row j) pairs.  The dispatcher
+// deals blocks to the 8 XCDs in turn, so with nqb a multiple of 8 XCD b % 8
+// takes q-blocks 8 g + b % 8 (its pass g) through the rows in order, and the
+// neighbour rows j - 1 .. j + 2 a block reads are in that XCD's L2 from the
+// blocks of the rows around it.
+__device__ __forceinline__ void xcd_rows(int b, int nqb, int nj, int& qb, int& j) {
+  if (nqb % 8) {
+    qb = b % nqb;
+    j = b / nqb;
+    return;
+  }
+  const int t = b >> 3;
+  j = t % nj;
+  qb = (t / nj) * 8 + (b & 7);
 }
 
 template <typename T, int NTH, int MODEL>
 __global__ __launch_bounds__(256) void k_amd(AmdArgs a) {
   using R = T;
   const Geo& g = a.g;
-  const long long q = (long long)blockIdx.x * 256 + threadIdx.x;
-  const int j = blockIdx.y;
+  int qb, j;
+  xcd_rows(blockIdx.x, (int)((g.pp + 255) / 256), g.ny2, qb, j);
+  const long long q = (long long)qb * 256 + threadIdx.x;
   if (q >= g.pp) return;
   AmdIn<T> in;
   for (int s = 0; s < 3 * (3 + NTH); ++s) in.v[s] = (const T*)a.f[s];
@@ -399,9 +433,16 @@ __global__ __launch_bounds__(256) void k_amd(AmdArgs a) {
   const int ny = g.ny;
   const long long o = (long long)j * g.pp + q;
   auto put = [&](void* p, R v) { ((T*)p)[o] = (T)v; };
+  // the model coefficient at GY j and j + 1, each evaluated once: nu_t, the two
+  // stresses at GY j and the GYF flux differences take them (row 1 of types 4
+  // and 5 needs none)
+  const bool need_j = j <= ny && (j >= 2 || (MODEL == 1 && j == 1));
+  const R cj = need_j ? model_gy<MODEL>(in, a, j, q) : R(0.0);
+  const R cj1 = (j >= a.jstart && j <= ny - 1) ? model_gy<MODEL>(in, a, j + 1, q) : R(0.0);
+  const double* rc = rq(a, j);
   // nu_t at GY j (zero on the wall rows of ghost_les_mpi)
   R nut = R(0.0);
-  if (j >= 3 && j <= ny - 1) nut = -R(0.5) * R(a.dlt[g.ny2 + j]) * model_gy<MODEL>(in, a, j, q);
+  if (j >= 3 && j <= ny - 1) nut = -R(0.5) * R(a.dlt[g.ny2 + j]) * cj;
   put(a.nut, nut);
   // GYF stresses, rows j1..j2 = jstart..Nyp-1
   R A0 = R(0.0), A1 = R(0.0), A2 = R(0.0), A3 = R(0.0), A4 = R(0.0);
@@ -411,15 +452,17 @@ __global__ __launch_bounds__(256) void k_amd(AmdArgs a) {
     A0 = dyf * in.at(1, j, q);
     A1 = dyf * (R(0.5) * (du1dz + du3dx));
     A2 = dyf * in.at(8, j, q);
-    A3 = (tau_gy<MODEL>(in, a, 0, j + 1, q) - tau_gy<MODEL>(in, a, 0, j, q)) / R(a.dyf[j]);
-    A4 = (tau_gy<MODEL>(in, a, 1, j + 1, q) - tau_gy<MODEL>(in, a, 1, j, q)) / R(a.dyf[j]);
+    const R rdyf = R(rc[RC_RDYF]);
+    A3 = (tau_gy<MODEL>(in, a, 0, j + 1, q, cj1) - tau_gy<MODEL>(in, a, 0, j, q, cj)) * rdyf;
+    A4 = (tau_gy<MODEL>(in, a, 1, j + 1, q, cj1) - tau_gy<MODEL>(in, a, 1, j, q, cj)) * rdyf;
   }
   put(a.aug[0], A0); put(a.aug[1], A1); put(a.aug[2], A2); put(a.aug[3], A3); put(a.aug[4], A4);
   // GY stresses and the explicit cross terms, rows 2..Nyp
   R A5 = R(0.0), A6 = R(0.0);
   if (j >= 2 && j <= ny) {
-    A5 = tau_gy<MODEL>(in, a, 0, j, q) - nut * (in.at(0, j, q) - in.at(0, j - 1, q)) / R(a.dy[j]);
-    A6 = tau_gy<MODEL>(in, a, 1, j, q) - nut * (in.at(6, j, q) - in.at(6, j - 1, q)) / R(a.dy[j]);
+    const R rdy = R(rc[RC_RDY]);
+    A5 = tau_gy<MODEL>(in, a, 0, j, q, cj) - nut * ((in.at(0, j, q) - in.at(0, j - 1, q)) * rdy);
+    A6 = tau_gy<MODEL>(in, a, 1, j, q, cj) - nut * ((in.at(6, j, q) - in.at(6, j - 1, q)) * rdy);
   }
   put(a.aug[5], A5); put(a.aug[6], A6);
   if constexpr (MODEL == 1 || MODEL == 4) {
@@ -476,8 +519,8 @@ __global__ __launch_bounds__(256) void k_amd(AmdArgs a) {
 // for/solver.f90:1691-1761, 1816-1852) solves rows 0..Nyp+1.
 // One thread per (column, field): the fields' systems are independent, so
 // blockIdx.y picks the field.  The thread keeps a register window of u and nu_t
-// over rows j-1..j+1 and issues the loads of rows j+2, j+3 while row j is
-// eliminated (the elimination chain is serial in j; the loads are not).
+// over rows j-1..j+1 and has the loads of the next rows in flight while row j
+// is eliminated (the elimination chain is serial in j; the loads are not).
 // The reciprocal pivot 1/b' and g' go to scratch; the back substitution reads
 // them in reverse with the same look-ahead and recomputes c from nu_t (the
 // same row function, so the same value).
@@ -485,9 +528,11 @@ __global__ __launch_bounds__(256) void k_amd(AmdArgs a) {
 // INC (the mixed-precision path): u is the state at the substep's start in
 // float and r holds r - u (float).  The solve is then for the increment
 // delta = u_new - u: right-hand side rhs - (l u_{j-1} + d u_j + c u_{j+1}) of the
-// same rows, in FP64, and delta is written back to r (0 outside [wlo, whi]),
-// so the state moves by a float transform of an O(dt) quantity.  Otherwise the
-// solution overwrites u on rows [wlo, whi].
+// same rows, eliminated in FP64, and delta is written back to r (0 outside
+// [wlo, whi]), so the state moves by a float transform of an O(dt) quantity.
+// The scratch 1/b', g' of the increment system is stored in float (g' is an
+// increment too; the rounding is of the order of the float transforms').
+// Otherwise the solution overwrites u on rows [wlo, whi].
 // ---------------------------------------------------------------------------
 template <typename TU, typename TV>
 struct PhysCnArgs {
@@ -501,11 +546,17 @@ struct PhysCnArgs {
   double temp2;
   const double* __restrict__ dy;
   const double* __restrict__ dyf;
+  const double* __restrict__ rowc;    // [ny2][RC_COUNT] per-row grid coefficients
   TU* u[MAXF];                        // physical field
   TU* r[MAXF];                        // right-hand side (INC: r - u, then delta)
   const TV* visc[MAXF];               // nu_t (velocity) or kappa_t_n (scalar)
   double* cp;                         // scratch, nf * 2 * ny2 planes: 1/b', g'
 };
+
+template <bool INC>
+struct ScratchType { using type = double; };
+template <>
+struct ScratchType<true> { using type = float; };
 
 template <typename TU, typename TV, bool INC>
 __global__ __launch_bounds__(256) void k_cn_phys(PhysCnArgs<TU, TV> a) {
@@ -517,43 +568,37 @@ __global__ __launch_bounds__(256) void k_cn_phys(PhysCnArgs<TU, TV> a) {
   const long long pp = g.pp;
   const double* __restrict__ dy = a.dy;
   const double* __restrict__ dyf = a.dyf;
+  const double* __restrict__ rowc = a.rowc;
   TU* __restrict__ u = a.u[e];
   TU* __restrict__ r = a.r[e];
   const TV* __restrict__ nt = a.visc[e];
-  double* __restrict__ sb = a.cp + (long long)e * 2 * n2 * pp;  // 1/b'
-  double* __restrict__ sg = sb + (long long)n2 * pp;            // g'
+  // scratch: FP64 for the FP64 solve, float for the increments (below)
+  using TS = typename ScratchType<INC>::type;
+  TS* __restrict__ sb = (TS*)a.cp + (long long)e * 2 * n2 * pp;  // 1/b'
+  TS* __restrict__ sg = sb + (long long)n2 * pp;                 // g'
   const double T = a.tcn[e], t2 = a.temp2;
   const int fld = a.field[e];
   const bool v2 = (fld == 1);
   const int blo = a.blo[e], bhi = a.bhi[e], wlo = a.wlo[e], whi = a.whi[e];
-  auto U = [&](int j) -> double { return (j >= 0 && j <= last) ? (double)u[(long long)j * pp + q] : 0.0; };
-  auto V = [&](int j) -> double { return (j >= 0 && j <= last) ? (double)nt[(long long)j * pp + q] : 0.0; };
-  auto Rr = [&](int j) -> double { return (j >= 0 && j <= last) ? (double)r[(long long)j * pp + q] : 0.0; };
   // row j of the system: (l, d, c | rhs) from the window values
   auto row = [&](int j, double um, double uc, double up, double nm, double nc, double np, double rv, double& l,
                  double& d, double& c, double& rhs) {
     l = 0.0; d = 1.0; c = 0.0; rhs = 0.0;
     if (j >= blo && j <= bhi) {
       rhs = INC ? rv + uc : rv;
-      if (!v2) {
-        l = -T / (dy[j] * dyf[j]);
-        // the u1/u3 builds write the diagonal as 1 - T(-a - b) (for/solver.f90:1136-1145)
-        d = (fld >= 3) ? 1. + T / (dy[j + 1] * dyf[j]) + T / (dy[j] * dyf[j])
-                       : 1. - T * (-1. / (dy[j + 1] * dyf[j]) - 1. / (dy[j] * dyf[j]));
-        c = -T / (dy[j + 1] * dyf[j]);
-        rhs = rhs + t2 * ((np * (up - uc) / dy[j + 1] - nc * (uc - um) / dy[j]) / dyf[j]);
-        l = l - t2 * nc / (dy[j] * dyf[j]);
-        d = d + t2 * np / (dy[j + 1] * dyf[j]) + t2 * nc / (dy[j] * dyf[j]);
-        c = c - t2 * np / (dy[j + 1] * dyf[j]);
-      } else {
-        l = -T / (dyf[j - 1] * dy[j]);
-        d = 1. + T / (dyf[j] * dy[j]) + T / (dyf[j - 1] * dy[j]);
-        c = -T / (dyf[j] * dy[j]);
-        rhs = rhs + t2 * ((0.5 * (nc + np) * (up - uc) / dyf[j] - 0.5 * (nc + nm) * (uc - um) / dyf[j - 1]) / dy[j]);
-        l = l - t2 * 0.5 * (nc + nm) / (dyf[j - 1] * dy[j]);
-        d = d + t2 * 0.5 * (nc + np) / (dyf[j] * dy[j]) + t2 * 0.5 * (nc + nm) / (dyf[j - 1] * dy[j]);
-        c = c - t2 * 0.5 * (nc + np) / (dyf[j] * dy[j]);
-      }
+      // The reference's rows (for/solver.f90:1136-1145 and the nu_t terms of
+      // les_chan) divide T and temp2 nu by dy(j) dyf(j) and dy(j+1) dyf(j)
+      // (GYF rows), dyf(j-1) dy(j) and dyf(j) dy(j) (u2's GY rows, nu_t averaged
+      // to the faces); the host table holds those reciprocals, so a row costs
+      // multiplications only (the quotients agree to rounding).
+      const double* k = rowc + (long long)j * RC_COUNT;
+      const double nlo = v2 ? 0.5 * (nc + nm) : nc, nhi = v2 ? 0.5 * (nc + np) : np;
+      const double ilo = k[v2 ? RC_GLO : RC_FLO], ihi = k[v2 ? RC_GHI : RC_FHI];
+      const double lo = (T + t2 * nlo) * ilo, hi = (T + t2 * nhi) * ihi;
+      l = -lo;
+      c = -hi;
+      d = 1.0 + lo + hi;
+      rhs = rhs + t2 * (nhi * (up - uc) * ihi - nlo * (uc - um) * ilo);
     }
     // BC rows (apply_BC_* of the reference, for/solver.f90:1901-2232)
     if (!v2) {
@@ -582,15 +627,24 @@ __global__ __launch_bounds__(256) void k_cn_phys(PhysCnArgs<TU, TV> a) {
     }
     if (INC) rhs = rhs - (l * um + d * uc + c * up);
   };
-  // forward elimination with a two-row look-ahead on the loads
-  double um = 0.0, uc = U(0), u1 = U(1), u2 = U(2);
-  double nm = 0.0, nc = V(0), n1 = V(1), n2v = V(2);
-  double r0 = Rr(0), r1 = Rr(1);
+  // forward elimination; the loads run L + 1 rows (u, nu_t) and L rows (r)
+  // ahead of the elimination, held raw in small register queues (float rows
+  // carry half the bytes per load, so the increment solve keeps more in flight)
+  constexpr int L = INC ? 4 : 2;
+  auto LU = [&](const TU* p, int j) -> TU { return (j >= 0 && j <= last) ? p[(long long)j * pp + q] : (TU)0; };
+  auto LV = [&](int j) -> TV { return (j >= 0 && j <= last) ? nt[(long long)j * pp + q] : (TV)0; };
+  double um = 0.0, uc = (double)LU(u, 0), up = (double)LU(u, 1);
+  double nm = 0.0, nc = (double)LV(0), np = (double)LV(1);
+  double rv = (double)LU(r, 0);
+  TU qu[L], qr[L];
+  TV qv[L];
+#pragma unroll
+  for (int i = 0; i < L; ++i) { qu[i] = LU(u, i + 2); qv[i] = LV(i + 2); qr[i] = LU(r, i + 1); }
   double binv = 0.0, cprev = 0.0, gprev = 0.0;
   for (int j = 0; j <= last; ++j) {
-    const double u3 = U(j + 3), n3 = V(j + 3), r2 = Rr(j + 2);
+    // here up = u(j+1), qu[i] = u(j+2+i) (nu_t alike), rv = r(j), qr[i] = r(j+1+i)
     double l, d, c, rhs;
-    row(j, um, uc, u1, nm, nc, n1, r0, l, d, c, rhs);
+    row(j, um, uc, up, nm, nc, np, rv, l, d, c, rhs);
     if (j > 0) {
       const double m = -l * binv;
       d = d + m * cprev;
@@ -598,34 +652,41 @@ __global__ __launch_bounds__(256) void k_cn_phys(PhysCnArgs<TU, TV> a) {
     }
     binv = 1.0 / d;
     const long long o = (long long)j * pp + q;
-    sb[o] = binv;
-    sg[o] = rhs;
+    sb[o] = (TS)binv;
+    sg[o] = (TS)rhs;
     cprev = c; gprev = rhs;
-    um = uc; uc = u1; u1 = u2; u2 = u3;
-    nm = nc; nc = n1; n1 = n2v; n2v = n3;
-    r0 = r1; r1 = r2;
+    um = uc; uc = up; up = (double)qu[0];
+    nm = nc; nc = np; np = (double)qv[0];
+    rv = (double)qr[0];
+#pragma unroll
+    for (int i = 0; i < L - 1; ++i) { qu[i] = qu[i + 1]; qv[i] = qv[i + 1]; qr[i] = qr[i + 1]; }
+    qu[L - 1] = LU(u, j + L + 2); qv[L - 1] = LV(j + L + 2); qr[L - 1] = LU(r, j + L + 1);
   }
-  // back substitution, rows last..0; c of row j from nu_t at j-1, j, j+1
-  auto S = [&](double* p, int j) -> double { return j >= 0 ? p[(long long)j * pp + q] : 0.0; };
-  double x = gprev * binv;
-  double b0 = S(sb, last - 1), g0 = S(sg, last - 1);
-  double b1 = S(sb, last - 2), g1 = S(sg, last - 2);
-  double vp = V(last), vc = V(last - 1), vm = V(last - 2), vmm = V(last - 3);
+  // back substitution, rows last..0; c of row j from nu_t at j, j+1
+  auto LS = [&](const TS* p, int j) -> TS { return j >= 0 ? p[(long long)j * pp + q] : (TS)0; };
   auto put = [&](int j, double v) {
     const long long o = (long long)j * pp + q;
     if (INC) r[o] = (TU)((j >= wlo && j <= whi) ? v : 0.0);
     else if (j >= wlo && j <= whi) u[o] = (TU)v;
   };
+  double x = gprev * binv;
   put(last, x);
+  double bc = (double)LS(sb, last - 1), gc = (double)LS(sg, last - 1);
+  double vp = (double)LV(last), vc = (double)LV(last - 1);
+  TS qb[L], qg[L];
+#pragma unroll
+  for (int i = 0; i < L; ++i) { qb[i] = LS(sb, last - 2 - i); qg[i] = LS(sg, last - 2 - i); qv[i] = LV(last - 2 - i); }
   for (int j = last - 1; j >= 0; --j) {
-    const double b2 = S(sb, j - 2), g2 = S(sg, j - 2), v3 = V(j - 3);
+    // here bc, gc = row j's 1/b', g'; qb[i] = row j-1-i's (nu_t alike: vc = nu(j), vp = nu(j+1))
     double l, d, c, rhs;
-    row(j, 0.0, 0.0, 0.0, vm, vc, vp, 0.0, l, d, c, rhs);
-    x = (g0 - c * x) * b0;
+    row(j, 0.0, 0.0, 0.0, 0.0, vc, vp, 0.0, l, d, c, rhs);
+    x = (gc - c * x) * bc;
     put(j, x);
-    b0 = b1; g0 = g1;
-    b1 = b2; g1 = g2;
-    vp = vc; vc = vm; vm = vmm; vmm = v3;
+    bc = (double)qb[0]; gc = (double)qg[0];
+    vp = vc; vc = (double)qv[0];
+#pragma unroll
+    for (int i = 0; i < L - 1; ++i) { qb[i] = qb[i + 1]; qg[i] = qg[i + 1]; qv[i] = qv[i + 1]; }
+    qb[L - 1] = LS(sb, j - 1 - L); qg[L - 1] = LS(sg, j - 1 - L); qv[L - 1] = LV(j - 1 - L);
   }
 }
 

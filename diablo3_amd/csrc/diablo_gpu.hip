@@ -556,8 +556,9 @@ int launch_fwdz_nl(dgpu_handle* h, NlArgs& a, int nj) {
     using I1 = std::integral_constant<int, (TI0 > 1 ? TI0 / 2 : 1)>;
     using P0 = std::false_type;
     using P1 = std::true_type;
-    // measured at 256^3: FP64 best persistent with TI columns, float with TI/2 per block
-    const int v = h->fwdz_variant >= 0 ? h->fwdz_variant : (sizeof(T) == 8 ? 2 : 1);
+    // measured: FP64 best persistent with TI columns; float with TI/2 per block at
+    // nz = 256 (256^3), persistent with TI columns at nz = 512 (512x257x512)
+    const int v = h->fwdz_variant >= 0 ? h->fwdz_variant : ((sizeof(T) == 8 || h->g.nz >= 512) ? 2 : 1);
     switch (v) {
       case 1: go(I1{}, P0{}); break;
       case 2: go(I0{}, P1{}); break;
@@ -924,7 +925,8 @@ AmdArgs amd_args(dgpu_handle* h) {
   aa.g = h->g; aa.nth = h->nth; aa.jstart = c.jstart; aa.jend = c.jend;
   for (int n = 0; n < MAXTH; ++n) { aa.jstart_th[n] = c.jstart_th[n]; aa.jend_th[n] = c.jend_th[n]; }
   aa.dxs = c.lx / c.nx * BETA_SGS; aa.dzs = c.lz / c.nz * BETA_SGS;
-  aa.dy = h->d_dy; aa.dyf = h->d_dyf; aa.dlt = h->d_dlt;
+  aa.zx = aa.dzs / aa.dxs; aa.xz = aa.dxs / aa.dzs;
+  aa.dy = h->d_dy; aa.dyf = h->d_dyf; aa.dlt = h->d_dlt; aa.rowc = h->d_rowc;
   for (int s = 0; s < 3 * (3 + h->nth); ++s) aa.f[s] = h->Pg[s];
   return aa;
 }
@@ -965,12 +967,14 @@ int substep_les(dgpu_handle* h, int rk, double delta_t, double time, int time_st
   aa.g = g; aa.nth = NTH; aa.jstart = c.jstart; aa.jend = c.jend;
   for (int n = 0; n < MAXTH; ++n) { aa.jstart_th[n] = c.jstart_th[n]; aa.jend_th[n] = c.jend_th[n]; }
   aa.dxs = c.lx / c.nx * BETA_SGS; aa.dzs = c.lz / c.nz * BETA_SGS;
-  aa.dy = h->d_dy; aa.dyf = h->d_dyf; aa.dlt = h->d_dlt;
+  aa.zx = aa.dzs / aa.dxs; aa.xz = aa.dxs / aa.dzs;
+  aa.dy = h->d_dy; aa.dyf = h->d_dyf; aa.dlt = h->d_dlt; aa.rowc = h->d_rowc;
   for (int s = 0; s < 3 * nf; ++s) aa.f[s] = h->Pg[s];
   aa.nut = h->NuT;
   for (int n = 0; n < NTH; ++n) aa.kap[n] = h->KapT[n];
   for (int m = 0; m < 7 + 2 * NTH; ++m) aa.aug[m] = h->Aug[m];
-  const dim3 agrid((unsigned)((g.pp + 255) / 256), g.ny2);
+  // one block per (256 points, row): xcd_rows maps the 1D index
+  const dim3 agrid((unsigned)(((g.pp + 255) / 256) * g.ny2));
   if constexpr (NTH == 1) {
     if (c.les_model_type == 6) {
       // the learned coefficients (AMD where a point leaves the envelope)
@@ -1064,7 +1068,7 @@ int substep_les(dgpu_handle* h, int rk, double delta_t, double time, int time_st
   // 7. per-column CN with nu_t / kappa_t (the field systems are independent; the
   //    reference solves the scalars first, for/solver.f90:1004-1007)
   PhysCnArgs<double, T> pa{};
-  pa.g = g; pa.nf = nf; pa.temp2 = temp2; pa.dy = h->d_dy; pa.dyf = h->d_dyf; pa.cp = h->CPs;
+  pa.g = g; pa.nf = nf; pa.temp2 = temp2; pa.dy = h->d_dy; pa.dyf = h->d_dyf; pa.rowc = h->d_rowc; pa.cp = h->CPs;
   for (int e0 = 0; e0 < nf; ++e0) {
     const int e = e0 < NTH ? 3 + e0 : e0 - NTH;  // slot e0 solves field e
     const SolveRows& R = ca.rows[e];
@@ -1087,7 +1091,7 @@ int substep_les(dgpu_handle* h, int rk, double delta_t, double time, int time_st
   const dim3 pgrid((unsigned)((g.pp + 255) / 256), nf);
   if (inc) {
     PhysCnArgs<float, T> pf{};
-    pf.g = pa.g; pf.nf = pa.nf; pf.temp2 = pa.temp2; pf.dy = pa.dy; pf.dyf = pa.dyf; pf.cp = pa.cp;
+    pf.g = pa.g; pf.nf = pa.nf; pf.temp2 = pa.temp2; pf.dy = pa.dy; pf.dyf = pa.dyf; pf.rowc = pa.rowc; pf.cp = pa.cp;
     for (int e0 = 0; e0 < nf; ++e0) {
       pf.field[e0] = pa.field[e0]; pf.blo[e0] = pa.blo[e0]; pf.bhi[e0] = pa.bhi[e0];
       pf.wlo[e0] = pa.wlo[e0]; pf.whi[e0] = pa.whi[e0]; pf.lo_kind[e0] = pa.lo_kind[e0];
@@ -1348,6 +1352,10 @@ int dgpu_create(const dgpu_config* cfg, dgpu_handle** out) {
     r[RC_RDYFM] = 1.0 / dyfm;
     r[RC_WTH] = dyfm / (2.0 * dy);
     r[RC_WTHM] = dyf / (2.0 * dy);
+    r[RC_FLO] = 1.0 / (dy * dyf);
+    r[RC_FHI] = 1.0 / (dyp * dyf);
+    r[RC_GLO] = 1.0 / (dyfm * dy);
+    r[RC_GHI] = 1.0 / (dyf * dy);
     if (cfg->ro_inv != 0.0)
       for (int n = 0; n < cfg->n_th; ++n) {
         const double yf = h->gyf[j] - 0.5 * cfg->ly, yg = h->gy[j] - 0.5 * cfg->ly;
@@ -1356,6 +1364,19 @@ int dgpu_create(const dgpu_config* cfg, dgpu_handle** out) {
         r[RC_TW0 + 4 * n + 2] = cfg->dthdx[n] * yg * cfg->delta / cfg->ro_inv;
         r[RC_TW0 + 4 * n + 3] = -1.0 * cfg->dthdz[n] * yg * cfg->delta / cfg->ro_inv;
       }
+  }
+  // the quotients the LES kernels take on the ghost rows too (dyf > 0 on rows
+  // 0..Nyp+1, dy > 0 on 1..Nyp+1), and the AMD closure's width ratios
+  rowc[RC_RDYF] = 1.0 / h->dyf[0];
+  rowc[(size_t)(cfg->ny + 1) * RC_COUNT + RC_RDY] = 1.0 / h->dy[cfg->ny + 1];
+  rowc[(size_t)(cfg->ny + 1) * RC_COUNT + RC_RDYF] = 1.0 / h->dyf[cfg->ny + 1];
+  if (cfg->use_les) {
+    const double ax = cfg->lx / cfg->nx * BETA_SGS, az = cfg->lz / cfg->nz * BETA_SGS;
+    for (int j = 1; j <= cfg->ny + 1; ++j) {
+      double* r = &rowc[(size_t)j * RC_COUNT];
+      const double dyl = h->dy[j] * 2.0;
+      r[RC_YX] = dyl / ax; r[RC_XY] = ax / dyl; r[RC_YZ] = dyl / az; r[RC_ZY] = az / dyl;
+    }
   }
   ok = ok && up(h->d_rowc, rowc.data(), sizeof(double) * rowc.size());
   std::vector<double> dlt(2 * (size_t)n2, 0.0);
@@ -1553,7 +1574,7 @@ int dgpu_les_features(dgpu_handle* h, double time, float* X, float* scale, float
   k.feat = 1;
   k.jf0 = jf0;
   k.X = dX; k.scale = dS; k.amd = dA; k.inside = dI;
-  const dim3 grid((unsigned)((g.pp + 255) / 256), g.ny2);
+  const dim3 grid((unsigned)(((g.pp + 255) / 256) * g.ny2));  // xcd_rows maps the 1D index
   bool ok = true;
   if (h->nth == 1) {
     if (mixed) k_closure<float, 1><<<grid, 256, 0, h->st>>>(k);
