@@ -130,8 +130,7 @@ __global__ __launch_bounds__(256) void k_closure(ClosureArgs c) {
   const long long q = live ? q0 : g.pp - 1;  // every lane takes part in the MFMA tiles
   const int ny = g.ny;
   AmdIn<T> in;
-  for (int s = 0; s < 3 * (3 + NTH); ++s) in.v[s] = (const T*)a.f[s];
-  in.pp = g.pp;
+  load_window<T, 3 * (3 + NTH)>(in, a, j, q);
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const double C2 = C_AMD * C_AMD;
   // step mode: GY rows 2..Nyp; feature mode: GYF rows jf0..Nyp-1

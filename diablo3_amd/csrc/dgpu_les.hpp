@@ -159,12 +159,32 @@ struct Gy {
   R a4, o4, a6, o6;
 };
 
+// The inputs one point reads: every slot on rows j - 1 .. j + 2, loaded into
+// registers before any arithmetic (the reads of the model's branches and of the
+// neighbour rows are then all in flight at once instead of in dependent groups).
+// at() is called with rows j + const, so the window index folds to a constant.
 template <typename T>
 struct AmdIn {
-  const T* __restrict__ v[3 * MAXF];
+  T w[3 * MAXF][4];
+  int j0;          // j - 1
   long long pp;
-  __device__ T at(int f, int j, long long q) const { return v[f][(long long)j * pp + q]; }
+  __device__ T at(int f, int row, long long) const { return w[f][row - j0]; }
 };
+// NS slots; rows outside 0..Nyp+1 are clamped (no caller uses such a row's value)
+template <typename T, int NS>
+__device__ __forceinline__ void load_window(AmdIn<T>& in, const AmdArgs& a, int j, long long q) {
+  const int last = a.g.ny + 1;
+  in.j0 = j - 1;
+  in.pp = a.g.pp;
+#pragma unroll
+  for (int s = 0; s < NS; ++s)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      int row = j - 1 + r;
+      row = row < 0 ? 0 : (row > last ? last : row);
+      in.w[s][r] = ((const T*)a.f[s])[(long long)row * a.g.pp + q];
+    }
+}
 
 // The reference divides by dy(j), dyf(j) and the filter widths at every point;
 // the row table holds those quotients (RowCoef), so the kernels multiply (the
@@ -428,8 +448,7 @@ __global__ __launch_bounds__(256) void k_amd(AmdArgs a) {
   const long long q = (long long)qb * 256 + threadIdx.x;
   if (q >= g.pp) return;
   AmdIn<T> in;
-  for (int s = 0; s < 3 * (3 + NTH); ++s) in.v[s] = (const T*)a.f[s];
-  in.pp = g.pp;
+  load_window<T, 3 * (3 + NTH)>(in, a, j, q);
   const int ny = g.ny;
   const long long o = (long long)j * g.pp + q;
   auto put = [&](void* p, R v) { ((T*)p)[o] = (T)v; };
