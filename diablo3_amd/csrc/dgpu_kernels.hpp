@@ -36,6 +36,19 @@ __device__ __forceinline__ int zpack(const Geo& g, int z) {
   return z <= g.nkz ? z : (z >= g.nz - g.nkz ? z - g.nz + g.kzr : -1);
 }
 
+// Tile order for the 8 XCDs: the dispatcher deals blocks (in linear order) to
+// the XCDs in turn, so within each group of 8 K consecutive blocks XCD x is
+// given the K consecutive tiles x K .. x K + K - 1.  Neighbouring column blocks
+// of a z transform each read half of the same 128-byte line of every z row;
+// ordered this way they run on one XCD together and the second half comes from
+// its L2.  Blocks past the last whole group keep their own index.
+__device__ __forceinline__ int xcd_tile(int b, int total, int K) {
+  const int C = 8 * K;
+  if (b >= (total / C) * C) return b;
+  return (b / C) * C + (b & 7) * K + (b % C) / 8;
+}
+constexpr int XCD_K = 16;
+
 // ---------------------------------------------------------------------------
 // z transforms: inverse (packed C -> H) and forward (H -> packed C)
 // ---------------------------------------------------------------------------
@@ -49,7 +62,10 @@ __global__ __launch_bounds__(NT) void k_inv_z(Ptrs<const cd*, MAXO> src, Ptrs<cx
   __shared__ cx<T> s[TI * LD];
   __shared__ cx<T> stw[NZ];  // twiddles
   stage_twiddles<T, NZ, NT>(stw, tw, threadIdx.x);
-  const int f = blockIdx.z, j = j0 + blockIdx.y, i0 = blockIdx.x * TI, tid = threadIdx.x;
+  // grid (column blocks, planes, fields), taken in xcd_tile order
+  const int nib = gridDim.x, nj = gridDim.y;
+  const int tt = xcd_tile(blockIdx.x + nib * (blockIdx.y + nj * blockIdx.z), nib * nj * gridDim.z, XCD_K);
+  const int f = tt / (nib * nj), j = j0 + (tt / nib) % nj, i0 = (tt % nib) * TI, tid = threadIdx.x;
   const cd* __restrict__ C = src.p[f] + (long long)j * g.pc;
   const bool dz = (dzmask >> f) & 1u;
   // every load of the thread first, then the LDS stores (the loads overlap)
@@ -89,7 +105,10 @@ __global__ __launch_bounds__(NT) void k_fwd_z(Ptrs<const cx<T>*, MAXO> src, Ptrs
   __shared__ cx<T> s[TI * LD];
   __shared__ cx<T> stw[NZ];  // twiddles
   stage_twiddles<T, NZ, NT>(stw, tw, threadIdx.x);
-  const int f = blockIdx.z, j = j0 + blockIdx.y, i0 = blockIdx.x * TI, tid = threadIdx.x;
+  // grid (column blocks, planes, fields), taken in xcd_tile order
+  const int nib = gridDim.x, nj = gridDim.y;
+  const int tt = xcd_tile(blockIdx.x + nib * (blockIdx.y + nj * blockIdx.z), nib * nj * gridDim.z, XCD_K);
+  const int f = tt / (nib * nj), j = j0 + (tt / nib) % nj, i0 = (tt % nib) * TI, tid = threadIdx.x;
   const cx<T>* __restrict__ H = src.p[f] + (long long)j * g.ph;
   for (int idx = tid; idx < TI * NZ; idx += NT) {
     const int ii = idx % TI, z = idx / TI, i = i0 + ii;
@@ -438,9 +457,12 @@ __device__ __forceinline__ void eq_rows(int e, int jstart, int jend, const int* 
   else { lo = jst[e - 3]; hi = jet[e - 3]; }
 }
 
-// Persistent blocks: tile t = (column block, plane, equation) runs t = blockIdx.x,
-// blockIdx.x + gridDim.x, ...; the next tile's spectra are loaded into registers
-// while the current tile is transformed, so the loads overlap the FFT.
+// Persistent blocks: tile t = (column block, plane, equation) runs t = b,
+// b + gridDim.x, ... with b the block's xcd_tile position (gridDim.x a multiple
+// of 8: XCD x works on a contiguous range of each round's tiles); the next
+// tile's spectra are loaded into registers while the current tile is
+// transformed, so the loads overlap the FFT.  One tile per block otherwise, in
+// xcd_tile order.
 template <typename T, int NZ, int TI, int NT, bool PERSIST>
 __global__ __launch_bounds__(NT) void k_fwdz_nl(NlArgs a, int ntiles) {
   constexpr int LD = NZ + 1;
@@ -481,7 +503,8 @@ __global__ __launch_bounds__(NT) void k_fwdz_nl(NlArgs a, int ntiles) {
     }
   };
   const int step = PERSIST ? (int)gridDim.x : ntiles;  // one tile per block unless persistent
-  int t = blockIdx.x;
+  int t = PERSIST ? ((gridDim.x & 7) ? (int)blockIdx.x : xcd_tile(blockIdx.x, gridDim.x, gridDim.x / 8))
+                  : xcd_tile(blockIdx.x, ntiles, XCD_K);
   load(t);
   __syncthreads();  // the twiddle table
   for (; t < ntiles; t += step) {

@@ -63,7 +63,8 @@ def run_oracle(p, s0, nsteps, variant=0, perturb=False):
         oracle.lib().oracle_set_variant(0)
 
 
-@pytest.mark.parametrize("nx,ny,nz,model", [(32, 33, 32, 5), (64, 33, 16, 5), (32, 33, 32, 1), (32, 33, 32, 4)])
+@pytest.mark.parametrize("nx,ny,nz,model", [(32, 33, 32, 5), (64, 33, 16, 5), (32, 33, 32, 1), (32, 33, 32, 4),
+                                             (16, 17, 512, 5)])
 def test_les_steps_match_oracle(nx, ny, nz, model):
     p, s0 = les_problem(nx, ny, nz, model=model)
     nsteps = 3
@@ -129,12 +130,13 @@ def test_les_gate_runs_dns_before_spin_up():
         assert np.array_equal(out[0][k], out[1][k]), k
 
 
-def test_les_mixed_precision_error():
+@pytest.mark.parametrize("nx,ny,nz", [(32, 33, 32), (16, 17, 512)])
+def test_les_mixed_precision_error(nx, ny, nz):
     """DGPU_MIXED LES (float gradients, model, products; FP64 state, solves,
     projection) against the FP64 oracle after 3 steps: the error a float
     transform carries, reported and bounded (u, theta <= 1e-6; nu_t <= 1e-5)."""
     from diablo3_amd.config import MIXED
-    p64, s0 = les_problem(32, 33, 32)
+    p64, s0 = les_problem(nx, ny, nz)
     cfg = p64.cfg.copy(precision=MIXED)
     p = Problem(cfg)
     g = Solver(p)
@@ -149,7 +151,7 @@ def test_les_mixed_precision_error():
     errs = {k: rel(fg[k], fo[k]) for k in fo}
     (ntg, _), (nto, _) = g.download_les(), o.download_les()
     errs["nu_t"] = rel(ntg, nto)
-    print("LES mixed x3", {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+    print(f"LES mixed {nx}x{ny}x{nz} x3", {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
     for k in ("cu1", "cu2", "cu3", "cth0"):
         assert errs[k] <= 1e-6, (k, errs[k])
     assert errs["nu_t"] <= 1e-5

@@ -166,6 +166,18 @@ def test_two_steps_dns_256_fp64():
     compare(g.download_state(), o.download_state(), 1e-11, "dns_256 x2", fl)
 
 
+@pytest.mark.parametrize("precision", [FP64, MIXED])
+def test_wide_z_steps(precision):
+    """16x17x512, 3 steps: the z transforms at nz = 512 take their widest column
+    tiles (one 128-byte row segment per z row) in both precisions."""
+    cfg = configs.small_dns(16, 17, 512)
+    p, g, o, fl = _run_pair(cfg, 3, seed=5, precision=precision)
+    if precision == FP64:
+        compare(g.download_state(), o.download_state(), 1e-11, "wide z x3", fl)
+    else:
+        compare(g.download_state(), o.download_state(), 1e-4, "wide z mixed x3")
+
+
 def test_mixed_precision_error_dns_64():
     cfg = configs.dns_64()
     p, g, o, _ = _run_pair(cfg, 10, precision=MIXED)
