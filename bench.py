@@ -137,7 +137,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="dns_256", choices=["dns_256", "dns_64", "les_512", "les_128"])
+    ap.add_argument("--workload", default="dns_256", choices=["dns_256", "dns_64", "les_512", "les_128", "les_1024"])
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--closure", default="amd", choices=["amd", "learned"],
                     help="LES workloads: the AMD closure (type 5) or the learned one (type 6)")

@@ -92,4 +92,9 @@ def les_128(precision: int = FP64) -> Config:
     return front_les(128, 129, 128, 5000.0, 2e-4, 1e-2, precision=precision)
 
 
-NAMED = {"dns_64": dns_64, "dns_256": dns_256, "les_128": les_128, "les_512": les_512}
+def les_1024(precision: int = FP64) -> Config:
+    """The front at 1024x385x1024: Re 2e5, N^2 1, dt 1e-4, amplitude 1e-2."""
+    return front_les(1024, 385, 1024, 2e5, 1e-4, 1e-2, precision=precision)
+
+
+NAMED = {"dns_64": dns_64, "dns_256": dns_256, "les_128": les_128, "les_512": les_512, "les_1024": les_1024}

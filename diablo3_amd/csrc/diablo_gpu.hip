@@ -520,7 +520,12 @@ int launch_fwd_x(dgpu_handle* h, int nf, T* const* src, void* const* dst, int j0
   return 0;
 }
 
-constexpr int xprod_nt(int nx) { return nx <= 32 ? 64 : (nx == 64 ? 128 : 256); }
+// threads per k_xprod block: 256 for the larger lines, 512 where the LDS tile
+// (> 80 KiB) leaves room for one block per CU only, so that a CU still holds
+// eight waves
+constexpr int xprod_nt(int nx, size_t lds) {
+  return nx <= 32 ? 64 : (nx == 64 ? 128 : (lds > 81920 ? 512 : 256));
+}
 
 template <typename T, int NTH, bool LES = false, bool SPLIT = false>
 int launch_xprod(dgpu_handle* h, XProdArgs& a, int nj) {
@@ -531,7 +536,7 @@ int launch_xprod(dgpu_handle* h, XProdArgs& a, int nj) {
     if constexpr (lds > 163840) {
       return fail(2, "this nx needs the mixed-precision mode (x pass exceeds LDS in FP64)");
     } else {
-      constexpr int NT = xprod_nt(N_);
+      constexpr int NT = xprod_nt(N_, lds);
       dim3 grid(h->g.nz / 2, nj, 1);
       k_xprod<T, N_, NT, NTH, LES, SPLIT><<<grid, NT, 0, h->st>>>(a);
     }
