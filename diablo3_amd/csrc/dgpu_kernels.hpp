@@ -285,7 +285,6 @@ __global__ __launch_bounds__(NT) void k_xprod(XProdArgs a) {
   stage_twiddles<T, NX, NT>(stw, (const cx<T>*)a.tw, threadIdx.x);
   const Geo& g = a.g;
   const int tid = threadIdx.x, z = 2 * blockIdx.x, j = a.jlo + blockIdx.y;
-  const cx<T>* __restrict__ tw = (const cx<T>*)a.tw;
   if constexpr (!SPLIT) {
     // inverse x transforms of (field f, plane j-1+d) into line 3f+d
     for (int idx = tid; idx < NL * NX; idx += NT) {

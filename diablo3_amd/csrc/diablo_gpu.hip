@@ -1825,6 +1825,12 @@ int dgpu_timing_read(dgpu_handle* h, int32_t kernel, double* total_ms, int64_t* 
   return 0;
 }
 
+int dgpu_get_sizes(const dgpu_handle* h, int32_t* nx, int32_t* ny, int32_t* nz, int32_t* nth) {
+  if (!h || !nx || !ny || !nz || !nth) return fail(1, "null argument");
+  *nx = h->c.nx; *ny = h->c.ny; *nz = h->c.nz; *nth = h->c.n_th;
+  return 0;
+}
+
 int dgpu_synchronize(dgpu_handle* h) {
   if (!h) return fail(1, "null handle");
   HIPCHK(hipStreamSynchronize(h->st));

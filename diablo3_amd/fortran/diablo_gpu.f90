@@ -38,12 +38,19 @@ module diablo_gpu
     type(c_ptr) :: kx, kz
   end type dgpu_config
 
+  ! the Timestep attributes of a flow file (include/dgpu_io.h)
+  type, bind(C), public :: dgpu_flow_times
+    real(c_double) :: time, save_flow_time, save_stats_time, save_movie_time
+    integer(c_int32_t) :: time_step
+  end type dgpu_flow_times
+
   public :: dgpu_create, dgpu_destroy, dgpu_upload_state, dgpu_download_state
   public :: dgpu_upload_rk, dgpu_download_rk, dgpu_rk_substep, dgpu_step, dgpu_rem_div
   public :: dgpu_fft_xz_to_fourier, dgpu_fft_xz_to_physical, dgpu_synchronize
   public :: dgpu_last_step_ms, dgpu_field_doubles, dgpu_last_error_f
   public :: dgpu_download_les, dgpu_stats, dgpu_stats_doubles
   public :: dgpu_set_variable_dt, dgpu_courant, dgpu_set_closure, dgpu_filter_chan
+  public :: dgpu_write_flow, dgpu_read_flow
 
   interface
     integer(c_int) function dgpu_create(cfg, h) bind(C, name="dgpu_create")
@@ -147,6 +154,24 @@ module diablo_gpu
     integer(c_int) function dgpu_set_closure(h, params, norm) bind(C, name="dgpu_set_closure")
       import :: c_int, c_ptr
       type(c_ptr), value :: h, params, norm
+    end function
+    ! flow checkpoints in the reference's HDF5 layout (libdgpu_io.so): they replace
+    ! WriteHDF5 / ReadHDF5 (for/phdf5.f90:918-1456) in save_flow / read_flow;
+    ! fname is passed as trim(fname)//c_null_char
+    integer(c_int) function dgpu_write_flow(h, fname, t, save_pressure) bind(C, name="dgpu_write_flow")
+      import :: c_int, c_int32_t, c_ptr, c_char, dgpu_flow_times
+      type(c_ptr), value :: h
+      character(kind=c_char), dimension(*), intent(in) :: fname
+      type(dgpu_flow_times), intent(in) :: t
+      integer(c_int32_t), value :: save_pressure
+    end function
+    integer(c_int) function dgpu_read_flow(h, fname, th_mask, t, read_pressure) bind(C, name="dgpu_read_flow")
+      import :: c_int, c_int32_t, c_ptr, c_char, dgpu_flow_times
+      type(c_ptr), value :: h
+      character(kind=c_char), dimension(*), intent(in) :: fname
+      integer(c_int32_t), value :: th_mask  ! bit n - 1: read TH n (not create_new_th(n))
+      type(dgpu_flow_times), intent(out) :: t
+      integer(c_int32_t), intent(out) :: read_pressure
     end function
     type(c_ptr) function dgpu_last_error_c() bind(C, name="dgpu_last_error")
       import :: c_ptr

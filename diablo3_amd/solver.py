@@ -84,12 +84,17 @@ def load():
     return L
 
 
-def exported_symbols() -> List[str]:
-    """Names of the functions the headers under include/ declare."""
-    import glob
+# headers whose functions libdiablo_gpu.so exports (include/dgpu_io.h belongs to
+# libdgpu_io.so)
+LIB_HEADERS = ("diablo_gpu.h", "dgpu_closure.h")
+
+
+def exported_symbols(headers=LIB_HEADERS) -> List[str]:
+    """Names of the functions the given headers under include/ declare."""
     import re
     names = set()
-    for path in glob.glob(os.path.join(os.path.dirname(HEADER_PATH), "*.h")):
+    for h in headers:
+        path = os.path.join(os.path.dirname(HEADER_PATH), h)
         names |= set(re.findall(r"\b(dgpu_[a-z0-9_]+)\s*\(", open(path).read()))
     return sorted(names)
 

@@ -200,6 +200,10 @@ int64_t dgpu_stats_doubles(const dgpu_config *cfg);
  * jstart_th..jend_th. */
 int dgpu_filter_chan(dgpu_handle *h, int32_t n);
 
+/* Grid sizes Nx, Ny, Nz and N_th of a handle's configuration (for buffers a
+ * caller sizes after creation, e.g. the checkpoint library). */
+int dgpu_get_sizes(const dgpu_handle *h, int32_t *nx, int32_t *ny, int32_t *nz, int32_t *nth);
+
 /* Text of the last error on this thread ("" if none). */
 const char *dgpu_last_error(void);
 

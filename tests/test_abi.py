@@ -27,6 +27,23 @@ def test_every_declared_symbol_is_exported(lib):
         assert hasattr(lib, n), n
 
 
+def test_io_library_exports_its_header(lib):
+    """libdgpu_io.so (checkpoints, statistics files) loads without a GPU and exports
+    every function include/dgpu_io.h declares; every header is covered by one of
+    the two libraries."""
+    import glob
+    from diablo3_amd import flow_io
+    if not os.path.exists(flow_io.IO_PATH):
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "diablo3_amd", "csrc")], check=True)
+    io = flow_io.load()
+    names = solver.exported_symbols(("dgpu_io.h",))
+    assert len(names) >= 8
+    for n in names:
+        assert hasattr(io, n), n
+    headers = {os.path.basename(p) for p in glob.glob(os.path.join(ROOT, "include", "*.h"))}
+    assert headers == set(solver.LIB_HEADERS) | {"dgpu_io.h"}
+
+
 def test_config_struct_layout_matches_header():
     # offsets the C compiler gives dgpu_config, from a tiny host program
     src = os.path.join(ROOT, "include", "diablo_gpu.h")

@@ -1,0 +1,146 @@
+"""libdgpu_io.so on the CPU: the flow file of WriteHDF5 / ReadHDF5
+(for/phdf5.f90:918-1456), the g2gf / gf2g maps (for/tools.f90:255-315) and the
+statistics series (for/phdf5.f90:7-278), checked against the numpy restatement
+(oracle/flow_io.py), closed forms, and the file as the HDF5 C library itself
+reads it (tests/h5raw.py).  No reference file exists to compare with; the layout
+is pinned by the reference's own calls (dataset names, dimension order, chunking,
+attribute types) as read back here."""
+import re
+
+import numpy as np
+import pytest
+
+import h5raw
+from oracle import flow_io as ofi
+
+pytestmark = pytest.mark.skipif(not h5raw.available(), reason="HDF5 C library not in the image")
+
+NX, NY, NZ = 8, 7, 6
+
+
+def fields(seed, nth=2):
+    rng = np.random.default_rng(seed)
+    shp = (NY + 2, NZ + 2, NX + 2)
+    return {"u1": rng.standard_normal(shp), "u2": rng.standard_normal(shp), "u3": rng.standard_normal(shp),
+            "th": [rng.standard_normal(shp) for _ in range(nth)], "p": rng.standard_normal(shp)}
+
+
+def io():
+    from diablo3_amd import flow_io
+    return flow_io
+
+
+def test_g2gf_matches_restatement_and_closed_forms():
+    fio = io()
+    v = fields(1)["u2"]
+    a = v.copy()
+    fio.g2gf(a, NX, NY, NZ)
+    assert np.array_equal(a, ofi.g2gf(v, NX, NY, NZ))
+    # closed form: a profile linear in the row index moves by half a row inside,
+    # and the wall rows keep the first and last interior values
+    lin = np.zeros_like(v)
+    lin[:, :NZ, :NX] = (3.0 + 0.25 * np.arange(NY + 2))[:, None, None]
+    b = lin.copy()
+    fio.g2gf(b, NX, NY, NZ)
+    rows = np.arange(NY + 2)
+    assert np.allclose(b[2:NY, :NZ, :NX], (3.0 + 0.25 * (rows[2:NY] + 0.5))[:, None, None], rtol=0, atol=1e-15)
+    assert np.all(b[1, :NZ, :NX] == lin[2, :NZ, :NX]) and np.all(b[NY, :NZ, :NX] == lin[NY, :NZ, :NX])
+    # x, z padding and row 0 are not touched
+    assert np.array_equal(b[:, NZ:, :], lin[:, NZ:, :]) and np.array_equal(b[0], lin[0])
+
+
+def test_gf2g_inverts_g2gf_and_sets_the_wall_rows():
+    fio = io()
+    v = fields(2)["u2"]
+    a = v.copy()
+    fio.g2gf(a, NX, NY, NZ)
+    fio.gf2g(a, NX, NY, NZ)
+    assert np.array_equal(a, ofi.gf2g(ofi.g2gf(v, NX, NY, NZ), NX, NY, NZ))
+    # interior rows come back (to rounding), the wall rows carry zero mass flux
+    assert np.allclose(a[2:NY + 1, :NZ, :NX], v[2:NY + 1, :NZ, :NX], rtol=0, atol=1e-14)
+    assert np.array_equal(a[1], -a[2]) and np.array_equal(a[NY + 1], -a[NY])
+
+
+def test_write_read_round_trip(tmp_path):
+    fio = io()
+    f = fields(3)
+    u2f = f["u2"].copy()
+    fio.g2gf(u2f, NX, NY, NZ)
+    path = str(tmp_path / "out.h5")
+    t = {"time": 1.25, "save_flow_time": 2.5, "save_stats_time": 0.125, "save_movie_time": 3.0, "time_step": 42}
+    fio.write_physical(path, NX, NY, NZ, f["u1"], u2f, f["u3"], f["th"], f["p"], t)
+    r = fio.read_physical(path, NX, NY, NZ, 2)
+    for k, src in (("u1", f["u1"]), ("u2f", u2f), ("u3", f["u3"]), ("p", f["p"])):
+        assert np.array_equal(r[k], ofi.swapyz(ofi.swapzy(src, NX, NY, NZ), NX, NY, NZ)), k
+    for n in range(2):
+        assert np.array_equal(r["th"][n], ofi.swapyz(ofi.swapzy(f["th"][n], NX, NY, NZ), NX, NY, NZ))
+    assert r["times"] == t
+
+
+def test_file_layout_as_hdf5_reads_it(tmp_path):
+    fio = io()
+    f = fields(4)
+    u2f = f["u2"].copy()
+    fio.g2gf(u2f, NX, NY, NZ)
+    path = str(tmp_path / "layout.h5")
+    t = {"time": 0.5, "save_flow_time": 1.0, "save_stats_time": 0.75, "save_movie_time": 2.0, "time_step": 7}
+    fio.write_physical(path, NX, NY, NZ, f["u1"], u2f, f["u3"], f["th"], f["p"], t)
+    h = h5raw.File(path)
+    try:
+        res, same, dims = h.attr("/", "Resolution", "i32")
+        assert list(res) == [NX, NY, NZ] and same and dims == (3,)
+        date, _, ddims = h.attr("/", "Date", "str")
+        assert ddims == (20,) and re.fullmatch(rb"\d\d-[A-Z]{3}-\d{4} \d\d:\d\d:\d\d", date), date
+        for name in ("Time", "Save_Flow_Time", "Save_Stats_Time", "Save_Movie_Time"):
+            v, same, d = h.attr("/Timestep", name, "f64")
+            key = {"Time": "time"}.get(name, name.lower())
+            assert same and d == () and v[0] == t[key], name
+        v, same, d = h.attr("/Timestep", "Time_Step", "i32")
+        assert same and d == () and v[0] == 7
+        # (Nx, Nyp, Nzp) Fortran dimensions = C order (Nz, Ny, Nx); one y plane per chunk
+        for name, src in (("U", f["u1"]), ("W", u2f), ("V", f["u3"]), ("TH1", f["th"][0]), ("TH2", f["th"][1]),
+                          ("P", f["p"])):
+            data, chunk, f64 = h.dataset("/Timestep/" + name)
+            assert data.shape == (NZ, NY, NX) and chunk == (NZ, 1, NX) and f64, name
+            # element (x+1, y, z+1) of the Fortran dataset = field(x, z, y)
+            assert np.array_equal(data, ofi.swapzy(src, NX, NY, NZ)), name
+            assert data[2, 0, 3] == src[1, 2, 3]
+        assert not h.exists("/Timestep/TH3")
+    finally:
+        h.close()
+
+
+def test_read_checks_resolution_and_masks_scalars(tmp_path):
+    fio = io()
+    f = fields(5)
+    path = str(tmp_path / "m.h5")
+    fio.write_physical(path, NX, NY, NZ, f["u1"], f["u2"], f["u3"], f["th"], None, {"time_step": 1})
+    with pytest.raises(RuntimeError, match="different resolution"):
+        fio.read_physical(path, NX + 2, NY, NZ, 2)
+    r = fio.read_physical(path, NX, NY, NZ, 2, th_mask=0b01)
+    assert r["p"] is None
+    assert np.array_equal(r["th"][0], ofi.swapyz(ofi.swapzy(f["th"][0], NX, NY, NZ), NX, NY, NZ))
+    assert not r["th"][1].any()  # scalar 2 not read: its array untouched (zeros here)
+
+
+def test_statistics_series(tmp_path):
+    fio = io()
+    path = str(tmp_path / "mean.h5")
+    profs = [np.linspace(0, 1, NY) * (k + 1) for k in range(3)]
+    for k, pr in enumerate(profs):
+        fio.append_profile(path, "ume", pr, 0.1 * (k + 1))
+        fio.append_real(path, "time", 0.1 * (k + 1))
+    h = h5raw.File(path)
+    try:
+        n, same, _ = h.attr("/ume", "SAMPLES", "i32")
+        assert n[0] == 3 and same
+        for k in range(3):
+            data, _, f64 = h.dataset(f"/ume/{k + 1:04d}")
+            assert f64 and np.array_equal(data, profs[k])
+            tv, _, _ = h.attr(f"/ume/{k + 1:04d}", "Time", "f64")
+            assert tv[0] == 0.1 * (k + 1)
+            v, _, _ = h.dataset(f"/time/{k + 1:04d}")
+            assert v.shape == (1,) and v[0] == 0.1 * (k + 1)
+        assert h.attr("/time", "SAMPLES", "i32")[0][0] == 3
+    finally:
+        h.close()

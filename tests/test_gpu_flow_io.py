@@ -1,0 +1,88 @@
+"""WriteHDF5 / ReadHDF5 with the device state (dgpu_write_flow, dgpu_read_flow)
+against the oracle's transforms and the numpy restatement of swapzy / g2gf /
+gf2g (oracle/flow_io.py), the file read back through the HDF5 C library itself.
+
+Bars: each dataset within 1e-13 (relative max) of the oracle's physical field,
+the writer's post-write state (the reference's g2gf / gf2g and transform round
+trip) within 1e-13 of the oracle's, and a fresh handle that reads the file holds
+the writer's post-write state on the rows the file carries, to 1e-15."""
+import numpy as np
+import pytest
+
+import h5raw
+import oracle
+from diablo3_amd import configs, layout
+from diablo3_amd.problem import Problem
+from diablo3_amd.solver import Solver
+from oracle import flow_io as ofi
+
+pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not h5raw.available(), reason="HDF5 C library not in the image")]
+
+
+def relmax(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def test_checkpoint_write_read(tmp_path):
+    from diablo3_amd import flow_io
+    cfg = configs.small_dns(32, 33, 32)
+    nx, ny, nz = cfg.nx, cfg.ny, cfg.nz
+    p = Problem(cfg)
+    g = Solver(p)
+    s0 = p.initial_state(seed=9, theta_noise=1e-3)
+    g.upload_state(s0)
+    g.rem_div()
+    for n in range(2):
+        g.step(time=n * cfg.delta_t, time_step=n + 1, create_new_flow=True)
+    s = g.download_state()
+    o = oracle.Oracle(p, threads=16)
+
+    def phys(four):  # the oracle's physical field in the Fortran physical layout
+        return layout.to_fortran_physical(cfg, o.fft_to_physical(four))
+
+    path = str(tmp_path / "out.h5")
+    times = {"time": 2 * cfg.delta_t, "save_flow_time": 1.0, "save_stats_time": 0.5, "save_movie_time": 0.25,
+             "time_step": 2}
+    print(f"write_flow {nx}x{ny}x{nz}", flush=True)
+    flow_io.write_flow(g, path, times, save_pressure=True)
+
+    u1, u2, u3, pp = phys(s.cu1), phys(s.cu2), phys(s.cu3), phys(s.cp)
+    th = phys(s.cth[0])
+    u2f = ofi.g2gf(u2, nx, ny, nz)
+    h = h5raw.File(path)
+    try:
+        assert list(h.attr("/", "Resolution", "i32")[0]) == [nx, ny, nz]
+        assert h.attr("/Timestep", "Time_Step", "i32")[0][0] == 2
+        for name, src in (("U", u1), ("W", u2f), ("V", u3), ("TH1", th), ("P", pp)):
+            data, chunk, _ = h.dataset("/Timestep/" + name)
+            err = relmax(data, ofi.swapzy(src, nx, ny, nz))
+            print(name, f"{err:.2e}", flush=True)
+            assert data.shape == (nz, ny, nx) and chunk == (nz, 1, nx) and err <= 1e-13, name
+    finally:
+        h.close()
+
+    # the writer's state afterwards: u2 through g2gf then gf2g, all fields through
+    # the transform round trip (for/phdf5.f90:1112-1114, 1171, 1187-1192)
+    s2 = g.download_state()
+    cp_back = layout.from_fortran_physical
+    want = {"cu1": o.fft_to_fourier(cp_back(cfg, u1)),
+            "cu2": o.fft_to_fourier(cp_back(cfg, ofi.gf2g(u2f, nx, ny, nz))),
+            "cu3": o.fft_to_fourier(cp_back(cfg, u3)), "cp": o.fft_to_fourier(cp_back(cfg, pp)),
+            "cth0": o.fft_to_fourier(cp_back(cfg, th))}
+    got = {"cu1": s2.cu1, "cu2": s2.cu2, "cu3": s2.cu3, "cp": s2.cp, "cth0": s2.cth[0]}
+    for k in want:
+        err = relmax(got[k], want[k])
+        print("post-write", k, f"{err:.2e}", flush=True)
+        assert err <= 1e-13, k
+
+    # a fresh handle reads the file
+    g2 = Solver(p)
+    t, had_p = flow_io.read_flow(g2, path)
+    assert had_p and t == times
+    s3 = g2.download_state()
+    for k, a, b in (("cu1", s3.cu1, s2.cu1), ("cu3", s3.cu3, s2.cu3), ("cp", s3.cp, s2.cp),
+                    ("cth0", s3.cth[0], s2.cth[0])):
+        assert relmax(a[1:ny + 1], b[1:ny + 1]) <= 1e-15, k
+    assert relmax(s3.cu2[1:ny + 2], s2.cu2[1:ny + 2]) <= 1e-15
+    g.close()
+    g2.close()
