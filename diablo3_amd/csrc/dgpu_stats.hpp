@@ -292,4 +292,117 @@ __global__ __launch_bounds__(256) void k_stat_plane(StatArgs a) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// z* statistics: the buoyancy-surface binning of compute_BPE and
+// Bin_Ystar_and_Write (for/diagnostics.f90:1733-1852, 1859-1953) for scalar 1.
+// Each point of rows jstart_th(1)..jend_th(1) falls in the bin of
+// s1 = th + dTHdX(1) (gx - Lx/2) (gx = x Lx / Nx): with x homogeneous the
+// uniform bins of [thmin, thmin + Nbin dTH); else bin = int(acos(-clip(s1) /
+// (Ro_inv + 1e-14)) Nbin / pi), pi the reference's single-precision value
+// (for/fft.f90:147).  Slot 0 accumulates dyf(j) (the PDF), slot 1 + f the point
+// field f times dyf(j): shear2 (for/diagnostics.f90:113-117), epsilon
+// (nu times the dissipation integrand, 843-1016, rows 2..Nyp), TKE (1273-1275),
+// KE (1354), and per scalar n th' w' (305-307) and |grad th + background|^2
+// (359-361).  The fields are those k_stat_zsum sums for the profiles.  One block
+// per (256 x, row): an LDS histogram (FP64 LDS atomics), added to the global one
+// at the end (so the order of the additions, and the last bits, vary run to run).
+struct ZstarArgs {
+  int nbin, nf, jlo, jhi, hx;
+  double thmin, thmax, dth, den, pi, dthdx0, lx;
+  double* hist;  // [nf][nbin]
+};
+
+__global__ __launch_bounds__(256) void k_stat_zstar(StatArgs a, ZstarArgs z) {
+  extern __shared__ double hs[];  // [nf][nbin]
+  const Geo& g = a.g;
+  const int t = threadIdx.x, x = blockIdx.x * 256 + t, j = blockIdx.y;
+  if (j < z.jlo || j > z.jhi) return;  // the whole block: j is the block's row
+  for (int i = t; i < z.nf * z.nbin; i += 256) hs[i] = 0.0;
+  __syncthreads();
+  const int ny = g.ny, nx = g.nx;
+  if (x < nx) {
+    const long long pl = (long long)g.ny2 * nx;
+    const long long pp = g.pp;
+    auto M = [&](int m, int jj) { return a.zm[m * pl + (long long)jj * nx + x]; };
+    auto F = [&](int s, int jj, int zz) { return a.f[s][(long long)jj * pp + (long long)zz * nx + x]; };
+    const double* dy = a.dy;
+    const double* dyf = a.dyf;
+    const double* gy = a.gy;
+    const double* gyf = a.gyf;
+    const bool gyf_row = (j >= 1 && j <= ny), gy_row = (j >= 2 && j <= ny);
+    const double wj = dyf[j];
+    const double gx = ((double)x * z.lx) / (double)nx;
+    const double bg = z.dthdx0 * (gx - 0.5 * z.lx);
+    const double mdx1 = a.hx ? 0.0 : M(SM_DXU1, j), mdx1m = a.hx ? 0.0 : (j >= 1 ? M(SM_DXU1, j - 1) : 0.0);
+    const double mdx2 = a.hx ? 0.0 : M(SM_DXU2, j);
+    const double mdx3 = a.hx ? 0.0 : M(SM_DXU3, j), mdx3m = a.hx ? 0.0 : (j >= 1 ? M(SM_DXU3, j - 1) : 0.0);
+    const double mu1 = M(SM_U1, j), mu2 = M(SM_U2, j), mu3 = M(SM_U3, j);
+    for (int zz = 0; zz < g.nz; ++zz) {
+      const double s1 = F(9, j, zz) + bg;
+      int b;
+      if (z.hx) {
+        b = (int)((s1 - z.thmin) / z.dth);  // truncation toward zero, as int()
+        b = b > z.nbin - 1 ? z.nbin - 1 : b;
+        b = b < 0 ? 0 : b;  // below thmin (outside the reference's bins): the first bin
+      } else {
+        const double c = fmin(fmax(s1, z.thmin), z.thmax);
+        b = (int)(acos(-c / z.den) * (double)z.nbin / z.pi);
+        b = b > z.nbin - 1 ? z.nbin - 1 : b;
+      }
+      const double u1 = F(0, j, zz), u2 = F(3, j, zz), u3 = F(6, j, zz);
+      const double f1 = u1 - mu1, f2 = u2 - mu2, f3 = u3 - mu3;
+      double sh = 0.0, tke = 0.0, eps = 0.0;
+      if (gyf_row) {
+        const double d1 = (F(0, j + 1, zz) - F(0, j - 1, zz)) / (2.0 * dyf[j]);
+        const double d3 = (F(6, j + 1, zz) - F(6, j - 1, zz)) / (2.0 * dyf[j]);
+        sh = d1 * d1 + d3 * d3;
+        tke = 0.5 * (f1 * f1) + 0.5 * (f2 * f2) + 0.5 * (f3 * f3);
+      }
+      if (gy_row) {
+        const double f1m = F(0, j - 1, zz) - M(SM_U1, j - 1), f3m = F(6, j - 1, zz) - M(SM_U3, j - 1);
+        auto itp = [&](double c, double m) { return (dyf[j - 1] * c * c + dyf[j] * m * m) / (2.0 * dy[j]); };
+        double e = itp(F(1, j, zz) - mdx1, F(1, j - 1, zz) - mdx1m);
+        const double a2 = F(4, j, zz) - mdx2;
+        e += a2 * a2;
+        const double t3 = (f1 - f1m) / dy[j];
+        e += t3 * t3;
+        e += itp(F(7, j, zz) - mdx3, F(7, j - 1, zz) - mdx3m);
+        e += itp(F(2, j, zz), F(2, j - 1, zz));
+        const double t6 = ((F(3, j + 1, zz) - M(SM_U2, j + 1)) - (F(3, j - 1, zz) - M(SM_U2, j - 1))) /
+                          (gy[j + 1] - gy[j - 1]);
+        e += t6 * t6;
+        const double t7 = (f3 - f3m) / dy[j];
+        e += t7 * t7;
+        const double a8 = F(5, j, zz);
+        e += a8 * a8;
+        e += itp(F(8, j, zz), F(8, j - 1, zz));
+        eps = a.nu * e;
+      }
+      const double ke = 0.5 * (u1 * u1 + u2 * u2 + u3 * u3);
+      atomicAdd(&hs[b], wj);
+      atomicAdd(&hs[1 * z.nbin + b], sh * wj);
+      atomicAdd(&hs[2 * z.nbin + b], eps * wj);
+      atomicAdd(&hs[3 * z.nbin + b], tke * wj);
+      atomicAdd(&hs[4 * z.nbin + b], ke * wj);
+      for (int n = 0; n < a.nth; ++n) {
+        double tw = 0.0, chi = 0.0;
+        if (gyf_row) {
+          const int ts = 3 * (3 + n);
+          const double ft = F(ts, j, zz) - M(SM_TH0 + n, j);
+          tw = ft * (0.5 * (u2 + F(3, j + 1, zz)) - 0.5 * (mu2 + M(SM_U2, j + 1)));
+          const double gx_ = F(ts + 1, j, zz) + a.dthdx[n];
+          const double gy_ = (F(9, j + 1, zz) - F(9, j - 1, zz)) / (gyf[j + 1] - gyf[j - 1]);
+          const double gz_ = F(ts + 2, j, zz) + a.dthdz[n];
+          chi = gx_ * gx_ + gy_ * gy_ + gz_ * gz_;
+        }
+        atomicAdd(&hs[(5 + 2 * n) * z.nbin + b], tw * wj);
+        atomicAdd(&hs[(6 + 2 * n) * z.nbin + b], chi * wj);
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = t; i < z.nf * z.nbin; i += 256)
+    if (hs[i] != 0.0) atomicAdd(&z.hist[i], hs[i]);
+}
+
 }  // namespace dgpu

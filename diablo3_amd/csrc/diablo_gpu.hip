@@ -136,6 +136,7 @@ struct dgpu_handle {
   double* d_cfl = nullptr;  // block minima of the last courant evaluation
   // statistics (allocated on the first dgpu_stats call of a DNS handle)
   double *d_zm = nullptr, *d_zs = nullptr, *d_prof = nullptr;
+  double* d_zhist = nullptr;  // z* histograms [5 + 2 MAXTH][nx]
   double* hstage = nullptr;  // pinned staging, one Fortran array
   size_t fort_doubles = 0;
   std::vector<void*> allocs;
@@ -1650,8 +1651,10 @@ int64_t dgpu_stats_doubles(const dgpu_config* cfg) {
   return (int64_t)(SP_NBASE + SP_PER_TH * cfg->n_th) * (cfg->ny + 2) + 1;
 }
 
-int dgpu_stats(dgpu_handle* h, double time, double* out) {
-  if (!h || !out) return fail(1, "null argument");
+namespace {
+// The first stage shared by the statistics entry points: the BC posts on the
+// state, the physical gradient fields and the z-line means (k_stat_mean).
+int stats_prepare(dgpu_handle* h, double time, StatArgs& a) {
   const dgpu_config& c = h->c;
   const Geo& g = h->g;
   const int nth = h->nth, n2 = g.ny2;
@@ -1669,7 +1672,7 @@ int dgpu_stats(dgpu_handle* h, double time, double* out) {
   // save_stats_chan starts with the BC posts on the state (for/diagnostics.f90:26-27)
   if ((rc = launch_bc_post(h, time, 0))) return rc;
   if ((rc = physical_gradients<double>(h, h->HinD, h->HinZD, h->PgD))) return rc;
-  StatArgs a{};
+  a = StatArgs{};
   a.g = g; a.nth = nth;
   a.hx = c.homogeneous_x != 0;
   a.nu = c.nu; a.ly = c.ly; a.lx = c.lx;
@@ -1683,6 +1686,20 @@ int dgpu_stats(dgpu_handle* h, double time, double* out) {
   a.zm = h->d_zm; a.zs = h->d_zs; a.prof = h->d_prof;
   const dim3 gxj((unsigned)((g.nx + 255) / 256), (unsigned)n2);
   k_stat_mean<<<gxj, 256, 0, h->st>>>(a);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+}  // namespace
+
+int dgpu_stats(dgpu_handle* h, double time, double* out) {
+  if (!h || !out) return fail(1, "null argument");
+  const dgpu_config& c = h->c;
+  const Geo& g = h->g;
+  const int nth = h->nth, n2 = g.ny2;
+  StatArgs a;
+  int rc = stats_prepare(h, time, a);
+  if (rc) return rc;
+  const dim3 gxj((unsigned)((g.nx + 255) / 256), (unsigned)n2);
   k_stat_zsum<<<gxj, 256, 0, h->st>>>(a);
   k_stat_plane<<<(unsigned)n2, 256, 0, h->st>>>(a);
   HIPCHK(hipGetLastError());
@@ -1694,6 +1711,72 @@ int dgpu_stats(dgpu_handle* h, double time, double* out) {
   double res = 0.0;
   for (int j = 2; j <= g.ny; ++j) res = res + 0.5 * (u1y[j] + u1y[j - 1]) * h->dy[j];
   out[np_] = res / c.ly;
+  return 0;
+}
+
+int64_t dgpu_stats_zstar_doubles(const dgpu_config* cfg) {
+  if (!cfg || cfg->n_th < 1 || cfg->nx <= 0) return -1;
+  const int64_t nb = cfg->nx;
+  return nb + (nb + 1) + 1 + (4 + 2 * (int64_t)cfg->n_th) * nb;
+}
+
+int dgpu_stats_zstar(dgpu_handle* h, double time, double* out) {
+  if (!h || !out) return fail(1, "null argument");
+  const dgpu_config& c = h->c;
+  const Geo& g = h->g;
+  const int nth = h->nth, nbin = g.nx, nf = 5 + 2 * nth;
+  if (nth < 1) return fail(2, "z* statistics need a scalar (N_th >= 1)");
+  const size_t lds = sizeof(double) * (size_t)nf * nbin;
+  if (lds > 65536) return fail(2, "z* statistics: Nx (5 + 2 N_th) bins exceed the 64 KiB LDS histogram");
+  StatArgs a;
+  int rc = stats_prepare(h, time, a);
+  if (rc) return rc;
+  if (!h->d_zhist) {
+    void* p = nullptr;
+    if ((rc = dalloc(h, &p, sizeof(double) * (5 + 2 * MAXTH) * (size_t)nbin))) return rc;
+    h->d_zhist = (double*)p;
+  }
+  HIPCHK(hipMemsetAsync(h->d_zhist, 0, lds, h->st));
+  ZstarArgs z{};
+  z.nbin = nbin; z.nf = nf; z.jlo = c.jstart_th[0]; z.jhi = c.jend_th[0]; z.hx = a.hx;
+  z.lx = c.lx; z.dthdx0 = c.dthdx[0];
+  z.pi = (double)(4.0f * std::atan(1.0f));  // the reference's pi (for/fft.f90:147)
+  if (z.hx) {
+    // -dTHdX(1) * 0.5*Lx * 1.1 with the single-precision literal 1.1
+    z.thmin = -(c.dthdx[0] * 0.5 * c.lx * (double)1.1f);
+    z.thmax = c.dthdx[0] * 0.5 * c.lx * (double)1.1f;
+    z.dth = (z.thmax - z.thmin) / nbin + 1.0e-14;
+  } else {
+    z.thmin = -c.ro_inv;
+    z.thmax = c.ro_inv;
+    z.den = c.ro_inv + 1.0e-14;
+  }
+  z.hist = h->d_zhist;
+  k_stat_zstar<<<dim3((unsigned)((g.nx + 255) / 256), (unsigned)g.ny2), 256, lds, h->st>>>(a, z);
+  HIPCHK(hipGetLastError());
+  std::vector<double> hist((size_t)nf * nbin);
+  HIPCHK(hipMemcpyAsync(hist.data(), h->d_zhist, lds, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  // compute_BPE (for/diagnostics.f90:1794-1830): PDF, bin edges, Y_r, BPE
+  double* pdf = out;
+  double* thb = out + nbin;
+  double* bpe = thb + nbin + 1;
+  double* fld = bpe + 1;
+  double tot = 0.0;
+  for (int i = 0; i < nbin; ++i) tot += hist[i];
+  for (int i = 0; i < nbin; ++i) pdf[i] = hist[i] / tot;
+  for (int i = 0; i <= nbin; ++i)
+    thb[i] = z.hx ? z.thmin + i * z.dth : -(c.ro_inv + 1.0e-14) * std::cos(i * z.pi / nbin);
+  std::vector<double> yr(nbin + 1, 0.0);
+  for (int i = 1; i <= nbin; ++i) yr[i] = yr[i - 1] + pdf[i - 1];
+  for (int i = 0; i <= nbin; ++i) yr[i] = yr[i] * c.ly;
+  double b = 0.0;
+  for (int i = 0; i < nbin; ++i)
+    b = b - (0.5 * (thb[i + 1] + thb[i]) * 0.5 * (yr[i + 1] + yr[i])) * (yr[i + 1] - yr[i]) / c.ly;
+  *bpe = b;
+  // Bin_Ystar_and_Write (for/diagnostics.f90:1936): the binned sums over the raw PDF
+  for (int f = 1; f < nf; ++f)
+    for (int i = 0; i < nbin; ++i) fld[(size_t)(f - 1) * nbin + i] = hist[(size_t)f * nbin + i] / hist[i];
   return 0;
 }
 

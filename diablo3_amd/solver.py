@@ -63,6 +63,10 @@ def load():
                                D, ctypes.POINTER(ctypes.c_int32)]
     L.dgpu_stats_doubles.argtypes = [P]
     L.dgpu_stats_doubles.restype = ctypes.c_int64
+    L.dgpu_stats_zstar.argtypes = [P, ctypes.c_double, D]
+    L.dgpu_stats_zstar_doubles.argtypes = [P]
+    L.dgpu_stats_zstar_doubles.restype = ctypes.c_int64
+    L.dgpu_get_sizes.argtypes = [P] + [ctypes.POINTER(ctypes.c_int32)] * 4
     L.dgpu_field_doubles.argtypes = [P]
     L.dgpu_field_doubles.restype = ctypes.c_int64
     F = ctypes.POINTER(ctypes.c_float)
@@ -213,6 +217,24 @@ class Solver:
         names = PROFILES + [f"{b}{k + 1:02d}" for k in range(self.cfg.n_th) for b in PER_SCALAR]
         res = {name: out[i * n2:(i + 1) * n2].copy() for i, name in enumerate(names)}
         res["u1z_x0"] = float(out[-1])
+        return res
+
+    def stats_zstar(self, time: float = 0.0):
+        """compute_BPE and Bin_Ystar_and_Write of save_stats_chan (scalar 1):
+        {'th1PDF', 'th1bin', 'BPE', 'shear2_zstar', 'epsilon_zstar', 'TKE_zstar',
+        'KE_zstar', 'thw_zstar01', 'chi_zstar01', ...}."""
+        n = load().dgpu_stats_zstar_doubles(ctypes.cast(self.p.cconfig.ptr, ctypes.c_void_p))
+        if n < 0:
+            raise RuntimeError("z* statistics need a scalar (n_th >= 1)")
+        out = np.zeros(n, np.float64)
+        self._check(load().dgpu_stats_zstar(self.h, time, _dp(out)))
+        nb = self.cfg.nx
+        res = {"th1PDF": out[:nb].copy(), "th1bin": out[nb:2 * nb + 1].copy(), "BPE": float(out[2 * nb + 1])}
+        names = ["shear2_zstar", "epsilon_zstar", "TKE_zstar", "KE_zstar"]
+        names += [f"{b}{k + 1:02d}" for k in range(self.cfg.n_th) for b in ("thw_zstar", "chi_zstar")]
+        base = 2 * nb + 2
+        for i, name in enumerate(names):
+            res[name] = out[base + i * nb:base + (i + 1) * nb].copy()
         return res
 
     # ---- time advance ----

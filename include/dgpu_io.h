@@ -60,9 +60,11 @@ int dgpu_h5_read_flow(const char *fname, int32_t nx, int32_t ny, int32_t nz, int
 void dgpu_g2gf(double *var, int32_t nx, int32_t ny, int32_t nz);
 void dgpu_gf2g(double *var, int32_t nx, int32_t ny, int32_t nz);
 
-/* WriteStatH5_Y (for/phdf5.f90:129-278): appends the profile diag[0..Nyp-1] as
- * dataset "%04d" (the next sample number) of group gname, with attribute Time,
- * and updates the group's SAMPLES counter; the file is created if absent. */
+/* WriteStatH5_Y (for/phdf5.f90:129-278) and WriteStatH5_X (285-431), which
+ * write the same layout at one rank: appends the n values of diag (Nyp for a y
+ * profile, Nbin for an x or z* profile) as dataset "%04d" (the next sample
+ * number) of group gname, with attribute Time, and updates the group's SAMPLES
+ * counter; the file is created if absent. */
 int dgpu_h5_append_profile(const char *fname, const char *gname, const double *diag, int32_t ny,
                            double time);
 

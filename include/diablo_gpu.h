@@ -190,8 +190,26 @@ int dgpu_courant(dgpu_handle *h, double time, double time_limit, double save_sta
  *   22 u1y_left
  * then per scalar n: thme, thrms, thv, thv_m, dthdy, pe_diss; then one double,
  * u1z_x0 (the y-average of u1y_left, integrate_y_var, for/diagnostics.f90:1964).
- * Not computed: movie planes, z* bins, the BPE PDF, FTx_uu, the vorticity rms. */
+ * z* bins and the BPE come from dgpu_stats_zstar.  Not computed: movie planes,
+ * FTx_uu, the vorticity rms. */
 int dgpu_stats(dgpu_handle *h, double time, double *out);
+
+/* The z* statistics of save_stats_chan for scalar 1 (N_th >= 1), on the device in
+ * FP64 from the current state after the same BC posts: compute_BPE
+ * (for/diagnostics.f90:1733-1852) and Bin_Ystar_and_Write (1859-1953) with
+ * Nbin = Nx bins of th + dTHdX(1) (gx - Lx/2) over rows jstart_th(1)..jend_th(1)
+ * (uniform bins when x is homogeneous, else the cos-spaced bins of the finite
+ * front).  out holds dgpu_stats_zstar_doubles(cfg) doubles:
+ *   [0, Nbin)            th1PDF (normalised, dyf-weighted)
+ *   [Nbin, 2 Nbin + 1)   th1bin, the Nbin + 1 bin edges (mean.h5 keeps the first Nbin)
+ *   [2 Nbin + 1]         BPE
+ *   then Nbin each, the z*-binned means (sum f dyf / sum dyf per bin; NaN in an
+ *   empty bin, as the reference's 0/0): shear2_zstar, epsilon_zstar, TKE_zstar,
+ *   KE_zstar, then per scalar n: thw_zstar(n), chi_zstar(n).
+ * The bin sums use atomic additions, so their last bits vary from call to call.
+ * Nx (5 + 2 N_th) must not exceed 8192 (the histogram lives in 64 KiB of LDS). */
+int dgpu_stats_zstar(dgpu_handle *h, double time, double *out);
+int64_t dgpu_stats_zstar_doubles(const dgpu_config *cfg);
 int64_t dgpu_stats_doubles(const dgpu_config *cfg);
 
 /* filter_chan(n) (for/solver.f90:1597-1685), as the driver calls it every

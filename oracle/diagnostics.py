@@ -31,10 +31,11 @@ def homogeneous_x(cfg):
     return cfg.ic_type not in (5, 6, 7, 8)
 
 
-def save_stats(o, time=0.0):
+def save_stats(o, time=0.0, points=False):
     """Statistics of the oracle's current state; applies the BC posts to it, as
     save_stats_chan does (for/diagnostics.f90:26-27).  Returns {name: profile}
-    plus 'u1z_x0' (a scalar)."""
+    plus 'u1z_x0' (a scalar); with points=True also '_points', the per-point fields
+    the reference bins on z* surfaces ([j][z][x] arrays) and th of scalar 1."""
     p = o.p
     cfg, met = p.cfg, p.met
     ny, nx, nz = cfg.ny, cfg.nx, cfg.nz
@@ -83,6 +84,7 @@ def save_stats(o, time=0.0):
     k_start = 0 if hx else 1
     interp = lambda f, j: (dyf[j - 1] * f[j] ** 2 + dyf[j] * f[j - 1] ** 2) / (2.0 * dy[j])
     eps = np.zeros(n2)
+    pts = {"epsilon_zstar": np.zeros_like(u1), "shear2_zstar": np.zeros_like(u1)}
     a1 = d_phys(1j * kx * cr1, k_start)
     a2 = d_phys(1j * kx * cr2, k_start, 2, ny)
     a4 = d_phys(1j * kx * cr3, k_start)
@@ -98,6 +100,7 @@ def save_stats(o, time=0.0):
         t = t + (((u3[j] - wme_xy[j][None, :]) - (u3[j - 1] - wme_xy[j - 1][None, :])) / dy[j]) ** 2
         t = t + a8[j] ** 2 + interp(a9, j)
         eps[j] = t.sum()
+        pts["epsilon_zstar"][j] = nu * t       # f1 = nu * f1 (for/diagnostics.f90:1021)
     out["epsilon"] = nu * eps / NxNz
 
     # ---- compute_MKE_diss (1097-1249); gradients of the z-mean fields
@@ -127,6 +130,9 @@ def save_stats(o, time=0.0):
     f3 = u3 - wme_xy[:, None, :]
     J = slice(1, ny + 1)
     uu_xy = np.zeros((n2, nx)); vv_xy = np.zeros((n2, nx)); ww_xy = np.zeros((n2, nx))
+    pts["TKE_zstar"] = np.zeros_like(u1)
+    pts["TKE_zstar"][J] = 0.5 * f1[J] ** 2 + 0.5 * f2[J] ** 2 + 0.5 * f3[J] ** 2   # (1273-1275)
+    pts["KE_zstar"] = 0.5 * (u1 ** 2 + u2 ** 2 + u3 ** 2)                          # (1354)
     for name, f, xy in (("urms", f1, uu_xy), ("vrms", f2, vv_xy), ("wrms", f3, ww_xy)):
         out[name][J] = np.sqrt((f[J] ** 2).sum(axis=(1, 2)) / NxNz)
         xy[J] = (f[J] ** 2).sum(axis=1) / nz
@@ -165,6 +171,7 @@ def save_stats(o, time=0.0):
         out["dwdy"][j] = (wme[j] - wme[j - 1]) / (gyf[j] - gyf[j - 1])
         sh = ((u1[j + 1] - u1[j - 1]) / (2.0 * dyf[j])) ** 2 + ((u3[j + 1] - u3[j - 1]) / (2.0 * dyf[j])) ** 2
         out["shear"][j] = sh.sum() / NxNz
+        pts["shear2_zstar"][j] = sh
 
     # ---- scalars (140-410)
     gxv = np.array([(i * cfg.lx) / nx for i in range(nx)])
@@ -178,14 +185,18 @@ def save_stats(o, time=0.0):
         r3 = d_phys(1j * kz * c)
         ft = t - thme_xy[n][:, None, :]
         out["thrms" + tag][J] = np.sqrt((ft[J] ** 2).sum(axis=(1, 2)) / NxNz)
+        pts["thw_zstar" + tag] = np.zeros_like(u1)
+        pts["chi_zstar" + tag] = np.zeros_like(u1)
         for j in range(1, ny + 1):
             w = 0.5 * (u2[j] + u2[j + 1]) - 0.5 * (vme_xy[j] + vme_xy[j + 1])[None, :]
             out["thv" + tag][j] = (ft[j] * w).sum() / NxNz
+            pts["thw_zstar" + tag][j] = ft[j] * w                                    # (305-307)
             out["thv_m" + tag][j] = ((thme_xy[n][j] + cfg.dthdx[n] * (gxv - 0.5 * cfg.lx)) *
                                      (0.5 * (vme_xy[j] + vme_xy[j + 1]))).sum() / float(nx)
             out["dthdy" + tag][j] = (thme[n][j] - thme[n][j - 1]) / (gyf[j] - gyf[j - 1])
             g2 = (r1[j] + cfg.dthdx[n]) ** 2 + r2[j] ** 2 + (r3[j] + cfg.dthdz[n]) ** 2
             out["pe_diss" + tag][j] = g2.sum() / NxNz
+            pts["chi_zstar" + tag][j] = g2                                           # (359-361)
         out["thme" + tag] = thme[n]
     # <u1 y> at the x = 0 edge, integrated in y (for/diagnostics.f90:388-402, 1964-1980)
     u1y = np.zeros(n2)
@@ -195,4 +206,61 @@ def save_stats(o, time=0.0):
     for j in range(2, ny + 1):
         res = res + 0.5 * (u1y[j] + u1y[j - 1]) * dy[j]
     out["u1z_x0"] = res / cfg.ly
+    if points:
+        pts["th0"] = th[0] if th else None
+        out["_points"] = pts
     return out
+
+
+# the reference's pi: 4 atan(1.0) in single precision (for/fft.f90:147)
+PI_F32 = float(np.float32(4.0) * np.arctan(np.float32(1.0)))
+ZSTAR_FIELDS = ["shear2_zstar", "epsilon_zstar", "TKE_zstar", "KE_zstar"]
+
+
+def zstar_bins(cfg, dyf, th0, fields, hx):
+    """compute_BPE and Bin_Ystar_and_Write (for/diagnostics.f90:1733-1852,
+    1859-1953) on given point fields: th0 and each field are [j][z][x] arrays
+    over rows 0..Nyp+1.  Returns {'th1PDF', 'th1bin', 'BPE', name: binned mean}."""
+    nx = th0.shape[2]
+    nb = nx
+    jlo, jhi = cfg.jstart_th[0], cfg.jend_th[0]
+    gxv = (np.arange(nx) * cfg.lx) / nx                       # gx(i) = (i Lx) / Nx
+    s1 = th0[jlo:jhi + 1] + cfg.dthdx[0] * (gxv - 0.5 * cfg.lx)[None, None, :]
+    if hx:
+        thmin = -(cfg.dthdx[0] * 0.5 * cfg.lx * float(np.float32(1.1)))  # the literal 1.1 is single
+        thmax = cfg.dthdx[0] * 0.5 * cfg.lx * float(np.float32(1.1))
+        dth = (thmax - thmin) / nb + 1.0e-14
+        th_bin = thmin + np.arange(nb + 1) * dth
+        b = np.minimum(np.trunc((s1 - thmin) / dth).astype(np.int64), nb - 1)
+        b = np.maximum(b, 0)  # below thmin the reference indexes outside its array; here the first bin
+    else:
+        den = cfg.ro_inv + 1.0e-14
+        th_bin = -den * np.cos(np.arange(nb + 1) * PI_F32 / nb)
+        c = np.minimum(np.maximum(s1, -cfg.ro_inv), cfg.ro_inv)
+        b = np.minimum(np.trunc(np.arccos(-c / den) * nb / PI_F32).astype(np.int64), nb - 1)
+    w = np.broadcast_to(dyf[jlo:jhi + 1][:, None, None], s1.shape)
+    pdf_raw = np.bincount(b.ravel(), weights=w.ravel(), minlength=nb)
+    res = {}
+    with np.errstate(invalid="ignore", divide="ignore"):
+        for name, f in fields.items():
+            sums = np.bincount(b.ravel(), weights=(f[jlo:jhi + 1] * w).ravel(), minlength=nb)
+            res[name] = sums / pdf_raw
+    pdf = pdf_raw / pdf_raw.sum()
+    yr = np.zeros(nb + 1)
+    for i in range(1, nb + 1):
+        yr[i] = yr[i - 1] + pdf[i - 1]
+    yr = yr * cfg.ly
+    bpe = 0.0
+    for i in range(nb):
+        bpe = bpe - (0.5 * (th_bin[i + 1] + th_bin[i]) * 0.5 * (yr[i + 1] + yr[i])) * (yr[i + 1] - yr[i]) / cfg.ly
+    res.update({"th1PDF": pdf, "th1bin": th_bin, "BPE": bpe, "_bin": b})
+    return res
+
+
+def zstar(o, time=0.0):
+    """The z* statistics of the oracle's state (after the BC posts of save_stats)."""
+    cfg = o.p.cfg
+    out = save_stats(o, time, points=True)
+    pts = out["_points"]
+    names = ZSTAR_FIELDS + [f"{b}{n + 1:02d}" for n in range(cfg.n_th) for b in ("thw_zstar", "chi_zstar")]
+    return zstar_bins(cfg, o.p.met["dyf"], pts["th0"], {k: pts[k] for k in names}, homogeneous_x(cfg))

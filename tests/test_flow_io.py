@@ -144,3 +144,40 @@ def test_statistics_series(tmp_path):
         assert h.attr("/time", "SAMPLES", "i32")[0][0] == 3
     finally:
         h.close()
+
+
+def test_mean_h5_sample(tmp_path):
+    """One save_stats sample in mean.h5 with the reference's group names (v and w
+    swapped in the names, for/diagnostics.f90:140-235, 453-477) and series
+    counters; the z* groups and BPE (1838-1850)."""
+    from diablo3_amd import stats_io
+    from diablo3_amd.stats import PROFILES, PER_SCALAR
+    ny, nth, nb = 9, 2, 12
+    rng = np.random.default_rng(8)
+    prof = {k: rng.standard_normal(ny + 2) for k in PROFILES}
+    prof.update({f"{b}{n + 1:02d}": rng.standard_normal(ny + 2) for n in range(nth) for b in PER_SCALAR})
+    prof["u1z_x0"] = 0.375
+    z = {k: rng.standard_normal(nb) for k in stats_io.ZSTAR_GROUPS}
+    z.update({f"{b}{n + 1:02d}": rng.standard_normal(nb) for n in range(nth) for b in ("thw_zstar", "chi_zstar")})
+    z.update({"th1PDF": rng.random(nb), "th1bin": np.linspace(-1, 1, nb + 1), "BPE": -0.125})
+    gyf = np.linspace(-1, 1, ny + 2)
+    path = str(tmp_path / "mean.h5")
+    stats_io.write_mean(path, 0.5, prof, gyf, ny, nth, z)
+    h = h5raw.File(path)
+    try:
+        J = slice(1, ny + 1)
+        assert np.array_equal(h.dataset("/wme/0001")[0], prof["vme"][J])
+        assert np.array_equal(h.dataset("/vrms/0001")[0], prof["wrms"][J])
+        assert np.array_equal(h.dataset("/ww_dwdz/0001")[0], prof["vv_dvdy"][J])
+        assert np.array_equal(h.dataset("/thw02_m/0001")[0], prof["thv_m02"][J])
+        assert np.array_equal(h.dataset("/dthdz01/0001")[0], prof["dthdy01"][J])
+        assert np.array_equal(h.dataset("/gzf/0001")[0], gyf[J])
+        assert h.dataset("/time/0001")[0][0] == 0.5 and h.dataset("/u1z_x0/0001")[0][0] == 0.375
+        assert h.dataset("/BPE/0001")[0][0] == -0.125
+        assert np.array_equal(h.dataset("/th1bin/0001")[0], z["th1bin"][:nb])
+        assert np.array_equal(h.dataset("/thw_zstar/0002")[0], z["thw_zstar02"])
+        assert h.attr("/thw_zstar", "SAMPLES", "i32")[0][0] == nth
+        assert h.attr("/ume", "SAMPLES", "i32")[0][0] == 1
+        assert h.attr("/ume/0001", "Time", "f64")[0][0] == 0.5
+    finally:
+        h.close()

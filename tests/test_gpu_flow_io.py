@@ -86,3 +86,38 @@ def test_checkpoint_write_read(tmp_path):
     assert relmax(s3.cu2[1:ny + 2], s2.cu2[1:ny + 2]) <= 1e-15
     g.close()
     g2.close()
+
+
+def test_save_stats_mean_h5(tmp_path):
+    """save_stats_chan's mean.h5 sample from the device statistics: the groups
+    hold the device profiles under the reference's names (a second evaluation of
+    the same state gives the same profiles; the z* sums to rounding)."""
+    from diablo3_amd import stats_io
+    cfg = configs.front_les(32, 33, 32, 5000.0, 2e-4, 1e-2)
+    cfg.use_les = False
+    ny = cfg.ny
+    p = Problem(cfg)
+    g = Solver(p)
+    g.upload_state(p.initial_state(seed=3, theta_noise=1e-2))
+    g.rem_div()
+    g.step(time=0.0, time_step=1, create_new_flow=True)
+    path = str(tmp_path / "mean.h5")
+    print("save_stats -> mean.h5", flush=True)
+    stats_io.save_stats(g, path, 0.25)
+    prof, z = g.stats(0.25), g.stats_zstar(0.25)
+    h = h5raw.File(path)
+    try:
+        J = slice(1, ny + 1)
+        for group, key in stats_io.PROFILE_GROUPS:
+            assert np.array_equal(h.dataset(f"/{group}/0001")[0], prof[key][J]), group
+        assert np.array_equal(h.dataset("/thw01_m/0001")[0], prof["thv_m01"][J])
+        assert abs(h.dataset("/BPE/0001")[0][0] - z["BPE"]) <= 1e-13 * abs(z["BPE"])
+        for name in ("th1PDF", "KE_zstar", "chi_zstar"):
+            a = h.dataset(f"/{name}/0001")[0]
+            b = z["chi_zstar01"] if name == "chi_zstar" else z[name]
+            m = ~np.isnan(b)
+            assert np.array_equal(np.isnan(a), ~m) and np.allclose(a[m], b[m], rtol=1e-12, atol=0), name
+        assert h.attr("/time", "SAMPLES", "i32")[0][0] == 1
+    finally:
+        h.close()
+    g.close()

@@ -65,3 +65,33 @@ def test_stats_match_oracle(which):
     assert so["epsilon"].max() > 0 and np.abs(so["thv01"]).max() > 0 and so["urms"].max() > 0
     if not diagnostics.homogeneous_x(cfg):
         assert so["epsilon_m"].max() > 0 and np.abs(so["uu_dudx"]).max() > 0
+
+
+@pytest.mark.parametrize("which", ["infinite_front_dns", "finite_front"])
+def test_zstar_matches_oracle(which):
+    """dgpu_stats_zstar (compute_BPE, Bin_Ystar_and_Write) against the oracle:
+    PDF, BPE and every binned mean within 1e-10 (relative to the largest
+    magnitude; NaN exactly where a bin is empty), the bin edges to 1e-15."""
+    if which == "infinite_front_dns":
+        cfg = configs.small_dns(32, 33, 32)
+    else:
+        cfg = configs.front_les(32, 33, 32, 5000.0, 2e-4, 1e-2)
+        cfg.use_les = False
+    p, o = prepared(cfg)
+    g = Solver(p)
+    g.upload_state(o.download_state())
+    zg = g.stats_zstar(time=0.0)
+    zo = diagnostics.zstar(o, time=0.0)
+    occupied = int(np.count_nonzero(zo["th1PDF"]))
+    print(which, f"BPE {zg['BPE']:.6e} / {zo['BPE']:.6e}, {occupied} of {cfg.nx} bins occupied", flush=True)
+    assert occupied > 2
+    assert np.abs(zg["th1bin"] - zo["th1bin"]).max() <= 1e-15 * np.abs(zo["th1bin"]).max()
+    assert abs(zg["BPE"] - zo["BPE"]) <= 1e-10 * abs(zo["BPE"])
+    for name in ["th1PDF"] + diagnostics.ZSTAR_FIELDS + ["thw_zstar01", "chi_zstar01"]:
+        a, b = zg[name], zo[name]
+        assert np.array_equal(np.isnan(a), np.isnan(b)), name
+        m = ~np.isnan(b)
+        scale = np.abs(b[m]).max()
+        err = np.abs(a[m] - b[m]).max() / (scale if scale > 0 else 1.0)
+        print(f"  {name:14s} max|ref| {scale:.3e} rel err {err:.2e}", flush=True)
+        assert err <= 1e-10, (name, err)

@@ -114,3 +114,41 @@ def test_dissipation_and_gradients_parseval():
     # the mean profiles are the (0,0) modes
     np.testing.assert_array_equal(st["ume"], s.cu1[:, 0, 0].real)
     np.testing.assert_array_equal(st["thme01"], s.cth[0][:, 0, 0].real)
+
+
+@pytest.mark.parametrize("hx", [True, False])
+def test_zstar_bins_closed_forms(hx):
+    """The oracle's z* binning (compute_BPE, Bin_Ystar_and_Write) on synthetic
+    fields: each point lies between the edges of its bin; the PDF sums to 1; a
+    constant field bins to that constant in every occupied bin (NaN in the empty
+    ones); with s1 constant all weight is in one bin b and
+    BPE = -Ly (edge_b + edge_b+1) / 4."""
+    from oracle import diagnostics as dg
+    cfg = configs.front_les(16, 9, 8, 5000.0, 2e-4, 1e-2)
+    p = Problem(cfg)
+    dyf = p.met["dyf"]
+    ny2, nz, nx = cfg.ny2, cfg.nz, cfg.nx
+    rng = np.random.default_rng(4)
+    gx = (np.arange(nx) * cfg.lx) / nx
+    # random scalar spanning the bins, plus two constant fields
+    # (x homogeneous: inside the uniform bins, |s1| < 0.55 dTHdX Lx; else past the
+    # ends too, where the acos bins clip)
+    th0 = rng.uniform(-1.0, 1.0, (ny2, nz, nx)) * (0.6 * cfg.ro_inv if not hx else 0.04 * cfg.dthdx[0] * cfg.lx)
+    r = dg.zstar_bins(cfg, dyf, th0, {"c": np.full((ny2, nz, nx), 2.5)}, hx)
+    jlo, jhi = cfg.jstart_th[0], cfg.jend_th[0]
+    s1 = th0[jlo:jhi + 1] + cfg.dthdx[0] * (gx - 0.5 * cfg.lx)
+    b, e = r["_bin"], r["th1bin"]
+    inside = (s1 >= e[b] - 1e-12) & ((s1 <= e[b + 1] + 1e-12) | (b == nx - 1))
+    if not hx:
+        inside |= (s1 <= -cfg.ro_inv) | (s1 >= cfg.ro_inv)  # clipped into the end bins
+    assert inside.all()
+    assert abs(r["th1PDF"].sum() - 1.0) < 1e-14
+    occ = r["th1PDF"] > 0
+    assert np.allclose(r["c"][occ], 2.5, rtol=1e-13, atol=0) and np.all(np.isnan(r["c"][~occ]))
+    # s1 constant: all the weight in one bin
+    th1 = np.broadcast_to(0.1 * cfg.ro_inv - cfg.dthdx[0] * (gx - 0.5 * cfg.lx), (ny2, nz, nx)).copy()
+    r1 = dg.zstar_bins(cfg, dyf, th1, {}, hx)
+    k = int(np.argmax(r1["th1PDF"]))
+    assert r1["th1PDF"][k] == 1.0 and np.count_nonzero(r1["th1PDF"]) == 1
+    e1 = r1["th1bin"]
+    assert abs(r1["BPE"] - (-cfg.ly * (e1[k] + e1[k + 1]) / 4.0)) <= 1e-14 * cfg.ly * abs(e1).max()
