@@ -19,13 +19,13 @@ namespace dgpu {
 // z-line mean slots
 enum { SM_U1 = 0, SM_U2, SM_U3, SM_DXU1, SM_DXU2, SM_DXU3, SM_TH0 };
 // z-sum slots
-enum { SZ_UU = 0, SZ_VV, SZ_WW, SZ_UW, SZ_UV, SZ_WV, SZ_EPS, SZ_SHEAR, SZ_U1, SZ_TH0 };
+enum { SZ_UU = 0, SZ_VV, SZ_WW, SZ_UW, SZ_UV, SZ_WV, SZ_EPS, SZ_SHEAR, SZ_U1, SZ_OX, SZ_OY, SZ_OZ, SZ_TH0 };
 constexpr int SZ_PER_TH = 3;  // (th')^2, th' w', |grad th|^2
 // output profiles (include/diablo_gpu.h: dgpu_stats)
 enum {
   SP_UME = 0, SP_VME, SP_WME, SP_MKE, SP_URMS, SP_VRMS, SP_WRMS, SP_UW, SP_UV, SP_WV,
   SP_UU_DUDX, SP_VV_DVDY, SP_WU_DWDX, SP_VU_DVDX, SP_UV_DUDY, SP_WV_DWDY,
-  SP_DUDY, SP_DWDY, SP_SHEAR, SP_EPS, SP_EPSM, SP_CP, SP_U1Y, SP_NBASE,
+  SP_DUDY, SP_DWDY, SP_SHEAR, SP_EPS, SP_EPSM, SP_CP, SP_U1Y, SP_OMX, SP_OMY, SP_OMZ, SP_NBASE,
   SP_PER_TH = 6  // thme, thrms, thv, thv_m, dthdy, pe_diss
 };
 
@@ -91,7 +91,7 @@ __global__ __launch_bounds__(256) void k_stat_zsum(StatArgs a) {
   const double* gy = a.gy;
   const double* gyf = a.gyf;
   const bool gyf_row = (j >= 1 && j <= ny), gy_row = (j >= 2 && j <= ny);
-  double uu = 0, vv = 0, ww = 0, uw = 0, uv = 0, wv = 0, eps = 0, sh = 0, s1 = 0;
+  double uu = 0, vv = 0, ww = 0, uw = 0, uv = 0, wv = 0, eps = 0, sh = 0, s1 = 0, sox = 0, soy = 0, soz = 0;
   double tt[MAXTH] = {}, tv[MAXTH] = {}, pe[MAXTH] = {};
   // the x derivatives whose kz = 0 row compute_TKE_diss drops when x is not
   // homogeneous (k_start = 1): the z-line mean of the derivative
@@ -109,6 +109,13 @@ __global__ __launch_bounds__(256) void k_stat_zsum(StatArgs a) {
         const double d3 = (F(6, j + 1, z) - F(6, j - 1, z)) / (2.0 * dyf[j]);
         sh += d1 * d1 + d3 * d3;
         s1 += u1;
+        // compute_Vorticity (for/diagnostics.f90:1534-1727): the three components,
+        // omega_x with the thermal-wind shear removed
+        const double ox = 0.5 * (F(5, j + 1, z) + F(5, j, z)) - (F(6, j + 1, z) - F(6, j - 1, z)) / (2.0 * dyf[j]) -
+                          a.twf;
+        const double oy = F(7, j, z) - F(2, j, z);
+        const double oz = (F(0, j + 1, z) - F(0, j - 1, z)) / (2.0 * dyf[j]) - 0.5 * (F(4, j + 1, z) + F(4, j, z));
+        sox += ox * ox; soy += oy * oy; soz += oz * oz;
       }
       if (gy_row) {
         const double f1m = F(0, j - 1, z) - M(SM_U1, j - 1), f3m = F(6, j - 1, z) - M(SM_U3, j - 1);
@@ -155,6 +162,7 @@ __global__ __launch_bounds__(256) void k_stat_zsum(StatArgs a) {
   a.zs[SZ_UU * pl + o] = uu; a.zs[SZ_VV * pl + o] = vv; a.zs[SZ_WW * pl + o] = ww;
   a.zs[SZ_UW * pl + o] = uw; a.zs[SZ_UV * pl + o] = uv; a.zs[SZ_WV * pl + o] = wv;
   a.zs[SZ_EPS * pl + o] = eps; a.zs[SZ_SHEAR * pl + o] = sh; a.zs[SZ_U1 * pl + o] = s1;
+  a.zs[SZ_OX * pl + o] = sox; a.zs[SZ_OY * pl + o] = soy; a.zs[SZ_OZ * pl + o] = soz;
   for (int n = 0; n < a.nth; ++n) {
     a.zs[(SZ_TH0 + SZ_PER_TH * n) * pl + o] = tt[n];
     a.zs[(SZ_TH0 + SZ_PER_TH * n + 1) * pl + o] = tv[n];
@@ -176,7 +184,7 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
   return r;
 }
 
-constexpr int NSTAT_X = 16 + 4 * MAXTH;
+constexpr int NSTAT_X = 19 + 4 * MAXTH;
 
 __global__ __launch_bounds__(256) void k_stat_plane(StatArgs a) {
   __shared__ double sh[256];
@@ -198,6 +206,7 @@ __global__ __launch_bounds__(256) void k_stat_plane(StatArgs a) {
     const double suv = S(SZ_UV, x), swv = S(SZ_WV, x);
     acc[0] += suu; acc[1] += svv; acc[2] += sww; acc[3] += suw; acc[4] += suv; acc[5] += swv;
     acc[6] += S(SZ_EPS, x); acc[7] += S(SZ_SHEAR, x);
+    acc[16] += S(SZ_OX, x); acc[17] += S(SZ_OY, x); acc[18] += S(SZ_OZ, x);
     if (gyf_row) {
       acc[8] += (suu / Nz) * M(SM_DXU1, j, x);     // uu_dudx
       acc[9] += (suw / Nz) * M(SM_DXU3, j, x);     // wu_dwdx
@@ -231,18 +240,18 @@ __global__ __launch_bounds__(256) void k_stat_plane(StatArgs a) {
                  (dyf[j - 1] * wm * wm + dyf[j] * wmm * wmm) / (2.0 * dy[j]) + vm * vm;
     }
     for (int n = 0; n < nth; ++n) {
-      acc[16 + 4 * n] += S(SZ_TH0 + SZ_PER_TH * n, x);
-      acc[17 + 4 * n] += S(SZ_TH0 + SZ_PER_TH * n + 1, x);
-      acc[18 + 4 * n] += S(SZ_TH0 + SZ_PER_TH * n + 2, x);
+      acc[19 + 4 * n] += S(SZ_TH0 + SZ_PER_TH * n, x);
+      acc[20 + 4 * n] += S(SZ_TH0 + SZ_PER_TH * n + 1, x);
+      acc[21 + 4 * n] += S(SZ_TH0 + SZ_PER_TH * n + 2, x);
       if (gyf_row) {
         const double gxv = ((double)x * a.lx) / (double)nx;
-        acc[19 + 4 * n] += (M(SM_TH0 + n, j, x) + a.dthdx[n] * (gxv - 0.5 * a.lx)) *
+        acc[22 + 4 * n] += (M(SM_TH0 + n, j, x) + a.dthdx[n] * (gxv - 0.5 * a.lx)) *
                            (0.5 * (M(SM_U2, j, x) + M(SM_U2, j + 1, x)));
       }
     }
   }
   double tot[NSTAT_X];
-  for (int k = 0; k < 16 + 4 * nth; ++k) tot[k] = block_sum(acc[k], sh);
+  for (int k = 0; k < 19 + 4 * nth; ++k) tot[k] = block_sum(acc[k], sh);
   if (t != 0) return;
   const int n2 = g.ny2;
   double* P = a.prof;
@@ -253,6 +262,7 @@ __global__ __launch_bounds__(256) void k_stat_plane(StatArgs a) {
   put(SP_CP, a.p[(long long)j * g.pc].x);
   double urms = 0, vrms = 0, wrms = 0, uw = 0, uv = 0, wv = 0, eps = 0, epsm = 0, shr = 0, mke = 0;
   double uud = 0, vvd = 0, wud = 0, vud = 0, uvd = 0, wvd = 0, dudy = 0, dwdy = 0, u1y = 0;
+  double omx = 0, omy = 0, omz = 0;
   if (gyf_row) {
     urms = sqrt(tot[0] / NxNz); vrms = sqrt(tot[1] / NxNz); wrms = sqrt(tot[2] / NxNz);
     uw = tot[3] / NxNz;
@@ -263,6 +273,7 @@ __global__ __launch_bounds__(256) void k_stat_plane(StatArgs a) {
     dudy = (ume - um1) / (gyf[j] - gyf[j - 1]);
     dwdy = (wme - wm1) / (gyf[j] - gyf[j - 1]);
     u1y = gyf[j] * S(SZ_U1, 0) / Nz;
+    omx = sqrt(tot[16] / NxNz); omy = sqrt(tot[17] / NxNz); omz = sqrt(tot[18] / NxNz);
   }
   if (gy_row) {
     uv = tot[4] / NxNz; wv = tot[5] / NxNz;
@@ -276,16 +287,17 @@ __global__ __launch_bounds__(256) void k_stat_plane(StatArgs a) {
   put(SP_UV_DUDY, uvd); put(SP_WV_DWDY, wvd);
   put(SP_DUDY, dudy); put(SP_DWDY, dwdy); put(SP_SHEAR, shr); put(SP_EPS, eps); put(SP_EPSM, epsm);
   put(SP_MKE, mke); put(SP_U1Y, u1y);
+  put(SP_OMX, omx); put(SP_OMY, omy); put(SP_OMZ, omz);
   for (int n = 0; n < nth; ++n) {
     const int b = SP_NBASE + SP_PER_TH * n;
     const cd* C = a.u[3 + n];
     const double thme = C[(long long)j * g.pc].x;
     double thrms = 0, thv = 0, thvm = 0, dthdy = 0, ped = 0;
     if (gyf_row) {
-      thrms = sqrt(tot[16 + 4 * n] / NxNz);
-      thv = tot[17 + 4 * n] / NxNz;
-      ped = tot[18 + 4 * n] / NxNz;
-      thvm = tot[19 + 4 * n] / Nx;
+      thrms = sqrt(tot[19 + 4 * n] / NxNz);
+      thv = tot[20 + 4 * n] / NxNz;
+      ped = tot[21 + 4 * n] / NxNz;
+      thvm = tot[22 + 4 * n] / Nx;
       dthdy = (thme - C[(long long)(j - 1) * g.pc].x) / (gyf[j] - gyf[j - 1]);
     }
     put(b + 0, thme); put(b + 1, thrms); put(b + 2, thv); put(b + 3, thvm); put(b + 4, dthdy); put(b + 5, ped);

@@ -18,6 +18,7 @@ PROFILE_GROUPS = [
     ("uu_dudx", "uu_dudx"), ("ww_dwdz", "vv_dvdy"), ("vu_dvdx", "wu_dwdx"), ("wu_dwdx", "vu_dvdx"),
     ("uw_dudz", "uv_dudy"), ("vw_dvdz", "wv_dwdy"), ("dudz", "dudy"), ("dvdz", "dwdy"),
     ("cp", "cp"), ("shear", "shear"),
+    ("omega_x", "omega_x"), ("omega_z", "omega_y"), ("omega_y", "omega_z"),   # (224-235)
     ("epsilon", "epsilon"),      # compute_TKE_diss (1031)
     ("epsilon_m", "epsilon_m"),  # compute_MKE_diss (1235)
 ]

@@ -187,11 +187,12 @@ int dgpu_courant(dgpu_handle *h, double time, double time_limit, double save_sta
  *   0 ume  1 vme  2 wme  3 mke  4 urms  5 vrms  6 wrms  7 uw  8 uv  9 wv
  *   10 uu_dudx  11 vv_dvdy  12 wu_dwdx  13 vu_dvdx  14 uv_dudy  15 wv_dwdy
  *   16 dudy  17 dwdy  18 shear  19 epsilon  20 epsilon_m  21 cp (mean mode)
- *   22 u1y_left
+ *   22 u1y_left  23 omega_x  24 omega_y  25 omega_z (compute_Vorticity, 1525-1727:
+ *   rms of the three vorticity components, omega_x less the thermal-wind shear)
  * then per scalar n: thme, thrms, thv, thv_m, dthdy, pe_diss; then one double,
  * u1z_x0 (the y-average of u1y_left, integrate_y_var, for/diagnostics.f90:1964).
- * z* bins and the BPE come from dgpu_stats_zstar.  Not computed: movie planes,
- * FTx_uu, the vorticity rms. */
+ * z* bins and the BPE come from dgpu_stats_zstar.  Not computed: movie planes
+ * and FTx_uu. */
 int dgpu_stats(dgpu_handle *h, double time, double *out);
 
 /* The z* statistics of save_stats_chan for scalar 1 (N_th >= 1), on the device in

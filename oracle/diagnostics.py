@@ -18,7 +18,8 @@ import numpy as np
 # profile order of dgpu_stats (include/diablo_gpu.h)
 BASE = ["ume", "vme", "wme", "mke", "urms", "vrms", "wrms", "uw", "uv", "wv",
         "uu_dudx", "vv_dvdy", "wu_dwdx", "vu_dvdx", "uv_dudy", "wv_dwdy",
-        "dudy", "dwdy", "shear", "epsilon", "epsilon_m", "cp", "u1y_left"]
+        "dudy", "dwdy", "shear", "epsilon", "epsilon_m", "cp", "u1y_left",
+        "omega_x", "omega_y", "omega_z"]
 PER_TH = ["thme", "thrms", "thv", "thv_m", "dthdy", "pe_diss"]
 
 
@@ -172,6 +173,23 @@ def save_stats(o, time=0.0, points=False):
         sh = ((u1[j + 1] - u1[j - 1]) / (2.0 * dyf[j])) ** 2 + ((u3[j + 1] - u3[j - 1]) / (2.0 * dyf[j])) ** 2
         out["shear"][j] = sh.sum() / NxNz
         pts["shear2_zstar"][j] = sh
+
+    # ---- compute_Vorticity (1525-1727): rms of the spectral curl, omega_x less
+    # the thermal-wind shear
+    def curl_rms(build, shift=0.0):
+        a = np.zeros_like(cr1)
+        for j in range(1, ny + 1):
+            a[j] = build(j)
+        f = phys(a)
+        r = np.zeros(n2)
+        for j in range(1, ny + 1):
+            r[j] = np.sqrt(((f[j] - shift) ** 2).sum() / NxNz)
+        return r
+    out["omega_x"] = curl_rms(lambda j: 1j * kz * 0.5 * (cr2[j + 1] + cr2[j])
+                              - (cr3[j + 1] - cr3[j - 1]) / (2.0 * dyf[j]), twf)
+    out["omega_y"] = curl_rms(lambda j: 1j * kx * cr3[j] - 1j * kz * cr1[j])
+    out["omega_z"] = curl_rms(lambda j: (cr1[j + 1] - cr1[j - 1]) / (2.0 * dyf[j])
+                              - 1j * kx * 0.5 * (cr2[j + 1] + cr2[j]))
 
     # ---- scalars (140-410)
     gxv = np.array([(i * cfg.lx) / nx for i in range(nx)])

@@ -152,3 +152,27 @@ def test_zstar_bins_closed_forms(hx):
     assert r1["th1PDF"][k] == 1.0 and np.count_nonzero(r1["th1PDF"]) == 1
     e1 = r1["th1bin"]
     assert abs(r1["BPE"] - (-cfg.ly * (e1[k] + e1[k + 1]) / 4.0)) <= 1e-14 * cfg.ly * abs(e1).max()
+
+
+def test_vorticity_closed_form():
+    """compute_Vorticity in the oracle on a pure shear u1 = a gyf (mean mode):
+    omega_z = a (gyf(j+1) - gyf(j-1)) / (2 dyf(j)) on every point, omega_x the
+    thermal-wind shear alone, omega_y = 0 (rows clear of the BC posts)."""
+    cfg = configs.front_les(16, 17, 8, 5000.0, 2e-4, 1e-2)
+    cfg.use_les = False
+    p = Problem(cfg)
+    o = oracle.Oracle(p)
+    s = p.initial_state(seed=1)
+    for arr in (s.cu1, s.cu2, s.cu3, s.cp, *s.cth):
+        arr[:] = 0
+    gyf, dyf = p.met["gyf"], p.met["dyf"]
+    a = 0.75
+    s.cu1[:, 0, 0] = a * gyf
+    o.upload_state(s)
+    out = diagnostics.save_stats(o)
+    twf = (1.0 / (cfg.ro_inv / cfg.delta)) * cfg.dthdx[0]
+    J = slice(3, cfg.ny - 1)
+    want_z = np.abs(a * (gyf[4:cfg.ny] - gyf[2:cfg.ny - 2]) / (2.0 * dyf[J]))
+    assert np.allclose(out["omega_z"][J], want_z, rtol=1e-13, atol=0)
+    assert np.allclose(out["omega_x"][J], abs(twf), rtol=1e-13, atol=0)
+    assert np.abs(out["omega_y"][J]).max() < 1e-14
