@@ -304,6 +304,25 @@ __global__ __launch_bounds__(256) void k_stat_plane(StatArgs a) {
   }
 }
 
+// FTx_uu (for/diagnostics.f90:641-663): per kx mode i < Nx/2 the y integral
+// (trapezoid over rows 1..Nyp) of |u1(i, kz = 0, j)|^2, over Ly; modes past
+// Nkx are zero in the state
+__global__ __launch_bounds__(256) void k_stat_ftx(StatArgs a, double* out, int nxp) {
+  const Geo& g = a.g;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= nxp) return;
+  double v = 0.0;
+  if (i < g.kxr) {
+    const cd* U = a.u[0];
+    for (int j = 2; j <= g.ny; ++j) {
+      const cd c = U[(long long)j * g.pc + i], cm = U[(long long)(j - 1) * g.pc + i];
+      const double p = c.x * c.x + c.y * c.y, pm = cm.x * cm.x + cm.y * cm.y;
+      v = v + 0.5 * (p + pm) * a.dy[j];
+    }
+  }
+  out[i] = v / a.ly;
+}
+
 // ---------------------------------------------------------------------------
 // z* statistics: the buoyancy-surface binning of compute_BPE and
 // Bin_Ystar_and_Write (for/diagnostics.f90:1733-1852, 1859-1953) for scalar 1.

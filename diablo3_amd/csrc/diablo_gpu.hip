@@ -137,6 +137,7 @@ struct dgpu_handle {
   // statistics (allocated on the first dgpu_stats call of a DNS handle)
   double *d_zm = nullptr, *d_zs = nullptr, *d_prof = nullptr;
   double* d_zhist = nullptr;  // z* histograms [5 + 2 MAXTH][nx]
+  double* d_ftx = nullptr;    // FTx_uu [nx / 2]
   double* hstage = nullptr;  // pinned staging, one Fortran array
   size_t fort_doubles = 0;
   std::vector<void*> allocs;
@@ -1648,7 +1649,7 @@ int dgpu_courant(dgpu_handle* h, double time, double time_limit, double save_sta
 
 int64_t dgpu_stats_doubles(const dgpu_config* cfg) {
   if (!cfg) return -1;
-  return (int64_t)(SP_NBASE + SP_PER_TH * cfg->n_th) * (cfg->ny + 2) + 1;
+  return (int64_t)(SP_NBASE + SP_PER_TH * cfg->n_th) * (cfg->ny + 2) + 1 + cfg->nx / 2;
 }
 
 namespace {
@@ -1702,9 +1703,19 @@ int dgpu_stats(dgpu_handle* h, double time, double* out) {
   const dim3 gxj((unsigned)((g.nx + 255) / 256), (unsigned)n2);
   k_stat_zsum<<<gxj, 256, 0, h->st>>>(a);
   k_stat_plane<<<(unsigned)n2, 256, 0, h->st>>>(a);
-  HIPCHK(hipGetLastError());
+  const int nxp = g.nx / 2;
   const size_t np_ = (size_t)(SP_NBASE + SP_PER_TH * nth) * n2;
+  // FTx_uu after the profiles (d_prof holds MAXTH scalars' room, enough for Nx/2 more)
+  double* ftx = h->d_ftx;
+  if (!ftx) {
+    void* q = nullptr;
+    if ((rc = dalloc(h, &q, sizeof(double) * (size_t)(nxp > 0 ? nxp : 1)))) return rc;
+    ftx = h->d_ftx = (double*)q;
+  }
+  k_stat_ftx<<<(unsigned)((nxp + 255) / 256), 256, 0, h->st>>>(a, ftx, nxp);
+  HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(out, h->d_prof, sizeof(double) * np_, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipMemcpyAsync(out + np_ + 1, ftx, sizeof(double) * nxp, hipMemcpyDeviceToHost, h->st));
   HIPCHK(hipStreamSynchronize(h->st));
   // u1z_x0: integrate_y_var of u1y_left (for/diagnostics.f90:1964-1980)
   const double* u1y = out + (size_t)SP_U1Y * n2;

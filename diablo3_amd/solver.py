@@ -216,7 +216,9 @@ class Solver:
         n2 = self.cfg.ny2
         names = PROFILES + [f"{b}{k + 1:02d}" for k in range(self.cfg.n_th) for b in PER_SCALAR]
         res = {name: out[i * n2:(i + 1) * n2].copy() for i, name in enumerate(names)}
-        res["u1z_x0"] = float(out[-1])
+        k = len(names) * n2
+        res["u1z_x0"] = float(out[k])
+        res["FTx_uu"] = out[k + 1:k + 1 + self.cfg.nx // 2].copy()
         return res
 
     def stats_zstar(self, time: float = 0.0):

@@ -42,6 +42,8 @@ def write_mean(fname: str, time: float, prof: Dict[str, np.ndarray], gyf: np.nda
         for fmt, key in SCALAR_GROUPS:
             flow_io.append_profile(fname, fmt.format(n + 1), prof[f"{key}{n + 1:02d}"][J], time)
     flow_io.append_real(fname, "u1z_x0", prof["u1z_x0"])
+    if "FTx_uu" in prof:
+        flow_io.append_profile(fname, "FTx_uu", prof["FTx_uu"], time)
     if zstar is not None:
         nb = len(zstar["th1PDF"])
         for name in ZSTAR_GROUPS:

@@ -190,9 +190,11 @@ int dgpu_courant(dgpu_handle *h, double time, double time_limit, double save_sta
  *   22 u1y_left  23 omega_x  24 omega_y  25 omega_z (compute_Vorticity, 1525-1727:
  *   rms of the three vorticity components, omega_x less the thermal-wind shear)
  * then per scalar n: thme, thrms, thv, thv_m, dthdy, pe_diss; then one double,
- * u1z_x0 (the y-average of u1y_left, integrate_y_var, for/diagnostics.f90:1964).
- * z* bins and the BPE come from dgpu_stats_zstar.  Not computed: movie planes
- * and FTx_uu. */
+ * u1z_x0 (the y-average of u1y_left, integrate_y_var, for/diagnostics.f90:1964);
+ * then Nx/2 doubles, FTx_uu (641-663: per kx mode the y-integral of |u1(kz = 0)|^2
+ * over Ly).
+ * z* bins and the BPE come from dgpu_stats_zstar.  Not computed: the movie
+ * planes. */
 int dgpu_stats(dgpu_handle *h, double time, double *out);
 
 /* The z* statistics of save_stats_chan for scalar 1 (N_th >= 1), on the device in

@@ -224,6 +224,13 @@ def save_stats(o, time=0.0, points=False):
     for j in range(2, ny + 1):
         res = res + 0.5 * (u1y[j] + u1y[j - 1]) * dy[j]
     out["u1z_x0"] = res / cfg.ly
+    # FTx_uu (641-663): kz = 0 row of u1, |c|^2 integrated in y by trapezoids
+    cuu = np.zeros((n2, nx // 2))
+    cuu[:, :cfg.kxr] = (s.cu1[:, 0, :] * np.conj(s.cu1[:, 0, :])).real
+    ftx = np.zeros(nx // 2)
+    for j in range(2, ny + 1):
+        ftx = ftx + 0.5 * (cuu[j] + cuu[j - 1]) * dy[j]
+    out["FTx_uu"] = ftx / cfg.ly
     if points:
         pts["th0"] = th[0] if th else None
         out["_points"] = pts

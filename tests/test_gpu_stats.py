@@ -57,6 +57,9 @@ def test_stats_match_oracle(which):
         print(f"  {name:10s} max|ref| {scale:.3e} rel err {err:.2e}", flush=True)
         assert err <= 1e-10, (name, err)
     assert abs(sg["u1z_x0"] - so["u1z_x0"]) <= 1e-10 * max(abs(so["u1z_x0"]), 1e-300)
+    ftx_err = np.abs(sg["FTx_uu"] - so["FTx_uu"]).max() / np.abs(so["FTx_uu"]).max()
+    print(f"  FTx_uu     max|ref| {np.abs(so['FTx_uu']).max():.3e} rel err {ftx_err:.2e}", flush=True)
+    assert sg["FTx_uu"].shape == (cfg.nx // 2,) and ftx_err <= 1e-10
     # the side effect on the state: the BC posts
     a, b = g.download_state(), o.download_state()
     for x, y in ((a.cu1, b.cu1), (a.cu2, b.cu2), (a.cu3, b.cu3), (a.cth[0], b.cth[0])):

@@ -176,3 +176,26 @@ def test_vorticity_closed_form():
     assert np.allclose(out["omega_z"][J], want_z, rtol=1e-13, atol=0)
     assert np.allclose(out["omega_x"][J], abs(twf), rtol=1e-13, atol=0)
     assert np.abs(out["omega_y"][J]).max() < 1e-14
+
+
+def test_ftx_uu_closed_form():
+    """FTx_uu in the oracle: a u1 with kz = 0 modes of constant amplitude c_i in y
+    (row 1 then zeroed by the bottom no-slip BC post of save_stats; the free-slip
+    top keeps row Nyp) gives |c_i|^2 (dy(2)/2 + dy(3) + ... + dy(Nyp)) / Ly per
+    mode (the trapezoid weights of a constant), zero past Nkx."""
+    cfg = configs.small_dns(16, 17, 8)
+    p = Problem(cfg)
+    o = oracle.Oracle(p)
+    s = p.initial_state(seed=1)
+    for arr in (s.cu1, s.cu2, s.cu3, s.cp, *s.cth):
+        arr[:] = 0
+    c = np.array([0.3, 0.2 + 0.1j, -0.05j, 0.07, 0.0, 0.01 + 0.02j])[:cfg.kxr]
+    s.cu1[:, 0, :len(c)] = c[None, :]
+    o.upload_state(s)
+    out = diagnostics.save_stats(o)
+    dy = p.met["dy"]
+    after = o.download_state().cu1
+    assert np.all(after[1, 0] == 0) and np.all(after[2:cfg.ny + 1, 0, :len(c)] == c[None, :])
+    want = np.zeros(cfg.nx // 2)
+    want[:len(c)] = np.abs(c) ** 2 * (0.5 * dy[2] + dy[3:cfg.ny + 1].sum()) / cfg.ly
+    assert np.allclose(out["FTx_uu"], want, rtol=1e-13, atol=1e-18)
