@@ -73,8 +73,9 @@ def kernel_bytes(cfg, mixed):
         out["k_cn"] = nf * Cj_t + (nf + 1) * C + (2.0 / 3.0) * nf * Cj + nf * Cj + nf * C
         npts = cfg.nx * cfg.nz * ny2
         if mixed:
-            # increments: u (float), r - u (float), nu in (twice); 1/b', g' out and back; delta out
-            out["k_cn_phys"] = nf * npts * (4 + 4 + 2 * s + 16 + 16 + 4)
+            # increments: u (float), r - u (float), nu in (twice); 1/b', g' (float) out and
+            # back; delta out
+            out["k_cn_phys"] = nf * npts * (4 + 4 + 2 * s + 8 + 8 + 4)
             # per launch (two per substep): C -> Hf -> Rf, then Rf -> Hf -> C (read + write)
             out["les_cn_fft"] = 0.5 * nf * ((C + 2 * Hf + Rf) + (Rf + 2 * Hf + 2 * C))
         else:
