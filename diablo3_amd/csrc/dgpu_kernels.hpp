@@ -726,6 +726,45 @@ __device__ __forceinline__ void assemble(const ColArgs& a, const RowIn<NTH>& r, 
   }
 }
 
+// The assembly of k_cn alone (the LES branch, whose CN solve runs per physical
+// column): cf and the right-hand side (or the right-hand side minus the state,
+// cr_minus_u) of every equation at one point per thread.  The columns of a row
+// are consecutive threads, so every load and store of a wave is one contiguous
+// run of the plane, and no LDS is held (the column kernel reserves its solve's).
+// Same rows, same load_row / assemble, same writes as k_cn with do_cn = 0.
+template <typename T, int NTH>
+__global__ __launch_bounds__(256) void k_assemble(ColArgs a) {
+  constexpr int NF = 3 + NTH;
+  const Geo& g = a.g;
+  const long long c = (long long)blockIdx.x * 256 + threadIdx.x;
+  const int j = blockIdx.y;
+  if (c >= g.pc) return;
+  const int i = (int)(c % g.kxp), k = (int)(c / g.kxp);
+  if (i >= g.kxr) return;  // padding columns: k_cn writes nothing there either
+  const long long pc = g.pc;
+  const double kxi = a.kx[i], kzk = a.kz[k];
+  const double k2 = kxi * kxi + kzk * kzk;
+  const double kk = (a.beta == 1.0) ? k2 : pow(k2, a.beta);
+  const long long o = (long long)j * pc + c;
+  const bool inner = j >= 1 && j <= g.ny;
+  RowIn<NTH> r;
+  if (inner)
+    load_row<T, NTH>(r, a, o, pc, a.u[0], a.u[1], a.u[2], NTH >= 1 ? a.u[3] : a.u[0], NTH >= 2 ? a.u[4] : a.u[0],
+                     a.p);
+#pragma unroll
+  for (int e = 0; e < NF; ++e) {
+    const SolveRows& SR = a.rows[e];
+    cd v = {0.0, 0.0};
+    if (inner && j >= SR.blo && j <= SR.bhi) {
+      cd cf;
+      assemble<NTH>(a, r, a.rowc + (long long)j * RC_COUNT, e, i, k, kxi, kzk, kk, cf, v);
+      a.cf[e][o] = cf;
+      if (a.cr_minus_u) v = v - a.u[e][o];
+    }
+    a.crout[e][o] = v;
+  }
+}
+
 // Assembly + Crank-Nicolson solves.  T: precision of the nonlinear terms;
 // TC columns x RG row groups per block; RPT >= ceil(ny2 / RG) rows per thread.
 // The CN solution overwrites the state rows [wlo, whi]; k_project follows.

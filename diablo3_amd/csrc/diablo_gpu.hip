@@ -1043,7 +1043,8 @@ int substep_les(dgpu_handle* h, int rk, double delta_t, double time, int time_st
   ca.cr_minus_u = inc ? 1 : 0;
   for (int f = 0; f < nf; ++f) ca.crout[f] = h->CR[f];
   if ((rc = tbeg(h, 3))) return rc;
-  if ((rc = launch_cn<T, NTH>(h, ca))) return rc;
+  k_assemble<T, NTH><<<dim3((unsigned)((g.pc + 255) / 256), (unsigned)g.ny2), 256, 0, h->st>>>(ca);
+  HIPCHK(hipGetLastError());
   if ((rc = tend(h))) return rc;
 
   // 6. right-hand sides to physical space: FP64, and on a mixed handle without
