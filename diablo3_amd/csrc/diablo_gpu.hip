@@ -522,11 +522,11 @@ int launch_fwd_x(dgpu_handle* h, int nf, T* const* src, void* const* dst, int j0
   return 0;
 }
 
-// threads per k_xprod block: 256 for the larger lines, 512 where the LDS tile
-// (> 80 KiB) leaves room for one block per CU only, so that a CU still holds
-// eight waves
+// threads per k_xprod block: 256 for the larger lines; where the LDS tile
+// (> 80 KiB) leaves room for one block per CU only, one thread per column of
+// the line (up to 1024), so that a CU still holds 8 or 16 waves
 constexpr int xprod_nt(int nx, size_t lds) {
-  return nx <= 32 ? 64 : (nx == 64 ? 128 : (lds > 81920 ? 512 : 256));
+  return nx <= 32 ? 64 : (nx == 64 ? 128 : (lds > 81920 ? (nx >= 1024 ? 1024 : 512) : 256));
 }
 
 template <typename T, int NTH, bool LES = false, bool SPLIT = false>

@@ -130,7 +130,7 @@ def test_les_gate_runs_dns_before_spin_up():
         assert np.array_equal(out[0][k], out[1][k]), k
 
 
-@pytest.mark.parametrize("nx,ny,nz", [(32, 33, 32), (16, 17, 512)])
+@pytest.mark.parametrize("nx,ny,nz", [(32, 33, 32), (16, 17, 512), (1024, 9, 16)])
 def test_les_mixed_precision_error(nx, ny, nz):
     """DGPU_MIXED LES (float gradients, model, products; FP64 state, solves,
     projection) against the FP64 oracle after 3 steps: the error a float
