@@ -390,6 +390,8 @@ int dgpu_read_flow(dgpu_handle* h, const char* fname, uint32_t th_mask, dgpu_flo
       return rc;
   if (has_p && (rc = dev(dgpu_fft_xz_to_fourier(h, p.data(), cp.data())))) return rc;
   if ((rc = dev(dgpu_upload_state(h, cu1.data(), cu2.data(), cu3.data(), cp.data(), cth.data())))) return rc;
+  // without P the reference recomputes the pressure (for/phdf5.f90:1452-1454)
+  if (!has_p && (rc = dev(dgpu_poisson_p(h)))) return rc;
   *read_pressure = has_p;
   return 0;
 }

@@ -619,6 +619,10 @@ struct ColArgs {
   cd* cf[MAXF];
   const void* nl[MAXF];
   cd* crout[MAXF];  // do_cn == 0: right-hand side rows [blo, bhi] of the physical-space solve
+  // k_project in its poisson_p_chan form: the transformed source (else null), the
+  // scalar count for its buoyancy term; the solution goes to p
+  const cd* prhs;
+  int pnth;
 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2).
@@ -965,7 +969,20 @@ __global__ __launch_bounds__(NT) void k_project(ColArgs a, int update_p) {
       cj = 1.0 / (a.dy[j + 1] * a.dyf[j]);
       if (active) {
         const long long o = (long long)j * pc + c;
-        d = ik(kxi, u1[o]) + (1.0 / a.dyf[j]) * (u2[o + pc] - u2[o]) + ik(kzk, u3[o]);
+        if (a.prhs) {
+          // poisson_p_chan (for/solver.f90:1382-1560): vec = -(source + the scalars'
+          // y gradient, except in the (0, 0) mode)
+          cd sv = a.prhs[o];
+          if (c != 0)
+            for (int n = 0; n < a.pnth; ++n) {
+              const cd dt = a.u[3 + n][o + pc] - a.u[3 + n][o - pc];
+              const double den = a.gyf[j + 1] - a.gyf[j - 1];
+              sv = sv + cd{dt.x / den, dt.y / den};
+            }
+          d = -1.0 * sv;
+        } else {
+          d = ik(kxi, u1[o]) + (1.0 / a.dyf[j]) * (u2[o + pc] - u2[o]) + ik(kzk, u3[o]);
+        }
       }
     }
     sA[q] = aj; sB[q] = bj; sC[q] = cj; sD[q] = d;
@@ -1023,6 +1040,10 @@ __global__ __launch_bounds__(NT) void k_project(ColArgs a, int update_p) {
     if (j < 1 || j > ny) continue;
     const long long o = (long long)j * pc + c;
     const cd phi = sD[j * TC + cc];
+    if (a.prhs) {  // cp(j) = vec(j), j = 1..Nyp
+      a.p[o] = phi;
+      continue;
+    }
     u1[o] = u1[o] - ik(kxi, phi);
     u3[o] = u3[o] - ik(kzk, phi);
     if (j >= 2) u2[o] = u2[o] - (1.0 / a.dy[j]) * (phi - sD[(j - 1) * TC + cc]);

@@ -304,6 +304,31 @@ __global__ __launch_bounds__(256) void k_stat_plane(StatArgs a) {
   }
 }
 
+// The source of poisson_p_chan (for/solver.f90:1390-1530) on rows 2..Nyp, in
+// physical space from the gradient fields: (du1/dx)^2 + ((u2(j+1) - u2(j))/dyf)^2
+// + (du3/dz)^2 + 2 du1/dy dv/dx + 2 du3/dy dv/dz + 2 du1/dz du3/dx, with the
+// reference's centred y differences and v averaged to GYF.  The reference
+// transforms each product separately; the sum of the transforms is the transform
+// of the sum.  Other rows are zero.
+__global__ __launch_bounds__(256) void k_pp_source(StatArgs a, double* __restrict__ out) {
+  const Geo& g = a.g;
+  const long long q = (long long)blockIdx.x * 256 + threadIdx.x;
+  const int j = blockIdx.y;
+  if (q >= g.pp) return;
+  const long long pp = g.pp, o = (long long)j * pp + q;
+  double v = 0.0;
+  if (j >= 2 && j <= g.ny) {
+    auto F = [&](int s, int jj) { return a.f[s][(long long)jj * pp + q]; };
+    const double r2f = 1.0 / (2.0 * a.dyf[j]);
+    const double f1 = F(1, j), f2 = (F(3, j + 1) - F(3, j)) / a.dyf[j], f3 = F(8, j);
+    const double g1 = (F(0, j + 1) - F(0, j - 1)) * r2f, g2 = 0.5 * (F(4, j) + F(4, j + 1));
+    const double g3 = (F(6, j + 1) - F(6, j - 1)) * r2f, h2 = 0.5 * (F(5, j) + F(5, j + 1));
+    const double h1 = F(2, j), h3 = F(7, j);
+    v = f1 * f1 + f2 * f2 + f3 * f3 + 2.0 * g1 * g2 + 2.0 * g3 * h2 + 2.0 * h1 * h3;
+  }
+  out[o] = v;
+}
+
 // FTx_uu (for/diagnostics.f90:641-663): per kx mode i < Nx/2 the y integral
 // (trapezoid over rows 1..Nyp) of |u1(i, kz = 0, j)|^2, over Ly; modes past
 // Nkx are zero in the state

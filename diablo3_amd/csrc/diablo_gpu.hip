@@ -1857,6 +1857,35 @@ int dgpu_filter_chan(dgpu_handle* h, int32_t n) {
   return 0;
 }
 
+int dgpu_poisson_p(dgpu_handle* h) {
+  if (!h) return fail(1, "null handle");
+  const Geo& g = h->g;
+  int rc = ensure_grad_buffers(h);
+  if (rc) return rc;
+  if ((rc = ensure_fft_scratch(h))) return rc;
+  // the FP64 physical gradient fields of the state, the source on rows 2..Nyp,
+  // its transform (fft_xz_to_fourier), then the projection's Poisson solve with
+  // that right-hand side (the same matrix and BC rows as rem_div_chan)
+  if ((rc = physical_gradients<double>(h, h->HinD, h->HinZD, h->PgD))) return rc;
+  StatArgs a{};
+  a.g = g; a.nth = h->nth; a.dyf = h->d_dyf;
+  for (int s = 0; s < 3 * (3 + h->nth); ++s) a.f[s] = (const double*)h->PgD[s];
+  k_pp_source<<<dim3((unsigned)((g.pp + 255) / 256), (unsigned)g.ny2), 256, 0, h->st>>>(a, h->d_R);
+  HIPCHK(hipGetLastError());
+  double* rs[1] = {h->d_R};
+  void* hs[1] = {h->d_H};
+  cd* cs[1] = {h->d_C};
+  if ((rc = launch_fwd_x<double>(h, 1, rs, hs, 0, g.ny2))) return rc;
+  if ((rc = launch_fwd_z<double>(h, 1, hs, cs, 0, g.ny2))) return rc;
+  ColArgs ca{};
+  fill_col_args(h, ca, 0.0);
+  ca.prhs = h->d_C;
+  ca.pnth = h->nth;
+  if ((rc = launch_project(h, ca, 0))) return rc;
+  HIPCHK(hipStreamSynchronize(h->st));
+  return 0;
+}
+
 int dgpu_fft_xz_to_fourier(dgpu_handle* h, const double* phys, double* four) {
   if (!h || !phys || !four) return fail(1, "null argument");
   const Geo& g = h->g;

@@ -50,7 +50,7 @@ module diablo_gpu
   public :: dgpu_last_step_ms, dgpu_field_doubles, dgpu_last_error_f
   public :: dgpu_download_les, dgpu_stats, dgpu_stats_doubles
   public :: dgpu_set_variable_dt, dgpu_courant, dgpu_set_closure, dgpu_filter_chan
-  public :: dgpu_write_flow, dgpu_read_flow
+  public :: dgpu_write_flow, dgpu_read_flow, dgpu_poisson_p
 
   interface
     integer(c_int) function dgpu_create(cfg, h) bind(C, name="dgpu_create")
@@ -92,6 +92,10 @@ module diablo_gpu
       integer(c_int32_t), value :: time_step, create_new_flow
     end function
     integer(c_int) function dgpu_rem_div(h) bind(C, name="dgpu_rem_div")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
+    integer(c_int) function dgpu_poisson_p(h) bind(C, name="dgpu_poisson_p")
       import :: c_int, c_ptr
       type(c_ptr), value :: h
     end function

@@ -47,6 +47,7 @@ def load():
                                   ctypes.c_int32, ctypes.c_int32]
     L.dgpu_step.argtypes = [P, ctypes.c_double, ctypes.c_double, ctypes.c_int32, ctypes.c_int32]
     L.dgpu_rem_div.argtypes = [P]
+    L.dgpu_poisson_p.argtypes = [P]
     L.dgpu_filter_chan.argtypes = [P, ctypes.c_int32]
     L.dgpu_fft_xz_to_fourier.argtypes = [P, D, D]
     L.dgpu_fft_xz_to_physical.argtypes = [P, D, D]
@@ -256,6 +257,11 @@ class Solver:
 
     def rem_div(self):
         self._check(load().dgpu_rem_div(self.h))
+
+    def poisson_p(self):
+        """poisson_p_chan of the reference (for/solver.f90:1382-1560): the pressure
+        recomputed from the device velocity and scalars (dgpu_poisson_p)."""
+        self._check(load().dgpu_poisson_p(self.h))
 
     def synchronize(self):
         self._check(load().dgpu_synchronize(self.h))

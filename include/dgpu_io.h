@@ -85,8 +85,8 @@ int dgpu_write_flow(dgpu_handle *h, const char *fname, const dgpu_flow_times *t,
 
 /* ReadHDF5 (for/phdf5.f90:1204-1456) into the handle's state: scalars whose bit
  * in th_mask is clear keep their state; without P in the file the pressure is
- * left as it was (the reference recomputes it with poisson_p_chan,
- * for/solver.f90:1382-1560, which this library does not provide) and
+ * recomputed from the velocity as the reference does (poisson_p_chan,
+ * for/solver.f90:1382-1560, on the device: dgpu_poisson_p) and
  * *read_pressure = 0.  ghost_chan_mpi is empty at one rank. */
 int dgpu_read_flow(dgpu_handle *h, const char *fname, uint32_t th_mask, dgpu_flow_times *t,
                    int32_t *read_pressure);

@@ -142,6 +142,12 @@ int dgpu_rem_div(dgpu_handle *h);
 int dgpu_fft_xz_to_fourier(dgpu_handle *h, const double *phys, double *four);
 int dgpu_fft_xz_to_physical(dgpu_handle *h, const double *four, double *phys);
 
+/* poisson_p_chan (for/solver.f90:1382-1560): the pressure cp from the velocity
+ * (and the scalars' y gradients) of the device state, rows 1..Nyp, by the
+ * pressure-Poisson solve of rem_div_chan's operator.  ReadHDF5 calls it for a
+ * flow file without P (dgpu_read_flow does). */
+int dgpu_poisson_p(dgpu_handle *h);
+
 /* Blocks until all queued work on the handle's stream is done. */
 int dgpu_synchronize(dgpu_handle *h);
 
