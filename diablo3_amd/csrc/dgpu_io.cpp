@@ -283,8 +283,11 @@ int dgpu_h5_read_flow(const char* fname, int32_t nx, int32_t ny, int32_t nz, int
   return 0;
 }
 
-int dgpu_h5_append_profile(const char* fname, const char* gname, const double* diag, int32_t ny, double time) {
-  if (!fname || !gname || !diag || ny <= 0) return fail(1, "dgpu_h5_append_profile: bad argument");
+namespace {
+// one more sample "%04d" (C dims `dims`, f64) with its Time attribute in the
+// series group gname, the group's SAMPLES counter advanced
+int append_series(const char* fname, const char* gname, const double* data, int rank, const hsize_t* dims,
+                  double time) {
   quiet();
   Hid f(open_series_file(fname), H5Fclose);
   if (!f.ok()) return fail(3, std::string("cannot open ") + fname);
@@ -294,14 +297,27 @@ int dgpu_h5_append_profile(const char* fname, const char* gname, const double* d
   Hid g(gid, H5Gclose);
   char dname[32];
   std::snprintf(dname, sizeof dname, "%04d", nsamp);  // (1I0.4)
-  const hsize_t n = (hsize_t)ny;
-  Hid sp(H5Screate_simple(1, &n, nullptr), H5Sclose);
+  Hid sp(H5Screate_simple(rank, dims, nullptr), H5Sclose);
   Hid ds(H5Dcreate2(g, dname, H5T_IEEE_F64LE, sp, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT), H5Dclose);
-  if (!ds.ok() || H5Dwrite(ds, H5T_NATIVE_DOUBLE, H5S_ALL, H5S_ALL, H5P_DEFAULT, diag) < 0)
+  if (!ds.ok() || H5Dwrite(ds, H5T_NATIVE_DOUBLE, H5S_ALL, H5S_ALL, H5P_DEFAULT, data) < 0)
     return fail(3, std::string("dataset ") + gname + "/" + dname);
   if (write_scalar_attr(ds, "Time", H5T_IEEE_F64LE, H5T_NATIVE_DOUBLE, &time) < 0)
     return fail(3, "attribute Time");
   return 0;
+}
+}  // namespace
+
+int dgpu_h5_append_profile(const char* fname, const char* gname, const double* diag, int32_t ny, double time) {
+  if (!fname || !gname || !diag || ny <= 0) return fail(1, "dgpu_h5_append_profile: bad argument");
+  const hsize_t n = (hsize_t)ny;
+  return append_series(fname, gname, diag, 1, &n, time);
+}
+
+int dgpu_h5_append_plane(const char* fname, const char* gname, const double* var2d, int32_t n1, int32_t n2,
+                         double time) {
+  if (!fname || !gname || !var2d || n1 <= 0 || n2 <= 0) return fail(1, "dgpu_h5_append_plane: bad argument");
+  const hsize_t dims[2] = {(hsize_t)n2, (hsize_t)n1};  // Fortran (n1, n2)
+  return append_series(fname, gname, var2d, 2, dims, time);
 }
 
 int dgpu_h5_append_real(const char* fname, const char* gname, double value) {

@@ -10,6 +10,8 @@
 //                x-homogeneous flow the plane means and zero gradients instead
 //  k_stat_zsum   per (x, j): the z sums of every fluctuation product
 //  k_stat_plane  per j: the x reductions and the x-only (mean-field) terms
+//  k_stat_xz     the z-mean (x, y) planes of mean_xz.h5 (movie output)
+//  k_stat_movie  the x-y, x-z and z-y slices of movie.h5
 // The z and x sums replace the reference's sequential loops (round-off only).
 #pragma once
 #include "dgpu_kernels.hpp"
@@ -19,7 +21,10 @@ namespace dgpu {
 // z-line mean slots
 enum { SM_U1 = 0, SM_U2, SM_U3, SM_DXU1, SM_DXU2, SM_DXU3, SM_TH0 };
 // z-sum slots
-enum { SZ_UU = 0, SZ_VV, SZ_WW, SZ_UW, SZ_UV, SZ_WV, SZ_EPS, SZ_SHEAR, SZ_U1, SZ_OX, SZ_OY, SZ_OZ, SZ_TH0 };
+// (SZ_THU: th_1' u1'; SZ_EPSM: the per-x mean-field dissipation integrand, written
+// by k_stat_plane)
+enum { SZ_UU = 0, SZ_VV, SZ_WW, SZ_UW, SZ_UV, SZ_WV, SZ_EPS, SZ_SHEAR, SZ_U1, SZ_OX, SZ_OY, SZ_OZ,
+       SZ_THU, SZ_EPSM, SZ_TH0 };
 constexpr int SZ_PER_TH = 3;  // (th')^2, th' w', |grad th|^2
 // output profiles (include/diablo_gpu.h: dgpu_stats)
 enum {
@@ -77,84 +82,113 @@ __global__ __launch_bounds__(256) void k_stat_mean(StatArgs a) {
   }
 }
 
+// The per-point quantities of save_stats_chan at (x, j, z), from the physical
+// fields and the z-line means; shared by the z sums, the z* bins and the movie
+// slices.
+struct StatPoint {
+  const StatArgs& a;
+  const int x;
+  const long long pl, pp;
+  __device__ StatPoint(const StatArgs& a_, int x_)
+      : a(a_), x(x_), pl((long long)a_.g.ny2 * a_.g.nx), pp(a_.g.pp) {}
+  __device__ double M(int m, int j) const { return a.zm[m * pl + (long long)j * a.g.nx + x]; }
+  __device__ double F(int s, int j, int z) const { return a.f[s][(long long)j * pp + (long long)z * a.g.nx + x]; }
+  // the dissipation integrand of compute_TKE_diss on GY rows 2..Nyp, zero elsewhere
+  // (for/diagnostics.f90:820-1016); when x is not homogeneous the kz = 0 row of
+  // each x derivative (k_start = 1), i.e. its z-line mean, is left out
+  __device__ double eps(int j, int z) const {
+    if (j < 2 || j > a.g.ny) return 0.0;
+    const double* dy = a.dy;
+    const double* dyf = a.dyf;
+    const double mdx1 = a.hx ? 0.0 : M(SM_DXU1, j), mdx1m = a.hx ? 0.0 : M(SM_DXU1, j - 1);
+    const double mdx2 = a.hx ? 0.0 : M(SM_DXU2, j);
+    const double mdx3 = a.hx ? 0.0 : M(SM_DXU3, j), mdx3m = a.hx ? 0.0 : M(SM_DXU3, j - 1);
+    const double f1 = F(0, j, z) - M(SM_U1, j), f3 = F(6, j, z) - M(SM_U3, j);
+    const double f1m = F(0, j - 1, z) - M(SM_U1, j - 1), f3m = F(6, j - 1, z) - M(SM_U3, j - 1);
+    auto itp = [&](double c, double m) { return (dyf[j - 1] * c * c + dyf[j] * m * m) / (2.0 * dy[j]); };
+    double e = itp(F(1, j, z) - mdx1, F(1, j - 1, z) - mdx1m);
+    const double a2 = F(4, j, z) - mdx2;
+    e += a2 * a2;
+    const double t3 = (f1 - f1m) / dy[j];
+    e += t3 * t3;
+    e += itp(F(7, j, z) - mdx3, F(7, j - 1, z) - mdx3m);
+    e += itp(F(2, j, z), F(2, j - 1, z));
+    const double t6 = ((F(3, j + 1, z) - M(SM_U2, j + 1)) - (F(3, j - 1, z) - M(SM_U2, j - 1))) /
+                      (a.gy[j + 1] - a.gy[j - 1]);
+    e += t6 * t6;
+    const double t7 = (f3 - f3m) / dy[j];
+    e += t7 * t7;
+    const double a8 = F(5, j, z);
+    e += a8 * a8;
+    e += itp(F(8, j, z), F(8, j - 1, z));
+    return e;
+  }
+  // compute_Vorticity (for/diagnostics.f90:1534-1727): the three components at a
+  // GYF row, omega_x with the thermal-wind shear removed
+  __device__ void omega(int j, int z, double& ox, double& oy, double& oz) const {
+    const double r2 = 1.0 / (2.0 * a.dyf[j]);
+    ox = 0.5 * (F(5, j + 1, z) + F(5, j, z)) - (F(6, j + 1, z) - F(6, j - 1, z)) * r2 - a.twf;
+    oy = F(7, j, z) - F(2, j, z);
+    oz = (F(0, j + 1, z) - F(0, j - 1, z)) * r2 - 0.5 * (F(4, j + 1, z) + F(4, j, z));
+  }
+  // th_n' w' at a GYF row (w averaged to GYF, for/diagnostics.f90:296-307)
+  __device__ double thw(int n, int j, int z) const {
+    const double ft = F(3 * (3 + n), j, z) - M(SM_TH0 + n, j);
+    return ft * (0.5 * (F(3, j, z) + F(3, j + 1, z)) - 0.5 * (M(SM_U2, j) + M(SM_U2, j + 1)));
+  }
+  // |grad th_n + background|^2 at a GYF row; the y difference is of scalar 1 for
+  // every n (crth(.., 1) in for/diagnostics.f90:147, 355-361)
+  __device__ double chi(int n, int j, int z) const {
+    const int ts = 3 * (3 + n);
+    const double gx_ = F(ts + 1, j, z) + a.dthdx[n];
+    const double gy_ = (F(9, j + 1, z) - F(9, j - 1, z)) / (a.gyf[j + 1] - a.gyf[j - 1]);
+    const double gz_ = F(ts + 2, j, z) + a.dthdz[n];
+    return gx_ * gx_ + gy_ * gy_ + gz_ * gz_;
+  }
+};
+
 __global__ __launch_bounds__(256) void k_stat_zsum(StatArgs a) {
   const Geo& g = a.g;
   const int x = blockIdx.x * 256 + threadIdx.x, j = blockIdx.y;
   if (x >= g.nx) return;
   const int ny = g.ny, nx = g.nx;
   const long long pl = (long long)g.ny2 * nx;
-  const long long pp = g.pp;
-  auto M = [&](int m, int jj) { return a.zm[m * pl + (long long)jj * nx + x]; };
-  auto F = [&](int s, int jj, int z) { return a.f[s][(long long)jj * pp + (long long)z * nx + x]; };
+  const StatPoint P(a, x);
   const double* dy = a.dy;
   const double* dyf = a.dyf;
-  const double* gy = a.gy;
-  const double* gyf = a.gyf;
   const bool gyf_row = (j >= 1 && j <= ny), gy_row = (j >= 2 && j <= ny);
   double uu = 0, vv = 0, ww = 0, uw = 0, uv = 0, wv = 0, eps = 0, sh = 0, s1 = 0, sox = 0, soy = 0, soz = 0;
+  double thu = 0;
   double tt[MAXTH] = {}, tv[MAXTH] = {}, pe[MAXTH] = {};
-  // the x derivatives whose kz = 0 row compute_TKE_diss drops when x is not
-  // homogeneous (k_start = 1): the z-line mean of the derivative
-  const double mdx1 = a.hx ? 0.0 : M(SM_DXU1, j), mdx1m = a.hx ? 0.0 : (j >= 1 ? M(SM_DXU1, j - 1) : 0.0);
-  const double mdx2 = a.hx ? 0.0 : M(SM_DXU2, j);
-  const double mdx3 = a.hx ? 0.0 : M(SM_DXU3, j), mdx3m = a.hx ? 0.0 : (j >= 1 ? M(SM_DXU3, j - 1) : 0.0);
   if (gyf_row || gy_row) {
-    const double mu1 = M(SM_U1, j), mu2 = M(SM_U2, j), mu3 = M(SM_U3, j);
+    const double mu1 = P.M(SM_U1, j), mu2 = P.M(SM_U2, j), mu3 = P.M(SM_U3, j);
     for (int z = 0; z < g.nz; ++z) {
-      const double u1 = F(0, j, z), u2 = F(3, j, z), u3 = F(6, j, z);
+      const double u1 = P.F(0, j, z), u2 = P.F(3, j, z), u3 = P.F(6, j, z);
       const double f1 = u1 - mu1, f2 = u2 - mu2, f3 = u3 - mu3;
       if (gyf_row) {
         uu += f1 * f1; vv += f2 * f2; ww += f3 * f3; uw += f1 * f3;
-        const double d1 = (F(0, j + 1, z) - F(0, j - 1, z)) / (2.0 * dyf[j]);
-        const double d3 = (F(6, j + 1, z) - F(6, j - 1, z)) / (2.0 * dyf[j]);
+        const double d1 = (P.F(0, j + 1, z) - P.F(0, j - 1, z)) / (2.0 * dyf[j]);
+        const double d3 = (P.F(6, j + 1, z) - P.F(6, j - 1, z)) / (2.0 * dyf[j]);
         sh += d1 * d1 + d3 * d3;
         s1 += u1;
-        // compute_Vorticity (for/diagnostics.f90:1534-1727): the three components,
-        // omega_x with the thermal-wind shear removed
-        const double ox = 0.5 * (F(5, j + 1, z) + F(5, j, z)) - (F(6, j + 1, z) - F(6, j - 1, z)) / (2.0 * dyf[j]) -
-                          a.twf;
-        const double oy = F(7, j, z) - F(2, j, z);
-        const double oz = (F(0, j + 1, z) - F(0, j - 1, z)) / (2.0 * dyf[j]) - 0.5 * (F(4, j + 1, z) + F(4, j, z));
+        double ox, oy, oz;
+        P.omega(j, z, ox, oy, oz);
         sox += ox * ox; soy += oy * oy; soz += oz * oz;
+        for (int n = 0; n < a.nth; ++n) {
+          const double ft = P.F(3 * (3 + n), j, z) - P.M(SM_TH0 + n, j);
+          tt[n] += ft * ft;
+          tv[n] += P.thw(n, j, z);
+          pe[n] += P.chi(n, j, z);
+          if (n == 0) thu += ft * f1;  // (th_1 - thme_xy) (u1 - ume_xy), for/diagnostics.f90:301-302
+        }
       }
       if (gy_row) {
-        const double f1m = F(0, j - 1, z) - M(SM_U1, j - 1), f3m = F(6, j - 1, z) - M(SM_U3, j - 1);
+        const double f1m = P.F(0, j - 1, z) - P.M(SM_U1, j - 1), f3m = P.F(6, j - 1, z) - P.M(SM_U3, j - 1);
         const double iu = (dyf[j - 1] * f1 + dyf[j] * f1m) / (2.0 * dy[j]);
         const double iw = (dyf[j - 1] * f3 + dyf[j] * f3m) / (2.0 * dy[j]);
         uv += iu * f2;
         wv += iw * f2;
-        // dissipation integrand (for/diagnostics.f90:834-1016)
-        auto itp = [&](double c, double m) { return (dyf[j - 1] * c * c + dyf[j] * m * m) / (2.0 * dy[j]); };
-        double e = itp(F(1, j, z) - mdx1, F(1, j - 1, z) - mdx1m);
-        const double a2 = F(4, j, z) - mdx2;
-        e += a2 * a2;
-        const double t3 = (f1 - f1m) / dy[j];
-        e += t3 * t3;
-        e += itp(F(7, j, z) - mdx3, F(7, j - 1, z) - mdx3m);
-        e += itp(F(2, j, z), F(2, j - 1, z));
-        const double t6 = ((F(3, j + 1, z) - M(SM_U2, j + 1)) - (F(3, j - 1, z) - M(SM_U2, j - 1))) /
-                          (gy[j + 1] - gy[j - 1]);
-        e += t6 * t6;
-        const double t7 = (f3 - f3m) / dy[j];
-        e += t7 * t7;
-        const double a8 = F(5, j, z);
-        e += a8 * a8;
-        e += itp(F(8, j, z), F(8, j - 1, z));
-        eps += e;
-      }
-      if (gyf_row) {
-        for (int n = 0; n < a.nth; ++n) {
-          const int ts = 3 * (3 + n);
-          const double ft = F(ts, j, z) - M(SM_TH0 + n, j);
-          tt[n] += ft * ft;
-          tv[n] += ft * (0.5 * (u2 + F(3, j + 1, z)) - 0.5 * (mu2 + M(SM_U2, j + 1)));
-          // |grad th + background|^2; the y difference is of scalar 1 for every n
-          // (crth(.., 1) in for/diagnostics.f90:147)
-          const double gx_ = F(ts + 1, j, z) + a.dthdx[n];
-          const double gy_ = (F(9, j + 1, z) - F(9, j - 1, z)) / (gyf[j + 1] - gyf[j - 1]);
-          const double gz_ = F(ts + 2, j, z) + a.dthdz[n];
-          pe[n] += gx_ * gx_ + gy_ * gy_ + gz_ * gz_;
-        }
+        eps += P.eps(j, z);
       }
     }
   }
@@ -163,6 +197,7 @@ __global__ __launch_bounds__(256) void k_stat_zsum(StatArgs a) {
   a.zs[SZ_UW * pl + o] = uw; a.zs[SZ_UV * pl + o] = uv; a.zs[SZ_WV * pl + o] = wv;
   a.zs[SZ_EPS * pl + o] = eps; a.zs[SZ_SHEAR * pl + o] = sh; a.zs[SZ_U1 * pl + o] = s1;
   a.zs[SZ_OX * pl + o] = sox; a.zs[SZ_OY * pl + o] = soy; a.zs[SZ_OZ * pl + o] = soz;
+  a.zs[SZ_THU * pl + o] = thu;
   for (int n = 0; n < a.nth; ++n) {
     a.zs[(SZ_TH0 + SZ_PER_TH * n) * pl + o] = tt[n];
     a.zs[(SZ_TH0 + SZ_PER_TH * n + 1) * pl + o] = tv[n];
@@ -207,6 +242,7 @@ __global__ __launch_bounds__(256) void k_stat_plane(StatArgs a) {
     acc[0] += suu; acc[1] += svv; acc[2] += sww; acc[3] += suw; acc[4] += suv; acc[5] += swv;
     acc[6] += S(SZ_EPS, x); acc[7] += S(SZ_SHEAR, x);
     acc[16] += S(SZ_OX, x); acc[17] += S(SZ_OY, x); acc[18] += S(SZ_OZ, x);
+    double em = 0.0;  // this x's mean-field dissipation integrand (the epsilon_m_xz plane)
     if (gyf_row) {
       acc[8] += (suu / Nz) * M(SM_DXU1, j, x);     // uu_dudx
       acc[9] += (suw / Nz) * M(SM_DXU3, j, x);     // wu_dwdx
@@ -224,8 +260,10 @@ __global__ __launch_bounds__(256) void k_stat_plane(StatArgs a) {
           e += t3 * t3 + t5 * t5 + t6 * t6;
         }
         acc[10] += e;
+        em = e;
       }
     }
+    a.zs[SZ_EPSM * pl + (long long)j * nx + x] = em;
     if (gy_row) {
       acc[11] += (suv / Nz) * M(SM_DXU2, j, x);    // vu_dvdx
       acc[12] += (M(SM_U2, j + 1, x) - M(SM_U2, j - 1, x)) / (gy[j + 1] - gy[j - 1]) * (svv / Nz);
@@ -377,24 +415,15 @@ __global__ __launch_bounds__(256) void k_stat_zstar(StatArgs a, ZstarArgs z) {
   __syncthreads();
   const int ny = g.ny, nx = g.nx;
   if (x < nx) {
-    const long long pl = (long long)g.ny2 * nx;
-    const long long pp = g.pp;
-    auto M = [&](int m, int jj) { return a.zm[m * pl + (long long)jj * nx + x]; };
-    auto F = [&](int s, int jj, int zz) { return a.f[s][(long long)jj * pp + (long long)zz * nx + x]; };
-    const double* dy = a.dy;
+    const StatPoint P(a, x);
     const double* dyf = a.dyf;
-    const double* gy = a.gy;
-    const double* gyf = a.gyf;
-    const bool gyf_row = (j >= 1 && j <= ny), gy_row = (j >= 2 && j <= ny);
+    const bool gyf_row = (j >= 1 && j <= ny);
     const double wj = dyf[j];
     const double gx = ((double)x * z.lx) / (double)nx;
     const double bg = z.dthdx0 * (gx - 0.5 * z.lx);
-    const double mdx1 = a.hx ? 0.0 : M(SM_DXU1, j), mdx1m = a.hx ? 0.0 : (j >= 1 ? M(SM_DXU1, j - 1) : 0.0);
-    const double mdx2 = a.hx ? 0.0 : M(SM_DXU2, j);
-    const double mdx3 = a.hx ? 0.0 : M(SM_DXU3, j), mdx3m = a.hx ? 0.0 : (j >= 1 ? M(SM_DXU3, j - 1) : 0.0);
-    const double mu1 = M(SM_U1, j), mu2 = M(SM_U2, j), mu3 = M(SM_U3, j);
+    const double mu1 = P.M(SM_U1, j), mu2 = P.M(SM_U2, j), mu3 = P.M(SM_U3, j);
     for (int zz = 0; zz < g.nz; ++zz) {
-      const double s1 = F(9, j, zz) + bg;
+      const double s1 = P.F(9, j, zz) + bg;
       int b;
       if (z.hx) {
         b = (int)((s1 - z.thmin) / z.dth);  // truncation toward zero, as int()
@@ -405,35 +434,16 @@ __global__ __launch_bounds__(256) void k_stat_zstar(StatArgs a, ZstarArgs z) {
         b = (int)(acos(-c / z.den) * (double)z.nbin / z.pi);
         b = b > z.nbin - 1 ? z.nbin - 1 : b;
       }
-      const double u1 = F(0, j, zz), u2 = F(3, j, zz), u3 = F(6, j, zz);
+      const double u1 = P.F(0, j, zz), u2 = P.F(3, j, zz), u3 = P.F(6, j, zz);
       const double f1 = u1 - mu1, f2 = u2 - mu2, f3 = u3 - mu3;
-      double sh = 0.0, tke = 0.0, eps = 0.0;
+      double sh = 0.0, tke = 0.0;
       if (gyf_row) {
-        const double d1 = (F(0, j + 1, zz) - F(0, j - 1, zz)) / (2.0 * dyf[j]);
-        const double d3 = (F(6, j + 1, zz) - F(6, j - 1, zz)) / (2.0 * dyf[j]);
+        const double d1 = (P.F(0, j + 1, zz) - P.F(0, j - 1, zz)) / (2.0 * dyf[j]);
+        const double d3 = (P.F(6, j + 1, zz) - P.F(6, j - 1, zz)) / (2.0 * dyf[j]);
         sh = d1 * d1 + d3 * d3;
         tke = 0.5 * (f1 * f1) + 0.5 * (f2 * f2) + 0.5 * (f3 * f3);
       }
-      if (gy_row) {
-        const double f1m = F(0, j - 1, zz) - M(SM_U1, j - 1), f3m = F(6, j - 1, zz) - M(SM_U3, j - 1);
-        auto itp = [&](double c, double m) { return (dyf[j - 1] * c * c + dyf[j] * m * m) / (2.0 * dy[j]); };
-        double e = itp(F(1, j, zz) - mdx1, F(1, j - 1, zz) - mdx1m);
-        const double a2 = F(4, j, zz) - mdx2;
-        e += a2 * a2;
-        const double t3 = (f1 - f1m) / dy[j];
-        e += t3 * t3;
-        e += itp(F(7, j, zz) - mdx3, F(7, j - 1, zz) - mdx3m);
-        e += itp(F(2, j, zz), F(2, j - 1, zz));
-        const double t6 = ((F(3, j + 1, zz) - M(SM_U2, j + 1)) - (F(3, j - 1, zz) - M(SM_U2, j - 1))) /
-                          (gy[j + 1] - gy[j - 1]);
-        e += t6 * t6;
-        const double t7 = (f3 - f3m) / dy[j];
-        e += t7 * t7;
-        const double a8 = F(5, j, zz);
-        e += a8 * a8;
-        e += itp(F(8, j, zz), F(8, j - 1, zz));
-        eps = a.nu * e;
-      }
+      const double eps = a.nu * P.eps(j, zz);
       const double ke = 0.5 * (u1 * u1 + u2 * u2 + u3 * u3);
       atomicAdd(&hs[b], wj);
       atomicAdd(&hs[1 * z.nbin + b], sh * wj);
@@ -441,16 +451,8 @@ __global__ __launch_bounds__(256) void k_stat_zstar(StatArgs a, ZstarArgs z) {
       atomicAdd(&hs[3 * z.nbin + b], tke * wj);
       atomicAdd(&hs[4 * z.nbin + b], ke * wj);
       for (int n = 0; n < a.nth; ++n) {
-        double tw = 0.0, chi = 0.0;
-        if (gyf_row) {
-          const int ts = 3 * (3 + n);
-          const double ft = F(ts, j, zz) - M(SM_TH0 + n, j);
-          tw = ft * (0.5 * (u2 + F(3, j + 1, zz)) - 0.5 * (mu2 + M(SM_U2, j + 1)));
-          const double gx_ = F(ts + 1, j, zz) + a.dthdx[n];
-          const double gy_ = (F(9, j + 1, zz) - F(9, j - 1, zz)) / (gyf[j + 1] - gyf[j - 1]);
-          const double gz_ = F(ts + 2, j, zz) + a.dthdz[n];
-          chi = gx_ * gx_ + gy_ * gy_ + gz_ * gz_;
-        }
+        const double tw = gyf_row ? P.thw(n, j, zz) : 0.0;
+        const double chi = gyf_row ? P.chi(n, j, zz) : 0.0;
         atomicAdd(&hs[(5 + 2 * n) * z.nbin + b], tw * wj);
         atomicAdd(&hs[(6 + 2 * n) * z.nbin + b], chi * wj);
       }
@@ -459,6 +461,80 @@ __global__ __launch_bounds__(256) void k_stat_zstar(StatArgs a, ZstarArgs z) {
   __syncthreads();
   for (int i = t; i < z.nf * z.nbin; i += 256)
     if (hs[i] != 0.0) atomicAdd(&z.hist[i], hs[i]);
+}
+
+
+// mean_xz.h5 (for/diagnostics.f90:283-378, 756-767, 1076-1087, 1238-1245, 1338-1346,
+// 1452-1460): the z-mean (x, y) planes, rows 1..Nyp, each [Nyp][Nx] in the order
+// ume, wme (v on GYF), vme (w), thme_1, uu, ww (v'v'), vv (w'w'), uw (u'v' on GY),
+// wv (w'v' on GY), uv (u'w'), epsilon, epsilon_m, thth, thu, thw, chi (scalar 1;
+// zero without a scalar).  The reference's plane names swap v and w.
+constexpr int NXZ_PLANES = 16;
+__global__ __launch_bounds__(256) void k_stat_xz(StatArgs a, double* __restrict__ out) {
+  const Geo& g = a.g;
+  const int x = blockIdx.x * 256 + threadIdx.x, j = blockIdx.y + 1;
+  if (x >= g.nx) return;
+  const int nx = g.nx;
+  const long long pl = (long long)g.ny2 * nx, o = (long long)j * nx + x;
+  const long long op = (long long)g.ny * nx, q = (long long)(j - 1) * nx + x;
+  const double rz = 1.0 / (double)g.nz;
+  auto Z = [&](int s) { return a.zs[s * pl + o] * rz; };
+  auto M = [&](int m, int jj) { return a.zm[m * pl + (long long)jj * nx + x]; };
+  const bool th = a.nth > 0;
+  double v[NXZ_PLANES];
+  v[0] = M(SM_U1, j);
+  v[1] = 0.5 * (M(SM_U2, j) + M(SM_U2, j + 1));
+  v[2] = M(SM_U3, j);
+  v[3] = th ? M(SM_TH0, j) : 0.0;
+  v[4] = Z(SZ_UU); v[5] = Z(SZ_VV); v[6] = Z(SZ_WW);
+  v[7] = Z(SZ_UV); v[8] = Z(SZ_WV); v[9] = Z(SZ_UW);
+  v[10] = a.nu * Z(SZ_EPS);
+  v[11] = a.nu * a.zs[SZ_EPSM * pl + o];
+  v[12] = th ? Z(SZ_TH0) : 0.0;
+  v[13] = th ? Z(SZ_THU) : 0.0;
+  v[14] = th ? Z(SZ_TH0 + 1) : 0.0;
+  v[15] = th ? Z(SZ_TH0 + 2) : 0.0;
+  for (int m = 0; m < NXZ_PLANES; ++m) out[m * op + q] = v[m];
+}
+
+// movie.h5 (for/diagnostics.f90:408-441, 485-617, 1041-1070, 1566-1727): per field
+// three slices, x-y at z = nzm (reference name *_xz, [Nyp][Nx] of rows 1..Nyp),
+// x-z at y = nym (*_xy, [Nz][Nx]; skipped when nym < 1) and z-y at x = nxm (*_yz,
+// [Nyp][Nz]).  Fields: u1, u2 averaged to GYF, u3, nu_t, epsilon (nu times the
+// integrand, zero on row 1), omega_x, omega_y, omega_z, then th_n.
+struct MovieArgs {
+  int nxm, nym, nzm, nfld;
+  const void* nut;  // [j][z][x] in the handle's precision (null: zeros)
+  int nut_f32;
+  double* out;
+};
+constexpr int MOVIE_BASE = 8;
+__global__ __launch_bounds__(256) void k_stat_movie(StatArgs a, MovieArgs m) {
+  const Geo& g = a.g;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const long long n1 = (long long)ny * nx, n2 = (long long)nz * nx, n3 = (long long)ny * nz;
+  const long long per = n1 + n2 + n3;
+  const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= per) return;
+  int x, j, z;
+  if (t < n1) { j = (int)(t / nx) + 1; x = (int)(t % nx); z = m.nzm; }
+  else if (t < n1 + n2) { if (m.nym < 1) return; z = (int)((t - n1) / nx); x = (int)((t - n1) % nx); j = m.nym; }
+  else { j = (int)((t - n1 - n2) / nz) + 1; z = (int)((t - n1 - n2) % nz); x = m.nxm; }
+  const StatPoint P(a, x);
+  const long long pt = (long long)j * g.pp + (long long)z * nx + x;
+  double ox, oy, oz;
+  P.omega(j, z, ox, oy, oz);
+  double nt = 0.0;
+  if (m.nut) nt = m.nut_f32 ? (double)((const float*)m.nut)[pt] : ((const double*)m.nut)[pt];
+  double v[MOVIE_BASE];
+  v[0] = P.F(0, j, z);
+  v[1] = 0.5 * (P.F(3, j, z) + P.F(3, j + 1, z));
+  v[2] = P.F(6, j, z);
+  v[3] = nt;
+  v[4] = a.nu * P.eps(j, z);
+  v[5] = ox; v[6] = oy; v[7] = oz;
+  for (int f = 0; f < MOVIE_BASE; ++f) m.out[f * per + t] = v[f];
+  for (int n = 0; n < a.nth; ++n) m.out[(MOVIE_BASE + n) * per + t] = P.F(3 * (3 + n), j, z);
 }
 
 }  // namespace dgpu

@@ -138,6 +138,7 @@ struct dgpu_handle {
   double *d_zm = nullptr, *d_zs = nullptr, *d_prof = nullptr;
   double* d_zhist = nullptr;  // z* histograms [5 + 2 MAXTH][nx]
   double* d_ftx = nullptr;    // FTx_uu [nx / 2]
+  double* d_movie = nullptr;  // mean_xz planes, then movie slices
   double* hstage = nullptr;  // pinned staging, one Fortran array
   size_t fort_doubles = 0;
   std::vector<void*> allocs;
@@ -1723,6 +1724,51 @@ int dgpu_stats(dgpu_handle* h, double time, double* out) {
   double res = 0.0;
   for (int j = 2; j <= g.ny; ++j) res = res + 0.5 * (u1y[j] + u1y[j - 1]) * h->dy[j];
   out[np_] = res / c.ly;
+  return 0;
+}
+
+int dgpu_stats_movie_sizes(const dgpu_config* cfg, int64_t* n_planes, int64_t* n_slices) {
+  if (!cfg || !n_planes || !n_slices || cfg->nx <= 0 || cfg->ny <= 0 || cfg->nz <= 0) return 1;
+  const int64_t nx = cfg->nx, ny = cfg->ny, nz = cfg->nz;
+  *n_planes = (int64_t)NXZ_PLANES * ny * nx;
+  *n_slices = (int64_t)(MOVIE_BASE + cfg->n_th) * (ny * nx + nz * nx + ny * nz);
+  return 0;
+}
+
+int dgpu_stats_movie(dgpu_handle* h, double time, int32_t nxm, int32_t nym, int32_t nzm, double* planes,
+                     double* slices) {
+  if (!h || !planes || !slices) return fail(1, "null argument");
+  const Geo& g = h->g;
+  if (nxm < 0 || nxm >= g.nx || nzm < 0 || nzm >= g.nz || nym == 0 || nym < -1 || nym > g.ny)
+    return fail(2, "movie indices out of range (0 <= NxMovie < Nx, 0 <= NzMovie < Nz, "
+                   "NyMovie in 1..Nyp or -1)");
+  int64_t npl = 0, nsl = 0;
+  dgpu_stats_movie_sizes(&h->c, &npl, &nsl);
+  int rc;
+  if (!h->d_movie) {
+    void* p = nullptr;
+    if ((rc = dalloc(h, &p, sizeof(double) * (size_t)(npl + nsl)))) return rc;
+    h->d_movie = (double*)p;
+  }
+  StatArgs a;
+  if ((rc = stats_prepare(h, time, a))) return rc;
+  const int n2 = g.ny2;
+  const dim3 gxj((unsigned)((g.nx + 255) / 256), (unsigned)n2);
+  k_stat_zsum<<<gxj, 256, 0, h->st>>>(a);
+  k_stat_plane<<<(unsigned)n2, 256, 0, h->st>>>(a);
+  k_stat_xz<<<dim3((unsigned)((g.nx + 255) / 256), (unsigned)g.ny), 256, 0, h->st>>>(a, h->d_movie);
+  MovieArgs m{};
+  m.nxm = nxm; m.nym = nym; m.nzm = nzm; m.nfld = MOVIE_BASE + h->nth;
+  m.nut = h->c.use_les ? h->NuT : nullptr;
+  m.nut_f32 = h->prec == DGPU_MIXED;
+  m.out = h->d_movie + npl;
+  const long long per = (long long)g.ny * g.nx + (long long)g.nz * g.nx + (long long)g.ny * g.nz;
+  if (nym < 1) HIPCHK(hipMemsetAsync(m.out, 0, sizeof(double) * (size_t)nsl, h->st));
+  k_stat_movie<<<(unsigned)((per + 255) / 256), 256, 0, h->st>>>(a, m);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(planes, h->d_movie, sizeof(double) * (size_t)npl, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipMemcpyAsync(slices, m.out, sizeof(double) * (size_t)nsl, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
   return 0;
 }
 

@@ -54,6 +54,7 @@ def load():
     L.dgpu_gf2g.restype = None
     L.dgpu_h5_append_profile.argtypes = [s, s, D, I32, ctypes.c_double]
     L.dgpu_h5_append_real.argtypes = [s, s, ctypes.c_double]
+    L.dgpu_h5_append_plane.argtypes = [s, s, D, I32, I32, ctypes.c_double]
     L.dgpu_write_flow.argtypes = [P, s, T, I32]
     L.dgpu_read_flow.argtypes = [P, s, ctypes.c_uint32, T, ctypes.POINTER(I32)]
     L.dgpu_io_last_error.restype = ctypes.c_char_p
@@ -121,6 +122,14 @@ def append_profile(fname: str, gname: str, diag: np.ndarray, time: float) -> Non
     """WriteStatH5_Y: one more sample of a y profile (Nyp values) in group gname."""
     d = np.ascontiguousarray(diag, dtype=np.float64)
     _check(load().dgpu_h5_append_profile(fname.encode(), gname.encode(), _dp(d), d.size, time))
+
+
+def append_plane(fname: str, gname: str, plane: np.ndarray, time: float) -> None:
+    """WriteHDF5_XYplane / XZplane / ZYplane: one more sample of a plane, given in
+    the file's C layout [n2][n1] (the Fortran var2d(n1, n2))."""
+    d = np.ascontiguousarray(plane, dtype=np.float64)
+    n2, n1 = d.shape
+    _check(load().dgpu_h5_append_plane(fname.encode(), gname.encode(), _dp(d), n1, n2, time))
 
 
 def append_real(fname: str, gname: str, value: float) -> None:

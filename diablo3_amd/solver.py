@@ -67,6 +67,8 @@ def load():
     L.dgpu_stats_zstar.argtypes = [P, ctypes.c_double, D]
     L.dgpu_stats_zstar_doubles.argtypes = [P]
     L.dgpu_stats_zstar_doubles.restype = ctypes.c_int64
+    L.dgpu_stats_movie.argtypes = [P, ctypes.c_double, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, D, D]
+    L.dgpu_stats_movie_sizes.argtypes = [P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]
     L.dgpu_get_sizes.argtypes = [P] + [ctypes.POINTER(ctypes.c_int32)] * 4
     L.dgpu_field_doubles.argtypes = [P]
     L.dgpu_field_doubles.restype = ctypes.c_int64
@@ -239,6 +241,32 @@ class Solver:
         for i, name in enumerate(names):
             res[name] = out[base + i * nb:base + (i + 1) * nb].copy()
         return res
+
+    def stats_movie(self, time: float, nxm: int, nym: int, nzm: int):
+        """The movie output of save_stats_chan (dgpu_stats_movie): (planes, slices),
+        each {group name of the reference: array in the file's C layout}.  planes:
+        the mean_xz.h5 z-mean planes [Nyp][Nx]; slices: the movie.h5 slices
+        <f>_xz [Nyp][Nx] at z = nzm, <f>_xy [Nz][Nx] at row nym (absent when
+        nym = -1), <f>_yz [Nyp][Nz] at x = nxm."""
+        from .stats_io import XZ_PLANES, movie_fields
+        cfg = self.cfg
+        npl, nsl = ctypes.c_int64(), ctypes.c_int64()
+        self._check(load().dgpu_stats_movie_sizes(ctypes.cast(self.p.cconfig.ptr, ctypes.c_void_p),
+                                                  ctypes.byref(npl), ctypes.byref(nsl)))
+        pl = np.zeros(npl.value, np.float64)
+        sl = np.zeros(nsl.value, np.float64)
+        self._check(load().dgpu_stats_movie(self.h, time, nxm, nym, nzm, _dp(pl), _dp(sl)))
+        nx, ny, nz = cfg.nx, cfg.ny, cfg.nz
+        planes = {name: pl[i * ny * nx:(i + 1) * ny * nx].reshape(ny, nx).copy() for i, name in enumerate(XZ_PLANES)}
+        per = ny * nx + nz * nx + ny * nz
+        slices = {}
+        for i, name in enumerate(movie_fields(cfg.n_th)):
+            f = sl[i * per:(i + 1) * per]
+            slices[name + "_xz"] = f[:ny * nx].reshape(ny, nx).copy()
+            if nym >= 1:
+                slices[name + "_xy"] = f[ny * nx:ny * nx + nz * nx].reshape(nz, nx).copy()
+            slices[name + "_yz"] = f[ny * nx + nz * nx:].reshape(ny, nz).copy()
+        return planes, slices
 
     # ---- time advance ----
     def rk_substep(self, rk_step: int, delta_t: Optional[float] = None, time: float = 0.0,

@@ -68,6 +68,13 @@ void dgpu_gf2g(double *var, int32_t nx, int32_t ny, int32_t nz);
 int dgpu_h5_append_profile(const char *fname, const char *gname, const double *diag, int32_t ny,
                            double time);
 
+/* WriteHDF5_XYplane, WriteHDF5_XZplane, WriteHDF5_ZYplane (for/phdf5.f90:438-589,
+ * 596-740, 745-896) at one rank: the same series form for a plane var2d(n1, n2)
+ * in Fortran order (n1 fastest), stored with C dimensions (n2, n1): (Nx, Nyp)
+ * for an x-y plane, (Nx, Nz) for x-z, (Nz, Nyp) for z-y. */
+int dgpu_h5_append_plane(const char *fname, const char *gname, const double *var2d, int32_t n1, int32_t n2,
+                         double time);
+
 /* WriteHDF5_real (for/phdf5.f90:7-128): the same series form for one value. */
 int dgpu_h5_append_real(const char *fname, const char *gname, double value);
 

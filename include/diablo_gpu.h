@@ -199,8 +199,8 @@ int dgpu_courant(dgpu_handle *h, double time, double time_limit, double save_sta
  * u1z_x0 (the y-average of u1y_left, integrate_y_var, for/diagnostics.f90:1964);
  * then Nx/2 doubles, FTx_uu (641-663: per kx mode the y-integral of |u1(kz = 0)|^2
  * over Ly).
- * z* bins and the BPE come from dgpu_stats_zstar.  Not computed: the movie
- * planes. */
+ * z* bins and the BPE come from dgpu_stats_zstar, the movie planes from
+ * dgpu_stats_movie. */
 int dgpu_stats(dgpu_handle *h, double time, double *out);
 
 /* The z* statistics of save_stats_chan for scalar 1 (N_th >= 1), on the device in
@@ -220,6 +220,26 @@ int dgpu_stats(dgpu_handle *h, double time, double *out);
 int dgpu_stats_zstar(dgpu_handle *h, double time, double *out);
 int64_t dgpu_stats_zstar_doubles(const dgpu_config *cfg);
 int64_t dgpu_stats_doubles(const dgpu_config *cfg);
+
+/* The movie output of save_stats_chan(movie = .true.) for the state at `time`
+ * (the BC posts are applied first, as dgpu_stats does), in FP64.
+ * planes (dgpu_stats_movie_sizes: n_planes doubles): the z-mean (x, y) planes of
+ *   mean_xz.h5 that reduce_and_write_XYplane writes (for/diagnostics.f90:283-378,
+ *   756-767, 1076-1087, 1238-1245, 1338-1346, 1452-1460), 16 planes of
+ *   [Nyp][Nx] (rows 1..Nyp): ume, wme (u2 on GYF), vme (u3), thme_1, uu, ww
+ *   (u2'u2'), vv (u3'u3'), uw (u1'u2' on GY), wv (u3'u2' on GY), uv (u1'u3'),
+ *   epsilon, epsilon_m, thth, thu, thw, chi (scalar 1; zero without scalars).
+ *   The group names are the reference's, whose v and w are u3 and u2.
+ * slices (n_slices doubles): the movie.h5 slices (408-441, 485-617, 1041-1070,
+ *   1566-1727) of 8 + N_th fields: u1, u2 averaged to GYF, u3, nu_t (zero
+ *   without LES), epsilon, omega_x, omega_y, omega_z, th_1..th_N.  Per field:
+ *   [Nyp][Nx] at z = nzm (the reference's *_xz), [Nz][Nx] at row y = nym (*_xy;
+ *   zeros when nym = -1, the reference's rankyMovie = -1), [Nyp][Nz] at x = nxm
+ *   (*_yz).  nxm, nym, nzm are NxMovie, NyMovie, NzMovie of init_chan_movie
+ *   (for/domain.f90:380-401). */
+int dgpu_stats_movie(dgpu_handle *h, double time, int32_t nxm, int32_t nym, int32_t nzm, double *planes,
+                     double *slices);
+int dgpu_stats_movie_sizes(const dgpu_config *cfg, int64_t *n_planes, int64_t *n_slices);
 
 /* filter_chan(n) (for/solver.f90:1597-1685), as the driver calls it every
  * filter_int(n) steps when filter_th(n) (for/diablo.f90:76-81): scalar n

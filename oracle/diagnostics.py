@@ -107,6 +107,7 @@ def save_stats(o, time=0.0, points=False):
     # ---- compute_MKE_diss (1097-1249); gradients of the z-mean fields
     dudx_m = np.zeros((n2, nx)); dvdx_m = np.zeros((n2, nx)); dwdx_m = np.zeros((n2, nx))
     epm = np.zeros(n2)
+    epm_xy = np.zeros((n2, nx))
     if not hx:
         def mean_dx(c):
             a = np.zeros_like(c)
@@ -122,6 +123,7 @@ def save_stats(o, time=0.0, points=False):
                 t = t + ((vme_xy[j + 1] - vme_xy[j - 1]) / (gy[j + 1] - gy[j - 1])) ** 2
                 t = t + ((wme_xy[j] - wme_xy[j - 1]) / dy[j] + twf) ** 2
             epm[j] = t.sum()
+            epm_xy[j] = t
         dudx_m[1:ny + 1], dvdx_m[1:ny + 1], dwdx_m[1:ny + 1] = m1[1:ny + 1], m2[1:ny + 1], m4[1:ny + 1]
     out["epsilon_m"] = nu * epm / float(nx)
 
@@ -184,12 +186,12 @@ def save_stats(o, time=0.0, points=False):
         r = np.zeros(n2)
         for j in range(1, ny + 1):
             r[j] = np.sqrt(((f[j] - shift) ** 2).sum() / NxNz)
-        return r
-    out["omega_x"] = curl_rms(lambda j: 1j * kz * 0.5 * (cr2[j + 1] + cr2[j])
-                              - (cr3[j + 1] - cr3[j - 1]) / (2.0 * dyf[j]), twf)
-    out["omega_y"] = curl_rms(lambda j: 1j * kx * cr3[j] - 1j * kz * cr1[j])
-    out["omega_z"] = curl_rms(lambda j: (cr1[j + 1] - cr1[j - 1]) / (2.0 * dyf[j])
-                              - 1j * kx * 0.5 * (cr2[j + 1] + cr2[j]))
+        return r, f - shift
+    out["omega_x"], pts["omega_x"] = curl_rms(lambda j: 1j * kz * 0.5 * (cr2[j + 1] + cr2[j])
+                                              - (cr3[j + 1] - cr3[j - 1]) / (2.0 * dyf[j]), twf)
+    out["omega_y"], pts["omega_y"] = curl_rms(lambda j: 1j * kx * cr3[j] - 1j * kz * cr1[j])
+    out["omega_z"], pts["omega_z"] = curl_rms(lambda j: (cr1[j + 1] - cr1[j - 1]) / (2.0 * dyf[j])
+                                              - 1j * kx * 0.5 * (cr2[j + 1] + cr2[j]))
 
     # ---- scalars (140-410)
     gxv = np.array([(i * cfg.lx) / nx for i in range(nx)])
@@ -234,7 +236,51 @@ def save_stats(o, time=0.0, points=False):
     if points:
         pts["th0"] = th[0] if th else None
         out["_points"] = pts
+        # mean_xz.h5 planes (283-378, 756-767, 1076-1087, 1238-1245, 1338-1346,
+        # 1452-1460), rows 1..Nyp
+        pl = {"ume_xz": ume_xy, "wme_xz": np.zeros((n2, nx)), "vme_xz": wme_xy,
+              "uu_xz": uu_xy, "ww_xz": vv_xy, "vv_xz": ww_xy, "uw_xz": uvar_xy, "wv_xz": wvar_xy,
+              "uv_xz": vvar_xy, "epsilon_xz": pts["epsilon_zstar"].sum(axis=1) / nz, "epsilon_m_xz": nu * epm_xy}
+        for j in range(1, ny + 1):
+            pl["wme_xz"][j] = (0.5 * (u2[j] + u2[j + 1])).sum(axis=0) / nz   # var_xy (723)
+        if th:
+            ft = th[0] - thme_xy[0][:, None, :]
+            pl["thme_xz"] = thme_xy[0]
+            pl["thth_xz"] = (ft ** 2).sum(axis=1) / nz
+            pl["thu_xz"] = (ft * f1).sum(axis=1) / nz
+            pl["thw_xz"] = pts["thw_zstar01"].sum(axis=1) / nz
+            pl["chi_xz"] = pts["chi_zstar01"].sum(axis=1) / nz
+        out["_planes"] = {k: v[1:ny + 1] for k, v in pl.items()}
+        pts["u1"], pts["u2"], pts["u3"], pts["th"] = u1, u2, u3, th
     return out
+
+
+def movie(o, nxm, nym, nzm, time=0.0):
+    """The movie output of save_stats_chan(movie = .true.) for the oracle's
+    state: (planes, slices) keyed by the reference's group names, in the files'
+    C layouts (mean_xz planes [Nyp][Nx]; slices <f>_xz [Nyp][Nx] at z = nzm,
+    <f>_xy [Nz][Nx] at row nym (absent for nym = -1), <f>_yz [Nyp][Nz] at
+    x = nxm), for/diagnostics.f90:408-441, 485-617, 1041-1070, 1566-1727."""
+    cfg = o.p.cfg
+    ny = cfg.ny
+    out = save_stats(o, time, points=True)
+    pts = out["_points"]
+    u2 = pts["u2"]
+    w = np.zeros_like(u2)
+    w[:ny + 1] = 0.5 * (u2[:ny + 1] + u2[1:ny + 2])
+    nut = o.download_les()[0] if cfg.use_les else np.zeros_like(u2)
+    fields = {"u": pts["u1"], "w": w, "v": pts["u3"], "nu_t": nut, "epsilon": pts["epsilon_zstar"],
+              "omegaX": pts["omega_x"], "omegaZ": pts["omega_y"], "omegaY": pts["omega_z"]}
+    for n, t in enumerate(pts["th"]):
+        fields[f"th{n + 1}"] = t
+    sl = {}
+    J = slice(1, ny + 1)
+    for name, a in fields.items():
+        sl[name + "_xz"] = a[J, nzm, :]
+        if nym >= 1:
+            sl[name + "_xy"] = a[nym, :, :]
+        sl[name + "_yz"] = a[J, :, nxm]
+    return out["_planes"], sl
 
 
 # the reference's pi: 4 atan(1.0) in single precision (for/fft.f90:147)

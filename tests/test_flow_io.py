@@ -181,3 +181,49 @@ def test_mean_h5_sample(tmp_path):
         assert h.attr("/ume/0001", "Time", "f64")[0][0] == 0.5
     finally:
         h.close()
+
+
+def test_plane_series(tmp_path):
+    """WriteHDF5_XYplane / XZplane / ZYplane: var2d(n1, n2) stored with C
+    dimensions (n2, n1), one sample per call, SAMPLES and Time as for profiles;
+    write_movie writes the groups save_stats_chan(movie) writes."""
+    fio = io()
+    path = str(tmp_path / "movie.h5")
+    a = np.arange(NY * NX, dtype=np.float64).reshape(NY, NX)   # [Nyp][Nx]: var2d(Nx, Nyp)
+    b = np.arange(NZ * NX, dtype=np.float64).reshape(NZ, NX) * 0.5
+    fio.append_plane(path, "u_xz", a, 0.25)
+    fio.append_plane(path, "u_xz", 2 * a, 0.5)
+    fio.append_plane(path, "u_xy", b, 0.5)
+    h = h5raw.File(path)
+    try:
+        assert h.attr("/u_xz", "SAMPLES", "i32")[0][0] == 2
+        d, _, f64 = h.dataset("/u_xz/0002")
+        assert f64 and d.shape == (NY, NX) and np.array_equal(d, 2 * a)
+        assert h.attr("/u_xz/0001", "Time", "f64")[0][0] == 0.25
+        assert np.array_equal(h.dataset("/u_xy/0001")[0], b)
+    finally:
+        h.close()
+    # the groups of one movie sample
+    from diablo3_amd import configs, stats_io
+    cfg = configs.front_les(NX, NY, NZ, 5000.0, 2e-4, 1e-2)
+    cfg.use_les = False
+    planes = {k: np.full((NY, NX), float(i)) for i, k in enumerate(stats_io.XZ_PLANES)}
+    slices = {}
+    for f in stats_io.movie_fields(1):
+        slices[f + "_xz"] = np.zeros((NY, NX))
+        slices[f + "_yz"] = np.zeros((NY, NZ))
+    mv, mx = str(tmp_path / "mv.h5"), str(tmp_path / "mean_xz.h5")
+    stats_io.write_movie(mv, mx, 1.0, planes, slices, cfg)
+    h = h5raw.File(mv)
+    try:
+        assert h.exists("/th1_yz") and h.exists("/omegaZ_xz") and not h.exists("/nu_t_xz") and not h.exists("/u_xy")
+        assert h.dataset("/epsilon_yz/0001")[0].shape == (NY, NZ)
+    finally:
+        h.close()
+    h = h5raw.File(mx)
+    try:
+        for k in stats_io.XZ_PLANES:
+            assert h.exists("/" + k), k   # finite front, Nz > 1: every plane
+        assert h.dataset("/chi_xz/0001")[0][0, 0] == stats_io.XZ_PLANES.index("chi_xz")
+    finally:
+        h.close()

@@ -176,3 +176,31 @@ def test_read_flow_without_pressure_recomputes_it(tmp_path):
     assert err <= 1e-11 and np.abs(s3.cp[1:ny + 1]).max() > 0
     g.close()
     g2.close()
+
+
+def test_save_movie_files(tmp_path):
+    """save_stats_chan's movie output through the device and the plane writers:
+    movie.h5 and mean_xz.h5 hold the arrays dgpu_stats_movie returns."""
+    from diablo3_amd import stats_io
+    cfg, p, g = _front_state(8)
+    nxm, nym, nzm = stats_io.movie_indices(cfg, p.met["gyf"], 0.5 * cfg.lx, 0.0, 0.5 * cfg.lz)
+    mv, mx = str(tmp_path / "movie.h5"), str(tmp_path / "mean_xz.h5")
+    print("save_movie", nxm, nym, nzm, flush=True)
+    stats_io.save_movie(g, 0.5, nxm, nym, nzm, mv, mx)
+    planes, slices = g.stats_movie(0.5, nxm, nym, nzm)
+    h = h5raw.File(mv)
+    try:
+        for name in ("u_xz", "w_xy", "th1_yz", "epsilon_xz", "omegaX_xy"):
+            d = h.dataset(f"/{name}/0001")[0]
+            assert np.allclose(d, slices[name], rtol=1e-13, atol=1e-300), name
+        assert h.attr("/v_yz/0001", "Time", "f64")[0][0] == 0.5
+    finally:
+        h.close()
+    h = h5raw.File(mx)
+    try:
+        for name in stats_io.written_planes(cfg):
+            d = h.dataset(f"/{name}/0001")[0]
+            assert d.shape == (cfg.ny, cfg.nx) and np.allclose(d, planes[name], rtol=1e-13, atol=1e-18), name
+    finally:
+        h.close()
+    g.close()

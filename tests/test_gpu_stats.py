@@ -98,3 +98,44 @@ def test_zstar_matches_oracle(which):
         err = np.abs(a[m] - b[m]).max() / (scale if scale > 0 else 1.0)
         print(f"  {name:14s} max|ref| {scale:.3e} rel err {err:.2e}", flush=True)
         assert err <= 1e-10, (name, err)
+
+
+@pytest.mark.parametrize("which", ["infinite_front_dns", "finite_front", "finite_front_les"])
+def test_movie_matches_oracle(which):
+    """dgpu_stats_movie: the mean_xz.h5 planes save_stats_chan writes for the
+    configuration and every movie.h5 slice, within 1e-10 of the oracle (relative
+    to each array's largest magnitude, with a 1e-8 floor for means at round-off).
+    The LES case compares after one LES step of both from the same state (nu_t)."""
+    from diablo3_amd import stats_io
+    if which == "infinite_front_dns":
+        cfg = configs.small_dns(32, 33, 32)
+    else:
+        cfg = configs.front_les(32, 33, 32, 5000.0, 2e-4, 1e-2)
+        cfg.use_les = which == "finite_front_les"
+    p, o = prepared(cfg) if not cfg.use_les else (Problem(cfg), None)
+    if cfg.use_les:
+        o = oracle.Oracle(p, threads=16)
+        o.upload_state(p.initial_state(seed=4, theta_noise=1e-2))
+        o.rem_div()
+    g = Solver(p)
+    g.upload_state(o.download_state())
+    if cfg.use_les:  # past the LES spin-up gate, so nu_t is live
+        g.step(time=0.0, time_step=1, create_new_flow=False)
+        o.step(time=0.0, time_step=1, create_new_flow=0)
+    nxm, nym, nzm = stats_io.movie_indices(cfg, p.met["gyf"], 0.3 * cfg.lx, 0.25, 0.6 * cfg.lz)
+    print(which, "movie indices", nxm, nym, nzm, flush=True)
+    pg, sg = g.stats_movie(0.0, nxm, nym, nzm)
+    po, so = diagnostics.movie(o, nxm, nym, nzm)
+    for name in stats_io.written_planes(cfg):
+        a, b = pg[name], po[name]
+        err = np.abs(a - b).max() / max(np.abs(b).max(), 1e-8)
+        print(f"  {name:13s} max|ref| {np.abs(b).max():.3e} rel err {err:.2e}", flush=True)
+        assert a.shape == (cfg.ny, cfg.nx) and err <= 1e-10, (name, err)
+    assert set(sg) == set(so)
+    for name in sorted(so):
+        a, b = sg[name], so[name]
+        err = np.abs(a - b).max() / max(np.abs(b).max(), 1e-8)
+        print(f"  {name:13s} max|ref| {np.abs(b).max():.3e} rel err {err:.2e}", flush=True)
+        assert a.shape == b.shape and err <= 1e-10, (name, err)
+    assert (np.abs(so["nu_t_xz"]).max() > 0) == cfg.use_les
+    g.close()
