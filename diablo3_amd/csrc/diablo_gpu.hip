@@ -102,6 +102,7 @@ struct dgpu_handle {
   int fwdz_variant = -1;
   // k_cn column tile (DGPU_CN_TC = 8, 4, 2 or 1; 0: the widest that fits)
   int cn_tc = 0;
+  int pj_tc = 0;  // k_project columns per block (0: the widest that fits)
   void* Px[MAXF] = {};
   void *twx64 = nullptr, *twz64 = nullptr, *twx32 = nullptr, *twz32 = nullptr;
   double* d_fac = nullptr;
@@ -601,12 +602,17 @@ ColShape col_shape(const dgpu_handle* h, bool cn) {
     }
     return cs;
   }
-  for (int tc : {8, 4, 2}) {
+  // k_project: 4 columns per block where that still gives two blocks per CU, else
+  // 2 (measured on MI355X: at 256x129x256, 4 columns 0.080 ms, 2 0.093, 8 0.106;
+  // at 64x65x64, 946 columns, 2 columns 0.011 ms, 4 0.013); h->pj_tc overrides
+  const long long want = h->resident_blocks > 0 ? h->resident_blocks : 512;
+  const int tc0 = h->g.pc / 4 >= want ? 4 : 2;
+  for (int tc : {h->pj_tc, tc0, 2}) {
+    if (tc <= 0) continue;
     const size_t lds = (size_t)tc * ny2 * 40;
     if (lds <= 80 * 1024 || (tc == 2 && lds <= 160 * 1024)) {
       const int rg = 256 / tc, need = (ny2 + rg - 1) / rg;
-      int rpt = 1;
-      while (rpt < need) rpt *= 2;
+      const int rpt = need <= 8 ? need : 16;
       cs = {tc, rpt, lds};
       break;
     }
@@ -644,8 +650,11 @@ int launch_project_t(dgpu_handle* h, ColArgs& a, size_t lds, int update_p) {
   return 0;
 }
 
-#define DG_SHAPES(X) X(8, 1) X(8, 2) X(8, 4) X(8, 8) X(4, 1) X(4, 2) X(4, 4) X(4, 8) \
-                     X(2, 1) X(2, 2) X(2, 4) X(2, 8) X(2, 16)
+// k_project shapes: every rows-per-thread count up to 8 (a power-of-two round-up
+// would leave up to 3 of 8 row slots idle in every PCR level), then 16 for TC = 2
+#define DG_SHAPES(X) X(8, 1) X(8, 2) X(8, 3) X(8, 4) X(8, 5) X(8, 6) X(8, 7) X(8, 8) \
+                     X(4, 1) X(4, 2) X(4, 3) X(4, 4) X(4, 5) X(4, 6) X(4, 7) X(4, 8) \
+                     X(2, 1) X(2, 2) X(2, 3) X(2, 4) X(2, 5) X(2, 6) X(2, 7) X(2, 8) X(2, 16)
 
 #define DG_CN_SHAPES(X) X(8, 1) X(8, 2) X(8, 4) X(8, 8) X(4, 1) X(4, 2) X(4, 4) X(4, 8) \
                         X(2, 1) X(2, 2) X(2, 4) X(2, 8) X(1, 1) X(1, 2) X(1, 4) X(1, 8)
@@ -1297,6 +1306,10 @@ int dgpu_create(const dgpu_config* cfg, dgpu_handle** out) {
   if (const char* ct = std::getenv("DGPU_CN_TC")) {
     const int v = std::atoi(ct);
     h->cn_tc = (v == 8 || v == 4 || v == 2 || v == 1) ? v : 0;
+  }
+  if (const char* pt = std::getenv("DGPU_PJ_TC")) {
+    const int v = std::atoi(pt);
+    h->pj_tc = (v == 8 || v == 4 || v == 2) ? v : 0;
   }
   // DGPU_XPASS=fused selects the single-kernel x pass (measurement of the two forms)
   if (const char* xp = std::getenv("DGPU_XPASS")) h->xsplit = std::strcmp(xp, "fused") != 0;
