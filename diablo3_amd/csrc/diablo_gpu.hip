@@ -596,8 +596,11 @@ ColShape col_shape(const dgpu_handle* h, bool cn) {
       if (tc <= 0) continue;
       const size_t lds = (size_t)tc * ny2 * 16 * nf;
       const int rg = 256 / tc, need = (ny2 + rg - 1) / rg;
+      // exact rows per thread for 8 columns (the chunk bound LMAX = 4 RPT then
+      // tracks the column: fewer unrolled Thomas steps), powers of two otherwise
       int rpt = 1;
       while (rpt < need) rpt *= 2;
+      if (tc == 8 && need >= 1 && need <= 8) rpt = need;
       if (lds <= 80 * 1024 && rpt <= 8) { cs = {tc, rpt, lds}; break; }
     }
     return cs;
@@ -656,7 +659,8 @@ int launch_project_t(dgpu_handle* h, ColArgs& a, size_t lds, int update_p) {
                      X(4, 1) X(4, 2) X(4, 3) X(4, 4) X(4, 5) X(4, 6) X(4, 7) X(4, 8) \
                      X(2, 1) X(2, 2) X(2, 3) X(2, 4) X(2, 5) X(2, 6) X(2, 7) X(2, 8) X(2, 16)
 
-#define DG_CN_SHAPES(X) X(8, 1) X(8, 2) X(8, 4) X(8, 8) X(4, 1) X(4, 2) X(4, 4) X(4, 8) \
+#define DG_CN_SHAPES(X) X(8, 1) X(8, 2) X(8, 3) X(8, 4) X(8, 5) X(8, 6) X(8, 7) X(8, 8) \
+                        X(4, 1) X(4, 2) X(4, 4) X(4, 8) \
                         X(2, 1) X(2, 2) X(2, 4) X(2, 8) X(1, 1) X(1, 2) X(1, 4) X(1, 8)
 
 template <typename T, int NTH>
