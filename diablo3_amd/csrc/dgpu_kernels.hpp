@@ -158,23 +158,6 @@ __device__ __forceinline__ void pair_split(const cx<T>* __restrict__ Zl, int i, 
   B = {T(0.5) * df.y, T(-0.5) * df.x};
 }
 
-// the pair spectrum of the x derivatives of lines z and z+1 (bins times i kx)
-template <typename T>
-__device__ __forceinline__ cx<T> pair_spectrum_dx(const cx<T>* __restrict__ A, const cx<T>* __restrict__ B,
-                                                  int n, int nx, int nkx, const double* __restrict__ kx) {
-  if (n >= 1 && n <= nkx) {
-    const T k = (T)kx[n];
-    const cx<T> a = ik(k, A[n]), b = ik(k, B[n]);
-    return {a.x - b.y, a.y + b.x};
-  }
-  if (n >= nx - nkx) {
-    const T k = (T)kx[nx - n];
-    const cx<T> a = ik(k, A[nx - n]), b = ik(k, B[nx - n]);
-    return {a.x + b.y, b.x - a.y};
-  }
-  return {T(0), T(0)};  // DC (kx = 0) and de-aliased bins
-}
-
 // Field f of a batch with bit f of dxmask set yields its x derivative.
 template <typename T, int NX, int ZP, int NT>
 __global__ __launch_bounds__(NT) void k_inv_x(Ptrs<const cx<T>*, MAXO> src, Ptrs<T*, MAXO> dst,
@@ -187,23 +170,45 @@ __global__ __launch_bounds__(NT) void k_inv_x(Ptrs<const cx<T>*, MAXO> src, Ptrs
   const int zb = blockIdx.x * ZP * 2;
   const cx<T>* __restrict__ H = src.p[f] + (long long)j * g.ph;
   const bool dx = (dxmask >> f) & 1u;
-  // every load of the thread first, then the LDS stores (the loads overlap)
-  constexpr int PER = (ZP * NX + NT - 1) / NT;
-  cx<T> v[PER];
+  // Each half-spectrum bin m = 0..NX/2 of the two lines is loaded once and gives
+  // both bins m and NX - m of the pair spectrum (pair_spectrum written out for the
+  // two halves; for an x derivative the bins times i kx, DC zero); every load of
+  // the thread first, then the LDS stores (the loads overlap).
+  constexpr int HB = NX / 2 + 1;
+  constexpr int PER = (ZP * HB + NT - 1) / NT;
+  cx<T> lo[PER], hi[PER];
 #pragma unroll
   for (int p = 0; p < PER; ++p) {
     const int idx = tid + p * NT;
-    if (PER * NT == ZP * NX || idx < ZP * NX) {
-      const int q = idx / NX, n = idx % NX, z = zb + 2 * q;
+    if (idx < ZP * HB) {
+      const int q = idx / HB, m = idx % HB, z = zb + 2 * q;
       const cx<T>* A = H + (long long)z * g.kxp;
       const cx<T>* B = H + (long long)(z + 1) * g.kxp;
-      v[p] = dx ? pair_spectrum_dx(A, B, n, NX, g.nkx, kx) : pair_spectrum(A, B, n, NX, g.nkx);
+      cx<T> l = {T(0), T(0)}, u = {T(0), T(0)};
+      if (m == 0) {
+        if (!dx) l = {A[0].x, B[0].x};  // DC: the imaginary parts dropped; d/dx of DC is 0
+      } else if (m <= g.nkx) {
+        cx<T> a = A[m], b = B[m];
+        if (dx) {
+          const T k = (T)kx[m];
+          a = ik(k, a);
+          b = ik(k, b);
+        }
+        l = {a.x - b.y, a.y + b.x};
+        u = {a.x + b.y, b.x - a.y};
+      }
+      lo[p] = l;
+      hi[p] = u;
     }
   }
 #pragma unroll
   for (int p = 0; p < PER; ++p) {
     const int idx = tid + p * NT;
-    if (PER * NT == ZP * NX || idx < ZP * NX) s[(idx / NX) * NX + swz<T>(idx % NX)] = v[p];
+    if (idx < ZP * HB) {
+      const int q = idx / HB, m = idx % HB;
+      s[q * NX + swz<T>(m)] = lo[p];
+      if (m > 0 && m < NX / 2) s[q * NX + swz<T>(NX - m)] = hi[p];
+    }
   }
   __syncthreads();
   fft_lds<T, NX, ZP, NT, +1>(s, NX, stw, tid);
