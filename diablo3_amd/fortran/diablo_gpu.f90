@@ -50,7 +50,7 @@ module diablo_gpu
   public :: dgpu_last_step_ms, dgpu_field_doubles, dgpu_last_error_f
   public :: dgpu_download_les, dgpu_stats, dgpu_stats_doubles
   public :: dgpu_set_variable_dt, dgpu_courant, dgpu_set_closure, dgpu_filter_chan
-  public :: dgpu_write_flow, dgpu_read_flow, dgpu_poisson_p
+  public :: dgpu_write_flow, dgpu_read_flow, dgpu_poisson_p, dgpu_stats_movie, dgpu_stats_movie_sizes
 
   interface
     integer(c_int) function dgpu_create(cfg, h) bind(C, name="dgpu_create")
@@ -133,6 +133,19 @@ module diablo_gpu
     integer(c_int64_t) function dgpu_stats_doubles(cfg) bind(C, name="dgpu_stats_doubles")
       import :: c_int64_t, dgpu_config
       type(dgpu_config), intent(in) :: cfg
+    end function
+    integer(c_int) function dgpu_stats_movie(h, time, nxm, nym, nzm, planes, slices) &
+        bind(C, name="dgpu_stats_movie")
+      import :: c_int, c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), value :: time
+      integer(c_int32_t), value :: nxm, nym, nzm
+      type(c_ptr), value :: planes, slices
+    end function
+    integer(c_int) function dgpu_stats_movie_sizes(cfg, n_planes, n_slices) bind(C, name="dgpu_stats_movie_sizes")
+      import :: c_int, c_int64_t, dgpu_config
+      type(dgpu_config), intent(in) :: cfg
+      integer(c_int64_t), intent(out) :: n_planes, n_slices
     end function
     integer(c_int) function dgpu_set_variable_dt(h, variable_dt, update_dt, cfl) bind(C, name="dgpu_set_variable_dt")
       import :: c_int, c_int32_t, c_ptr, c_double

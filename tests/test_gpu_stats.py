@@ -139,3 +139,19 @@ def test_movie_matches_oracle(which):
         assert a.shape == b.shape and err <= 1e-10, (name, err)
     assert (np.abs(so["nu_t_xz"]).max() > 0) == cfg.use_les
     g.close()
+
+
+def test_movie_index_checks_and_missing_row():
+    """dgpu_stats_movie refuses slice indices outside the grid (before any launch)
+    and, for NyMovie = -1 (the row not found), returns no x-z slices."""
+    cfg = configs.small_dns(16, 17, 16)
+    p = Problem(cfg)
+    g = Solver(p)
+    g.upload_state(p.initial_state(seed=1, theta_noise=1e-2))
+    g.rem_div()
+    for bad in ((cfg.nx, 3, 0), (0, 0, 0), (0, cfg.ny + 1, 0), (0, 3, cfg.nz), (-1, 3, 0)):
+        with pytest.raises(RuntimeError, match="movie indices"):
+            g.stats_movie(0.0, *bad)
+    planes, sl = g.stats_movie(0.0, 1, -1, 2)
+    assert "u_xz" in sl and "th1_yz" in sl and not any(k.endswith("_xy") for k in sl)
+    g.close()
