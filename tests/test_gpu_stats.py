@@ -155,3 +155,24 @@ def test_movie_index_checks_and_missing_row():
     planes, sl = g.stats_movie(0.0, 1, -1, 2)
     assert "u_xz" in sl and "th1_yz" in sl and not any(k.endswith("_xy") for k in sl)
     g.close()
+
+
+def test_movie_nu_t_slices_mixed_precision():
+    """The nu_t slices of a mixed-precision LES handle (nu_t held in FP32) are the
+    handle's own nu_t at the slice indices, exactly."""
+    from diablo3_amd.config import MIXED
+    cfg = configs.front_les(32, 33, 32, 5000.0, 2e-4, 1e-2, precision=MIXED)
+    p = Problem(cfg)
+    g = Solver(p)
+    g.upload_state(p.initial_state(seed=2, theta_noise=1e-2))
+    g.rem_div()
+    g.step(time=0.0, time_step=1, create_new_flow=False)
+    nut = g.download_les()[0]
+    nxm, nym, nzm = 7, 12, 20
+    _, sl = g.stats_movie(0.0, nxm, nym, nzm)
+    ny = cfg.ny
+    assert np.abs(nut).max() > 0
+    assert np.array_equal(sl["nu_t_xz"], nut[1:ny + 1, nzm, :])
+    assert np.array_equal(sl["nu_t_xy"], nut[nym])
+    assert np.array_equal(sl["nu_t_yz"], nut[1:ny + 1, :, nxm])
+    g.close()
