@@ -105,6 +105,9 @@ def read_traffic(kernel, workload):
     return float(m.group(1)) if m else None
 
 
+TIMING_EVERY = 10
+
+
 def cpu_baseline(cfg, threads, steps=1):
     """The oracle on this host's cores: one full step (DNS) or one substep scaled
     by 3 (LES, whose substep costs the same at every rk_step)."""
@@ -182,6 +185,10 @@ def main():
         g.step(time=t, time_step=n + 1, create_new_flow=cnf)
         t += cfg.delta_t
     g.timing(True)
+    # kernel events on one step in TIMING_EVERY: the per-kernel durations come from
+    # those steps, and the timed loop carries the events' cost (36 records per
+    # step, about 6% of a 256x129x256 step) on them only
+    g.timing_every(TIMING_EVERY)
     g.synchronize()
     t0 = time.perf_counter()
     for n in range(args.steps):

@@ -145,6 +145,9 @@ struct dgpu_handle {
   std::vector<void*> allocs;
   // per-kernel timing (measurement only)
   bool timing = false;
+  int timing_every = 1;       // record the kernel events on every n-th dgpu_step call
+  long long timing_calls = 0;
+  bool timing_now = true;     // this call records
   std::vector<hipEvent_t> ev_pool;
   struct Mark { int id; hipEvent_t a, b; };
   std::vector<Mark> marks;
@@ -404,7 +407,7 @@ hipEvent_t pool_event(dgpu_handle* h) {
   return e;
 }
 int tbeg(dgpu_handle* h, int id) {
-  if (!h->timing) return 0;
+  if (!h->timing || !h->timing_now) return 0;
   dgpu_handle::Mark m{id, pool_event(h), pool_event(h)};
   if (!m.a || !m.b) return fail(10, "hipEventCreate");
   HIPCHK(hipEventRecord(m.a, h->st));
@@ -412,7 +415,7 @@ int tbeg(dgpu_handle* h, int id) {
   return 0;
 }
 int tend(dgpu_handle* h) {
-  if (!h->timing) return 0;
+  if (!h->timing || !h->timing_now) return 0;
   HIPCHK(hipEventRecord(h->marks.back().b, h->st));
   return 0;
 }
@@ -1858,6 +1861,7 @@ int dgpu_stats_zstar(dgpu_handle* h, double time, double* out) {
 int dgpu_rk_substep(dgpu_handle* h, int32_t rk_step, double delta_t, double time, int32_t time_step,
                     int32_t create_new_flow) {
   if (!h) return fail(1, "null handle");
+  h->timing_now = true;
   HIPCHK(hipEventRecord(h->e0, h->st));
   int rc = substep(h, rk_step, delta_t, time, time_step, create_new_flow);
   if (rc) return rc;
@@ -1868,6 +1872,7 @@ int dgpu_rk_substep(dgpu_handle* h, int32_t rk_step, double delta_t, double time
 
 int dgpu_step(dgpu_handle* h, double delta_t, double time, int32_t time_step, int32_t create_new_flow) {
   if (!h) return fail(1, "null handle");
+  h->timing_now = (h->timing_calls++ % h->timing_every) == 0;
   HIPCHK(hipEventRecord(h->e0, h->st));
   for (int rk = 1; rk <= 3; ++rk) {
     int rc = substep(h, rk, delta_t, time, time_step, create_new_flow);
@@ -1997,6 +2002,17 @@ int dgpu_timing_enable(dgpu_handle* h, int32_t enable) {
   if (rc) return rc;
   h->timing = enable != 0;
   for (int k = 0; k < NKERN; ++k) { h->t_ms[k] = 0.0; h->t_n[k] = 0; }
+  return 0;
+}
+
+int dgpu_timing_every(dgpu_handle* h, int32_t every) {
+  if (!h || every < 1) return fail(1, "dgpu_timing_every: every must be >= 1");
+  HIPCHK(hipStreamSynchronize(h->st));
+  int rc = harvest(h);
+  if (rc) return rc;
+  h->timing_every = every;
+  h->timing_calls = 0;
+  h->timing_now = true;
   return 0;
 }
 

@@ -47,6 +47,7 @@ def load():
                                   ctypes.c_int32, ctypes.c_int32]
     L.dgpu_step.argtypes = [P, ctypes.c_double, ctypes.c_double, ctypes.c_int32, ctypes.c_int32]
     L.dgpu_rem_div.argtypes = [P]
+    L.dgpu_timing_every.argtypes = [P, ctypes.c_int32]
     L.dgpu_poisson_p.argtypes = [P]
     L.dgpu_filter_chan.argtypes = [P, ctypes.c_int32]
     L.dgpu_fft_xz_to_fourier.argtypes = [P, D, D]
@@ -302,6 +303,10 @@ class Solver:
     # ---- measurement ----
     def timing(self, enable: bool):
         self._check(load().dgpu_timing_enable(self.h, int(enable)))
+
+    def timing_every(self, every: int):
+        """Record the kernel events on every `every`-th step only (dgpu_timing_every)."""
+        self._check(load().dgpu_timing_every(self.h, int(every)))
 
     def timing_read(self):
         """{kernel name: (total device ms, launches)} since timing(True), launched kernels only."""

@@ -168,6 +168,10 @@ int dgpu_last_step_ms(dgpu_handle *h, double *ms);
  * dgpu_timing_read returns the
  * device time and launch count accumulated since the last enable, and the name. */
 int dgpu_timing_enable(dgpu_handle *h, int32_t enable);
+/* Measurement only (no counterpart in the reference): with timing enabled, record
+ * the kernel events on every `every`-th dgpu_step call only (1: every call, the
+ * default), so that a timed loop carries the events' own cost on few steps. */
+int dgpu_timing_every(dgpu_handle *h, int32_t every);
 int dgpu_timing_read(dgpu_handle *h, int32_t kernel, double *total_ms, int64_t *launches,
                      const char **name);
 
