@@ -57,6 +57,8 @@ def lib():
                                          ctypes.c_double, D, D]
         L.oracle_grid_metrics.argtypes = [ctypes.c_int, D, D, D, D, D]
         L.oracle_cn_matrix.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_double, D, D, D, D]
+        L.oracle_cn_matrix_nut.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_double, D, D, D, D, D]
+        L.oracle_debug_rhs.argtypes = [P, ctypes.c_int, D]
         L.oracle_thomas_real.argtypes = [ctypes.c_int, D, D, D, D]
         L.oracle_thomas_complex.argtypes = [ctypes.c_int, D, D, D, D]
         _lib = L
@@ -182,6 +184,24 @@ class Oracle:
         a, b, c, v = (np.zeros(n2) for _ in range(4))
         self._check(lib().oracle_cn_matrix(self.h, which, rk_step, dt, _dp(a), _dp(b), _dp(c), _dp(v)))
         return a, b, c, v
+
+    def cn_matrix_nut(self, which, rk_step, nut, delta_t=None):
+        """CN system of one column with the eddy-viscosity column nut (rows 0..Nyp+1)."""
+        n2 = self.cfg.ny2
+        dt = self.cfg.delta_t if delta_t is None else delta_t
+        nut = np.ascontiguousarray(nut, dtype=np.float64)
+        assert nut.shape == (n2,)
+        a, b, c, v = (np.zeros(n2) for _ in range(4))
+        self._check(lib().oracle_cn_matrix_nut(self.h, which, rk_step, dt, _dp(nut), _dp(a), _dp(b),
+                                               _dp(c), _dp(v)))
+        return a, b, c, v
+
+    def debug_rhs(self, which):
+        """The physical CN right-hand side of field `which` (0 u1, 1 u2, 2 u3, 3+n scalar n)
+        left by the last substep, as a compact [j][z][x] array."""
+        out = np.zeros(self._layout.fortran_physical_shape(self.cfg), np.float64)
+        self._check(lib().oracle_debug_rhs(self.h, which, _dp(out)))
+        return self._layout.from_fortran_physical(self.cfg, out)
 
 
 def wavenumbers(nx, nz, lx, lz):

@@ -562,10 +562,32 @@ static void build_cn(const oracle_t *o, int which, double temp1, double temp2,
 
 int oracle_cn_matrix(oracle_t *o, int which, int rk_step, double delta_t, double *matl,
                      double *matd, double *matu, double *vec_bc) {
+  return oracle_cn_matrix_nut(o, which, rk_step, delta_t, NULL, matl, matd, matu, vec_bc);
+}
+
+int oracle_cn_matrix_nut(oracle_t *o, int which, int rk_step, double delta_t, const double *nut,
+                         double *matl, double *matd, double *matu, double *vec_bc) {
   const double h[3] = {delta_t * (8.0 / 15.0), delta_t * (2.0 / 15.0), delta_t * (5.0 / 15.0)};
   if (rk_step < 1 || rk_step > 3 || which < 0 || which >= 3 + o->nth) return 1;
+  if (nut && !o->c.use_les) { set_err("a nu_t column needs use_les"); return 1; }
   const double temp1 = 0.5 * o->c.nu * h[rk_step - 1], temp2 = 0.5 * h[rk_step - 1];
-  build_cn(o, which, temp1, temp2, NULL, NULL, 0, 0.0, matl, matd, matu, vec_bc);
+  build_cn(o, which, temp1, temp2, nut, NULL, 1, 0.0, matl, matd, matu, vec_bc);
+  return 0;
+}
+
+/* The physical CN right-hand side r1, r2, r3 or rth(n) (which = 0, 1, 2, 3+n) as
+ * the last substep left it: the inverse transform of cr plus the explicit half of
+ * the vertical viscous terms (for/solver.f90:841-903, 967-1002). */
+int oracle_debug_rhs(oracle_t *o, int which, double *out) {
+  const double *src;
+  if (which == 0) src = o->r1;
+  else if (which == 1) src = o->r2;
+  else if (which == 2) src = o->r3;
+  else if (which >= 3 && which < 3 + o->nth) src = o->rth + (size_t)(which - 3) * o->np;
+  else { set_err("which out of range"); return 1; }
+  for (int j = 0; j < o->ny2; j++)
+    for (int z = 0; z < o->nz; z++)
+      for (int x = 0; x < o->nx; x++) out[pfort(o, x, z, j)] = src[PI(j, z, x)];
   return 0;
 }
 

@@ -65,6 +65,12 @@ void oracle_grid_metrics(int ny, const double *gy_file, double *gy, double *gyf,
  * matl/matd/matu have ny+2 entries (rows 0..Nyp+1). */
 int oracle_cn_matrix(oracle_t *o, int which, int rk_step, double delta_t,
                      double *matl, double *matd, double *matu, double *vec_bc);
+/* The same with an eddy-viscosity column nut(0:Nyp+1) (use_les handles only). */
+int oracle_cn_matrix_nut(oracle_t *o, int which, int rk_step, double delta_t, const double *nut,
+                         double *matl, double *matd, double *matu, double *vec_bc);
+/* The physical CN right-hand side of field `which` left by the last substep
+ * (physical Fortran layout), so tests can pin the explicit half on its own. */
+int oracle_debug_rhs(oracle_t *o, int which, double *out);
 /* Thomas solves exactly as thomas_forward/backward_*_mpi at one rank. */
 void oracle_thomas_real(int ny, double *a, double *b, const double *c, double *g);
 void oracle_thomas_complex(int ny, double *a, double *b, const double *c, double *g);
