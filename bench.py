@@ -8,7 +8,9 @@ One "step" is the full RKW3 step (three rk_chan_1 substeps) through the C ABI.
 Printed JSON line: value = Nx*Ny*Nz / t_step in Mpoint-updates/s; roofline for the
 dominant kernel (algorithmic bytes per launch / its mean HIP-event duration over
 the timed region); cpu_baseline = the CPU oracle timed on this host's cores on a
-bounded sample (one full step of the same workload).
+bounded sample (the median substep of three steps on all cores, and of one step
+on one core); learned_vs_amd_les = the learned-closure mixed LES step against the
+FP64 and mixed AMD LES steps at 512x257x512 (the metric's third part).
 """
 import argparse
 import json
@@ -108,9 +110,16 @@ def read_traffic(kernel, workload):
 TIMING_EVERY = 10
 
 
-def cpu_baseline(cfg, threads, steps=1):
-    """The oracle on this host's cores: one full step (DNS) or one substep scaled
-    by 3 (LES, whose substep costs the same at every rk_step)."""
+def log(msg):
+    print(f"bench.py: {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(cfg, threads, steps):
+    """The oracle on this host's cores, timed per substep: the median substep
+    time x 3 over `steps` RKW3 steps (every rk_step costs the same: the same
+    transforms and solves), the reference's own per-iteration wall time
+    (for/diablo.f90:104-109) taken as a median."""
+    import statistics
     import oracle
     from diablo3_amd.problem import Problem
     p = Problem(cfg.copy(precision=0))
@@ -118,22 +127,76 @@ def cpu_baseline(cfg, threads, steps=1):
     o.upload_state(p.initial_state(seed=0, theta_noise=1e-3))
     o.rem_div()
     cnf = 0 if cfg.use_les else 1
-    t0 = time.perf_counter()
-    if cfg.use_les:
-        o.rk_substep(1, time=0.0, time_step=1, create_new_flow=cnf)
-        dt = 3.0 * (time.perf_counter() - t0)
-        what = "one LES substep (x3)"
-    else:
-        for n in range(steps):
-            o.step(time=n * cfg.delta_t, time_step=n + 1, create_new_flow=cnf)
-        dt = (time.perf_counter() - t0) / steps
-        what = f"{steps} full RKW3 step(s)"
-    return {"value": cfg.nx * cfg.ny * cfg.nz / dt / 1e6, "unit": "Mpoint-updates/s", "cores": threads,
+    times = []
+    for n in range(steps):
+        for rk in (1, 2, 3):
+            t0 = time.perf_counter()
+            o.rk_substep(rk, time=n * cfg.delta_t, time_step=n + 1, create_new_flow=cnf)
+            times.append(time.perf_counter() - t0)
+    dt = 3.0 * statistics.median(times)
+    return {"value": round(cfg.nx * cfg.ny * cfg.nz / dt / 1e6, 3), "unit": "Mpoint-updates/s", "cores": threads,
             "kind": "port",
-            "sample": f"{what} of the same {cfg.nx}x{cfg.ny}x{cfg.nz} FP64 workload, "
-                      f"C restatement of rk_chan_1 (physical-space CN, "
-                      f"{59 if cfg.use_les else 23} transforms/substep), "
-                      f"OpenMP {threads} threads, {dt:.2f} s/step"}
+            "sample": f"median of {len(times)} substeps x 3 ({steps} RKW3 step(s)) of the same "
+                      f"{cfg.nx}x{cfg.ny}x{cfg.nz} FP64 workload, C restatement of rk_chan_1 "
+                      f"(physical-space CN, {59 if cfg.use_les else 23} transforms/substep), "
+                      f"{'OpenMP ' if threads > 1 else ''}{threads} thread(s), {dt:.2f} s/step"}
+
+
+def copy_calibration(nbytes, reps=3):
+    """A plain device copy of known size (read nbytes, write nbytes per launch):
+    the reference point that scales rocprofv3's FETCH_SIZE / WRITE_SIZE counters
+    to bytes."""
+    import torch
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        torch.neg(a, out=b)
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / reps
+    print(json.dumps({"copy_calibration_bytes": nbytes, "launches": reps,
+                      "GB/s_read_plus_write": round(2 * nbytes / t / 1e9, 1)}), flush=True)
+
+
+def les_speedup(steps, warmup):
+    """The metric's third part on the 512x257x512 finite-front LES: ms per step of
+    the learned-closure step in mixed precision against the plain FP64 AMD step
+    (and the mixed AMD step), each from the same seeded state, SGS on from step 1."""
+    from diablo3_amd import closure, configs
+    from diablo3_amd.config import FP64, MIXED
+    from diablo3_amd.problem import Problem
+    from diablo3_amd.solver import Solver
+    base = configs.les_512(FP64)
+    p0 = Problem(base)
+    s0 = p0.initial_state(seed=0, theta_noise=1e-3)
+    params, norm, src = closure.load_params()
+    out = {}
+    for name, prec, model in (("fp64_amd", FP64, 5), ("mixed_amd", MIXED, 5), ("mixed_learned", MIXED, 6)):
+        p = Problem(base.copy(precision=prec, les_model_type=model))
+        g = Solver(p)
+        if model == 6:
+            g.set_closure(params, norm)
+        g.upload_state(s0)
+        g.rem_div()
+        t = 0.0
+        for n in range(warmup):
+            g.step(time=t, time_step=n + 1, create_new_flow=False)
+            t += base.delta_t
+        g.synchronize()
+        t0 = time.perf_counter()
+        for n in range(steps):
+            g.step(time=t, time_step=warmup + n + 1, create_new_flow=False)
+            t += base.delta_t
+        g.synchronize()
+        out[name + "_ms_per_step"] = round((time.perf_counter() - t0) / steps * 1e3, 3)
+        g.close()
+        log(f"LES 512 {name}: {out[name + '_ms_per_step']} ms/step")
+    out["speedup_learned_mixed_vs_fp64_amd"] = round(out["fp64_amd_ms_per_step"] / out["mixed_learned_ms_per_step"], 3)
+    out["speedup_mixed_amd_vs_fp64_amd"] = round(out["fp64_amd_ms_per_step"] / out["mixed_amd_ms_per_step"], 3)
+    out["workload"] = (f"front LES {base.nx}x{base.ny}x{base.nz}, Re {base.re}, dt {base.delta_t}, "
+                       f"{steps} timed steps after {warmup}; learned closure {src}")
+    return out
 
 
 def main():
@@ -150,11 +213,19 @@ def main():
                          "committed front closure if present, else a seeded network")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-les", action="store_true",
+                    help="skip the learned-vs-AMD LES leg (512x257x512) of the default DNS run")
+    ap.add_argument("--les-steps", type=int, default=10)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run a plain device copy of BYTES (rocprofv3 counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only (no collectives, no sharding); "
               "use --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
 
     from diablo3_amd import configs
     from diablo3_amd.config import FP64, MIXED
@@ -233,9 +304,19 @@ def main():
         "step_hbm_frac_model_v1": round(step_model / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "kernels": per_kernel,
     }
+    g.close()
+    log(f"{args.workload} {args.precision}: {ms:.4f} ms/step over {args.steps} steps")
+    if not args.no_les and args.workload == "dns_256":
+        out["learned_vs_amd_les"] = les_speedup(args.les_steps, 2)
+        log(f"LES leg: {out['learned_vs_amd_les']}")
     if not args.no_cpu:
         threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
-        out["cpu_baseline"] = cpu_baseline(configs.NAMED[args.workload](FP64), threads)
+        cfg64 = configs.NAMED[args.workload](FP64)
+        out["cpu_baseline"] = cpu_baseline(cfg64, threads, steps=3)
+        log(f"CPU baseline, {threads} threads: {out['cpu_baseline']['sample']}")
+        # one core, as the reference's one-rank run: one step's three substeps
+        out["cpu_baseline"]["one_core"] = cpu_baseline(cfg64, 1, steps=1)
+        log(f"CPU baseline, 1 thread: {out['cpu_baseline']['one_core']['sample']}")
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

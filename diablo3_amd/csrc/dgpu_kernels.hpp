@@ -272,44 +272,27 @@ struct XProdArgs {
   // (tau_11, tau_13, tau_33, the two stress fluxes, the two GY stresses, the
   // scalar fluxes); null for DNS
   const void* aug[7 + 2 * MAXTH];
-  // SPLIT: the fields in physical space ([j][z][x], T), all planes; the kernel
-  // then reads them instead of inverse-transforming hin
+  // the fields in physical space ([j][z][x], T), all planes, written by the
+  // inverse x transform; the kernel reads the three planes of its x columns
   const void* phys[MAXF];
 };
 
-// SPLIT = false: each block inverse-transforms planes j-1, j, j+1 of every field
-// itself (no physical-space traffic).  SPLIT = true: a separate inverse x
-// transform has written the physical fields once per plane and the block reads
-// the three planes of its x columns from them; only the product lines need LDS.
-template <typename T, int NX, int NT, int NTH, bool LES, bool SPLIT = false>
+// A separate inverse x transform has written the physical fields once per
+// plane; each block reads the three planes j-1..j+1 of its line pair (z, z+1),
+// forms the products and forward-transforms them in LDS.
+template <typename T, int NX, int NT, int NTH, bool LES>
 __global__ __launch_bounds__(NT) void k_xprod(XProdArgs a) {
-  constexpr int NF = 3 + NTH, NL = 3 * NF, NP = 8 + 3 * NTH, NO = 6 + 2 * NTH;
-  static_assert(NP <= NL, "product lines overlay input lines");
-  __shared__ cx<T> s[(SPLIT ? NP : NL) * NX];
+  constexpr int NP = 8 + 3 * NTH, NO = 6 + 2 * NTH;
+  __shared__ cx<T> s[NP * NX];
   __shared__ cx<T> stw[NX];  // twiddles
   stage_twiddles<T, NX, NT>(stw, (const cx<T>*)a.tw, threadIdx.x);
   const Geo& g = a.g;
   const int tid = threadIdx.x, z = 2 * blockIdx.x, j = a.jlo + blockIdx.y;
-  if constexpr (!SPLIT) {
-    // inverse x transforms of (field f, plane j-1+d) into line 3f+d
-    for (int idx = tid; idx < NL * NX; idx += NT) {
-      const int l = idx / NX, n = idx % NX, f = l / 3, jj = j - 1 + (l - 3 * f);
-      const cx<T>* H = (const cx<T>*)a.hin[f] + (long long)jj * g.ph;
-      s[l * NX + swz<T>(n)] = pair_spectrum(H + (long long)z * g.kxp, H + (long long)(z + 1) * g.kxp, n, NX, g.nkx);
-    }
-    __syncthreads();
-    fft_lds<T, NX, NL, NT, +1>(s, NX, stw, tid);
-  } else {
-    __syncthreads();  // the twiddle table
-  }
+  __syncthreads();  // the twiddle table
   // (field f, plane j-1+d) of the line pair (z, z+1) at column x
   auto field = [&](int f, int d, int x) -> cx<T> {
-    if constexpr (SPLIT) {
-      const T* R = (const T*)a.phys[f] + (long long)(j - 1 + d) * g.pp + (long long)z * NX + x;
-      return cx<T>{R[0], R[NX]};
-    } else {
-      return s[(f * 3 + d) * NX + swz<T>(x)];
-    }
+    const T* R = (const T*)a.phys[f] + (long long)(j - 1 + d) * g.pp + (long long)z * NX + x;
+    return cx<T>{R[0], R[NX]};
   };
   const bool vrow = (j >= a.jstart && j <= a.jend);
   const bool v2row = (j >= 2 && j <= g.ny);
@@ -398,8 +381,7 @@ __global__ __launch_bounds__(NT) void k_xprod(XProdArgs a) {
         P[9 + 3 * n] = P[9 + 3 * n] + A(8 + 2 * n);
       }
     }
-    // each thread rewrites only column x, which only it has read (SPLIT: the
-    // lines are free)
+    // each thread writes only column x of the product lines
 #pragma unroll
     for (int p = 0; p < NP; ++p) s[p * NX + swz<T>(x)] = P[p];
   }

@@ -91,18 +91,11 @@ struct dgpu_handle {
   void* HinZD[MAXF] = {};
   void* PgD[3 * MAXF] = {};
   void* Hout[MAXO] = {};
-  // the x pass in two kernels (inverse x transforms, then products reading the
-  // physical fields) or fused (each block inverse-transforms its three planes);
-  // DNS handles keep the physical fields in Px (T), LES handles use Pg
-  int xsplit = 1;
+  // the x pass runs in two kernels (inverse x transforms, then the products
+  // reading the physical fields): DNS handles keep the physical fields in Px (T),
+  // LES handles use Pg
   // persistent kernels: blocks to launch (a few per CU)
   int resident_blocks = 1024;
-  // k_fwdz_nl launch variant (DGPU_FWDZ: 0 columns per block TI, 1 TI/2,
-  // 2 TI persistent, 3 TI/2 persistent; -1 the measured default per precision)
-  int fwdz_variant = -1;
-  // k_cn column tile (DGPU_CN_TC = 8, 4, 2 or 1; 0: the widest that fits)
-  int cn_tc = 0;
-  int pj_tc = 0;  // k_project columns per block (0: the widest that fits)
   void* Px[MAXF] = {};
   void *twx64 = nullptr, *twz64 = nullptr, *twx32 = nullptr, *twz32 = nullptr;
   double* d_fac = nullptr;
@@ -534,18 +527,18 @@ constexpr int xprod_nt(int nx, size_t lds) {
   return nx <= 32 ? 64 : (nx == 64 ? 128 : (lds > 81920 ? (nx >= 1024 ? 1024 : 512) : 256));
 }
 
-template <typename T, int NTH, bool LES = false, bool SPLIT = false>
+template <typename T, int NTH, bool LES = false>
 int launch_xprod(dgpu_handle* h, XProdArgs& a, int nj) {
   a.tw = sizeof(T) == 8 ? h->twx64 : h->twx32;
   DG_SWITCH_N(h->g.nx, {
-    // input (or product) lines + twiddle table
-    constexpr size_t lds = sizeof(cx<T>) * ((SPLIT ? 8 + 3 * NTH : 3 * (3 + NTH)) + 1) * N_;
+    // product lines + twiddle table
+    constexpr size_t lds = sizeof(cx<T>) * ((8 + 3 * NTH) + 1) * N_;
     if constexpr (lds > 163840) {
       return fail(2, "this nx needs the mixed-precision mode (x pass exceeds LDS in FP64)");
     } else {
       constexpr int NT = xprod_nt(N_, lds);
       dim3 grid(h->g.nz / 2, nj, 1);
-      k_xprod<T, N_, NT, NTH, LES, SPLIT><<<grid, NT, 0, h->st>>>(a);
+      k_xprod<T, N_, NT, NTH, LES><<<grid, NT, 0, h->st>>>(a);
     }
   });
   HIPCHK(hipGetLastError());
@@ -570,13 +563,8 @@ int launch_fwdz_nl(dgpu_handle* h, NlArgs& a, int nj) {
     using P1 = std::true_type;
     // measured: FP64 best persistent with TI columns; float with TI/2 per block at
     // nz = 256 (256^3), persistent with TI columns at nz = 512 (512x257x512)
-    const int v = h->fwdz_variant >= 0 ? h->fwdz_variant : ((sizeof(T) == 8 || h->g.nz >= 512) ? 2 : 1);
-    switch (v) {
-      case 1: go(I1{}, P0{}); break;
-      case 2: go(I0{}, P1{}); break;
-      case 3: go(I1{}, P1{}); break;
-      default: go(I0{}, P0{}); break;
-    }
+    if (sizeof(T) == 8 || h->g.nz >= 512) go(I0{}, P1{});
+    else go(I1{}, P0{});
   });
   HIPCHK(hipGetLastError());
   return 0;
@@ -594,9 +582,8 @@ ColShape col_shape(const dgpu_handle* h, bool cn) {
   if (cn) {
     // k_cn: TC columns x all fields in LDS; one wave solves one field of the TC
     // columns in 64/TC chunks of L <= 32 rows (LMAX = 4 RPT >= L); the widest TC
-    // that fits, or h->cn_tc when it fits
-    for (int tc : {h->cn_tc, 8, 4, 2, 1}) {
-      if (tc <= 0) continue;
+    // that fits
+    for (int tc : {8, 4, 2, 1}) {
       const size_t lds = (size_t)tc * ny2 * 16 * nf;
       const int rg = 256 / tc, need = (ny2 + rg - 1) / rg;
       // exact rows per thread for 8 columns (the chunk bound LMAX = 4 RPT then
@@ -610,11 +597,10 @@ ColShape col_shape(const dgpu_handle* h, bool cn) {
   }
   // k_project: 4 columns per block where that still gives two blocks per CU, else
   // 2 (measured on MI355X: at 256x129x256, 4 columns 0.080 ms, 2 0.093, 8 0.106;
-  // at 64x65x64, 946 columns, 2 columns 0.011 ms, 4 0.013); h->pj_tc overrides
+  // at 64x65x64, 946 columns, 2 columns 0.011 ms, 4 0.013)
   const long long want = h->resident_blocks > 0 ? h->resident_blocks : 512;
   const int tc0 = h->g.pc / 4 >= want ? 4 : 2;
-  for (int tc : {h->pj_tc, tc0, 2}) {
-    if (tc <= 0) continue;
+  for (int tc : {tc0, 2}) {
     const size_t lds = (size_t)tc * ny2 * 40;
     if (lds <= 80 * 1024 || (tc == 2 && lds <= 160 * 1024)) {
       const int rg = 256 / tc, need = (ny2 + rg - 1) / rg;
@@ -1031,8 +1017,7 @@ int substep_les(dgpu_handle* h, int rk, double delta_t, double time, int time_st
     if (!xa.phys[f]) return fail(12, "x pass: physical field buffer missing");
   }
   if ((rc = tbeg(h, 1))) return rc;
-  if ((rc = h->xsplit ? launch_xprod<T, NTH, true, true>(h, xa, nj) : launch_xprod<T, NTH, true>(h, xa, nj)))
-    return rc;
+  if ((rc = launch_xprod<T, NTH, true>(h, xa, nj))) return rc;
   if ((rc = tend(h))) return rc;
   NlArgs na{};
   na.g = g; na.jlo = jlo; na.jstart = c.jstart; na.jend = c.jend;
@@ -1193,21 +1178,17 @@ int substep(dgpu_handle* h, int rk, double delta_t, double time, int time_step, 
   xa.dy = h->d_dy; xa.dyf = h->d_dyf; xa.kx = h->d_kx;
   for (int f = 0; f < nf; ++f) xa.hin[f] = h->Hin[f];
   for (int o = 0; o < 6 + 2 * h->nth; ++o) xa.hout[o] = h->Hout[o];
-  if (h->xsplit) {
-    // the physical fields once per plane, then the products read them
-    for (int f = 0; f < nf; ++f)
-      if (!h->Px[f]) return fail(12, "x pass: physical field buffer missing");
-    if ((rc = tbeg(h, 11))) return rc;
-    rc = mixed ? launch_inv_x<float>(h, nf, h->Hin, (float* const*)h->Px, 0, g.ny2)
-               : launch_inv_x<double>(h, nf, h->Hin, (double* const*)h->Px, 0, g.ny2);
-    if (rc || (rc = tend(h))) return rc;
-    for (int f = 0; f < nf; ++f) xa.phys[f] = h->Px[f];
-  }
+  // the physical fields once per plane, then the products read them
+  for (int f = 0; f < nf; ++f)
+    if (!h->Px[f]) return fail(12, "x pass: physical field buffer missing");
+  if ((rc = tbeg(h, 11))) return rc;
+  rc = mixed ? launch_inv_x<float>(h, nf, h->Hin, (float* const*)h->Px, 0, g.ny2)
+             : launch_inv_x<double>(h, nf, h->Hin, (double* const*)h->Px, 0, g.ny2);
+  if (rc || (rc = tend(h))) return rc;
+  for (int f = 0; f < nf; ++f) xa.phys[f] = h->Px[f];
   if ((rc = tbeg(h, 1))) return rc;
   rc = by_nth(h->nth, [&](auto NTHc) {
     constexpr int NTH = decltype(NTHc)::value;
-    if (h->xsplit)
-      return mixed ? launch_xprod<float, NTH, false, true>(h, xa, nj) : launch_xprod<double, NTH, false, true>(h, xa, nj);
     return mixed ? launch_xprod<float, NTH>(h, xa, nj) : launch_xprod<double, NTH>(h, xa, nj);
   });
   if (rc || (rc = tend(h))) return rc;
@@ -1309,18 +1290,7 @@ int dgpu_create(const dgpu_config* cfg, dgpu_handle** out) {
         hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && ncu > 0)
       h->resident_blocks = 2 * ncu;
   }
-  if (const char* fv = std::getenv("DGPU_FWDZ")) h->fwdz_variant = std::atoi(fv);
-  if (const char* ct = std::getenv("DGPU_CN_TC")) {
-    const int v = std::atoi(ct);
-    h->cn_tc = (v == 8 || v == 4 || v == 2 || v == 1) ? v : 0;
-  }
-  if (const char* pt = std::getenv("DGPU_PJ_TC")) {
-    const int v = std::atoi(pt);
-    h->pj_tc = (v == 8 || v == 4 || v == 2) ? v : 0;
-  }
-  // DGPU_XPASS=fused selects the single-kernel x pass (measurement of the two forms)
-  if (const char* xp = std::getenv("DGPU_XPASS")) h->xsplit = std::strcmp(xp, "fused") != 0;
-  if (h->xsplit && !cfg->use_les) {
+  if (!cfg->use_les) {
     const size_t pbytes = (h->prec == DGPU_MIXED ? sizeof(float) : sizeof(double)) * g.pp * n2;
     for (int f = 0; f < nf; ++f) ALLOC(h->Px[f], pbytes);
   }
@@ -1630,6 +1600,30 @@ int dgpu_set_variable_dt(dgpu_handle* h, int32_t variable_dt, int32_t update_dt,
   h->update_dt = update_dt < 1 ? 1 : update_dt;
   h->cfl = cfl;
   h->cfl_ran = 0;
+  return 0;
+}
+
+int dgpu_courant_physical(dgpu_handle* h, const double* u1, const double* u2, const double* u3) {
+  if (!h || !u1 || !u2 || !u3) return fail(1, "null argument");
+  if (!h->variable_dt) return 0;
+  const Geo& g = h->g;
+  int rc = ensure_grad_buffers(h);
+  if (rc) return rc;
+  // physical Fortran layout u(0:Nx+1, 0:Nzp+1, 0:Nyp+1) -> [j][z][x] in the
+  // gradient stage's FP64 buffers
+  std::vector<double> tmp((size_t)g.pp * g.ny2);
+  const double* src[3] = {u1, u2, u3};
+  double* dst[3] = {(double*)h->PgD[0], (double*)h->PgD[3], (double*)h->PgD[6]};
+  for (int f = 0; f < 3; ++f) {
+    for (int j = 0; j < g.ny2; ++j)
+      for (int z = 0; z < g.nz; ++z)
+        std::memcpy(&tmp[((size_t)j * g.nz + z) * g.nx], src[f] + ((size_t)j * (g.nz + 2) + z) * (g.nx + 2),
+                    sizeof(double) * g.nx);
+    HIPCHK(hipMemcpyAsync(dst[f], tmp.data(), tmp.size() * 8, hipMemcpyHostToDevice, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));  // tmp is refilled for the next field
+  }
+  if ((rc = launch_courant(h, dst[0], dst[1], dst[2]))) return rc;
+  HIPCHK(hipStreamSynchronize(h->st));
   return 0;
 }
 

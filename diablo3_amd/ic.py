@@ -135,10 +135,12 @@ def create_flow_physical(cfg: Config, gyf: np.ndarray, rng: np.random.Generator)
         raise ValueError("unsupported IC_type in create_flow")
     if cfg.physical_noise:
         rng.random(3)  # the three discarded draws of for/flow.f90:409-411
-        r = rng.random((cfg.ny2, nz, nx, 3))
-        u1 += cfg.kick * (r[..., 0] - 0.5)
-        u2 += cfg.kick * (r[..., 1] - 0.5)
-        u3 += cfg.kick * (r[..., 2] - 0.5)
+        # draws in the loop order j, z, x, (u1, u2, u3), one plane at a time
+        for j in range(cfg.ny2):
+            r = rng.random((nz, nx, 3))
+            u1[j] += cfg.kick * (r[..., 0] - 0.5)
+            u2[j] += cfg.kick * (r[..., 1] - 0.5)
+            u3[j] += cfg.kick * (r[..., 2] - 0.5)
     return u1, u2, u3
 
 

@@ -63,6 +63,7 @@ def load():
     L.dgpu_set_variable_dt.argtypes = [P, ctypes.c_int32, ctypes.c_int32, ctypes.c_double]
     L.dgpu_courant.argtypes = [P, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                D, ctypes.POINTER(ctypes.c_int32)]
+    L.dgpu_courant_physical.argtypes = [P, D, D, D]
     L.dgpu_stats_doubles.argtypes = [P]
     L.dgpu_stats_doubles.restype = ctypes.c_int64
     L.dgpu_stats_zstar.argtypes = [P, ctypes.c_double, D]

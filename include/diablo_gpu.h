@@ -185,6 +185,13 @@ int dgpu_timing_read(dgpu_handle *h, int32_t kernel, double *total_ms, int64_t *
 int dgpu_set_variable_dt(dgpu_handle *h, int32_t variable_dt, int32_t update_dt, double cfl);
 int dgpu_courant(dgpu_handle *h, double time, double time_limit, double save_stats_time,
                  double save_flow_time, double *delta_t, int32_t *ran);
+/* courant on a physical velocity the caller holds (FP64, the physical Fortran
+ * layout u(0:Nx+1, 0:Nzp+1, 0:Nyp+1), u2 at GY points), as ReadHDF5 calls it on
+ * the velocity it has just read (for/phdf5.f90:1437-1439): with variable_dt set
+ * the CFL reduction runs on the device and the next dgpu_courant returns the
+ * delta_t (ran = 1); without variable_dt it does nothing.  dgpu_read_flow
+ * (dgpu_io.h) calls it. */
+int dgpu_courant_physical(dgpu_handle *h, const double *u1, const double *u2, const double *u3);
 
 /* Energy-budget statistics of save_stats_chan (for/diagnostics.f90:3-669, with
  * compute_averages, compute_TKE_diss, compute_MKE_diss, compute_TKE and

@@ -59,6 +59,8 @@ def lib():
         L.oracle_cn_matrix.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_double, D, D, D, D]
         L.oracle_cn_matrix_nut.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_double, D, D, D, D, D]
         L.oracle_debug_rhs.argtypes = [P, ctypes.c_int, D]
+        L.oracle_les_capture.argtypes = [P, D]
+        L.oracle_les_override.argtypes = [P, D, D]
         L.oracle_thomas_real.argtypes = [ctypes.c_int, D, D, D, D]
         L.oracle_thomas_complex.argtypes = [ctypes.c_int, D, D, D, D]
         _lib = L
@@ -195,6 +197,25 @@ class Oracle:
         self._check(lib().oracle_cn_matrix_nut(self.h, which, rk_step, dt, _dp(nut), _dp(a), _dp(b),
                                                _dp(c), _dp(v)))
         return a, b, c, v
+
+    def les_capture(self, on=True):
+        """Capture array [ny2][nz][nx][16] the next LES substeps fill (see diablo_oracle.h)."""
+        if not on:
+            self._check(lib().oracle_les_capture(self.h, None))
+            self._cap = None
+            return None
+        self._cap = np.zeros((self.cfg.ny2, self.cfg.nz, self.cfg.nx, 16), np.float64)
+        self._check(lib().oracle_les_capture(self.h, _dp(self._cap)))
+        return self._cap
+
+    def les_override(self, cgy, kgy):
+        """GY coefficients [ny2][nz][nx] used as les_model_type 6 uses them (None: off)."""
+        if cgy is None:
+            self._ov = None
+            self._check(lib().oracle_les_override(self.h, None, None))
+            return
+        self._ov = (np.ascontiguousarray(cgy, np.float64), np.ascontiguousarray(kgy, np.float64))
+        self._check(lib().oracle_les_override(self.h, _dp(self._ov[0]), _dp(self._ov[1])))
 
     def debug_rhs(self, which):
         """The physical CN right-hand side of field `which` (0 u1, 1 u2, 2 u3, 3+n scalar n)

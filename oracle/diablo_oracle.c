@@ -127,7 +127,16 @@ struct oracle {
   double *les_p;   /* NLESP physical arrays, see les_phys() */
   zc *les_c;       /* one Fourier scratch array */
   fftplan px, pz;
+  /* test hooks of the learned closure (les_model_type 6), caller-owned arrays:
+   * cap [ny2][nz][nx][CAP_N] receives the type-5 GY quantities of the next LES
+   * substep; ov_cgy, ov_kgy [ny2][nz][nx] replace the GY coefficients */
+  double *cap;
+  const double *ov_cgy, *ov_kgy;
 };
+/* capture slots per GY point: the scaled strain and rotation entries a1..a6,
+ * o4..o6 the GY coefficient is formed from, that coefficient, the scaled scalar
+ * gradient ax, ay, az, the unscaled th_y and the scalar coefficient */
+enum { CAP_A1 = 0, CAP_O4 = 6, CAP_V = 9, CAP_BX = 10, CAP_TY = 13, CAP_VT = 14, CAP_N = 16 };
 /* physical LES arrays: Sij1..6, SIij1..6, Oij4..6, du1dx du2dy du3dz du2dx du3dx
  * du1dz du2dz du1dy du3dy, temp, temp_th, s1_th, then dthetadx/dy/dz per scalar */
 enum { L_S1 = 0, L_SI1 = 6, L_O4 = 12, L_DU1DX = 15, L_DU2DY, L_DU3DZ, L_DU2DX, L_DU3DX,
@@ -1261,7 +1270,24 @@ static void les_amd(oracle_t *o, double time) {
           v = v / (a1 * a1 + a2 * a2 + a3 * a3 + 2.0 * (a4 * a4) + 2.0 * (o4 * o4) + 2.0 * (a5 * a5) +
                    2.0 * (o5 * o5) + 2.0 * (a6 * a6) + 2.0 * (o6 * o6));
         temp[p] = v;
+        if (o->cap) {
+          double *cp = o->cap + p * CAP_N;
+          cp[CAP_A1] = a1; cp[CAP_A1 + 1] = a2; cp[CAP_A1 + 2] = a3;
+          cp[CAP_A1 + 3] = a4; cp[CAP_A1 + 4] = a5; cp[CAP_A1 + 5] = a6;
+          cp[CAP_O4] = o4; cp[CAP_O4 + 1] = o5; cp[CAP_O4 + 2] = o6;
+          cp[CAP_V] = v;
+        }
       }
+  }
+  if (o->ov_cgy) {
+    /* les_model_type 6: the GY coefficient |c|; at GYF the mean of the two GY
+     * neighbours when neither carries the fallback sign, else the type-5 value */
+    const double *cg = o->ov_cgy;
+    LOOP_PX(js, je) {
+      const size_t p = PI(j, z, x), q = PI(j + 1, z, x);
+      if (!signbit(cg[p]) && !signbit(cg[q])) s1[p] = 0.5 * (cg[p] + cg[q]);
+    }
+    LOOP_PX(2, ny) temp[PI(j, z, x)] = fabs(cg[PI(j, z, x)]);
   }
   /* coefficient times S_ij (for/les.f90:745-773); temp(Nyp+1) is never set by
    * the reference (a stale value that feeds only planes nothing reads): 0 here */
@@ -1343,7 +1369,21 @@ static void les_amd(oracle_t *o, double time) {
             v = v / (ax * ax + ay * ay + az * az);
           }
           tth[p] = v;
+          if (o->cap && n == 0) {
+            double *cp = o->cap + p * CAP_N;
+            cp[CAP_BX] = dxs * S2 / den; cp[CAP_BX + 1] = dyl * ty[p]; cp[CAP_BX + 2] = dzs * S5 / den;
+            cp[CAP_TY] = ty[p];
+            cp[CAP_VT] = v;
+          }
         }
+    }
+    if (o->ov_kgy) {
+      const double *kg = o->ov_kgy;
+      LOOP_PX(jst, jet) {
+        const size_t p = PI(j, z, x), q = PI(j + 1, z, x);
+        if (!signbit(kg[p]) && !signbit(kg[q])) s1th[p] = 0.5 * (kg[p] + kg[q]);
+      }
+      LOOP_PX(2, ny) tth[PI(j, z, x)] = fabs(kg[PI(j, z, x)]);
     }
     LOOP_PX(jst, jet) {
       const size_t p = PI(j, z, x);
@@ -1372,6 +1412,19 @@ static void les_amd(oracle_t *o, double time) {
 }
 #undef LOOP_PX
 #undef LOOP_FX
+
+int oracle_les_capture(oracle_t *o, double *cap) {
+  if (!o->c.use_les || o->c.les_model_type != 5) { set_err("capture needs an les_model_type 5 handle"); return 1; }
+  o->cap = cap;
+  return 0;
+}
+
+int oracle_les_override(oracle_t *o, const double *cgy, const double *kgy) {
+  if (!o->c.use_les || o->c.les_model_type != 5) { set_err("override needs an les_model_type 5 handle"); return 1; }
+  o->ov_cgy = cgy;
+  o->ov_kgy = kgy;
+  return 0;
+}
 
 /* apply_BC_vel_mpi_post + apply_BC_th_mpi_post on the Fourier state, as
  * save_stats_chan starts (for/diagnostics.f90:26-27) */

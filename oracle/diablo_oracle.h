@@ -71,6 +71,19 @@ int oracle_cn_matrix_nut(oracle_t *o, int which, int rk_step, double delta_t, co
 /* The physical CN right-hand side of field `which` left by the last substep
  * (physical Fortran layout), so tests can pin the explicit half on its own. */
 int oracle_debug_rhs(oracle_t *o, int which, double *out);
+/* Test hooks of the learned closure (les_model_type 6, no counterpart in the
+ * reference), on an les_model_type 5 handle.  oracle_les_capture: the next LES
+ * substeps write, per GY point of rows 2..Nyp, 16 doubles into cap
+ * ([ny2][nz][nx][16], compact physical order): the scaled strain and rotation
+ * entries a1..a6, o4..o6 of the type-5 GY coefficient, that coefficient, the
+ * scaled scalar gradient (dx b, 2 dy, dz b) x grad th, th_y and the scalar
+ * coefficient (slots 0-8, 9, 10-12, 13, 14).  oracle_les_override: the GY
+ * velocity and scalar coefficients cgy, kgy ([ny2][nz][nx]) replace the type-5
+ * ones as les_model_type 6 uses them: |c| at GY; at GYF the mean of the two GY
+ * neighbours when neither has its sign bit set, else the type-5 GYF value.
+ * NULL switches a hook off; the arrays stay the caller's. */
+int oracle_les_capture(oracle_t *o, double *cap);
+int oracle_les_override(oracle_t *o, const double *cgy, const double *kgy);
 /* Thomas solves exactly as thomas_forward/backward_*_mpi at one rank. */
 void oracle_thomas_real(int ny, double *a, double *b, const double *c, double *g);
 void oracle_thomas_complex(int ny, double *a, double *b, const double *c, double *g);

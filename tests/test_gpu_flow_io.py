@@ -88,6 +88,46 @@ def test_checkpoint_write_read(tmp_path):
     g2.close()
 
 
+def test_read_flow_with_variable_dt_runs_courant(tmp_path):
+    """ReadHDF5 calls courant on the velocity it has read when variable_dt is set
+    (for/phdf5.f90:1437-1439): after dgpu_read_flow the next dgpu_courant returns
+    courant's delta_t for that velocity, as the oracle's courant of the state read
+    (limits, caps and event scheduling) gives it; without variable_dt it has not run."""
+    from diablo3_amd import flow_io
+    cfg = configs.small_dns(32, 33, 32)
+    p = Problem(cfg)
+    g = Solver(p)
+    s0 = p.initial_state(seed=4, theta_noise=1e-3)
+    for a in (s0.cu1, s0.cu2, s0.cu3):
+        a *= 500.0  # velocities large enough that the CFL limit is the smallest
+    g.upload_state(s0)
+    g.rem_div()
+    g.step(time=0.0, time_step=1, create_new_flow=True)
+    path = str(tmp_path / "restart.h5")
+    times = {"time": 0.5, "save_flow_time": 0.75, "save_stats_time": 0.6, "save_movie_time": 0.25,
+             "time_step": 1}
+    print("write_flow, then read_flow with and without variable_dt", flush=True)
+    flow_io.write_flow(g, path, times, save_pressure=True)
+    g.close()
+    g0 = Solver(p)
+    flow_io.read_flow(g0, path)
+    assert g0.courant(0.5, 10.0, 0.6, 0.75) == (0.0, False)
+    g0.close()
+    g2 = Solver(p)
+    cfl = 0.5
+    g2.set_variable_dt(True, 1, cfl)
+    t, _ = flow_io.read_flow(g2, path)
+    dt, ran = g2.courant(t["time"], 10.0, t["save_stats_time"], t["save_flow_time"])
+    o = oracle.Oracle(p, threads=16)
+    o.upload_state(g2.download_state())
+    lim = o.courant_dt_state(cfl)
+    want = oracle.lib().oracle_courant_schedule(lim, t["time"], 10.0, t["save_stats_time"], t["save_flow_time"])
+    print(f"courant after read: device dt {dt:.15e}, oracle {want:.15e} (limit {lim:.6e})", flush=True)
+    assert ran and abs(dt - want) <= 1e-12 * want
+    assert lim < 0.5 * (0.5 * (min(cfg.lx / cfg.nx, cfg.lz / cfg.nz) ** 2) / cfg.nu)  # the CFL limit is active
+    g2.close()
+
+
 def test_save_stats_mean_h5(tmp_path):
     """save_stats_chan's mean.h5 sample from the device statistics: the groups
     hold the device profiles under the reference's names (a second evaluation of

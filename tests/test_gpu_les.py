@@ -1,10 +1,10 @@
 """Parity of the HIP LES branch (les_model_type 1 constant Smagorinsky, 4 and 5
 the two AMD forms) with the CPU oracle, through the C ABI.
 
-Bars: after 3 RKW3 steps past the spin-up gate, u1, u2, u3 and theta match the
-oracle to relative L2 <= 1e-11; nu_t and kappa_t of the last substep to 1e-11;
-cp to max(1e-11, 30 x the oracle's own round-off disagreement on cp), for the
-reason the DNS parity tests give (cp = phi / h_bar amplifies round-off).
+Bars: after 3 RKW3 steps past the spin-up gate, u1, u2, u3, theta and cp match
+the oracle to relative L2 <= 1e-11; nu_t and kappa_t of the last substep to
+1e-11.  The oracle's own round-off disagreement on cp (cp = phi / h_bar amplifies
+round-off) is printed beside the GPU's error.
 """
 import numpy as np
 import pytest
@@ -78,10 +78,8 @@ def test_les_steps_match_oracle(nx, ny, nz, model):
         t += p.cfg.delta_t
     o = run_oracle(p, s0, nsteps)
     fg, fo = fields(g.download_state()), fields(o.download_state())
-    base = fo
-    fa = fields(run_oracle(p, s0, nsteps, perturb=True).download_state())
     fb = fields(run_oracle(p, s0, nsteps, variant=1).download_state())
-    cp_floor = max(rel(fa["cp"], base["cp"]), rel(fb["cp"], base["cp"]))
+    cp_floor = rel(fb["cp"], fo["cp"])
     errs = {k: rel(fg[k], fo[k]) for k in fo}
     (ntg, ktg), (nto, kto) = g.download_les(), o.download_les()
     errs["nu_t"] = rel(ntg, nto)
@@ -90,8 +88,7 @@ def test_les_steps_match_oracle(nx, ny, nz, model):
           f"nu_t max {nto.max():.3e}, active {(nto > 0).mean():.2f}", flush=True)
     assert nto.max() > 0 and (nto > 0).mean() > 0.1 and (kto[0] > 0).mean() > 0.05
     for k, e in errs.items():
-        bar = max(1e-11, 30 * cp_floor) if k == "cp" else 1e-11
-        assert e <= bar, (k, e, bar)
+        assert e <= 1e-11, (k, e)
 
 
 def test_les_substep_rk_memory():

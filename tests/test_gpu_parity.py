@@ -1,18 +1,18 @@
 """Parity of the HIP step with the CPU oracle, through the C ABI.
 
-Bars (written in each test): u1, u2, u3 and theta match the oracle to relative
-L2 <= 1e-11 after 10 RKW3 steps; transforms and the projection to ~1e-14; the
-closed-form RKW3 cases to 1e-14; the mixed-precision path within the error a
-float32 transform carries (reported, bounded at 1e-4).
+Bars (written in each test): u1, u2, u3, theta and the pressure cp match the
+oracle to relative L2 <= 1e-11 after 10 RKW3 steps (64x65x64 and 256x129x256);
+transforms and the projection to ~1e-14; the closed-form RKW3 cases to 1e-14; the
+energy-budget profiles of the GPU trajectory within 1e-10 of those of the oracle's
+own trajectory; the mixed-precision path reported per field against the FP64
+oracle and bounded at the error a float32 transform carries.
 
 The pressure cp is recovered as phi / h_bar (for/solver.f90:1261-1267), so the
 O(1e-16 |u|) round-off of the projected velocity's divergence reaches cp
-amplified by 1/h_bar (7500 at dt = 1e-3, rk 2).  The oracle disagrees with itself
-on cp by ~1e-12 (64^3, 10 steps) under either of two legal perturbations: one
-inverse + forward transform of its initial state, or back substitution by
-reciprocal multiplication instead of division (``noise_floor``), while u, v, w,
-theta move by ~1e-15.  cp is therefore held to max(1e-11, 30 x that measured
-self-disagreement), computed in the same test; u1, u2, u3, theta keep 1e-11.
+amplified by 1/h_bar (7500 at dt = 1e-3, rk 2).  Each test prints the oracle's
+disagreement with itself on cp under a legal round-off perturbation
+(``noise_floor``: back substitution by reciprocal multiplication, or one transform
+round trip of the initial state) beside the GPU's error; the bar stays fixed.
 """
 import numpy as np
 import pytest
@@ -35,26 +35,26 @@ def fields(s):
     return {"cu1": s.cu1, "cu2": s.cu2, "cu3": s.cu3, "cp": s.cp, **{f"cth{n}": t for n, t in enumerate(s.cth)}}
 
 
-def compare(sg, so, tol, what="", floor=None):
-    """Relative L2 per field; cp is held to max(tol, 10 x floor['cp']) when a floor is given."""
+def compare(sg, so, tol, what="", floor=None, bars=None):
+    """Relative L2 per field against a fixed bar (tol, or bars[field]); the oracle's
+    own round-off disagreement (floor) is printed beside it, not used as a bar."""
     fg, fo = fields(sg), fields(so)
     errs = {k: rel(fg[k], fo[k]) for k in fo}
-    bars = {k: tol for k in fo}
-    if floor is not None:
-        bars["cp"] = max(tol, 30.0 * floor["cp"])
+    bar = {k: (bars or {}).get(k, tol) for k in fo}
     print(what, {k: f"{v:.2e}" for k, v in errs.items()},
-          "" if floor is None else {"cp_floor": f"{floor['cp']:.2e}", "cp_bar": f"{bars['cp']:.2e}"},
+          "" if floor is None else {"oracle_self_floor": {k: f"{v:.1e}" for k, v in floor.items()}},
           flush=True)
     for k, e in errs.items():
-        assert e <= bars[k], (what, k, e, bars[k])
+        assert e <= bar[k], (what, k, e, bar[k])
     return errs
 
 
-def noise_floor(p, s0, substeps):
+def noise_floor(p, s0, substeps, round_trip=True):
     """The oracle's disagreement with itself: per field, the larger relative change
-    after `substeps` under (a) one fft_xz_to_physical + fft_xz_to_fourier of every
-    field of the initial state, (b) back substitution by reciprocal multiplication
-    instead of division.  Both are legal round-off perturbations of the algorithm."""
+    after `substeps` under (a) back substitution by reciprocal multiplication
+    instead of division, (b) (round_trip) one fft_xz_to_physical +
+    fft_xz_to_fourier of every field of the initial state.  Both are legal
+    round-off perturbations of the algorithm."""
     dt = p.cfg.delta_t
 
     def run(state, variant):
@@ -68,13 +68,44 @@ def noise_floor(p, s0, substeps):
         finally:
             oracle.lib().oracle_set_variant(0)
 
-    ot = oracle.Oracle(p, threads=16)
-    s1 = s0.copy()
-    for name in ("cu1", "cu2", "cu3"):
-        setattr(s1, name, ot.fft_to_fourier(ot.fft_to_physical(getattr(s1, name))))
-    s1.cth = [ot.fft_to_fourier(ot.fft_to_physical(t)) for t in s1.cth]
-    base, fa, fb = run(s0, 0), run(s1, 0), run(s0, 1)
-    return {k: max(rel(fa[k], base[k]), rel(fb[k], base[k])) for k in base}
+    base, fb = run(s0, 0), run(s0, 1)
+    out = {k: rel(fb[k], base[k]) for k in base}
+    if round_trip:
+        ot = oracle.Oracle(p, threads=16)
+        s1 = s0.copy()
+        for name in ("cu1", "cu2", "cu3"):
+            setattr(s1, name, ot.fft_to_fourier(ot.fft_to_physical(getattr(s1, name))))
+        s1.cth = [ot.fft_to_fourier(ot.fft_to_physical(t)) for t in s1.cth]
+        fa = run(s1, 0)
+        out = {k: max(out[k], rel(fa[k], base[k])) for k in base}
+    return out
+
+
+def divergence(p, s):
+    """ikx u1 + (u2(j+1) - u2(j))/dyf(j) + ikz u3 on rows 1..Nyp."""
+    cfg = p.cfg
+    kx, kz, dyf, ny = p.kx[:cfg.kxr], p.kz, p.met["dyf"], cfg.ny
+    return (1j * kx[None, None, :] * s.cu1[1:ny + 1]
+            + (s.cu2[2:ny + 2] - s.cu2[1:ny + 1]) / dyf[1:ny + 1, None, None]
+            + 1j * kz[None, :, None] * s.cu3[1:ny + 1])
+
+
+def compare_budget(g, o, what, bar=1e-10):
+    """save_stats_chan's energy-budget profiles: the device statistics of the GPU
+    state against the oracle's restatement on the oracle's own state."""
+    from oracle import diagnostics
+    sg = g.stats(time=0.0)
+    so = diagnostics.save_stats(o, time=0.0)
+    worst, wname = 0.0, ""
+    for name in diagnostics.names(g.cfg.n_th):
+        a, b = sg[name], so[name]
+        scale = np.abs(b).max()
+        err = np.abs(a - b).max() / (scale if scale > 0 else 1.0)
+        if err > worst:
+            worst, wname = err, name
+        assert err <= bar, (what, name, err)
+    print(what, f"energy budget: worst profile {wname} rel err {worst:.2e} (bar {bar:.0e})", flush=True)
+    return worst
 
 
 @pytest.mark.parametrize("nx,ny,nz", [(16, 17, 16), (64, 33, 32), (32, 9, 128), (256, 5, 64), (512, 5, 16), (16, 5, 512), (1024, 5, 32), (32, 5, 1024)])
@@ -118,7 +149,7 @@ def test_each_substep_matches_oracle(nx, ny, nz):
         g.rk_substep(rk, time_step=1)
         o.rk_substep(rk, time_step=1)
         fl = noise_floor(p, s0, [(1, r) for r in range(1, rk + 1)])
-        compare(g.download_state(), o.download_state(), 1e-12, f"rk{rk}", fl)
+        compare(g.download_state(), o.download_state(), 1e-12, f"rk{rk}", fl, bars={"cp": 1e-11})
         cg, co = g.download_rk(), o.download_rk()
         # RK memory on the rows each equation owns
         js, je, ny_ = cfg.jstart, cfg.jend, cfg.ny
@@ -127,43 +158,84 @@ def test_each_substep_matches_oracle(nx, ny, nz):
         assert rel(cg[0][2][js:je + 1], co[0][2][js:je + 1]) < 1e-12
 
 
-def _run_pair(cfg, steps, seed=0, precision=FP64):
-    p = Problem(cfg.copy(precision=precision))
+def _oracle_run(cfg, steps, seed=0):
     po = Problem(cfg.copy(precision=FP64))
-    g, o = Solver(p), oracle.Oracle(po, threads=16)
+    o = oracle.Oracle(po, threads=16)
     s0 = po.initial_state(seed=seed, theta_noise=1e-3)
     o.upload_state(s0)
     o.rem_div()
     s0 = o.download_state()
+    t = 0.0
+    for n in range(steps):
+        o.step(time=t, time_step=n + 1)
+        t += cfg.delta_t
+    return po, o, s0
+
+
+def _gpu_run(cfg, s0, steps, precision=FP64):
+    p = Problem(cfg.copy(precision=precision))
+    g = Solver(p)
     g.upload_state(s0)
     t = 0.0
     for n in range(steps):
         g.step(time=t, time_step=n + 1)
-        o.step(time=t, time_step=n + 1)
         t += cfg.delta_t
-    floor = noise_floor(po, s0, [(n + 1, rk) for n in range(steps) for rk in (1, 2, 3)])
-    return p, g, o, floor
+    return p, g
+
+
+def _run_pair(cfg, steps, seed=0, precision=FP64, floor=True):
+    po, o, s0 = _oracle_run(cfg, steps, seed)
+    p, g = _gpu_run(cfg, s0, steps, precision)
+    fl = noise_floor(po, s0, [(n + 1, rk) for n in range(steps) for rk in (1, 2, 3)]) if floor else None
+    return p, g, o, fl
 
 
 def test_ten_steps_dns_64_fp64():
-    """64x65x64 front DNS, 10 RKW3 steps: relative L2 <= 1e-11 on every field."""
+    """64x65x64 front DNS, 10 RKW3 steps: relative L2 <= 1e-11 on every field, cp
+    included; divergence at round-off; energy-budget profiles within 1e-10 of the
+    oracle's own trajectory's."""
     cfg = configs.dns_64()
     p, g, o, fl = _run_pair(cfg, 10)
     sg, so = g.download_state(), o.download_state()
     compare(sg, so, 1e-11, "dns_64 x10", fl)
     # discrete divergence at round-off on rows 2..Nyp-1
-    kx, kz, dyf, ny = p.kx[:cfg.kxr], p.kz, p.met["dyf"], cfg.ny
-    d = (1j * kx[None, None, :] * sg.cu1[1:ny + 1]
-         + (sg.cu2[2:ny + 2] - sg.cu2[1:ny + 1]) / dyf[1:ny + 1, None, None]
-         + 1j * kz[None, :, None] * sg.cu3[1:ny + 1])
-    assert np.abs(d[1:-1]).max() < 1e-13 * np.abs(sg.cu1).max() * kx.max()
+    d = divergence(p, sg)
+    assert np.abs(d[1:-1]).max() < 1e-13 * np.abs(sg.cu1).max() * p.kx.max()
+    compare_budget(g, o, "dns_64 x10")
 
 
-def test_two_steps_dns_256_fp64():
-    """256x129x256 front DNS, 2 steps against the (OpenMP) oracle."""
+def test_ten_steps_dns_256_fp64_and_mixed():
+    """256x129x256 front DNS, 10 RKW3 steps from one seeded state: the FP64 path
+    within 1e-11 of the oracle on every field, cp included, with the divergence at
+    round-off and the energy-budget profiles within 1e-10; the mixed path (FP32
+    transforms and products, FP64 solves, projection and mean mode) reported per
+    field against the same FP64 oracle state and bounded at 1e-6."""
     cfg = configs.dns_256()
-    p, g, o, fl = _run_pair(cfg, 2)
-    compare(g.download_state(), o.download_state(), 1e-11, "dns_256 x2", fl)
+    po, o, s0 = _oracle_run(cfg, 10)
+    so = o.download_state()
+    fl = noise_floor(po, s0, [(n + 1, rk) for n in range(10) for rk in (1, 2, 3)], round_trip=False)
+    p, g = _gpu_run(cfg, s0, 10, FP64)
+    sg = g.download_state()
+    compare(sg, so, 1e-11, "dns_256 fp64 x10", fl)
+    d = divergence(p, sg)
+    assert np.abs(d[1:-1]).max() < 1e-13 * np.abs(sg.cu1).max() * p.kx.max()
+    compare_budget(g, o, "dns_256 fp64 x10")
+    del g
+    pm, gm = _gpu_run(cfg, s0, 10, MIXED)
+    errs = compare(gm.download_state(), so, 1e-6, "dns_256 mixed x10 (error vs FP64 oracle)")
+    assert max(errs.values()) > 1e-13  # it really ran in float32
+    dm = divergence(pm, gm.download_state())
+    assert np.abs(dm[1:-1]).max() < 1e-13 * np.abs(so.cu1).max() * p.kx.max()
+
+
+@pytest.mark.parametrize("ny", [257, 385])
+def test_tall_columns(ny):
+    """128 x ny x 128 front DNS, 2 steps, FP64: the column kernels take the shapes
+    the 512x257x512 and 1024x385x1024 grids select (k_project 4 columns per block
+    with 5 and 7 rows per thread; k_cn 4 and 2 columns per block)."""
+    cfg = configs.small_dns(128, ny, 128)
+    p, g, o, _ = _run_pair(cfg, 2, seed=6, floor=False)
+    compare(g.download_state(), o.download_state(), 1e-11, f"128x{ny}x128 x2")
 
 
 @pytest.mark.parametrize("precision", [FP64, MIXED])
@@ -171,7 +243,7 @@ def test_wide_z_steps(precision):
     """16x17x512, 3 steps: the z transforms at nz = 512 take their widest column
     tiles (one 128-byte row segment per z row) in both precisions."""
     cfg = configs.small_dns(16, 17, 512)
-    p, g, o, fl = _run_pair(cfg, 3, seed=5, precision=precision)
+    p, g, o, fl = _run_pair(cfg, 3, seed=5, precision=precision, floor=precision == FP64)
     if precision == FP64:
         compare(g.download_state(), o.download_state(), 1e-11, "wide z x3", fl)
     else:
