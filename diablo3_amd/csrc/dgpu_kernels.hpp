@@ -918,6 +918,16 @@ __global__ __launch_bounds__(NT) void k_cn(ColArgs a) {
 // rem_div_chan (for/solver.f90:1302-1379, BC rows 2330-2371): Poisson rows 1..Nyp
 // by PCR with per-column coefficients, velocity correction, and with
 // update_p the pressure update of rk_chan_1 (for/solver.f90:1261-1267).
+//
+// Every row is kept with off-diagonals a, c >= 0 and diagonal b <= 0 (the bottom
+// BC row is negated), and stored by its row sum s = a + b + c instead of b: the
+// interior rows sum to exactly -(kx^2 + kz^2) and the BC rows to 0 (or -1 for
+// the Dirichlet row of the (0,0) mode).  A reduction step then forms
+//   s' = s + al s(j-s) + ga s(j+s),   b = s - a - c,
+// sums of terms of one sign (al, ga >= 0), where the usual update
+// b' = b + al c(j-s) + ga a(j+s) cancels almost all of b for the weakly dominant
+// rows of small kx^2 + kz^2 (on a 257-row stretched grid plain PCR lost 3-4
+// digits there against the Thomas sweep; this form stays within 10x of it).
 template <int TC, int NT, int RPT>
 __global__ __launch_bounds__(NT) void k_project(ColArgs a, int update_p) {
   constexpr int RG = NT / TC;
@@ -933,8 +943,8 @@ __global__ __launch_bounds__(NT) void k_project(ColArgs a, int update_p) {
   const long long pc = g.pc;
   cd* sD = (cd*)smem;
   double* sA = (double*)(sD + TC * ny2);
-  double* sB = sA + TC * ny2;
-  double* sC = sB + TC * ny2;
+  double* sS = sA + TC * ny2;  // row sums a + b + c
+  double* sC = sS + TC * ny2;
   cd* __restrict__ u1 = a.u[0];
   cd* __restrict__ u2 = a.u[1];
   cd* __restrict__ u3 = a.u[2];
@@ -944,15 +954,17 @@ __global__ __launch_bounds__(NT) void k_project(ColArgs a, int update_p) {
     const int j = rg + m * RG;
     if (j >= ny2) break;
     const int q = j * TC + cc;
-    double aj = 0.0, bj = 1.0, cj = 0.0;
+    double aj = 0.0, sj = 1.0, cj = 0.0;
     cd d = {0.0, 0.0};
     if (j == 1) {
-      cj = (c == 0) ? 0.0 : -1.0;
+      // (1, -1) Neumann, or (1) for phi(1) = 0 in the (0,0) mode; negated
+      cj = (c == 0) ? 0.0 : 1.0;
+      sj = (c == 0) ? -1.0 : 0.0;
     } else if (j == ny) {
-      aj = 1.0; bj = -1.0;
+      aj = 1.0; sj = 0.0;  // (1, -1)
     } else if (j > 1 && j < ny) {
       aj = 1.0 / (a.dy[j] * a.dyf[j]);
-      bj = k2 - 1.0 / (a.dy[j + 1] * a.dyf[j]) - 1.0 / (a.dy[j] * a.dyf[j]);
+      sj = k2;
       cj = 1.0 / (a.dy[j + 1] * a.dyf[j]);
       if (active) {
         const long long o = (long long)j * pc + c;
@@ -972,12 +984,12 @@ __global__ __launch_bounds__(NT) void k_project(ColArgs a, int update_p) {
         }
       }
     }
-    sA[q] = aj; sB[q] = bj; sC[q] = cj; sD[q] = d;
+    sA[q] = aj; sS[q] = sj; sC[q] = cj; sD[q] = d;
   }
   __syncthreads();
   for (int l = 0; l < a.plevels; ++l) {
     const int s = 1 << l;
-    double na[RPT], nb[RPT], nc[RPT];
+    double na[RPT], ns[RPT], nc[RPT];
     cd nd[RPT];
 #pragma unroll
     for (int m = 0; m < RPT; ++m) {
@@ -985,21 +997,23 @@ __global__ __launch_bounds__(NT) void k_project(ColArgs a, int update_p) {
       if (j >= 1 && j <= ny) {
         const int q = j * TC + cc, qm = q - s * TC, qp = q + s * TC;
         const int jm = j - s, jp = j + s;
-        double a2 = 0.0, c2 = 0.0, b2 = sB[q];
+        double a2 = 0.0, c2 = 0.0, s2 = sS[q];
         cd d2 = sD[q];
         if (jm >= 1) {
-          const double al = -sA[q] / sB[qm];
+          const double bm = sS[qm] - sA[qm] - sC[qm];
+          const double al = -sA[q] / bm;
           a2 = al * sA[qm];
-          b2 = b2 + al * sC[qm];
+          s2 = s2 + al * sS[qm];
           d2 = d2 + al * sD[qm];
         }
         if (jp <= ny) {
-          const double ga = -sC[q] / sB[qp];
+          const double bp = sS[qp] - sA[qp] - sC[qp];
+          const double ga = -sC[q] / bp;
           c2 = ga * sC[qp];
-          b2 = b2 + ga * sA[qp];
+          s2 = s2 + ga * sS[qp];
           d2 = d2 + ga * sD[qp];
         }
-        na[m] = a2; nb[m] = b2; nc[m] = c2; nd[m] = d2;
+        na[m] = a2; ns[m] = s2; nc[m] = c2; nd[m] = d2;
       }
     }
     __syncthreads();
@@ -1008,7 +1022,7 @@ __global__ __launch_bounds__(NT) void k_project(ColArgs a, int update_p) {
       const int j = rg + m * RG;
       if (j >= 1 && j <= ny) {
         const int q = j * TC + cc;
-        sA[q] = na[m]; sB[q] = nb[m]; sC[q] = nc[m]; sD[q] = nd[m];
+        sA[q] = na[m]; sS[q] = ns[m]; sC[q] = nc[m]; sD[q] = nd[m];
       }
     }
     __syncthreads();
@@ -1017,7 +1031,7 @@ __global__ __launch_bounds__(NT) void k_project(ColArgs a, int update_p) {
     const int j = rg + m * RG;
     if (j >= ny2) break;
     const int q = j * TC + cc;
-    sD[q] = (j >= 1 && j <= ny) ? (1.0 / sB[q]) * sD[q] : cd{0.0, 0.0};
+    sD[q] = (j >= 1 && j <= ny) ? (1.0 / (sS[q] - sA[q] - sC[q])) * sD[q] : cd{0.0, 0.0};
   }
   __syncthreads();
   if (!active) return;

@@ -1874,7 +1874,10 @@ int dgpu_step(dgpu_handle* h, double delta_t, double time, int32_t time_step, in
   }
   HIPCHK(hipEventRecord(h->e1, h->st));
   HIPCHK(hipStreamSynchronize(h->st));
-  return harvest(h);
+  const int rc = harvest(h);
+  // calls between steps (rem_div, poisson_p, filter_chan) always record
+  h->timing_now = true;
+  return rc;
 }
 
 int dgpu_rem_div(dgpu_handle* h) {
@@ -1995,6 +1998,9 @@ int dgpu_timing_enable(dgpu_handle* h, int32_t enable) {
   int rc = harvest(h);
   if (rc) return rc;
   h->timing = enable != 0;
+  h->timing_every = 1;
+  h->timing_calls = 0;
+  h->timing_now = true;
   for (int k = 0; k < NKERN; ++k) { h->t_ms[k] = 0.0; h->t_n[k] = 0; }
   return 0;
 }

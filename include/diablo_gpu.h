@@ -162,11 +162,9 @@ int dgpu_last_step_ms(dgpu_handle *h, double *ms);
  * LES substeps add 5 = BC posts, 6 = gradient fields, 7 = AMD model,
  * 8 = physical-column CN solves, 9 = right-hand-side and solution transforms,
  * 10 = the learned closure's coefficients (les_model_type 6, dgpu_closure.h),
- * 11 = the x inverse transforms of the state ahead of the products (DNS; the
- * environment variable DGPU_XPASS=fused at dgpu_create selects the single-kernel
- * x pass instead).
- * dgpu_timing_read returns the
- * device time and launch count accumulated since the last enable, and the name. */
+ * 11 = the x inverse transforms of the state ahead of the products.
+ * dgpu_timing_read returns the device time and launch count accumulated since the
+ * last enable, and the name.  Enabling resets the dgpu_step sampling to every call. */
 int dgpu_timing_enable(dgpu_handle *h, int32_t enable);
 /* Measurement only (no counterpart in the reference): with timing enabled, record
  * the kernel events on every `every`-th dgpu_step call only (1: every call, the

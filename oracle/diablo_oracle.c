@@ -41,7 +41,9 @@ typedef double complex zc;
 
 static __thread char g_err[512];
 /* Arithmetic variant switch for round-off sensitivity studies: bit 0 makes the
- * back substitutions multiply by 1/b instead of dividing by b.  0 = the reference. */
+ * back substitutions multiply by 1/b instead of dividing by b; bit 1 solves the
+ * projection's complex systems with their rows in reverse order (elimination from
+ * the top wall down).  0 = the reference. */
 static int g_variant = 0;
 void oracle_set_variant(int flags) { g_variant = flags; }
 static void set_err(const char *m) { snprintf(g_err, sizeof g_err, "%s", m); }
@@ -432,6 +434,18 @@ void oracle_thomas_real(int ny, double *a, double *b, const double *c, double *g
 
 /* thomas_forward/backward_complex_mpi (for/solver.f90:1764-1813, 1855-1896): rows 1..Nyp */
 static void thomas_complex(int ny, double *a, double *b, const double *c, zc *g) {
+  if (g_variant & 2) {
+    /* the same system, eliminated upward: row j (1 <= j < ny) loses its
+     * coefficient of x(j+1) against row j+1, then x(1) and forward substitution */
+    for (int j = ny - 1; j >= 1; j--) {
+      const double w = c[j] / b[j + 1];
+      b[j] = b[j] - w * a[j + 1];
+      g[j] = g[j] - w * g[j + 1];
+    }
+    g[1] = g[1] / b[1];
+    for (int j = 2; j <= ny; j++) g[j] = (g[j] - a[j] * g[j - 1]) / b[j];
+    return;
+  }
   for (int j = 2; j <= ny; j++) {
     a[j] = -a[j] / b[j - 1];
     b[j] = b[j] + a[j] * c[j - 1];

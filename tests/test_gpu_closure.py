@@ -86,8 +86,12 @@ def test_type6_network_points_match_oracle():
     state's own features, so that points both inside and outside are exercised.
     Bars (BF16-aware: a feature that differs in its last float bit can round to a
     neighbouring BF16 input): the inside mask identical on all but 0.1% of the
-    points; nu_t and kappa_t after one substep within 2e-3 (relative L2); the state
-    within 1e-9."""
+    points; nu_t and kappa_t after one substep within 2e-3 (relative L2).  The
+    state moves with the eddy viscosity linearly over one substep, so its error
+    is held to 10x the linear response of the nu_t difference: the state change
+    between the oracle's network and type-5 substeps, scaled by the ratio of the
+    device-vs-oracle nu_t difference to the network-vs-type-5 one (and never
+    tighter than the 1e-11 parity bar)."""
     from oracle import closure as ocl
     p5, s0 = les_problem(32, 33, 32)
     cfg = p5.cfg
@@ -125,7 +129,12 @@ def test_type6_network_points_match_oracle():
     errs.update({k: rel(sg[k], so[k]) for k in so})
     print({k: f"{v:.2e}" for k, v in errs.items()}, f"nu_t ratio median {np.median(dfac):.6f}", flush=True)
     # against the pure type-5 step the network changed the eddy viscosity
-    assert rel(nto, o.download_les()[0]) > 1e-3 and fall[j].max() > 0
+    d_nu = rel(nto, o.download_les()[0])
+    assert d_nu > 1e-3 and fall[j].max() > 0
     assert errs["nu_t"] <= 2e-3 and errs["kappa_t"] <= 2e-3
+    s5 = fields(o.download_state())
     for k in so:
-        assert errs[k] <= 1e-9, (k, errs[k])
+        bar = max(1e-11, 10.0 * rel(so[k], s5[k]) * errs["nu_t"] / d_nu)
+        print(f"{k}: device vs oracle {errs[k]:.2e}, network vs type 5 {rel(so[k], s5[k]):.2e}, "
+              f"bar {bar:.2e}", flush=True)
+        assert errs[k] <= bar, (k, errs[k], bar)
