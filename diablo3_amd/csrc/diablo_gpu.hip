@@ -101,6 +101,8 @@ struct dgpu_handle {
   double* d_fac = nullptr;
   double fac_dt = NAN;
   double* d_rowc = nullptr;  // [ny2][RC_COUNT] assembly row coefficients
+  int dev = 0;             // the device current at dgpu_create
+  unsigned char* noalloc = nullptr;  // that device's address of `scratch`
   double* d_R = nullptr;   // one physical FP64 field (transform entry points)
   cd* d_H = nullptr;       // one FP64 x-spectrum field
   cd* d_C = nullptr;       // one packed field
@@ -148,7 +150,60 @@ struct dgpu_handle {
   long long t_n[16] = {};
 };
 
+// Transform scratch of dgpu_fft_xz_to_fourier_noalloc: the compact physical field,
+// the x-spectra and the packed spectrum of one grid, in one file-scope device
+// array (each GPU the library runs on holds its own copy until the process
+// ends).  Host code never passes `scratch` itself to a kernel: its per-device
+// address comes from hipGetSymbolAddress at dgpu_create.
+constexpr size_t kNoallocScratchBytes = (size_t)DGPU_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+
 namespace {
+
+constexpr int kMaxDevices = 64;
+unsigned char* g_scratch_addr[kMaxDevices] = {};
+
+// The address of `scratch` on device `dev`, which must be current; resolved once
+// per device.
+int scratch_address(int dev, unsigned char** out) {
+  if (dev < 0 || dev >= kMaxDevices) return fail(2, "device ordinal out of range for the transform scratch");
+  if (!g_scratch_addr[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(10, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_addr[dev] = (unsigned char*)ptr;
+  }
+  *out = g_scratch_addr[dev];
+  return 0;
+}
+
+// Fortran physical array u(0:Nx+1, 0:Nzp+1, 0:Nyp+1) -> compact [j][z][x]
+__global__ void k_pack_phys(const double* __restrict__ fort, double* __restrict__ out, Geo g) {
+  const long long n = (long long)g.ny2 * g.nz * g.nx;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+    const int x = (int)(t % g.nx);
+    const long long r = t / g.nx;
+    const int z = (int)(r % g.nz), j = (int)(r / g.nz);
+    out[t] = fort[((long long)j * (g.nz + 2) + z) * (g.nx + 2) + x];
+  }
+}
+
+// packed spectrum [j][k][kxp] -> Fortran cu(0:Nxp, 0:Nz+1, 0:Nyp+1) on the rows
+// k < 2 Nkz + 1, the columns past Nkx zeroed (the host unpack_fourier's map)
+__global__ void k_unpack_four(const cd* __restrict__ comp, cd* __restrict__ fort, Geo g) {
+  const int ldi = g.nx / 2 + 1;
+  const long long n = (long long)g.ny2 * g.kzr * ldi;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(t % ldi);
+    const long long r = t / ldi;
+    const int k = (int)(r % g.kzr), j = (int)(r / g.kzr);
+    fort[((long long)j * (g.nz + 2) + k) * ldi + i] =
+        i < g.kxr ? comp[(long long)j * g.pc + (long long)k * g.kxp + i] : cd{0.0, 0.0};
+  }
+}
+
+size_t round256(size_t b) { return (b + 255) / 256 * 256; }
 
 int dalloc(dgpu_handle* h, void** p, size_t bytes) {
   hipError_t e = hipMalloc(p, bytes ? bytes : 16);
@@ -1286,9 +1341,11 @@ int dgpu_create(const dgpu_config* cfg, dgpu_handle** out) {
   for (int o = 0; o < no; ++o) ALLOC(h->Hout[o], hbytes);
   {
     int dev = 0, ncu = 0;
-    if (hipGetDevice(&dev) == hipSuccess &&
-        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && ncu > 0)
+    if (hipGetDevice(&dev) != hipSuccess) return bail(fail(10, "hipGetDevice"));
+    h->dev = dev;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && ncu > 0)
       h->resident_blocks = 2 * ncu;
+    if ((rc = scratch_address(dev, &h->noalloc))) return bail(rc);
   }
   if (!cfg->use_les) {
     const size_t pbytes = (h->prec == DGPU_MIXED ? sizeof(float) : sizeof(double)) * g.pp * n2;
@@ -1989,6 +2046,37 @@ int dgpu_fft_xz_to_physical(dgpu_handle* h, const double* four, double* phys) {
     for (int z = 0; z < g.nz; ++z)
       std::memcpy(phys + ((size_t)j * (g.nz + 2) + z) * (g.nx + 2), &tmp[((size_t)j * g.nz + z) * g.nx],
                   sizeof(double) * g.nx);
+  return 0;
+}
+
+int dgpu_fft_xz_to_fourier_noalloc(dgpu_handle* h, const double* phys, double* four) {
+  if (!h || !phys || !four) return fail(1, "null argument");
+  const Geo& g = h->g;
+  int dev = -1;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev != h->dev) return fail(2, "dgpu_fft_xz_to_fourier_noalloc: the current device is not the handle's");
+  const size_t rb = round256(sizeof(double) * g.pp * g.ny2);
+  const size_t hb = round256(sizeof(cd) * g.ph * g.ny2);
+  const size_t cb = round256(sizeof(cd) * g.pc * g.ny2);
+  if (rb + hb + cb > kNoallocScratchBytes)
+    return fail(2, "dgpu_fft_xz_to_fourier_noalloc: this grid needs " + std::to_string(rb + hb + cb) +
+                       " bytes of transform scratch; the allocation-free entry point holds " +
+                       std::to_string(kNoallocScratchBytes));
+  double* R = (double*)h->noalloc;
+  cd* H = (cd*)(h->noalloc + rb);
+  cd* C = (cd*)(h->noalloc + rb + hb);
+  const int nb = 1024;
+  k_pack_phys<<<nb, 256, 0, h->st>>>(phys, R, g);
+  HIPCHK(hipGetLastError());
+  double* rs[1] = {R};
+  void* hs[1] = {H};
+  cd* cs[1] = {C};
+  int rc;
+  if ((rc = launch_fwd_x<double>(h, 1, rs, hs, 0, g.ny2))) return rc;
+  if ((rc = launch_fwd_z<double>(h, 1, hs, cs, 0, g.ny2))) return rc;
+  k_unpack_four<<<nb, 256, 0, h->st>>>(C, (cd*)four, g);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(h->st));
   return 0;
 }
 

@@ -52,6 +52,7 @@ def load():
     L.dgpu_filter_chan.argtypes = [P, ctypes.c_int32]
     L.dgpu_fft_xz_to_fourier.argtypes = [P, D, D]
     L.dgpu_fft_xz_to_physical.argtypes = [P, D, D]
+    L.dgpu_fft_xz_to_fourier_noalloc.argtypes = [P, P, P]
     L.dgpu_synchronize.argtypes = [P]
     L.dgpu_last_step_ms.argtypes = [P, D]
     L.dgpu_last_error.restype = ctypes.c_char_p

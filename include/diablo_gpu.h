@@ -142,6 +142,22 @@ int dgpu_rem_div(dgpu_handle *h);
 int dgpu_fft_xz_to_fourier(dgpu_handle *h, const double *phys, double *four);
 int dgpu_fft_xz_to_physical(dgpu_handle *h, const double *four, double *phys);
 
+/* No counterpart in the reference: dgpu_fft_xz_to_fourier for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The same transform, with phys and four in
+ * the same Fortran layouts but in DEVICE memory (they may alias, as the
+ * reference's u/cu views do); rows k >= 2 Nkz + 1 of four are left untouched.
+ * It calls no hipMalloc or hipFree: the compact field, the x-spectra and the
+ * packed spectrum live in a library-static device buffer of
+ * DGPU_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends, and grids
+ * needing more (8 Nx Nz (Ny+2) + 16 kxp (Nz + 2 Nkz + 1)(Ny+2) bytes, kxp = Nkx + 1
+ * rounded up to a multiple of 8; 128x129x128 needs 38.6 MB) are rejected with
+ * status 2.  The device current at the call must be the handle's.  Every call
+ * on one GPU shares that buffer: calls that overlap on one GPU (two threads,
+ * or two handles' streams) race. */
+enum { DGPU_NOALLOC_SCRATCH_BYTES = 64 << 20 };
+int dgpu_fft_xz_to_fourier_noalloc(dgpu_handle *h, const double *phys, double *four);
+
 /* poisson_p_chan (for/solver.f90:1382-1560): the pressure cp from the velocity
  * (and the scalars' y gradients) of the device state, rows 1..Nyp, by the
  * pressure-Poisson solve of rem_div_chan's operator.  ReadHDF5 calls it for a
