@@ -46,9 +46,10 @@ def kernel_bytes(cfg, mixed):
     Rtj = cfg.nx * cfg.nz * s * nj
     out = {
         "k_inv_z": nf * C + nf * H,
-        # x pass in two kernels: inverse x transforms (x-spectra in, physical
-        # fields out), then the products reading the physical fields once
-        "k_inv_x": nf * H + nf * Rt,
+        # DNS x pass marching in y: the x-spectra of planes jlo-1..ny+1 in, the
+        # product spectra of planes jlo..ny out (physical fields stay on chip)
+        "k_xmarch": nf * cfg.kxr * cfg.nz * 2 * s * (nj + 2) + no * Hj,
+        # LES x pass: physical fields (and SGS fields) in, product spectra out
         "k_xprod": nf * Rtj + no * Hj,
         "k_fwdz_nl": no * Hj + nf * Cj_t,
         # nonlinear terms, state, cp and (rk 2, 3: 2 of 3 launches) old cf in; cf and state out

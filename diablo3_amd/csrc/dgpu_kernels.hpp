@@ -258,12 +258,36 @@ __global__ __launch_bounds__(NT) void k_fwd_x(Ptrs<const T*, MAXO> src, Ptrs<cx<
 //   Nt = -ikx F(th u1) - F(fluxth)   Zt = F(th u3)       (per scalar)
 // so that the nonlinear tendency of equation e is Fz(Ne) - ikz Fz(Ze).
 // ---------------------------------------------------------------------------
+// Per-row coefficients of the assembly, computed once on the host from the grid
+// (the reference evaluates them inline; the host table holds the same quotients).
+enum RowCoef {
+  RC_RDY = 0,    // 1/dy(j)
+  RC_RDYP,       // 1/dy(j+1)
+  RC_RDYF,       // 1/dyf(j)
+  RC_RDYFM,      // 1/dyf(j-1)
+  RC_WTH,        // dyf(j-1)/(2 dy(j)): weight of th(j) in the GY interpolation
+  RC_WTHM,       // dyf(j)/(2 dy(j)):   weight of th(j-1)
+  RC_FLO,        // 1/(dy(j) dyf(j)):    lower coupling of a GYF row (u1, u3, scalars)
+  RC_FHI,        // 1/(dy(j+1) dyf(j)):  upper coupling of a GYF row
+  RC_GLO,        // 1/(dyf(j-1) dy(j)):  lower coupling of a GY row (u2)
+  RC_GHI,        // 1/(dyf(j) dy(j)):    upper coupling of a GY row
+  RC_YX,         // 2 dy(j) / (dx beta_sgs):  the AMD closure's width ratios at GY j
+  RC_XY,         // (dx beta_sgs) / (2 dy(j))
+  RC_YZ,         // 2 dy(j) / (dz beta_sgs)
+  RC_ZY,         // (dz beta_sgs) / (2 dy(j))
+  RC_TW0,        // per scalar n, 4 entries at RC_TW0 + 4n:
+  RC_COUNT = RC_TW0 + 4 * MAXTH
+  // +0 dthdx (gyf-Ly/2) delta/Ro_inv, +1 -dthdz (gyf-Ly/2) delta/Ro_inv,
+  // +2 dthdx (gy-Ly/2) delta/Ro_inv,  +3 -dthdz (gy-Ly/2) delta/Ro_inv
+};
+
 struct XProdArgs {
   Geo g;
   int jlo;
   int jstart, jend, jstart_th[MAXTH], jend_th[MAXTH];
   const double* __restrict__ dy;
   const double* __restrict__ dyf;
+  const double* __restrict__ rowc;  // [ny2][RC_COUNT] grid quotients
   const double* __restrict__ kx;
   const void* hin[MAXF];   // x-spectra of u1, u2, u3, th_n (all planes)
   void* hout[MAXO];        // N1, N2, N3, Z1, Z2, Z3, (Nt, Zt) per scalar
@@ -277,12 +301,125 @@ struct XProdArgs {
   const void* phys[MAXF];
 };
 
+// The NP = 8 + 3 NTH product lines of one column x at plane j, from the physical
+// fields of planes j-1, j, j+1 (index 0, 1, 2) of the line pair (the .x and .y
+// components), with rk_chan_1's y stencils (for/solver.f90:655-815, 914-964):
+//   0 u3u1, 1 u1u1, 2 u3u3 (GYF, rows jstart..jend), 3 u1u2i, 4 u3u2i (GY, rows
+//   2..Nyp), 5 flux of u1, 6 flux of u3 (GYF), 7 the GY derivative of u2u2, and
+//   per scalar n: 8+3n th u1, 9+3n th u3, 10+3n flux of th (rows jstart_th..jend_th).
+template <typename T, int NTH>
+__device__ __forceinline__ void xprod_products(const XProdArgs& a, int j, const cx<T>* U1, const cx<T>* U2,
+                                               const cx<T>* U3, const cx<T> (*TH)[3], cx<T>* P) {
+  const bool vrow = (j >= a.jstart && j <= a.jend);
+  const bool v2row = (j >= 2 && j <= a.g.ny);
+  const double* rc = a.rowc + (long long)j * RC_COUNT;
+  const T rdy = (T)rc[RC_RDY], rdyf = (T)rc[RC_RDYF];
+  const T dyfj = (T)a.dyf[j], dyfm = (T)a.dyf[j - 1];
+  const T cint = T(0.5) * rdy;  // interpolation to GY(j): (dyf_j u_j + dyf_{j-1} u_{j-1})/(2 dy_j)
+  // the two real lines of the pair are the .x and .y components
+#define DG_PROD(expr_x, expr_y) cx<T>{expr_x, expr_y}
+#define C(v, c) (v).c
+  if (vrow) {
+    P[0] = DG_PROD(C(U3[1], x) * C(U1[1], x), C(U3[1], y) * C(U1[1], y));
+    P[1] = DG_PROD(C(U1[1], x) * C(U1[1], x), C(U1[1], y) * C(U1[1], y));
+    P[2] = DG_PROD(C(U3[1], x) * C(U3[1], x), C(U3[1], y) * C(U3[1], y));
+    const T r2f = T(0.5) * rdyf;
+    P[5] = DG_PROD((C(U1[2], x) * C(U2[2], x) + C(U1[1], x) * C(U2[2], x) - C(U1[1], x) * C(U2[1], x) -
+                    C(U1[0], x) * C(U2[1], x)) * r2f,
+                   (C(U1[2], y) * C(U2[2], y) + C(U1[1], y) * C(U2[2], y) - C(U1[1], y) * C(U2[1], y) -
+                    C(U1[0], y) * C(U2[1], y)) * r2f);
+    P[6] = DG_PROD((C(U3[2], x) * C(U2[2], x) + C(U3[1], x) * C(U2[2], x) - C(U3[1], x) * C(U2[1], x) -
+                    C(U3[0], x) * C(U2[1], x)) * r2f,
+                   (C(U3[2], y) * C(U2[2], y) + C(U3[1], y) * C(U2[2], y) - C(U3[1], y) * C(U2[1], y) -
+                    C(U3[0], y) * C(U2[1], y)) * r2f);
+  } else {
+    P[0] = P[1] = P[2] = P[5] = P[6] = cx<T>{T(0), T(0)};
+  }
+  if (v2row) {
+    P[3] = DG_PROD(((dyfj * C(U1[1], x) + dyfm * C(U1[0], x)) * cint) * C(U2[1], x),
+                   ((dyfj * C(U1[1], y) + dyfm * C(U1[0], y)) * cint) * C(U2[1], y));
+    P[4] = DG_PROD(((dyfj * C(U3[1], x) + dyfm * C(U3[0], x)) * cint) * C(U2[1], x),
+                   ((dyfj * C(U3[1], y) + dyfm * C(U3[0], y)) * cint) * C(U2[1], y));
+    const T apx = C(U2[1], x) + C(U2[2], x), amx = C(U2[1], x) + C(U2[0], x);
+    const T apy = C(U2[1], y) + C(U2[2], y), amy = C(U2[1], y) + C(U2[0], y);
+    P[7] = DG_PROD((T(0.25) * (apx * apx) - T(0.25) * (amx * amx)) * rdy,
+                   (T(0.25) * (apy * apy) - T(0.25) * (amy * amy)) * rdy);
+  } else {
+    P[3] = P[4] = P[7] = cx<T>{T(0), T(0)};
+  }
+#pragma unroll
+  for (int n = 0; n < NTH; ++n) {
+    const bool trow = (j >= a.jstart_th[n] && j <= a.jend_th[n]);
+    if (trow) {
+      const T r2f = T(0.5) * rdyf;
+      P[8 + 3 * n] = DG_PROD(C(TH[n][1], x) * C(U1[1], x), C(TH[n][1], y) * C(U1[1], y));
+      P[9 + 3 * n] = DG_PROD(C(TH[n][1], x) * C(U3[1], x), C(TH[n][1], y) * C(U3[1], y));
+      P[10 + 3 * n] = DG_PROD(
+          (C(TH[n][2], x) * C(U2[2], x) + C(TH[n][1], x) * C(U2[2], x) - C(TH[n][1], x) * C(U2[1], x) -
+           C(TH[n][0], x) * C(U2[1], x)) * r2f,
+          (C(TH[n][2], y) * C(U2[2], y) + C(TH[n][1], y) * C(U2[2], y) - C(TH[n][1], y) * C(U2[1], y) -
+           C(TH[n][0], y) * C(U2[1], y)) * r2f);
+    } else {
+      P[8 + 3 * n] = P[9 + 3 * n] = P[10 + 3 * n] = cx<T>{T(0), T(0)};
+    }
+  }
+#undef DG_PROD
+#undef C
+}
+
+// The product lines of the pair, forward-transformed in LDS (line p at s + p NX),
+// to the 6 + 2 NTH x-spectra of plane j: the two real lines split apart, the
+// derivative lines times -ikx less the flux lines, 1/Nx.  hout: the output
+// pointers, kxt: kx (either may be a copy in LDS).
+template <typename T, int NX, int NT, int NTH, typename KT>
+__device__ __forceinline__ void xprod_store(const cx<T>* __restrict__ s, const Geo& g, void* const* hout,
+                                            const KT* __restrict__ kxt, int z, int j, int tid) {
+  constexpr int NO = 6 + 2 * NTH;
+  const T rnx = T(1) / T(NX);
+  for (int idx = tid; idx < NO * g.kxr; idx += NT) {
+    const int o = idx / g.kxr, i = idx % g.kxr;
+    const T kxi = (T)kxt[i];
+    cx<T> A0, B0, A1, B1;
+    int pd, pf;  // derivative product (times -ikx) and flux product; pf < 0: plain copy of pd
+    if (o == 0) { pd = 1; pf = 5; }
+    else if (o == 1) { pd = 3; pf = 7; }
+    else if (o == 2) { pd = 0; pf = 6; }
+    else if (o == 3) { pd = 0; pf = -1; }
+    else if (o == 4) { pd = 4; pf = -1; }
+    else if (o == 5) { pd = 2; pf = -1; }
+    else {
+      const int n = (o - 6) >> 1;
+      if (((o - 6) & 1) == 0) { pd = 8 + 3 * n; pf = 10 + 3 * n; }
+      else { pd = 9 + 3 * n; pf = -1; }
+    }
+    pair_split(s + pd * NX, i, NX, A0, B0);
+    cx<T> ra, rb;
+    if (pf >= 0) {
+      pair_split(s + pf * NX, i, NX, A1, B1);
+      // -ikx F - Fflux
+      ra = cx<T>{kxi * A0.y, -(kxi * A0.x)} - A1;
+      rb = cx<T>{kxi * B0.y, -(kxi * B0.x)} - B1;
+    } else {
+      ra = A0;
+      rb = B0;
+    }
+    // device memory (the pointer may come from LDS: say so, for global stores)
+    typedef T v2 __attribute__((ext_vector_type(2)));
+    typedef __attribute__((address_space(1))) v2 gv2;
+    gv2* H = (gv2*)((cx<T>*)hout[o] + (long long)j * g.ph);
+    H[(long long)z * g.kxp + i] = v2{rnx * ra.x, rnx * ra.y};
+    H[(long long)(z + 1) * g.kxp + i] = v2{rnx * rb.x, rnx * rb.y};
+  }
+}
+
 // A separate inverse x transform has written the physical fields once per
 // plane; each block reads the three planes j-1..j+1 of its line pair (z, z+1),
-// forms the products and forward-transforms them in LDS.
+// forms the products and forward-transforms them in LDS.  (The LES x pass: the
+// physical fields exist anyway for the closure, and the SGS fields are added to
+// the products.)
 template <typename T, int NX, int NT, int NTH, bool LES>
 __global__ __launch_bounds__(NT) void k_xprod(XProdArgs a) {
-  constexpr int NP = 8 + 3 * NTH, NO = 6 + 2 * NTH;
+  constexpr int NP = 8 + 3 * NTH;
   __shared__ cx<T> s[NP * NX];
   __shared__ cx<T> stw[NX];  // twiddles
   stage_twiddles<T, NX, NT>(stw, (const cx<T>*)a.tw, threadIdx.x);
@@ -294,11 +431,6 @@ __global__ __launch_bounds__(NT) void k_xprod(XProdArgs a) {
     const T* R = (const T*)a.phys[f] + (long long)(j - 1 + d) * g.pp + (long long)z * NX + x;
     return cx<T>{R[0], R[NX]};
   };
-  const bool vrow = (j >= a.jstart && j <= a.jend);
-  const bool v2row = (j >= 2 && j <= g.ny);
-  const T dyj = (T)a.dy[j];
-  const T dyfj = (T)a.dyf[j], dyfm = (T)a.dyf[j - 1];
-  const T cint = T(1) / (T(2) * dyj);  // interpolation to GY(j): (dyf_j u_j + dyf_{j-1} u_{j-1})/(2 dy_j)
   for (int x = tid; x < NX; x += NT) {
     cx<T> U1[3], U2[3], U3[3];
 #pragma unroll
@@ -313,55 +445,7 @@ __global__ __launch_bounds__(NT) void k_xprod(XProdArgs a) {
 #pragma unroll
       for (int d = 0; d < 3; ++d) TH[n][d] = field(3 + n, d, x);
     cx<T> P[NP];
-    // the two real lines of the pair are the .x and .y components
-#define DG_PROD(expr_x, expr_y) cx<T>{expr_x, expr_y}
-#define C(v, c) (v).c
-    if (vrow) {
-      P[0] = DG_PROD(C(U3[1], x) * C(U1[1], x), C(U3[1], y) * C(U1[1], y));
-      P[1] = DG_PROD(C(U1[1], x) * C(U1[1], x), C(U1[1], y) * C(U1[1], y));
-      P[2] = DG_PROD(C(U3[1], x) * C(U3[1], x), C(U3[1], y) * C(U3[1], y));
-      const T r2f = T(1) / (T(2) * dyfj);
-      P[5] = DG_PROD((C(U1[2], x) * C(U2[2], x) + C(U1[1], x) * C(U2[2], x) - C(U1[1], x) * C(U2[1], x) -
-                      C(U1[0], x) * C(U2[1], x)) * r2f,
-                     (C(U1[2], y) * C(U2[2], y) + C(U1[1], y) * C(U2[2], y) - C(U1[1], y) * C(U2[1], y) -
-                      C(U1[0], y) * C(U2[1], y)) * r2f);
-      P[6] = DG_PROD((C(U3[2], x) * C(U2[2], x) + C(U3[1], x) * C(U2[2], x) - C(U3[1], x) * C(U2[1], x) -
-                      C(U3[0], x) * C(U2[1], x)) * r2f,
-                     (C(U3[2], y) * C(U2[2], y) + C(U3[1], y) * C(U2[2], y) - C(U3[1], y) * C(U2[1], y) -
-                      C(U3[0], y) * C(U2[1], y)) * r2f);
-    } else {
-      P[0] = P[1] = P[2] = P[5] = P[6] = cx<T>{T(0), T(0)};
-    }
-    if (v2row) {
-      P[3] = DG_PROD(((dyfj * C(U1[1], x) + dyfm * C(U1[0], x)) * cint) * C(U2[1], x),
-                     ((dyfj * C(U1[1], y) + dyfm * C(U1[0], y)) * cint) * C(U2[1], y));
-      P[4] = DG_PROD(((dyfj * C(U3[1], x) + dyfm * C(U3[0], x)) * cint) * C(U2[1], x),
-                     ((dyfj * C(U3[1], y) + dyfm * C(U3[0], y)) * cint) * C(U2[1], y));
-      const T apx = C(U2[1], x) + C(U2[2], x), amx = C(U2[1], x) + C(U2[0], x);
-      const T apy = C(U2[1], y) + C(U2[2], y), amy = C(U2[1], y) + C(U2[0], y);
-      P[7] = DG_PROD((T(0.25) * (apx * apx) - T(0.25) * (amx * amx)) / dyj,
-                     (T(0.25) * (apy * apy) - T(0.25) * (amy * amy)) / dyj);
-    } else {
-      P[3] = P[4] = P[7] = cx<T>{T(0), T(0)};
-    }
-#pragma unroll
-    for (int n = 0; n < NTH; ++n) {
-      const bool trow = (j >= a.jstart_th[n] && j <= a.jend_th[n]);
-      if (trow) {
-        const T r2f = T(1) / (T(2) * dyfj);
-        P[8 + 3 * n] = DG_PROD(C(TH[n][1], x) * C(U1[1], x), C(TH[n][1], y) * C(U1[1], y));
-        P[9 + 3 * n] = DG_PROD(C(TH[n][1], x) * C(U3[1], x), C(TH[n][1], y) * C(U3[1], y));
-        P[10 + 3 * n] = DG_PROD(
-            (C(TH[n][2], x) * C(U2[2], x) + C(TH[n][1], x) * C(U2[2], x) - C(TH[n][1], x) * C(U2[1], x) -
-             C(TH[n][0], x) * C(U2[1], x)) * r2f,
-            (C(TH[n][2], y) * C(U2[2], y) + C(TH[n][1], y) * C(U2[2], y) - C(TH[n][1], y) * C(U2[1], y) -
-             C(TH[n][0], y) * C(U2[1], y)) * r2f);
-      } else {
-        P[8 + 3 * n] = P[9 + 3 * n] = P[10 + 3 * n] = cx<T>{T(0), T(0)};
-      }
-    }
-#undef DG_PROD
-#undef C
+    xprod_products<T, NTH>(a, j, U1, U2, U3, TH, P);
     if constexpr (LES) {
       const long long pz = (long long)j * g.pp + (long long)z * NX + x;
       auto A = [&](int m) {
@@ -387,37 +471,138 @@ __global__ __launch_bounds__(NT) void k_xprod(XProdArgs a) {
   }
   __syncthreads();
   fft_lds<T, NX, NP, NT, -1>(s, NX, stw, tid);
-  const T rnx = T(1) / T(NX);
-  for (int idx = tid; idx < NO * g.kxr; idx += NT) {
-    const int o = idx / g.kxr, i = idx % g.kxr;
-    const T kxi = (T)a.kx[i];
-    cx<T> A0, B0, A1, B1;
-    int pd, pf;  // derivative product (times -ikx) and flux product; pf < 0: plain copy of pd
-    if (o == 0) { pd = 1; pf = 5; }
-    else if (o == 1) { pd = 3; pf = 7; }
-    else if (o == 2) { pd = 0; pf = 6; }
-    else if (o == 3) { pd = 0; pf = -1; }
-    else if (o == 4) { pd = 4; pf = -1; }
-    else if (o == 5) { pd = 2; pf = -1; }
-    else {
-      const int n = (o - 6) >> 1;
-      if (((o - 6) & 1) == 0) { pd = 8 + 3 * n; pf = 10 + 3 * n; }
-      else { pd = 9 + 3 * n; pf = -1; }
+  xprod_store<T, NX, NT, NTH>(s, g, a.hout, a.kx, z, j, tid);
+}
+
+// The DNS x pass in one kernel, marching in y.  Block (z pair, segment) owns the
+// line pair (z, z+1) over the planes j0..j1 of its segment and keeps the physical
+// fields of planes j-1, j, j+1 at its columns in registers: each plane's
+// x-spectra are read once (the segment's two halo planes aside), inverse-
+// transformed, multiplied out and forward-transformed without the physical
+// fields ever reaching HBM.  One LDS FFT per plane serves both directions: the
+// NP product lines of plane j and, conjugated (the inverse transform is
+// conj(F(conj(X))), the same arithmetic as the +i transform), the NF field lines
+// of plane j+2; the raw spectra of plane j+3 are loaded into registers meanwhile
+// (unconditional loads from per-thread addresses fixed for the whole block, so
+// that nothing waits on them before they are used one plane later; the output
+// pointers and kx sit in LDS, so the stores wait on no global load either).
+template <typename T, int NX, int NT, int NTH>
+__global__ __launch_bounds__(NT) void k_xmarch(XProdArgs a, int seg) {
+  constexpr int NF = 3 + NTH, NP = 8 + 3 * NTH, NL = NP + NF, NO = 6 + 2 * NTH;
+  static_assert(3 * NF <= NL, "the three first planes share one FFT batch");
+  static_assert(NX <= NT, "one column per thread");
+  constexpr int HB = NX / 2 + 1;                  // bins 0..NX/2 of a line pair
+  constexpr int PREF = (NF * HB + NT - 1) / NT;   // loaded bins per thread
+  constexpr bool ALLX = NX == NT;
+  __shared__ cx<T> s[NL * NX];
+  __shared__ cx<T> stw[NX];  // twiddles
+  __shared__ T skx[NX / 2];
+  __shared__ void* sout[NO];
+  stage_twiddles<T, NX, NT>(stw, (const cx<T>*)a.tw, threadIdx.x);
+  const Geo& g = a.g;
+  const int tid = threadIdx.x, z = 2 * blockIdx.x, x = tid;
+  for (int i = tid; i < g.kxr; i += NT) skx[i] = (T)a.kx[i];
+  if (tid < NO) sout[tid] = a.hout[tid];
+  const int j0 = a.jlo + blockIdx.y * seg;
+  const int j1 = min(j0 + seg - 1, g.ny);
+  // bin m of field f per prefetch slot; the slot reads bins min(m, Nkx) of the
+  // two lines (in bounds for every slot; unused values are dropped at put time)
+  const cx<T>* pb[PREF];
+  int pm[PREF];
+#pragma unroll
+  for (int p = 0; p < PREF; ++p) {
+    const int idx = tid + p * NT;
+    const bool live = idx < NF * HB;
+    const int f = live ? idx / HB : 0;
+    const int m = live ? idx % HB : -1;
+    pm[p] = m;
+    pb[p] = (const cx<T>*)a.hin[f] + (long long)z * g.kxp + (m > 0 ? min(m, g.nkx) : 0);
+  }
+  cx<T> ra[PREF], rb[PREF];
+  auto load_plane = [&](int jp) {
+    const long long off = (long long)jp * g.ph;
+#pragma unroll
+    for (int p = 0; p < PREF; ++p) {
+      ra[p] = pb[p][off];
+      rb[p] = pb[p][off + g.kxp];
     }
-    pair_split(s + pd * NX, i, NX, A0, B0);
-    cx<T> ra, rb;
-    if (pf >= 0) {
-      pair_split(s + pf * NX, i, NX, A1, B1);
-      // -ikx F - Fflux
-      ra = cx<T>{kxi * A0.y, -(kxi * A0.x)} - A1;
-      rb = cx<T>{kxi * B0.y, -(kxi * B0.x)} - B1;
-    } else {
-      ra = A0;
-      rb = B0;
+  };
+  // the pair spectrum (bins m and NX - m, as k_inv_x: i > Nkx zero, the DC bins'
+  // imaginary parts dropped), conjugated, into lines line0 + f
+  auto put_plane = [&](int line0) {
+#pragma unroll
+    for (int p = 0; p < PREF; ++p) {
+      const int m = pm[p];
+      if (m >= 0) {
+        const int f = (tid + p * NT) / HB;
+        const cx<T> u = ra[p], v = rb[p];
+        cx<T> lo = {T(0), T(0)}, hi = {T(0), T(0)};
+        if (m == 0) {
+          lo = {u.x, v.x};
+        } else if (m <= g.nkx) {
+          lo = {u.x - v.y, u.y + v.x};
+          hi = {u.x + v.y, v.x - u.y};
+        }
+        s[(line0 + f) * NX + swz<T>(m)] = conjg(lo);
+        if (m > 0 && m < NX / 2) s[(line0 + f) * NX + swz<T>(NX - m)] = conjg(hi);
+      }
     }
-    cx<T>* H = (cx<T>*)a.hout[o] + (long long)j * g.ph;
-    H[(long long)z * g.kxp + i] = rnx * ra;
-    H[(long long)(z + 1) * g.kxp + i] = rnx * rb;
+  };
+  cx<T> W[3][NF];  // fields at column x, planes j-1, j, j+1
+  auto get_plane = [&](int line0, cx<T>* w) {
+#pragma unroll
+    for (int f = 0; f < NF; ++f) w[f] = conjg(s[(line0 + f) * NX + swz<T>(x)]);
+  };
+  // the window's first three planes
+  load_plane(j0 - 1);
+  put_plane(0);
+  load_plane(j0);
+  put_plane(NF);
+  load_plane(j0 + 1);
+  put_plane(2 * NF);
+  if (j0 + 2 <= j1 + 1) load_plane(j0 + 2);
+  __syncthreads();
+  fft_lds<T, NX, 3 * NF, NT, -1>(s, NX, stw, tid);
+  if (ALLX || x < NX) {
+    get_plane(0, W[0]);
+    get_plane(NF, W[1]);
+    get_plane(2 * NF, W[2]);
+  }
+  __syncthreads();
+  for (int j = j0; j <= j1; ++j) {
+    const bool next = j + 2 <= j1 + 1;  // plane j+2 enters the window
+    if (ALLX || x < NX) {
+      cx<T> U1[3] = {W[0][0], W[1][0], W[2][0]};
+      cx<T> U2[3] = {W[0][1], W[1][1], W[2][1]};
+      cx<T> U3[3] = {W[0][2], W[1][2], W[2][2]};
+      cx<T> TH[NTH > 0 ? NTH : 1][3];
+#pragma unroll
+      for (int n = 0; n < NTH; ++n) {
+        TH[n][0] = W[0][3 + n];
+        TH[n][1] = W[1][3 + n];
+        TH[n][2] = W[2][3 + n];
+      }
+      cx<T> P[NP];
+      xprod_products<T, NTH>(a, j, U1, U2, U3, TH, P);
+#pragma unroll
+      for (int p = 0; p < NP; ++p) s[p * NX + swz<T>(x)] = P[p];
+    }
+    if (next) {
+      put_plane(NP);
+      if (j + 3 <= j1 + 1) load_plane(j + 3);
+    }
+    __syncthreads();
+    fft_lds<T, NX, NL, NT, -1>(s, NX, stw, tid);
+    xprod_store<T, NX, NT, NTH>(s, g, sout, skx, z, j, tid);
+    if (next && (ALLX || x < NX)) {
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        W[0][f] = W[1][f];
+        W[1][f] = W[2][f];
+      }
+      get_plane(NP, W[2]);
+    }
+    __syncthreads();
   }
 }
 
@@ -555,29 +740,6 @@ struct SolveRows {
   int wlo, whi;      // rows written back
   int kind[5];       // rows 0, 1, 2, ny, ny+1: -1 = build-range rule, 0 = zero, 1 = BC value
   double val[5];     // BC value of the mean mode at those rows
-};
-
-// Per-row coefficients of the assembly, computed once on the host from the grid
-// (the reference evaluates them inline; the host table holds the same quotients).
-enum RowCoef {
-  RC_RDY = 0,    // 1/dy(j)
-  RC_RDYP,       // 1/dy(j+1)
-  RC_RDYF,       // 1/dyf(j)
-  RC_RDYFM,      // 1/dyf(j-1)
-  RC_WTH,        // dyf(j-1)/(2 dy(j)): weight of th(j) in the GY interpolation
-  RC_WTHM,       // dyf(j)/(2 dy(j)):   weight of th(j-1)
-  RC_FLO,        // 1/(dy(j) dyf(j)):    lower coupling of a GYF row (u1, u3, scalars)
-  RC_FHI,        // 1/(dy(j+1) dyf(j)):  upper coupling of a GYF row
-  RC_GLO,        // 1/(dyf(j-1) dy(j)):  lower coupling of a GY row (u2)
-  RC_GHI,        // 1/(dyf(j) dy(j)):    upper coupling of a GY row
-  RC_YX,         // 2 dy(j) / (dx beta_sgs):  the AMD closure's width ratios at GY j
-  RC_XY,         // (dx beta_sgs) / (2 dy(j))
-  RC_YZ,         // 2 dy(j) / (dz beta_sgs)
-  RC_ZY,         // (dz beta_sgs) / (2 dy(j))
-  RC_TW0,        // per scalar n, 4 entries at RC_TW0 + 4n:
-  RC_COUNT = RC_TW0 + 4 * MAXTH
-  // +0 dthdx (gyf-Ly/2) delta/Ro_inv, +1 -dthdz (gyf-Ly/2) delta/Ro_inv,
-  // +2 dthdx (gy-Ly/2) delta/Ro_inv,  +3 -dthdz (gy-Ly/2) delta/Ro_inv
 };
 
 struct ColArgs {

@@ -91,12 +91,8 @@ struct dgpu_handle {
   void* HinZD[MAXF] = {};
   void* PgD[3 * MAXF] = {};
   void* Hout[MAXO] = {};
-  // the x pass runs in two kernels (inverse x transforms, then the products
-  // reading the physical fields): DNS handles keep the physical fields in Px (T),
-  // LES handles use Pg
   // persistent kernels: blocks to launch (a few per CU)
   int resident_blocks = 1024;
-  void* Px[MAXF] = {};
   void *twx64 = nullptr, *twz64 = nullptr, *twx32 = nullptr, *twz32 = nullptr;
   double* d_fac = nullptr;
   double fac_dt = NAN;
@@ -441,8 +437,9 @@ int ensure_fft_scratch(dgpu_handle* h) {
 // per-kernel timing
 // --------------------------------------------------------------------------------
 constexpr int NKERN = 12;
-const char* const kKernelNames[NKERN] = {"k_inv_z",   "k_xprod",  "k_fwdz_nl", "k_cn",      "k_project", "k_bc_post",
-                                         "les_grad", "k_amd",    "k_cn_phys", "les_cn_fft", "k_closure", "k_inv_x"};
+const char* const kKernelNames[NKERN] = {"k_inv_z",   "k_xprod",  "k_fwdz_nl", "k_cn",      "k_project",
+                                         "k_bc_post", "les_grad", "k_amd",     "k_cn_phys", "les_cn_fft",
+                                         "k_closure", "k_xmarch"};
 
 hipEvent_t pool_event(dgpu_handle* h) {
   if (!h->ev_pool.empty()) {
@@ -594,6 +591,32 @@ int launch_xprod(dgpu_handle* h, XProdArgs& a, int nj) {
       constexpr int NT = xprod_nt(N_, lds);
       dim3 grid(h->g.nz / 2, nj, 1);
       k_xprod<T, N_, NT, NTH, LES><<<grid, NT, 0, h->st>>>(a);
+    }
+  });
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// The DNS x pass: one thread per column (64 for the short lines), the y planes
+// split into segments so that the grid is about two blocks per CU
+template <typename T, int NTH>
+int launch_xmarch(dgpu_handle* h, XProdArgs& a, int nj) {
+  a.tw = sizeof(T) == 8 ? h->twx64 : h->twx32;
+  DG_SWITCH_N(h->g.nx, {
+    constexpr int NT = N_ < 64 ? 64 : N_;
+    // NP product lines + NF field lines + the twiddle table, kx, output pointers
+    constexpr size_t lds = sizeof(cx<T>) * ((11 + 4 * NTH) + 1) * N_ + sizeof(T) * (N_ / 2) +
+                           sizeof(void*) * (6 + 2 * NTH);
+    if constexpr (lds > 163840) {
+      return fail(2, "this nx needs the mixed-precision mode (x pass exceeds LDS in FP64)");
+    } else {
+      const int pairs = h->g.nz / 2;
+      const int want = h->resident_blocks > 0 ? h->resident_blocks : 512;
+      int nseg = std::min(nj, std::max(1, (want + pairs - 1) / pairs));
+      const int seg = (nj + nseg - 1) / nseg;
+      nseg = (nj + seg - 1) / seg;
+      dim3 grid(pairs, nseg, 1);
+      k_xmarch<T, N_, NT, NTH><<<grid, NT, 0, h->st>>>(a, seg);
     }
   });
   HIPCHK(hipGetLastError());
@@ -1063,7 +1086,7 @@ int substep_les(dgpu_handle* h, int rk, double delta_t, double time, int time_st
   XProdArgs xa{};
   xa.g = g; xa.jlo = jlo; xa.jstart = c.jstart; xa.jend = c.jend;
   for (int n = 0; n < MAXTH; ++n) { xa.jstart_th[n] = c.jstart_th[n]; xa.jend_th[n] = c.jend_th[n]; }
-  xa.dy = h->d_dy; xa.dyf = h->d_dyf; xa.kx = h->d_kx;
+  xa.dy = h->d_dy; xa.dyf = h->d_dyf; xa.kx = h->d_kx; xa.rowc = h->d_rowc;
   for (int f = 0; f < nf; ++f) xa.hin[f] = h->Hin[f];
   for (int o = 0; o < 6 + 2 * NTH; ++o) xa.hout[o] = h->Hout[o];
   for (int m = 0; m < 7 + 2 * NTH; ++m) xa.aug[m] = h->Aug[m];
@@ -1226,25 +1249,17 @@ int substep(dgpu_handle* h, int rk, double delta_t, double time, int time_step, 
   rc = mixed ? launch_inv_z<float>(h, nf, srcs, h->Hin, 0, g.ny2)
              : launch_inv_z<double>(h, nf, srcs, h->Hin, 0, g.ny2);
   if (rc || (rc = tend(h))) return rc;
-  // 2. fused x pass
+  // 2. x pass marching in y: x inverse transforms, products, x forward transforms
   XProdArgs xa{};
   xa.g = g; xa.jlo = jlo; xa.jstart = c.jstart; xa.jend = c.jend;
   for (int n = 0; n < MAXTH; ++n) { xa.jstart_th[n] = c.jstart_th[n]; xa.jend_th[n] = c.jend_th[n]; }
-  xa.dy = h->d_dy; xa.dyf = h->d_dyf; xa.kx = h->d_kx;
+  xa.dy = h->d_dy; xa.dyf = h->d_dyf; xa.kx = h->d_kx; xa.rowc = h->d_rowc;
   for (int f = 0; f < nf; ++f) xa.hin[f] = h->Hin[f];
   for (int o = 0; o < 6 + 2 * h->nth; ++o) xa.hout[o] = h->Hout[o];
-  // the physical fields once per plane, then the products read them
-  for (int f = 0; f < nf; ++f)
-    if (!h->Px[f]) return fail(12, "x pass: physical field buffer missing");
   if ((rc = tbeg(h, 11))) return rc;
-  rc = mixed ? launch_inv_x<float>(h, nf, h->Hin, (float* const*)h->Px, 0, g.ny2)
-             : launch_inv_x<double>(h, nf, h->Hin, (double* const*)h->Px, 0, g.ny2);
-  if (rc || (rc = tend(h))) return rc;
-  for (int f = 0; f < nf; ++f) xa.phys[f] = h->Px[f];
-  if ((rc = tbeg(h, 1))) return rc;
   rc = by_nth(h->nth, [&](auto NTHc) {
     constexpr int NTH = decltype(NTHc)::value;
-    return mixed ? launch_xprod<float, NTH>(h, xa, nj) : launch_xprod<double, NTH>(h, xa, nj);
+    return mixed ? launch_xmarch<float, NTH>(h, xa, nj) : launch_xmarch<double, NTH>(h, xa, nj);
   });
   if (rc || (rc = tend(h))) return rc;
   // 3. z forward transform of the products -> nonlinear tendencies
@@ -1347,10 +1362,6 @@ int dgpu_create(const dgpu_config* cfg, dgpu_handle** out) {
       h->resident_blocks = 2 * ncu;
     if ((rc = scratch_address(dev, &h->noalloc))) return bail(rc);
   }
-  if (!cfg->use_les) {
-    const size_t pbytes = (h->prec == DGPU_MIXED ? sizeof(float) : sizeof(double)) * g.pp * n2;
-    for (int f = 0; f < nf; ++f) ALLOC(h->Px[f], pbytes);
-  }
   ALLOC(h->twx64, sizeof(cd) * g.nx); ALLOC(h->twz64, sizeof(cd) * g.nz);
   ALLOC(h->twx32, sizeof(cf) * g.nx); ALLOC(h->twz32, sizeof(cf) * g.nz);
   ALLOC(h->d_fac, sizeof(double) * 3 * MAXF * 5 * n2);
@@ -1371,9 +1382,6 @@ int dgpu_create(const dgpu_config* cfg, dgpu_handle** out) {
     for (int m = 0; m < 7 + 2 * h->nth; ++m) ALLOC(h->Aug[m], tbytes);
     ALLOC(h->CPs, 2 * (size_t)nf * rbytes);
     ALLOC(h->d_dlt, sizeof(double) * 2 * n2);
-    // the DNS branch of an LES handle (before the spin-up gate) keeps its physical
-    // fields in the gradient stage's buffers
-    for (int f = 0; f < nf; ++f) h->Px[f] = h->Pg[3 * f];
     if (cfg->les_model_type == 6) {
       ALLOC(h->Cgy, tbytes);
       ALLOC(h->Kgy, tbytes);

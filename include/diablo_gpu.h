@@ -173,12 +173,13 @@ int dgpu_last_step_ms(dgpu_handle *h, double *ms);
 /* Per-kernel device timing, for measurement only (no counterpart in the
  * reference).  While enabled, every launch of the step's kernels is bracketed by
  * HIP events on the handle's stream.  Kernel ids: 0 = z inverse transform of the
- * state, 1 = fused x pass (products), 2 = z forward transform of the products,
+ * state, 1 = the LES x pass (products), 2 = z forward transform of the products,
  * 3 = Fourier assembly + Crank-Nicolson solves, 4 = projection + pressure update;
  * LES substeps add 5 = BC posts, 6 = gradient fields, 7 = AMD model,
  * 8 = physical-column CN solves, 9 = right-hand-side and solution transforms,
  * 10 = the learned closure's coefficients (les_model_type 6, dgpu_closure.h),
- * 11 = the x inverse transforms of the state ahead of the products.
+ * 11 = the DNS x pass (x inverse transforms, products and x forward transforms
+ * in one kernel marching in y).
  * dgpu_timing_read returns the device time and launch count accumulated since the
  * last enable, and the name.  Enabling resets the dgpu_step sampling to every call. */
 int dgpu_timing_enable(dgpu_handle *h, int32_t enable);
