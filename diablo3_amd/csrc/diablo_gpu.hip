@@ -93,6 +93,8 @@ struct dgpu_handle {
   void* Hout[MAXO] = {};
   // persistent kernels: blocks to launch (a few per CU)
   int resident_blocks = 1024;
+  int cus = 256;        // compute units of the device
+  int xm_resident = 0;  // k_xmarch blocks resident at once on the device
   void *twx64 = nullptr, *twz64 = nullptr, *twx32 = nullptr, *twz32 = nullptr;
   double* d_fac = nullptr;
   double fac_dt = NAN;
@@ -134,6 +136,12 @@ struct dgpu_handle {
   double* hstage = nullptr;  // pinned staging, one Fortran array
   size_t fort_doubles = 0;
   std::vector<void*> allocs;
+  // dgpu_step as a captured HIP graph: the three substeps' launches, replayed
+  // while nothing that enters their arguments changes (delta_t, the LES gate)
+  bool graph_on = true;
+  hipGraphExec_t gexec = nullptr;
+  double g_dt = NAN;
+  int g_les = -1;
   // per-kernel timing (measurement only)
   bool timing = false;
   int timing_every = 1;       // record the kernel events on every n-th dgpu_step call
@@ -597,8 +605,10 @@ int launch_xprod(dgpu_handle* h, XProdArgs& a, int nj) {
   return 0;
 }
 
-// The DNS x pass: one thread per column (64 for the short lines), the y planes
-// split into segments so that the grid is about two blocks per CU
+// The DNS x pass: one thread per column (64 for the short lines).  The planes are
+// split into segments of as few planes as keep every block resident at once
+// (a block's planes run one after another; a segment costs two halo planes of
+// inverse transforms): one plane per block on small grids, 32 at 256x129x256.
 template <typename T, int NTH>
 int launch_xmarch(dgpu_handle* h, XProdArgs& a, int nj) {
   a.tw = sizeof(T) == 8 ? h->twx64 : h->twx32;
@@ -610,11 +620,15 @@ int launch_xmarch(dgpu_handle* h, XProdArgs& a, int nj) {
     if constexpr (lds > 163840) {
       return fail(2, "this nx needs the mixed-precision mode (x pass exceeds LDS in FP64)");
     } else {
+      if (h->xm_resident <= 0) {
+        int per_cu = 0;
+        HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_xmarch<T, N_, NT, NTH>, NT, 0));
+        h->xm_resident = std::max(1, per_cu) * h->cus;
+      }
       const int pairs = h->g.nz / 2;
-      const int want = h->resident_blocks > 0 ? h->resident_blocks : 512;
-      int nseg = std::min(nj, std::max(1, (want + pairs - 1) / pairs));
-      const int seg = (nj + nseg - 1) / nseg;
-      nseg = (nj + seg - 1) / seg;
+      const long long work = (long long)pairs * nj;
+      const int seg = (int)std::min<long long>(nj, std::max<long long>(1, (work + h->xm_resident - 1) / h->xm_resident));
+      const int nseg = (nj + seg - 1) / seg;
       dim3 grid(pairs, nseg, 1);
       k_xmarch<T, N_, NT, NTH><<<grid, NT, 0, h->st>>>(a, seg);
     }
@@ -1359,7 +1373,7 @@ int dgpu_create(const dgpu_config* cfg, dgpu_handle** out) {
     if (hipGetDevice(&dev) != hipSuccess) return bail(fail(10, "hipGetDevice"));
     h->dev = dev;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && ncu > 0)
-      h->resident_blocks = 2 * ncu;
+      h->resident_blocks = 2 * ncu, h->cus = ncu;
     if ((rc = scratch_address(dev, &h->noalloc))) return bail(rc);
   }
   ALLOC(h->twx64, sizeof(cd) * g.nx); ALLOC(h->twz64, sizeof(cd) * g.nz);
@@ -1480,6 +1494,7 @@ int dgpu_destroy(dgpu_handle* h) {
   for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
   if (h->e0) (void)hipEventDestroy(h->e0);
   if (h->e1) (void)hipEventDestroy(h->e1);
+  if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
   if (h->st) (void)hipStreamDestroy(h->st);
   delete h;
   return 0;
@@ -1565,6 +1580,10 @@ int dgpu_download_les(dgpu_handle* h, double* nu_t, double* kappa_t) {
 int dgpu_set_closure(dgpu_handle* h, const float* params, const float* norm) {
   if (!h || !params || !norm) return fail(1, "null argument");
   if (!h->c.use_les || h->c.les_model_type != 6) return fail(2, "dgpu_set_closure needs an LES handle of type 6");
+  if (h->gexec) {  // the closure's branch changes the captured launches
+    HIPCHK(hipGraphExecDestroy(h->gexec));
+    h->gexec = nullptr;
+  }
   // BF16 copies of the forward weights: w1t [64][16], w2t [64][64], w3t [16][64]
   // (round to nearest even, as the device's own conversion)
   auto bf = [](float x) -> unsigned short {
@@ -1655,6 +1674,17 @@ int dgpu_les_features(dgpu_handle* h, double time, float* X, float* scale, float
   ok = ok && hipStreamSynchronize(h->st) == hipSuccess;
   cleanup();
   if (!ok) return fail(10, "dgpu_les_features: kernel or download failed");
+  return 0;
+}
+
+int dgpu_set_graph(dgpu_handle* h, int32_t enable) {
+  if (!h) return fail(1, "null handle");
+  h->graph_on = enable != 0;
+  if (!h->graph_on && h->gexec) {
+    HIPCHK(hipStreamSynchronize(h->st));
+    HIPCHK(hipGraphExecDestroy(h->gexec));
+    h->gexec = nullptr;
+  }
   return 0;
 }
 
@@ -1932,6 +1962,46 @@ int dgpu_rk_substep(dgpu_handle* h, int32_t rk_step, double delta_t, double time
 int dgpu_step(dgpu_handle* h, double delta_t, double time, int32_t time_step, int32_t create_new_flow) {
   if (!h) return fail(1, "null handle");
   h->timing_now = (h->timing_calls++ % h->timing_every) == 0;
+  const dgpu_config& c = h->c;
+  // The step's launches depend on the time only through the f_type 2 and 4
+  // forcings, on host decisions inside the step only with variable_dt (courant
+  // reads dt back in rk 3), and carry per-kernel events only while timing: every
+  // other step replays one captured graph per (delta_t, LES gate).
+  const bool les_on = c.use_les && (!create_new_flow || time_step > 100);
+  if (h->graph_on && !h->variable_dt && c.f_type != 2 && c.f_type != 4 && !(h->timing && h->timing_now)) {
+    int rc;
+    if (!h->gexec || h->g_dt != delta_t || h->g_les != (int)les_on) {
+      if (h->gexec) {
+        HIPCHK(hipGraphExecDestroy(h->gexec));
+        h->gexec = nullptr;
+      }
+      if ((rc = ensure_factors(h, delta_t))) return rc;  // host work and copies: before the capture
+      HIPCHK(hipStreamBeginCapture(h->st, hipStreamCaptureModeThreadLocal));
+      for (int rk = 1; rk <= 3 && !rc; ++rk) rc = substep(h, rk, delta_t, time, time_step, create_new_flow);
+      hipGraph_t graph = nullptr;
+      const hipError_t ec = hipStreamEndCapture(h->st, &graph);
+      if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+      }
+      if (ec != hipSuccess) return fail(10, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+      const hipError_t ei = hipGraphInstantiate(&h->gexec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (ei != hipSuccess) {
+        h->gexec = nullptr;
+        return fail(10, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+      }
+      h->g_dt = delta_t;
+      h->g_les = (int)les_on;
+    }
+    HIPCHK(hipEventRecord(h->e0, h->st));
+    HIPCHK(hipGraphLaunch(h->gexec, h->st));
+    HIPCHK(hipEventRecord(h->e1, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    rc = harvest(h);
+    h->timing_now = true;
+    return rc;
+  }
   HIPCHK(hipEventRecord(h->e0, h->st));
   for (int rk = 1; rk <= 3; ++rk) {
     int rc = substep(h, rk, delta_t, time, time_step, create_new_flow);

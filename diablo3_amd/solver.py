@@ -47,6 +47,7 @@ def load():
                                   ctypes.c_int32, ctypes.c_int32]
     L.dgpu_step.argtypes = [P, ctypes.c_double, ctypes.c_double, ctypes.c_int32, ctypes.c_int32]
     L.dgpu_rem_div.argtypes = [P]
+    L.dgpu_set_graph.argtypes = [P, ctypes.c_int32]
     L.dgpu_timing_every.argtypes = [P, ctypes.c_int32]
     L.dgpu_poisson_p.argtypes = [P]
     L.dgpu_filter_chan.argtypes = [P, ctypes.c_int32]
@@ -285,6 +286,10 @@ class Solver:
     def filter_chan(self, n: int = 0):
         """filter_chan(n+1) of the reference on the device state (dgpu_filter_chan)."""
         self._check(load().dgpu_filter_chan(self.h, n))
+
+    def set_graph(self, enable: bool):
+        """dgpu_step as a captured HIP graph (default) or as direct launches."""
+        self._check(load().dgpu_set_graph(self.h, int(bool(enable))))
 
     def rem_div(self):
         self._check(load().dgpu_rem_div(self.h))

@@ -132,6 +132,14 @@ int dgpu_rk_substep(dgpu_handle *h, int32_t rk_step, double delta_t, double time
 int dgpu_step(dgpu_handle *h, double delta_t, double time, int32_t time_step,
               int32_t create_new_flow);
 
+/* dgpu_step runs as a captured HIP graph of its three substeps (on by default;
+ * no counterpart in the reference): captured on the first step, replayed while
+ * delta_t and the LES spin-up gate are unchanged.  Steps whose launches depend
+ * on the time (f_type 2 and 4 forcings), on variable_dt, or that record per-kernel
+ * events run their launches directly.  enable = 0 turns the graph off (and frees
+ * it); the results are the same either way. */
+int dgpu_set_graph(dgpu_handle *h, int32_t enable);
+
 /* Projection of the device velocity onto divergence-free fields: rem_div_chan
  * (for/solver.f90:1302-1379), called by create_flow_chan (for/flow.f90:487). */
 int dgpu_rem_div(dgpu_handle *h);
