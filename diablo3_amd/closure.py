@@ -33,20 +33,22 @@ def _check(rc):
         raise RuntimeError(f"libdiablo_gpu error {rc}: {load().dgpu_last_error().decode()}")
 
 
-def make_norm(X: np.ndarray, envelope_quantile: float = 0.0) -> np.ndarray:
+def make_norm(X: np.ndarray, envelope_quantile: float = 0.0, max_rows: int = 10_000_000) -> np.ndarray:
     """Normaliser from training features X [n][16]: shift = mean, scale = 1/std
     on the 13 live features, and the envelope [lo, hi] = the [q, 1-q] quantiles
-    (q = 0: min and max)."""
-    X = np.asarray(X, dtype=np.float64)[:, :NFEAT]
+    (q = 0: min and max, over all rows).  Mean, std and quantiles of more than
+    max_rows rows come from an evenly strided subset of max_rows rows."""
+    step = max(1, X.shape[0] // max_rows)
+    Xs = np.asarray(X[::step, :NFEAT], dtype=np.float64)
     n = np.zeros(NNORM, dtype=np.float32)
-    sd = X.std(axis=0)
-    n[:NFEAT] = X.mean(axis=0)
+    sd = Xs.std(axis=0)
+    n[:NFEAT] = Xs.mean(axis=0)
     n[16:16 + NFEAT] = np.where(sd > 0, 1.0 / np.where(sd > 0, sd, 1.0), 0.0)
     if envelope_quantile > 0:
-        lo = np.quantile(X, envelope_quantile, axis=0)
-        hi = np.quantile(X, 1.0 - envelope_quantile, axis=0)
+        lo = np.quantile(Xs, envelope_quantile, axis=0)
+        hi = np.quantile(Xs, 1.0 - envelope_quantile, axis=0)
     else:
-        lo, hi = X.min(axis=0), X.max(axis=0)
+        lo, hi = X[:, :NFEAT].min(axis=0), X[:, :NFEAT].max(axis=0)
     n[32:48] = -np.inf
     n[48:64] = np.inf
     n[32:32 + NFEAT] = lo

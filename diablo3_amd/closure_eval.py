@@ -20,7 +20,7 @@ from typing import Dict, List, Optional
 
 import numpy as np
 
-from . import closure, sgs_data, train_closure
+from . import closure, train_closure
 from .problem import Problem
 from .solver import Solver
 
@@ -61,10 +61,7 @@ def filtered_states(dns_cfg, les_cfg, seed: int, steps: List[int], log=print):
         g.step(time=t, time_step=n, create_new_flow=True)
         t += dns_cfg.delta_t
         if n in steps:
-            s = g.download_state()
-            phys = {"u1": g.fft_to_physical(s.cu1), "u2": g.fft_to_physical(s.cu2),
-                    "u3": g.fft_to_physical(s.cu3), "th": g.fft_to_physical(s.cth[0])}
-            out[n] = sgs_data.filter_snapshot(dns_cfg, pd.met, phys, les_cfg, pl.met).state
+            out[n] = train_closure.filtered(g, pd, pl).state
             log(f"  filtered DNS seed {seed} at step {n}")
     g.close()
     return out

@@ -59,17 +59,17 @@ def small_dns(nx: int = 16, ny: int = 17, nz: int = 16, precision: int = FP64) -
 
 
 def front_les(nx: int, ny: int, nz: int, re: float, dt: float, amp: float, n2: float = 1.0,
-              precision: int = FP64) -> Config:
+              precision: int = FP64, length: float = 8.0 * math.pi) -> Config:
     """Finite front with the AMD closure (les_model_type 5, C_AMD 0.2887 of
-    for/les.f90:25): Lx = Lz = 8 pi, f = Ro_inv / delta = 1 with Ro_inv = delta =
-    0.5, so the tanh front of IC type 6 (for/flow.f90:629-646, 366-385) has width
-    0.5 and M^2 = 1 at its core, stratified with Ri(1) = N^2, in thermal-wind
-    balance; the background gradient dTHdX = 2 Ro_inv / Lx and the top Neumann
-    compensation of w come from input_chan (for/parameters.f90:266-273).  Bottom
-    no-slip, top free-slip, scalar no-flux walls."""
+    for/les.f90:25): Lx = Lz = `length` (8 pi), f = Ro_inv / delta = 1 with
+    Ro_inv = delta = 0.5, so the tanh front of IC type 6 (for/flow.f90:629-646,
+    366-385) has width 0.5 and M^2 = 1 at its core, stratified with Ri(1) = N^2,
+    in thermal-wind balance; the background gradient dTHdX = 2 Ro_inv / Lx and the
+    top Neumann compensation of w come from input_chan (for/parameters.f90:266-273).
+    Bottom no-slip, top free-slip, scalar no-flux walls."""
     return Config(
         nx=nx, ny=ny, nz=nz, n_th=1, flavor="Front", use_les=True, les_model_type=5,
-        re=re, beta=1.0, lx=8.0 * math.pi, ly=2.0, lz=8.0 * math.pi, nu_v_scale=1.0,
+        re=re, beta=1.0, lx=length, ly=2.0, lz=length, nu_v_scale=1.0,
         delta_t=dt, ri=[n2], pr=[1.0], ic_type=6, kick=2.0 * amp, physical_noise=True,
         ro=2.0, delta=0.5, grav_x=0.0, grav_y=1.0, grav_z=0.0, f_type=3,
         u_bc_ymin=DIRICHLET, u_bc_ymin_c1=0.0, w_bc_ymin=DIRICHLET, w_bc_ymin_c1=0.0,

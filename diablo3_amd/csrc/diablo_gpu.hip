@@ -21,6 +21,7 @@
 #include "dgpu_closure.hpp"
 #include "dgpu_stats.hpp"
 #include "dgpu_courant.hpp"
+#include "dgpu_boxfilter.hpp"
 #include "dgpu_error.hpp"
 
 #include <algorithm>
@@ -93,6 +94,8 @@ struct dgpu_handle {
   void* Hout[MAXO] = {};
   // persistent kernels: blocks to launch (a few per CU)
   int resident_blocks = 1024;
+  void* d_box = nullptr;  // dgpu_box_filter: y weights, row ranges, output
+  size_t box_bytes = 0;
   int cus = 256;        // compute units of the device
   int xm_resident = 0;  // k_xmarch blocks resident at once on the device
   void *twx64 = nullptr, *twz64 = nullptr, *twx32 = nullptr, *twz32 = nullptr;
@@ -1495,6 +1498,7 @@ int dgpu_destroy(dgpu_handle* h) {
   if (h->e0) (void)hipEventDestroy(h->e0);
   if (h->e1) (void)hipEventDestroy(h->e1);
   if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+  if (h->d_box) (void)hipFree(h->d_box);
   if (h->st) (void)hipStreamDestroy(h->st);
   delete h;
   return 0;
@@ -1674,6 +1678,70 @@ int dgpu_les_features(dgpu_handle* h, double time, float* X, float* scale, float
   ok = ok && hipStreamSynchronize(h->st) == hipSuccess;
   cleanup();
   if (!ok) return fail(10, "dgpu_les_features: kernel or download failed");
+  return 0;
+}
+
+int dgpu_box_filter(dgpu_handle* h, int32_t fx, int32_t fz, int32_t nl, const double* mf, const double* mg,
+                    double* out) {
+  if (!h || !mf || !mg || !out) return fail(1, "null argument");
+  const Geo& g = h->g;
+  if (h->nth < 1) return fail(2, "dgpu_box_filter needs a scalar (n_th >= 1)");
+  if (fx < 2 || fz < 2 || fx % 2 || fz % 2 || g.nx % fx || g.nz % fz)
+    return fail(2, "dgpu_box_filter: fx, fz must be even divisors of nx, nz");
+  if (nl < 2) return fail(2, "dgpu_box_filter: nl must be >= 2");
+  const int nl2 = nl + 2, nxl = g.nx / fx, nzl = g.nz / fz;
+  int rc = ensure_grad_buffers(h);
+  if (rc) return rc;
+  // physical u1, u2, u3, th of the state, FP64
+  const cd* cs[4] = {h->U[0], h->U[1], h->U[2], h->U[3]};
+  void* hs[4] = {h->HinD[0], h->HinD[1], h->HinD[2], h->HinD[3]};
+  double* rs[4] = {(double*)h->PgD[0], (double*)h->PgD[3], (double*)h->PgD[6], (double*)h->PgD[9]};
+  if ((rc = launch_inv_z<double>(h, 4, cs, hs, 0, g.ny2))) return rc;
+  if ((rc = launch_inv_x<double>(h, 4, hs, rs, 0, g.ny2))) return rc;
+  // the y weights, their nonzero row ranges and the output in one device buffer
+  const size_t mbytes = sizeof(double) * (size_t)nl2 * g.ny2;
+  const size_t rbytes = sizeof(int) * 2 * (size_t)nl2;
+  const size_t obytes = sizeof(double) * BOX_FIELDS * (size_t)nl2 * nzl * nxl;
+  const size_t need = 2 * round256(mbytes) + 2 * round256(rbytes) + obytes;
+  if (h->box_bytes < need) {
+    HIPCHK(hipStreamSynchronize(h->st));
+    if (h->d_box) HIPCHK(hipFree(h->d_box));
+    h->d_box = nullptr;
+    h->box_bytes = 0;
+    HIPCHK(hipMalloc(&h->d_box, need));
+    h->box_bytes = need;
+  }
+  std::vector<int> ranges(4 * (size_t)nl2);
+  for (int m = 0; m < 2; ++m) {
+    const double* M = m ? mg : mf;
+    for (int j = 0; j < nl2; ++j) {
+      int lo = 1, hi = 0;
+      for (int k = 0; k < g.ny2; ++k)
+        if (M[(size_t)j * g.ny2 + k] != 0.0) {
+          if (lo > hi) lo = k;
+          hi = k;
+        }
+      ranges[(size_t)m * 2 * nl2 + 2 * j] = lo;
+      ranges[(size_t)m * 2 * nl2 + 2 * j + 1] = hi;
+    }
+  }
+  unsigned char* b = (unsigned char*)h->d_box;
+  BoxArgs a{};
+  a.g = g; a.fx = fx; a.fz = fz; a.nxl = nxl; a.nzl = nzl; a.nl2 = nl2;
+  for (int f = 0; f < 4; ++f) a.u[f] = rs[f];
+  a.mf = (const double*)b;
+  a.mg = (const double*)(b + round256(mbytes));
+  a.rf = (const int*)(b + 2 * round256(mbytes));
+  a.rg = (const int*)(b + 2 * round256(mbytes) + round256(rbytes));
+  a.out = (double*)(b + 2 * round256(mbytes) + 2 * round256(rbytes));
+  HIPCHK(hipMemcpyAsync((void*)a.mf, mf, mbytes, hipMemcpyHostToDevice, h->st));
+  HIPCHK(hipMemcpyAsync((void*)a.mg, mg, mbytes, hipMemcpyHostToDevice, h->st));
+  HIPCHK(hipMemcpyAsync((void*)a.rf, ranges.data(), rbytes, hipMemcpyHostToDevice, h->st));
+  HIPCHK(hipMemcpyAsync((void*)a.rg, ranges.data() + 2 * (size_t)nl2, rbytes, hipMemcpyHostToDevice, h->st));
+  k_box_filter<<<dim3((unsigned)((nxl * nzl + 255) / 256), (unsigned)nl2), 256, 0, h->st>>>(a);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, a.out, obytes, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
   return 0;
 }
 

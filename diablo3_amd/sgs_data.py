@@ -101,9 +101,17 @@ class Filtered:
     kappa_star: np.ndarray
 
 
-def filter_snapshot(dns_cfg: Config, dns_met: dict, phys: Dict[str, np.ndarray],
-                    les_cfg: Config, les_met: dict) -> Filtered:
-    """phys: u1, u2, u3, th as [j][z][x] physical arrays of the DNS (rows 0..ny+1)."""
+# the filtered fields, in the order of dgpu_box_filter (include/dgpu_closure.h):
+# u1, u2 at GY, u2 at GYF, u3, th, then the products of (u1, u2 at GYF, u3) and th
+BOX_FIELDS = ("u1", "u2g", "u2", "u3", "th", "u1u1", "u1u2", "u1u3", "u2u2", "u2u3", "u3u3",
+              "u1th", "u2th", "u3th")
+
+
+def filter_fields(dns_cfg: Config, dns_met: dict, phys: Dict[str, np.ndarray],
+                  les_cfg: Config, les_met: dict) -> np.ndarray:
+    """The 14 filtered fields [14][ny_l + 2][nz_l][nx_l] of dgpu_box_filter, in
+    numpy (the restatement the device filter is tested against).  phys: u1, u2,
+    u3, th as [j][z][x] physical arrays of the DNS (rows 0..ny+1)."""
     fx, fz = dns_cfg.nx // les_cfg.nx, dns_cfg.nz // les_cfg.nz
     assert fx * les_cfg.nx == dns_cfg.nx and fz * les_cfg.nz == dns_cfg.nz and fx % 2 == 0 and fz % 2 == 0
     yf = y_filters(dns_met, les_met)
@@ -116,16 +124,30 @@ def filter_snapshot(dns_cfg: Config, dns_met: dict, phys: Dict[str, np.ndarray],
     def F(a, gy=False):
         return apply_y(yf.gy if gy else yf.gyf, tophat_xz(a, fx, fz))
 
-    b1, b2g, b3, bt = F(u1), F(u2, gy=True), F(u3), F(th)
-    b2 = F(u2f)
+    return np.stack([F(u1), F(u2, gy=True), F(u2f), F(u3), F(th),
+                     F(u1 * u1), F(u1 * u2f), F(u1 * u3), F(u2f * u2f), F(u2f * u3), F(u3 * u3),
+                     F(u1 * th), F(u2f * th), F(u3 * th)])
+
+
+def y_weights(dns_met: dict, les_met: dict):
+    """(mf, mg): the row weights dgpu_box_filter takes, C-contiguous."""
+    yf = y_filters(dns_met, les_met)
+    return np.ascontiguousarray(yf.gyf), np.ascontiguousarray(yf.gy)
+
+
+def targets(les_cfg: Config, les_met: dict, fields: np.ndarray) -> "Filtered":
+    """From the 14 filtered fields: the filtered state, the subgrid stress and flux,
+    and the least-squares nu*, kappa* on rows jstart..ny-1 of the LES grid."""
+    d = dict(zip(BOX_FIELDS, fields))
+    b1, b2g, b2, b3, bt = d["u1"], d["u2g"], d["u2"], d["u3"], d["th"]
     # GYF-located filtered velocities, for the stresses and the strain
     vel = [b1, b2, b3]
-    raw = [u1, u2f, u3]
+    names = ("u1", "u2", "u3")
     tau = {}
     for i in range(3):
         for k in range(i, 3):
-            tau[i, k] = F(raw[i] * raw[k]) - vel[i] * vel[k]
-    q = [F(raw[i] * th) - vel[i] * bt for i in range(3)]
+            tau[i, k] = d[names[i] + names[k]] - vel[i] * vel[k]
+    q = [d[names[i] + "th"] - vel[i] * bt for i in range(3)]
     # strain and scalar gradient of the filtered field on the LES grid
     ny_l = les_cfg.ny
     rows = slice(les_cfg.jstart, ny_l)
@@ -159,6 +181,12 @@ def filter_snapshot(dns_cfg: Config, dns_met: dict, phys: Dict[str, np.ndarray],
         cu3=layout.fft_xz_to_fourier(les_cfg, b3), cp=layout.compact_zeros(les_cfg),
         cth=[layout.fft_xz_to_fourier(les_cfg, bt)])
     return Filtered(st, nu[rows], ka[rows])
+
+
+def filter_snapshot(dns_cfg: Config, dns_met: dict, phys: Dict[str, np.ndarray],
+                    les_cfg: Config, les_met: dict) -> Filtered:
+    """phys: u1, u2, u3, th as [j][z][x] physical arrays of the DNS (rows 0..ny+1)."""
+    return targets(les_cfg, les_met, filter_fields(dns_cfg, dns_met, phys, les_cfg, les_met))
 
 
 def make_samples(X: np.ndarray, scale: np.ndarray, inside: np.ndarray, nu: np.ndarray, ka: np.ndarray,

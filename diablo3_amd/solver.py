@@ -89,6 +89,7 @@ def load():
     L.dgpu_mlp_train.argtypes = [P, ctypes.c_int32, ctypes.c_int32, ctypes.c_float, ctypes.c_uint64, F, F]
     L.dgpu_mlp_predict.argtypes = [P, F, I64, F]
     L.dgpu_set_closure.argtypes = [P, F, F]
+    L.dgpu_box_filter.argtypes = [P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, D, D, D]
     L.dgpu_les_features.argtypes = [P, ctypes.c_double, F, F, F, ctypes.POINTER(ctypes.c_int32),
                                     ctypes.POINTER(I64)]
     _lib = L
@@ -203,6 +204,19 @@ class Solver:
                                         am.ctypes.data_as(F), ins.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
                                         ctypes.byref(n)))
         return X, sc, am, ins
+
+    def box_filter(self, fx: int, fz: int, mf: np.ndarray, mg: np.ndarray) -> np.ndarray:
+        """dgpu_box_filter: the state filtered onto a grid coarser by fx, fz in x, z
+        and with the LES rows of the y weights mf, mg ([nl+2][Ny+2]): the 14
+        fields of sgs_data.BOX_FIELDS as [14][nl+2][Nz/fz][Nx/fx]."""
+        cfg = self.cfg
+        mf = np.ascontiguousarray(mf, np.float64)
+        mg = np.ascontiguousarray(mg, np.float64)
+        assert mf.shape == mg.shape and mf.shape[1] == cfg.ny2
+        nl2 = mf.shape[0]
+        out = np.zeros((14, nl2, cfg.nz // fz, cfg.nx // fx), np.float64)
+        self._check(load().dgpu_box_filter(self.h, fx, fz, nl2 - 2, _dp(mf), _dp(mg), _dp(out)))
+        return out
 
     def set_variable_dt(self, variable_dt: bool, update_dt: int = 1, cfl: float = 0.5):
         self._check(load().dgpu_set_variable_dt(self.h, int(variable_dt), update_dt, cfl))

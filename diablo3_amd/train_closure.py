@@ -11,6 +11,7 @@ and the a-priori errors of the network and of AMD on held-out snapshots.
 from __future__ import annotations
 
 import argparse
+import math
 import time as _time
 from dataclasses import dataclass
 from typing import List, Optional, Sequence
@@ -23,11 +24,21 @@ from .problem import Problem
 from .solver import Solver
 
 
-def front_pair(dns_shape, les_shape, re: float, n2: float, amp: float, dt: float):
-    """A DNS config and its LES grid of the same finite front (front_les physics)."""
-    d = configs.front_les(*dns_shape, re, dt, amp, n2=n2).copy(use_les=False).derive()
-    l = configs.front_les(*les_shape, re, dt, amp, n2=n2).copy(les_model_type=6).derive()
+def front_pair(dns_shape, les_shape, re: float, n2: float, amp: float, dt: float,
+               length: float = 8.0 * math.pi):
+    """A DNS config and its LES grid of the same finite front (front_les physics,
+    Lx = Lz = length)."""
+    d = configs.front_les(*dns_shape, re, dt, amp, n2=n2, length=length).copy(use_les=False)
+    l = configs.front_les(*les_shape, re, dt, amp, n2=n2, length=length).copy(les_model_type=6)
     return d, l
+
+
+def filtered(g: Solver, pd: Problem, pl: Problem) -> sgs_data.Filtered:
+    """The DNS handle's state box-filtered onto the LES grid on the device
+    (dgpu_box_filter), then targets and the filtered state on the host."""
+    mf, mg = sgs_data.y_weights(pd.met, pl.met)
+    f = g.box_filter(pd.cfg.nx // pl.cfg.nx, pd.cfg.nz // pl.cfg.nz, mf, mg)
+    return sgs_data.targets(pl.cfg, pl.met, f)
 
 
 @dataclass
@@ -56,10 +67,7 @@ def snapshots(dns_cfg: Config, les_cfg: Config, seed: int, steps: int, every: in
         g.step(time=t, time_step=n, create_new_flow=True)
         t += dns_cfg.delta_t
         if n > start and n % every == 0:
-            s = g.download_state()
-            phys = {"u1": g.fft_to_physical(s.cu1), "u2": g.fft_to_physical(s.cu2),
-                    "u3": g.fft_to_physical(s.cu3), "th": g.fft_to_physical(s.cth[0])}
-            f = sgs_data.filter_snapshot(dns_cfg, pd.met, phys, les_cfg, pl.met)
+            f = filtered(g, pd, pl)
             gl.upload_state(f.state)
             X, sc, am, _ = gl.les_features(t)
             out.append(Snapshot(X, sc, am, f.nu_star.ravel(), f.kappa_star.ravel(), seed, n))

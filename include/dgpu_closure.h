@@ -106,6 +106,23 @@ int dgpu_set_closure(dgpu_handle *h, const float *params, const float *norm);
 int dgpu_les_features(dgpu_handle *h, double time, float *X, float *scale, float *amd, int32_t *inside,
                       int64_t *n_out);
 
+/* Training data: the handle's state (a DNS) box-filtered onto a coarser grid of
+ * the same domain, with the products of the subgrid stress and flux.  No
+ * counterpart in the reference's solver (its post-processing filters DNS
+ * offline, proc/functions/coarsen.m:1-25).  In x and z a top-hat of width fx dx
+ * (fz dz) centred on every fx-th (fz-th) point, trapezoid weights over its
+ * fx + 1 (fz + 1) points, periodic (fx, fz even divisors of Nx, Nz); in y the
+ * rows of an LES grid with nl interior rows: mf [nl+2][Ny+2] weights of the DNS
+ * rows for the GYF fields, mg [nl+2][Ny+2] for the GY field (row-major; the
+ * caller's choice, e.g. the overlap of the LES cells with the DNS cells).  out:
+ * [14][nl+2][Nz/fz][Nx/fx] doubles, the filtered
+ *   0 u1, 1 u2 (mg), 2 u2 at GYF, 3 u3, 4 th,
+ *   5 u1u1, 6 u1u2, 7 u1u3, 8 u2u2, 9 u2u3, 10 u3u3, 11 u1 th, 12 u2 th, 13 u3 th
+ * with u2 at GYF the DNS's own (u2(j) + u2(j+1))/2 on rows 1..Ny, th scalar 1.
+ * Needs n_th >= 1.  The state is not changed. */
+int dgpu_box_filter(dgpu_handle *h, int32_t fx, int32_t fz, int32_t nl, const double *mf, const double *mg,
+                    double *out);
+
 #ifdef __cplusplus
 }
 #endif
