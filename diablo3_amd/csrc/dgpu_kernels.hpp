@@ -632,8 +632,9 @@ __device__ __forceinline__ void eq_rows(int e, int jstart, int jend, const int* 
 // b + gridDim.x, ... with b the block's xcd_tile position (gridDim.x a multiple
 // of 8: XCD x works on a contiguous range of each round's tiles); the next
 // tile's spectra are loaded into registers while the current tile is
-// transformed, so the loads overlap the FFT.  One tile per block otherwise, in
-// xcd_tile order.
+// transformed and stored, so the loads overlap the FFT and the stores (kz sits
+// in LDS: a global load of kz in the store loop made every store wait for the
+// whole prefetch).  One tile per block otherwise, in xcd_tile order.
 template <typename T, int NZ, int TI, int NT, bool PERSIST>
 __global__ __launch_bounds__(NT) void k_fwdz_nl(NlArgs a, int ntiles) {
   constexpr int LD = NZ + 1;
@@ -641,9 +642,11 @@ __global__ __launch_bounds__(NT) void k_fwdz_nl(NlArgs a, int ntiles) {
   constexpr int PER = (NE + NT - 1) / NT;      // per thread
   __shared__ cx<T> s[2 * TI * LD];
   __shared__ cx<T> stw[NZ];  // twiddles
+  __shared__ T skz[NZ];      // kz of the packed rows (kzr < NZ)
   stage_twiddles<T, NZ, NT>(stw, (const cx<T>*)a.tw, threadIdx.x);
   const Geo& g = a.g;
   const int tid = threadIdx.x;
+  for (int k = tid; k < g.kzr; k += NT) skz[k] = (T)a.kz[k];
   const int nib = (g.kxr + TI - 1) / TI, nj = g.ny - a.jlo + 1;
   const T rnz = T(1) / T(NZ);
   // tile -> (equation e, plane j, first column i0); live: j is one of e's rows
@@ -699,7 +702,7 @@ __global__ __launch_bounds__(NT) void k_fwdz_nl(NlArgs a, int ntiles) {
         if (i >= g.kxr) continue;
         const int zp = zpos(g, k);
         const cx<T> fn = s[ii * LD + swz<T>(zp)], fz = s[(TI + ii) * LD + swz<T>(zp)];
-        NL[(long long)k * g.kxp + i] = rnz * (fn - ik((T)a.kz[k], fz));
+        NL[(long long)k * g.kxp + i] = rnz * (fn - ik(skz[k], fz));
       }
     }
     __syncthreads();  // s is rewritten by the next tile
