@@ -315,8 +315,8 @@ def main():
         cfg64 = configs.NAMED[args.workload](FP64)
         out["cpu_baseline"] = cpu_baseline(cfg64, threads, steps=3)
         log(f"CPU baseline, {threads} threads: {out['cpu_baseline']['sample']}")
-        # one core, as the reference's one-rank run: one step's three substeps
-        out["cpu_baseline"]["one_core"] = cpu_baseline(cfg64, 1, steps=1)
+        # one core, as the reference's one-rank run: the median over three steps
+        out["cpu_baseline"]["one_core"] = cpu_baseline(cfg64, 1, steps=3)
         log(f"CPU baseline, 1 thread: {out['cpu_baseline']['one_core']['sample']}")
     else:
         out["cpu_baseline"] = None
