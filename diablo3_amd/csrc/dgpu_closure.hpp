@@ -29,19 +29,38 @@ struct ClosureArgs {
   void* cgy;      // [ny2][pp] outputs at GY (step mode), precision T; sign bit set
   void* kgy;      // where the point took the AMD value
   int feat, jf0;  // feature mode and its first row
+  // per-launch constants: -1/(2 C^2), sqrt(2)/nu, 1/dxs, 1/dzs
+  double md2;
+  float rnu2, rdxs, rdzs;
   float* X;       // [(ny - jf0) pp][16]
   float* scale;   // C^2 D2 sqrt(2 s2)
   float* amd;     // [n][2]: type-5 velocity and scalar coefficients / sqrt(2 s2)
   int* inside;
 };
 
+// n / d where d > 0, else 0, with the hardware reciprocal (1 ulp): the features
+// become BF16 network inputs, whose resolution is 2^-8
+__device__ __forceinline__ float fq(float n, float d) { return d > 0.f ? n * __builtin_amdgcn_rcpf(d) : 0.f; }
+// tanh from one exp2 and one reciprocal: (1 - e^-2a) / (1 + e^-2a), and a - a^3/3
+// below a = 2^-8 (relative error under 1e-5 either way)
+__device__ __forceinline__ float tanh_fast(float v) {
+  const float a = fabsf(v);
+  const float t = __builtin_amdgcn_exp2f(-2.8853900817779268f * a);
+  const float r = a < 0.00390625f ? a * (1.f - 0.33333333f * a * a) : (1.f - t) * __builtin_amdgcn_rcpf(1.f + t);
+  return copysignf(r, v);
+}
+// log(1 + y), y >= 0, from the hardware log2 (y (1 - y/2) below 2^-10)
+__device__ __forceinline__ float log1p_fast(float y) {
+  return y < 0.0009765625f ? y * (1.f - 0.5f * y) : __builtin_amdgcn_logf(1.f + y) * 0.69314718055994531f;
+}
+
 // the 13 features (raw) of one point from the scaled strain (a1..a6), rotation
 // (o4..o6), the scaled scalar gradient b, the unscaled th_y and |grad th|, the
-// filter width D2 and the y width delta_y, in float; returns s2 = S:S
+// filter width term red = D2 sqrt(2) / nu and the row's log(delta_y^2 / (dxs dzs)),
+// in float; returns s2 = S:S
 __device__ __forceinline__ float closure_features(float a1, float a2, float a3, float a4, float a5, float a6,
                                                   float o4, float o5, float o6, float bx, float by, float bz,
-                                                  float ty, float gth, float D2, float dyl, float dxs, float dzs,
-                                                  float nu, float gyb, float* f) {
+                                                  float ty, float gth, float red, float f12, float gyb, float* f) {
   const float s2 = a1 * a1 + a2 * a2 + a3 * a3 + 2.f * (a4 * a4 + a5 * a5 + a6 * a6);
   const float w2 = 2.f * (o4 * o4 + o5 * o5 + o6 * o6);
   const float e = s2 + w2;
@@ -78,20 +97,19 @@ __device__ __forceinline__ float closure_features(float a1, float a2, float a3, 
       bPb += b[i] * P[i][k] * b[k];
     }
   const float rs = sqrtf(s2);
-  auto q = [](float n, float d) { return d > 0.f ? n / d : 0.f; };
-  f[0] = q(w2 - s2, e);
-  f[1] = q(trS3, s2 * rs);
-  f[2] = q(trSP, rs * w2);
-  f[3] = q(-(trS3 + trSP), rs * e);
-  f[4] = q(trS2P, s2 * w2);
-  f[5] = q(by * by, b2);
-  f[6] = q(bSb, b2 * rs);
-  f[7] = q(bS2b, b2 * s2);
-  f[8] = q(bPb, b2 * w2);
-  f[9] = tanhf(q(gyb * ty, s2));
-  f[10] = tanhf(q(fabsf(gyb) * gth, s2));
-  f[11] = log1pf(D2 * sqrtf(2.f * s2) / nu);
-  f[12] = logf(dyl * dyl / (dxs * dzs));
+  f[0] = fq(w2 - s2, e);
+  f[1] = fq(trS3, s2 * rs);
+  f[2] = fq(trSP, rs * w2);
+  f[3] = fq(-(trS3 + trSP), rs * e);
+  f[4] = fq(trS2P, s2 * w2);
+  f[5] = fq(by * by, b2);
+  f[6] = fq(bSb, b2 * rs);
+  f[7] = fq(bS2b, b2 * s2);
+  f[8] = fq(bPb, b2 * w2);
+  f[9] = tanh_fast(fq(gyb * ty, s2));
+  f[10] = tanh_fast(fq(fabsf(gyb) * gth, s2));
+  f[11] = log1p_fast(red * rs);
+  f[12] = f12;
   f[13] = f[14] = f[15] = 0.f;
   return s2;
 }
@@ -132,7 +150,6 @@ __global__ __launch_bounds__(256) void k_closure(ClosureArgs c) {
   AmdIn<T> in;
   load_window<T, 3 * (3 + NTH)>(in, a, j, q);
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const double C2 = C_AMD * C_AMD;
   // step mode: GY rows 2..Nyp; feature mode: GYF rows jf0..Nyp-1
   const bool on = live && (c.feat ? (j >= c.jf0 && j <= ny - 1) : (j >= 2 && j <= ny));
   float fe[MLP_INP];
@@ -154,21 +171,25 @@ __global__ __launch_bounds__(256) void k_closure(ClosureArgs c) {
       o6 = T(0.5) * (p.o6 + m.o6);
       if constexpr (NTH > 0) t = amd_th_gyf(in, a, 9, j, q);
       dyl = a.dyf[j] * 2.0;
-      D2 = -a.dlt[j] / (2.0 * C2);
+      D2 = a.dlt[j] * c.md2;
     } else {
       const GyEntries<T> E = gy_entries(in, a, j, q);
       a1 = E.a1; a2 = E.a2; a3 = E.a3; a4 = E.a4; a5 = E.a5; a6 = E.a6; o4 = E.o4; o5 = E.o5; o6 = E.o6;
       if constexpr (NTH > 0) t = amd_th_gy(in, a, 9, j, q);
       dyl = a.dy[j] * 2.0;
-      D2 = -a.dlt[g.ny2 + j] / (2.0 * C2);
+      D2 = a.dlt[g.ny2 + j] * c.md2;
     }
     amd_v = amd_coef(a1, a2, a3, a4, a5, a6, o4, o5, o6);
     amd_t = t.c;
-    const float tx = (float)t.bx / (float)a.dxs, tz = (float)t.bz / (float)a.dzs, ty = (float)t.ty;
+    const float tx = (float)t.bx * c.rdxs, tz = (float)t.bz * c.rdzs, ty = (float)t.ty;
     const float gth = sqrtf(tx * tx + ty * ty + tz * tz);
+    // constant on a row, and an envelope bound of the training data is exactly
+    // one row's value: the same float division and logf as the training features
+    const float fy = (float)dyl;
+    const float f12 = logf(fy * fy / ((float)a.dxs * (float)a.dzs));
     s2 = closure_features((float)a1, (float)a2, (float)a3, (float)a4, (float)a5, (float)a6, (float)o4, (float)o5,
-                          (float)o6, (float)t.bx, (float)t.by, (float)t.bz, ty, gth, (float)D2, (float)dyl,
-                          (float)a.dxs, (float)a.dzs, (float)c.nu, (float)c.gyb, fe);
+                          (float)o6, (float)t.bx, (float)t.by, (float)t.bz, ty, gth, (float)D2 * c.rnu2, f12,
+                          (float)c.gyb, fe);
   }
   // network: 4 tiles of 16 samples per wave
 #pragma unroll

@@ -82,10 +82,16 @@ __device__ __forceinline__ bf16x4 ld4(const unsigned short* p) {
   r[2] = (short)(u.y & 0xffffu); r[3] = (short)(u.y >> 16);
   return r;
 }
-typedef __bf16 bfv4 __attribute__((ext_vector_type(4)));
-// four values at once with gfx950's v_cvt_pk_bf16_f32 (round to nearest even,
-// the same result as f2bf)
-__device__ __forceinline__ bf16x4 tobf(f32x4 v) { return __builtin_bit_cast(bf16x4, __builtin_convertvector(v, bfv4)); }
+typedef __bf16 bfv2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+// four values as two pairs, each one gfx950 v_cvt_pk_bf16_f32 (round to nearest
+// even, the same result as f2bf)
+__device__ __forceinline__ bf16x4 tobf(f32x4 v) {
+  const bfv2 lo = __builtin_convertvector(f32x2{v[0], v[1]}, bfv2);
+  const bfv2 hi = __builtin_convertvector(f32x2{v[2], v[3]}, bfv2);
+  const unsigned long long ul = __builtin_bit_cast(unsigned int, lo), uh = __builtin_bit_cast(unsigned int, hi);
+  return __builtin_bit_cast(bf16x4, ul | (uh << 32));
+}
 
 // activations of one 16-sample tile in the accumulator layout
 struct MlpAct {

@@ -1002,6 +1002,16 @@ int courant_dns(dgpu_handle* h) {
 // One substep of the LES branch: rk_chan_1 with les_chan / les_chan_th
 // (for/solver.f90:611-623) and the physical-space Crank–Nicolson solves with the
 // eddy viscosity (for/solver.f90:875-903, 990-1233).
+// k_closure's per-launch constants
+void closure_consts(dgpu_handle* h, ClosureArgs& k) {
+  k.nu = h->c.nu;
+  k.gyb = h->c.grav_y;
+  k.md2 = -1.0 / (2.0 * C_AMD * C_AMD);
+  k.rnu2 = (float)(std::sqrt(2.0) / h->c.nu);
+  k.rdxs = (float)(1.0 / k.a.dxs);
+  k.rdzs = (float)(1.0 / k.a.dzs);
+}
+
 // arguments of k_closure from the handle's network and an AmdArgs already filled
 ClosureArgs closure_args(dgpu_handle* h, const AmdArgs& aa) {
   ClosureArgs k{};
@@ -1017,8 +1027,7 @@ ClosureArgs closure_args(dgpu_handle* h, const AmdArgs& aa) {
   w.b3 = h->d_netp + MLP_OB3;
   w.norm = h->d_netnorm;
   k.have_net = h->have_net;
-  k.nu = h->c.nu;
-  k.gyb = h->c.grav_y;
+  closure_consts(h, k);
   k.cgy = h->Cgy; k.kgy = h->Kgy;
   return k;
 }
@@ -1655,8 +1664,7 @@ int dgpu_les_features(dgpu_handle* h, double time, float* X, float* scale, float
   } else {
     k.a = aa;
     k.have_net = 0;
-    k.nu = h->c.nu;
-    k.gyb = h->c.grav_y;
+    closure_consts(h, k);
   }
   k.feat = 1;
   k.jf0 = jf0;
