@@ -677,7 +677,9 @@ ColShape col_shape(const dgpu_handle* h, bool cn) {
   if (cn) {
     // k_cn: TC columns x all fields in LDS; one wave solves one field of the TC
     // columns in 64/TC chunks of L <= 32 rows (LMAX = 4 RPT >= L); the widest TC
-    // that fits
+    // that fits and still gives the CUs 1.5 blocks each, else the narrowest that fits
+    // (measured on MI355X: at 64x65x64, 946 columns, 8 columns per block 0.0228 ms,
+    // 4 0.0184, 2 0.0160, 1 0.0231; at 256x129x256, 8 0.196, 4 0.243)
     for (int tc : {8, 4, 2, 1}) {
       const size_t lds = (size_t)tc * ny2 * 16 * nf;
       const int rg = 256 / tc, need = (ny2 + rg - 1) / rg;
@@ -686,7 +688,10 @@ ColShape col_shape(const dgpu_handle* h, bool cn) {
       int rpt = 1;
       while (rpt < need) rpt *= 2;
       if (tc == 8 && need >= 1 && need <= 8) rpt = need;
-      if (lds <= 80 * 1024 && rpt <= 8) { cs = {tc, rpt, lds}; break; }
+      if (lds <= 80 * 1024 && rpt <= 8 && h->g.pc % tc == 0) {
+        cs = {tc, rpt, lds};
+        if (2 * (h->g.pc / tc) >= 3 * h->cus) break;
+      }
     }
     return cs;
   }
