@@ -98,6 +98,7 @@ struct dgpu_handle {
   size_t box_bytes = 0;
   int cus = 256;        // compute units of the device
   int xm_resident = 0;  // k_xmarch blocks resident at once on the device
+  int fz_resident = 0;  // persistent k_fwdz_nl blocks resident at once
   void *twx64 = nullptr, *twz64 = nullptr, *twx32 = nullptr, *twz32 = nullptr;
   double* d_fac = nullptr;
   double fac_dt = NAN;
@@ -649,8 +650,16 @@ int launch_fwdz_nl(dgpu_handle* h, NlArgs& a, int nj) {
       constexpr int TI = decltype(TIc)::value;
       constexpr bool PERSIST = decltype(Pc)::value;
       const int ntiles = (h->g.kxr + TI - 1) / TI * nj * (3 + h->nth);
-      const int nblk = PERSIST ? std::min(ntiles, h->resident_blocks) : ntiles;
+      // persistent: as many blocks as are resident at once (one per CU for the FP64
+      // 72 KiB tile at nz = 256: 0.126 -> 0.118 ms against two per CU)
+      if (PERSIST && h->fz_resident <= 0) {
+        int per_cu = 0;
+        HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fwdz_nl<T, N_, TI, 256, PERSIST>, 256, 0));
+        h->fz_resident = std::max(1, per_cu) * h->cus;
+      }
+      const int nblk = PERSIST ? std::min(ntiles, h->fz_resident) : ntiles;
       k_fwdz_nl<T, N_, TI, 256, PERSIST><<<nblk, 256, 0, h->st>>>(a, ntiles);
+      return 0;
     };
     using I0 = std::integral_constant<int, TI0>;
     using I1 = std::integral_constant<int, (TI0 > 1 ? TI0 / 2 : 1)>;
@@ -658,8 +667,8 @@ int launch_fwdz_nl(dgpu_handle* h, NlArgs& a, int nj) {
     using P1 = std::true_type;
     // measured: FP64 best persistent with TI columns; float with TI/2 per block at
     // nz = 256 (256^3), persistent with TI columns at nz = 512 (512x257x512)
-    if (sizeof(T) == 8 || h->g.nz >= 512) go(I0{}, P1{});
-    else go(I1{}, P0{});
+    const int rc = (sizeof(T) == 8 || h->g.nz >= 512) ? go(I0{}, P1{}) : go(I1{}, P0{});
+    if (rc) return rc;
   });
   HIPCHK(hipGetLastError());
   return 0;
