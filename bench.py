@@ -191,6 +191,11 @@ def les_speedup(steps, warmup):
             t += base.delta_t
         g.synchronize()
         out[name + "_ms_per_step"] = round((time.perf_counter() - t0) / steps * 1e3, 3)
+        if model == 6:
+            # the share of the closure's points that took the network in the last
+            # substep (the rest left the training envelope and took AMD)
+            nn, npts = g.closure_counts()
+            out["learned_network_fraction"] = round(nn / max(npts, 1), 4)
         g.close()
         log(f"LES 512 {name}: {out[name + '_ms_per_step']} ms/step")
     out["speedup_learned_mixed_vs_fp64_amd"] = round(out["fp64_amd_ms_per_step"] / out["mixed_learned_ms_per_step"], 3)

@@ -191,11 +191,14 @@ __global__ __launch_bounds__(256) void k_closure(ClosureArgs c) {
                           (float)o6, (float)t.bx, (float)t.by, (float)t.bz, ty, gth, (float)D2 * c.rnu2, f12,
                           (float)c.gyb, fe);
   }
-  // network: 4 tiles of 16 samples per wave
+  // network: 4 tiles of 16 samples per wave, skipped by a wave none of whose
+  // points is inside the envelope (the log width ratio is constant on a row, so
+  // whole rows fall outside on stretched grids)
+  const bool inside = c.have_net && s2 > 0.f && closure_inside(c.w.norm, fe);
 #pragma unroll
   for (int k = 0; k < MLP_INP; ++k) sF[wave][lane][k] = fe[k];
   __syncthreads();
-  if (c.have_net && !c.feat) {
+  if (c.have_net && !c.feat && __any(inside)) {
     const int r = lane & 15, gq = lane >> 4;
     MlpReg m;
     mlp_load(c.w, m);
@@ -209,7 +212,6 @@ __global__ __launch_bounds__(256) void k_closure(ClosureArgs c) {
     }
   }
   __syncthreads();
-  const bool inside = c.have_net && s2 > 0.f && closure_inside(c.w.norm, fe);
   if (c.feat) {
     if (on) {
       const long long s = (long long)(j - c.jf0) * g.pp + q;
@@ -229,6 +231,27 @@ __global__ __launch_bounds__(256) void k_closure(ClosureArgs c) {
   const float r2 = sqrtf(2.f * s2);
   ((T*)c.cgy)[o] = inside ? (T)(r2 * fmaxf(0.f, sY[wave][lane][0])) : -amd_v;
   ((T*)c.kgy)[o] = inside ? (T)(r2 * fmaxf(0.f, sY[wave][lane][1])) : -amd_t;
+}
+
+// The GY points of rows 2..Nyp whose coefficient (velocity) came from the
+// network: the sign bit of cgy is clear there (k_closure stores the AMD fallback
+// negated).  One atomic per block into count.
+template <typename T>
+__global__ __launch_bounds__(256) void k_closure_count(const T* __restrict__ cgy, long long pp, int ny,
+                                                        unsigned long long* count) {
+  const long long n = (long long)(ny - 1) * pp, base = 2 * pp;
+  int c = 0;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+    c += signbit(cgy[base + i]) ? 0 : 1;
+  // block sum through LDS
+  __shared__ int sc[256];
+  sc[threadIdx.x] = c;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sc[threadIdx.x] += sc[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && sc[0]) atomicAdd(count, (unsigned long long)sc[0]);
 }
 
 }  // namespace dgpu

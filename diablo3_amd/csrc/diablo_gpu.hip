@@ -128,6 +128,7 @@ struct dgpu_handle {
   float* d_netnorm = nullptr;
   int have_net = 0;
   void *Cgy = nullptr, *Kgy = nullptr;  // coefficients at GY, precision T
+  unsigned long long* d_ccount = nullptr;  // dgpu_closure_counts' sum
   // variable_dt: courant's CFL limit, reduced per block on the device at rk_step 3
   int variable_dt = 0, update_dt = 1, cfl_ran = 0;
   double cfl = 0.5;
@@ -1425,6 +1426,7 @@ int dgpu_create(const dgpu_config* cfg, dgpu_handle** out) {
     if (cfg->les_model_type == 6) {
       ALLOC(h->Cgy, tbytes);
       ALLOC(h->Kgy, tbytes);
+      ALLOC(h->d_ccount, sizeof(unsigned long long));
       ALLOC(h->d_netw, sizeof(unsigned short) * (64 * 16 + 64 * 64 + 16 * 64));
       ALLOC(h->d_netp, sizeof(float) * MLP_NP);
       ALLOC(h->d_netnorm, sizeof(float) * MLP_NNORM);
@@ -1639,6 +1641,24 @@ int dgpu_set_closure(dgpu_handle* h, const float* params, const float* norm) {
   HIPCHK(hipMemcpyAsync(h->d_netnorm, norm, sizeof(float) * MLP_NNORM, hipMemcpyHostToDevice, h->st));
   HIPCHK(hipStreamSynchronize(h->st));
   h->have_net = 1;
+  return 0;
+}
+
+int dgpu_closure_counts(dgpu_handle* h, int64_t* network, int64_t* points) {
+  if (!h || !network || !points) return fail(1, "null argument");
+  if (!h->d_ccount) return fail(2, "dgpu_closure_counts needs an les_model_type 6 handle");
+  unsigned long long c = 0;
+  HIPCHK(hipMemsetAsync(h->d_ccount, 0, sizeof(c), h->st));
+  const Geo& g = h->g;
+  if (h->prec == DGPU_MIXED)
+    k_closure_count<float><<<1024, 256, 0, h->st>>>((const float*)h->Cgy, g.pp, g.ny, h->d_ccount);
+  else
+    k_closure_count<double><<<1024, 256, 0, h->st>>>((const double*)h->Cgy, g.pp, g.ny, h->d_ccount);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(&c, h->d_ccount, sizeof(c), hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  *network = (int64_t)c;
+  *points = (int64_t)(g.ny - 1) * g.pp;
   return 0;
 }
 

@@ -90,6 +90,7 @@ def load():
     L.dgpu_mlp_predict.argtypes = [P, F, I64, F]
     L.dgpu_set_closure.argtypes = [P, F, F]
     L.dgpu_box_filter.argtypes = [P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, D, D, D]
+    L.dgpu_closure_counts.argtypes = [P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]
     L.dgpu_les_features.argtypes = [P, ctypes.c_double, F, F, F, ctypes.POINTER(ctypes.c_int32),
                                     ctypes.POINTER(I64)]
     _lib = L
@@ -204,6 +205,13 @@ class Solver:
                                         am.ctypes.data_as(F), ins.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
                                         ctypes.byref(n)))
         return X, sc, am, ins
+
+    def closure_counts(self):
+        """dgpu_closure_counts: (points whose coefficient came from the network,
+        points the closure evaluated) in the last LES substep."""
+        a, b = ctypes.c_int64(), ctypes.c_int64()
+        self._check(load().dgpu_closure_counts(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
 
     def box_filter(self, fx: int, fz: int, mf: np.ndarray, mg: np.ndarray) -> np.ndarray:
         """dgpu_box_filter: the state filtered onto a grid coarser by fx, fz in x, z

@@ -95,6 +95,13 @@ int dgpu_mlp_predict(dgpu_mlp *m, const float *X, int64_t n, float *Y);
  * are copied; later changes to a dgpu_mlp do not reach the handle. */
 int dgpu_set_closure(dgpu_handle *h, const float *params, const float *norm);
 
+/* Of the GY points of rows 2..Nyp the closure evaluated in the last LES
+ * substep (*points = (Ny - 1) Nz Nx), those whose velocity coefficient came from
+ * the network (*network); the rest took the AMD fallback.  Meaningful after an
+ * LES substep of the handle.  No counterpart in the reference (a statistic of
+ * the learned closure). */
+int dgpu_closure_counts(dgpu_handle *h, int64_t *network, int64_t *points);
+
 /* Features of the current state at the GYF points of rows jstart..Nyp-1 (the
  * rows where les_chan forms the GYF stresses), after the BC posts of an LES
  * substep at `time`: X [n][16], scale [n] = C^2 D2 sqrt(2 s2), amd [n][2] = the

@@ -5,6 +5,8 @@
   parity bar (1e-11 on every field, cp included).
 * dgpu_les_features: the AMD column times the scale is the type-5 eddy
   viscosity at GYF, and the features are finite and bounded where defined.
+* The in-envelope step against the oracle's network restatement, with
+  dgpu_closure_counts against the oracle's envelope mask.
 """
 import numpy as np
 import pytest
@@ -77,13 +79,17 @@ def test_features_amd_column_is_type5_viscosity():
     print("features: s2>0 on", f"{live.mean():.3f}", "AMD active on", f"{(am[:, 0] > 0).mean():.3f}", flush=True)
 
 
-def test_type6_network_points_match_oracle():
+@pytest.mark.parametrize("rows_out", [False, True])
+def test_type6_network_points_match_oracle(rows_out):
     """The in-envelope path: the device's features, network, envelope test and
     coefficient assembly (k_closure, k_amd<..., 6>) against the oracle's
     restatement (oracle/closure.py on the type-5 quantities the oracle captures,
     oracle/mlp.py's BF16-emulating forward pass, oracle_les_override).  The
     committed network with an envelope of the 1st-99th percentiles of this
-    state's own features, so that points both inside and outside are exercised.
+    state's own features, so that points both inside and outside are exercised;
+    with rows_out the upper bound of the row-constant width-ratio feature is its
+    median instead, so that whole rows leave the envelope (k_closure's waves with
+    no point inside skip the network).
     Bars (BF16-aware: a feature that differs in its last float bit can round to a
     neighbouring BF16 input): the inside mask identical on all but 0.1% of the
     points; nu_t and kappa_t after one substep within 2e-3 (relative L2).  The
@@ -106,10 +112,16 @@ def test_type6_network_points_match_oracle():
     env = np.asarray(norm, np.float32).copy()
     env[32:32 + ocl.mlp.NIN] = np.quantile(F, 0.01, axis=0)
     env[48:48 + ocl.mlp.NIN] = np.quantile(F, 0.99, axis=0)
+    if rows_out:
+        env[48 + 12] = np.median(F[:, 12])
     cgy, kgy, inside = ocl.coefficients(cap, cfg, p5.met["dy"], params, env)
     frac = float(inside[live].mean())
-    print(f"network {src}: {live.sum()} GY points, inside the envelope {frac:.3f}", flush=True)
-    assert 0.3 < frac < 0.99
+    rows_in = int(inside.any(axis=(1, 2)).sum())
+    print(f"network {src}: {live.sum()} GY points, inside the envelope {frac:.3f}, "
+          f"rows with a point inside {rows_in} of {cfg.ny - 1}", flush=True)
+    assert 0.2 < frac < 0.99
+    if rows_out:
+        assert rows_in < (cfg.ny - 1) * 0.7
     ob = oracle.Oracle(p5, threads=16)
     ob.upload_state(s0)
     ob.les_override(cgy, kgy)
@@ -118,6 +130,14 @@ def test_type6_network_points_match_oracle():
     g.set_closure(params, env)
     g.upload_state(s0)
     g.rk_substep(1, time=0.0, time_step=1, create_new_flow=False)
+    # dgpu_closure_counts: every GY point of rows 2..Nyp evaluated once, and the
+    # network taken where the oracle's envelope test passes (up to the points a
+    # last-bit feature difference moves across an envelope bound)
+    nn, npts = g.closure_counts()
+    n_in = int(inside.sum())
+    print(f"closure counts: network {nn} of {npts}, oracle inside {n_in}", flush=True)
+    assert npts == (cfg.ny - 1) * cfg.nz * cfg.nx
+    assert abs(nn - n_in) <= max(2, int(1e-3 * npts))
     (ntg, ktg), (nto, kto) = g.download_les(), ob.download_les()
     # which GY points took the network: nu_t of the device at GY over the
     # oracle's two candidates (network or fallback) on rows 3..Nyp-1
