@@ -215,6 +215,75 @@ __device__ __forceinline__ void fft_stages(cx<T>* __restrict__ s, int ld, const 
   }
 }
 
+// The same stage with every line owned by one wave (line l by wave l % NW): a
+// wave's LDS reads and writes are processed in order, so the stage needs no
+// barrier.  Every stage must split a line into whole butterflies of one wave
+// (64 % (N / R) == 0: N <= 256).
+template <typename T, int N, int LINES, int NT, int DIR, int R, int Ns>
+__device__ __forceinline__ void fft_stage_wave(cx<T>* __restrict__ s, int ld,
+                                               const cx<T>* __restrict__ tw, int tid) {
+  constexpr int NW = NT / 64, BL = N / R;
+  static_assert(NT % 64 == 0 && 64 % BL == 0, "a wave takes whole lines");
+  constexpr int LW = (LINES + NW - 1) / NW;  // lines of the busiest wave
+  constexpr int PER = (LW * BL + 63) / 64;
+  const int w = tid >> 6, lane = tid & 63;
+  cx<T> v[PER][R];
+  int ob[PER], ol[PER];
+#pragma unroll
+  for (int p = 0; p < PER; ++p) {
+    const int b = lane + p * 64;
+    const int li = b / BL, j = b - li * BL;
+    const int l = w + li * NW;
+    ob[p] = -1;
+    ol[p] = 0;
+    if (li < LW && l < LINES) {
+      const cx<T>* line = s + l * ld;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[p][r] = line[swz<T>(j + r * (N / R))];
+      const int k = j & (Ns - 1);
+      constexpr int tstep = N / (Ns * R);
+      if constexpr (Ns > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          cx<T> tw_ = tw[r * k * tstep];
+          if constexpr (DIR > 0) tw_ = conjg(tw_);
+          v[p][r] = v[p][r] * tw_;
+        }
+      }
+      if constexpr (R == 16) {
+        dft16<DIR>(v[p]);
+      } else if constexpr (R == 8) {
+        dft8<DIR>(v[p]);
+      } else if constexpr (R == 4) {
+        dft4<DIR>(v[p][0], v[p][1], v[p][2], v[p][3]);
+      } else {
+        cx<T> a = v[p][0], c = v[p][1];
+        v[p][0] = a + c;
+        v[p][1] = a - c;
+      }
+      ob[p] = (j / Ns) * Ns * R + k;
+      ol[p] = l * ld;
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < PER; ++p) {
+    if (ob[p] >= 0) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) s[ol[p] + swz<T>(ob[p] + r * Ns)] = v[p][r];
+    }
+  }
+}
+
+template <typename T, int N, int LINES, int NT, int DIR, int ST, int NST, int Ns>
+__device__ __forceinline__ void fft_stages_wave(cx<T>* __restrict__ s, int ld, const cx<T>* __restrict__ tw,
+                                                int tid) {
+  if constexpr (ST < NST) {
+    constexpr int R = stage_radix(ilog2(N), ST);
+    fft_stage_wave<T, N, LINES, NT, DIR, R, Ns>(s, ld, tw, tid);
+    fft_stages_wave<T, N, LINES, NT, DIR, ST + 1, NST, Ns * R>(s, ld, tw, tid);
+  }
+}
+
 // Batched in-place FFT of LINES lines of length N (power of two, >= 2) in LDS,
 // element n of line l at s[l*ld + swz<T>(n)], with the N twiddles tw (in LDS:
 // see stage_twiddles).  Caller must __syncthreads() after filling s and tw; the
@@ -223,6 +292,16 @@ template <typename T, int N, int LINES, int NT, int DIR>
 __device__ __forceinline__ void fft_lds(cx<T>* __restrict__ s, int ld,
                                         const cx<T>* __restrict__ tw, int tid) {
   fft_stages<T, N, LINES, NT, DIR, 0, stage_count(ilog2(N)), 1>(s, ld, tw, tid);
+}
+
+// The same transform with the lines shared out among the waves (line l to wave
+// l % (NT / 64)) and no barrier inside, for N <= 256 (64 % (N / R) == 0 at every
+// stage).  Same arithmetic, so the same result as fft_lds.  Caller must
+// __syncthreads() after filling s and tw, and before other waves read the lines.
+template <typename T, int N, int LINES, int NT, int DIR>
+__device__ __forceinline__ void fft_lds_wave(cx<T>* __restrict__ s, int ld,
+                                             const cx<T>* __restrict__ tw, int tid) {
+  fft_stages_wave<T, N, LINES, NT, DIR, 0, stage_count(ilog2(N)), 1>(s, ld, tw, tid);
 }
 
 // copy the N twiddles exp(-2 pi i m / N) from global memory into the kernel's LDS
