@@ -304,6 +304,19 @@ __device__ __forceinline__ void fft_lds_wave(cx<T>* __restrict__ s, int ld,
   fft_stages_wave<T, N, LINES, NT, DIR, 0, stage_count(ilog2(N)), 1>(s, ld, tw, tid);
 }
 
+// fft_lds_wave and one barrier where it applies (64 <= N <= 256), else fft_lds:
+// either way the lines are complete for every thread on return
+template <typename T, int N, int LINES, int NT, int DIR>
+__device__ __forceinline__ void fft_lds_sync(cx<T>* __restrict__ s, int ld,
+                                             const cx<T>* __restrict__ tw, int tid) {
+  if constexpr (N >= 64 && N <= 256 && NT % 64 == 0) {
+    fft_lds_wave<T, N, LINES, NT, DIR>(s, ld, tw, tid);
+    __syncthreads();
+  } else {
+    fft_lds<T, N, LINES, NT, DIR>(s, ld, tw, tid);
+  }
+}
+
 // copy the N twiddles exp(-2 pi i m / N) from global memory into the kernel's LDS
 // table (before the caller's first barrier)
 template <typename T, int N, int NT>

@@ -494,9 +494,6 @@ __global__ __launch_bounds__(NT) void k_xmarch(XProdArgs a, int seg) {
   constexpr int HB = NX / 2 + 1;                  // bins 0..NX/2 of a line pair
   constexpr int PREF = (NF * HB + NT - 1) / NT;   // loaded bins per thread
   constexpr bool ALLX = NX == NT;
-  // the plane's FFT without block barriers inside: every stage of an Nx <= 256
-  // transform splits a line into butterflies of one wave
-  constexpr bool WAVEFFT = NX >= 64 && NX <= 256 && NT % 64 == 0;
   __shared__ cx<T> s[NL * NX];
   __shared__ cx<T> stw[NX];  // twiddles
   __shared__ T skx[NX / 2];
@@ -595,12 +592,7 @@ __global__ __launch_bounds__(NT) void k_xmarch(XProdArgs a, int seg) {
       if (j + 3 <= j1 + 1) load_plane(j + 3);
     }
     __syncthreads();
-    if constexpr (WAVEFFT) {
-      fft_lds_wave<T, NX, NL, NT, -1>(s, NX, stw, tid);
-      __syncthreads();
-    } else {
-      fft_lds<T, NX, NL, NT, -1>(s, NX, stw, tid);
-    }
+    fft_lds_sync<T, NX, NL, NT, -1>(s, NX, stw, tid);
     xprod_store<T, NX, NT, NTH>(s, g, sout, skx, z, j, tid);
     if (next && (ALLX || x < NX)) {
 #pragma unroll
@@ -702,7 +694,7 @@ __global__ __launch_bounds__(NT) void k_fwdz_nl(NlArgs a, int ntiles) {
     }
     __syncthreads();
     if (PERSIST) load(t + step);
-    fft_lds<T, NZ, 2 * TI, NT, -1>(s, LD, stw, tid);
+    fft_lds_sync<T, NZ, 2 * TI, NT, -1>(s, LD, stw, tid);
     if (live) {
       cx<T>* NL = (cx<T>*)a.nl[e] + (long long)j * g.pc;
       for (int idx = tid; idx < TI * g.kzr; idx += NT) {
