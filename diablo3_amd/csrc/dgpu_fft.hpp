@@ -217,13 +217,13 @@ __device__ __forceinline__ void fft_stages(cx<T>* __restrict__ s, int ld, const 
 
 // The same stage with every line owned by one wave (line l by wave l % NW): a
 // wave's LDS reads and writes are processed in order, so the stage needs no
-// barrier.  Every stage must split a line into whole butterflies of one wave
-// (64 % (N / R) == 0: N <= 256).
+// barrier.  A wave runs the N / R butterflies of each of its lines, PER per
+// lane.
 template <typename T, int N, int LINES, int NT, int DIR, int R, int Ns>
 __device__ __forceinline__ void fft_stage_wave(cx<T>* __restrict__ s, int ld,
                                                const cx<T>* __restrict__ tw, int tid) {
   constexpr int NW = NT / 64, BL = N / R;
-  static_assert(NT % 64 == 0 && 64 % BL == 0, "a wave takes whole lines");
+  static_assert(NT % 64 == 0, "whole waves");
   constexpr int LW = (LINES + NW - 1) / NW;  // lines of the busiest wave
   constexpr int PER = (LW * BL + 63) / 64;
   const int w = tid >> 6, lane = tid & 63;
@@ -295,21 +295,24 @@ __device__ __forceinline__ void fft_lds(cx<T>* __restrict__ s, int ld,
 }
 
 // The same transform with the lines shared out among the waves (line l to wave
-// l % (NT / 64)) and no barrier inside, for N <= 256 (64 % (N / R) == 0 at every
-// stage).  Same arithmetic, so the same result as fft_lds.  Caller must
-// __syncthreads() after filling s and tw, and before other waves read the lines.
+// l % (NT / 64)) and no barrier inside.  Same arithmetic, so the same result as
+// fft_lds.  Caller must __syncthreads() after filling s and tw, and before other
+// waves read the lines.
 template <typename T, int N, int LINES, int NT, int DIR>
 __device__ __forceinline__ void fft_lds_wave(cx<T>* __restrict__ s, int ld,
                                              const cx<T>* __restrict__ tw, int tid) {
   fft_stages_wave<T, N, LINES, NT, DIR, 0, stage_count(ilog2(N)), 1>(s, ld, tw, tid);
 }
 
-// fft_lds_wave and one barrier where it applies (64 <= N <= 256), else fft_lds:
-// either way the lines are complete for every thread on return
+// fft_lds_wave and one barrier where every wave has a line (LINES >= NT / 64)
+// and a line fills a wave (N >= 64), else fft_lds: either way the lines are
+// complete for every thread on return.  Above N = 256 only in FP64 (measured at
+// 512x257x512: FP64 k_xprod 3.27 -> 3.12 ms and k_fwdz_nl 1.04 -> 0.91 ms, but
+// float k_xprod 1.37 -> 1.41 and k_fwdz_nl 0.73 -> 0.82).
 template <typename T, int N, int LINES, int NT, int DIR>
 __device__ __forceinline__ void fft_lds_sync(cx<T>* __restrict__ s, int ld,
                                              const cx<T>* __restrict__ tw, int tid) {
-  if constexpr (N >= 64 && N <= 256 && NT % 64 == 0) {
+  if constexpr (N >= 64 && NT % 64 == 0 && LINES >= NT / 64 && (N <= 256 || sizeof(T) == 8)) {
     fft_lds_wave<T, N, LINES, NT, DIR>(s, ld, tw, tid);
     __syncthreads();
   } else {

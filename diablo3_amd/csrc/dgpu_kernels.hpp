@@ -211,7 +211,7 @@ __global__ __launch_bounds__(NT) void k_inv_x(Ptrs<const cx<T>*, MAXO> src, Ptrs
     }
   }
   __syncthreads();
-  fft_lds<T, NX, ZP, NT, +1>(s, NX, stw, tid);
+  fft_lds_sync<T, NX, ZP, NT, +1>(s, NX, stw, tid);
   T* __restrict__ R = dst.p[f] + (long long)j * g.pp;
   for (int idx = tid; idx < ZP * NX; idx += NT) {
     const int q = idx / NX, n = idx % NX, z = zb + 2 * q;
@@ -235,7 +235,7 @@ __global__ __launch_bounds__(NT) void k_fwd_x(Ptrs<const T*, MAXO> src, Ptrs<cx<
     s[q * NX + swz<T>(n)] = {R[(long long)z * NX + n], R[(long long)(z + 1) * NX + n]};
   }
   __syncthreads();
-  fft_lds<T, NX, ZP, NT, -1>(s, NX, stw, tid);
+  fft_lds_sync<T, NX, ZP, NT, -1>(s, NX, stw, tid);
   cx<T>* __restrict__ H = dst.p[f] + (long long)j * g.ph;
   const T rnx = T(1) / T(NX);
   for (int idx = tid; idx < ZP * g.kxr; idx += NT) {
@@ -470,7 +470,7 @@ __global__ __launch_bounds__(NT) void k_xprod(XProdArgs a) {
     for (int p = 0; p < NP; ++p) s[p * NX + swz<T>(x)] = P[p];
   }
   __syncthreads();
-  fft_lds<T, NX, NP, NT, -1>(s, NX, stw, tid);
+  fft_lds_sync<T, NX, NP, NT, -1>(s, NX, stw, tid);
   xprod_store<T, NX, NT, NTH>(s, g, a.hout, a.kx, z, j, tid);
 }
 
