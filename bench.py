@@ -226,7 +226,7 @@ def main():
                     help="only run a plain device copy of BYTES (rocprofv3 counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build runs on one GPU only (no collectives, no sharding); "
+        print("bench.py: one GPU only (no collectives, no sharding); "
               "use --gpus 1.", file=sys.stderr)
         sys.exit(2)
     if args.copy_calibration:

@@ -14,7 +14,7 @@ learned closure (AMD fallback outside the training envelope), with AMD, and
 with no model, each compared with the filtered DNS by the mean buoyancy profile
 and the TKE spectrum (closure_eval.compare) every `compare_every` steps.
 
-    python -m diablo3_amd.closure_c4 --out-dir gpurun_out/c4
+    python -m diablo3_amd.closure_c4 --out-dir c4_out
 """
 from __future__ import annotations
 

@@ -93,7 +93,7 @@ class Config:
     # apply the IC-type Neumann compensation of input_chan (False when the BC values
     # already hold it)
     bc_compensation: bool = True
-    # this build's switches
+    # the library's own switches
     precision: int = FP64
     # derived (filled by derive())
     nu: float = 0.0

@@ -1,4 +1,4 @@
-"""The named workloads of this build, as :class:`Config` factories.
+"""The named workloads, as :class:`Config` factories.
 
 Code axes throughout: (x, y, z) with y wall-normal; y in [-1, 1] is GRID_TYPE 1 of
 proc/init/create_grid.m with Lz(user) = 2 and stretch CS = 1.75.

@@ -398,7 +398,7 @@ int dgpu_read_flow(dgpu_handle* h, const char* fname, uint32_t th_mask, dgpu_flo
     return rc;
   dgpu_gf2g(u2.data(), nx, ny, nz);
   // with variable_dt, courant on the velocity read (for/phdf5.f90:1437-1439):
-  // the driver's next dgpu_courant returns its delta_t
+  // the time loop's next dgpu_courant returns its delta_t
   if ((rc = dev(dgpu_courant_physical(h, u1.data(), u2.data(), u3.data())))) return rc;
   if ((rc = dev(dgpu_fft_xz_to_fourier(h, u1.data(), cu1.data()))) ||
       (rc = dev(dgpu_fft_xz_to_fourier(h, u2.data(), cu2.data()))) ||

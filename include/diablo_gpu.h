@@ -275,7 +275,7 @@ int dgpu_stats_movie(dgpu_handle *h, double time, int32_t nxm, int32_t nym, int3
                      double *slices);
 int dgpu_stats_movie_sizes(const dgpu_config *cfg, int64_t *n_planes, int64_t *n_slices);
 
-/* filter_chan(n) (for/solver.f90:1597-1685), as the driver calls it every
+/* filter_chan(n) (for/solver.f90:1597-1685), as diablo.f90's time loop calls it every
  * filter_int(n) steps when filter_th(n) (for/diablo.f90:76-81): scalar n
  * (0-based) times the sharpened raised-cosine filter sigma(kx, kz) on rows
  * jstart_th..jend_th. */

@@ -1,4 +1,4 @@
-"""The driver-side hooks and the forcing types on the device against the oracle,
+"""The time-loop hooks and the forcing types on the device against the oracle,
 through the C ABI: filter_chan (for/solver.f90:1597-1685) on a random scalar,
 and 3 RKW3 steps with f_type 1 (constant pressure gradient), 2 (oscillatory
 pressure gradient; for/solver.f90:464-476) and 4 (oscillatory surface forcing of
