@@ -307,6 +307,37 @@ int append_series(const char* fname, const char* gname, const double* data, int 
 }
 }  // namespace
 
+int dgpu_h5_write_grid(const char* fname, const double* gy, int32_t n) {
+  quiet();
+  if (!fname || !gy || n < 2) return fail(1, "bad argument");
+  Hid f(H5Fcreate(fname, H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT), H5Fclose);
+  if (!f.ok()) return fail(3, std::string("cannot create ") + fname);
+  Hid g(H5Gcreate2(f, "grids", H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT), H5Gclose);
+  if (!g.ok()) return fail(3, "group /grids");
+  const hsize_t dims[1] = {(hsize_t)n};
+  Hid sp(H5Screate_simple(1, dims, nullptr), H5Sclose);
+  Hid ds(H5Dcreate2(g, "y", H5T_IEEE_F64LE, sp, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT), H5Dclose);
+  if (!ds.ok() || H5Dwrite(ds, H5T_NATIVE_DOUBLE, H5S_ALL, H5S_ALL, H5P_DEFAULT, gy) < 0)
+    return fail(3, "dataset /grids/y");
+  return 0;
+}
+
+int dgpu_h5_read_grid(const char* fname, double* gy, int32_t* n) {
+  quiet();
+  if (!fname || !n) return fail(1, "null argument");
+  Hid f(H5Fopen(fname, H5F_ACC_RDONLY, H5P_DEFAULT), H5Fclose);
+  if (!f.ok()) return fail(3, std::string("cannot open ") + fname);
+  Hid ds(H5Dopen2(f, "/grids/y", H5P_DEFAULT), H5Dclose);
+  if (!ds.ok()) return fail(3, "no dataset /grids/y");
+  Hid sp(H5Dget_space(ds), H5Sclose);
+  hsize_t dims[1] = {0};
+  if (!sp.ok() || H5Sget_simple_extent_ndims(sp) != 1 || H5Sget_simple_extent_dims(sp, dims, nullptr) != 1)
+    return fail(3, "/grids/y is not one-dimensional");
+  *n = (int32_t)dims[0];
+  if (gy && H5Dread(ds, H5T_NATIVE_DOUBLE, H5S_ALL, H5S_ALL, H5P_DEFAULT, gy) < 0) return fail(3, "read /grids/y");
+  return 0;
+}
+
 int dgpu_h5_append_profile(const char* fname, const char* gname, const double* diag, int32_t ny, double time) {
   if (!fname || !gname || !diag || ny <= 0) return fail(1, "dgpu_h5_append_profile: bad argument");
   const hsize_t n = (hsize_t)ny;

@@ -54,6 +54,8 @@ def load():
     L.dgpu_gf2g.restype = None
     L.dgpu_h5_append_profile.argtypes = [s, s, D, I32, ctypes.c_double]
     L.dgpu_h5_append_real.argtypes = [s, s, ctypes.c_double]
+    L.dgpu_h5_write_grid.argtypes = [s, D, I32]
+    L.dgpu_h5_read_grid.argtypes = [s, D, ctypes.POINTER(I32)]
     L.dgpu_h5_append_plane.argtypes = [s, s, D, I32, I32, ctypes.c_double]
     L.dgpu_write_flow.argtypes = [P, s, T, I32]
     L.dgpu_read_flow.argtypes = [P, s, ctypes.c_uint32, T, ctypes.POINTER(I32)]
@@ -78,6 +80,21 @@ def _times(t) -> FlowTimes:
     if isinstance(t, FlowTimes):
         return t
     return FlowTimes(**{k: t.get(k, 0) for k, _ in FlowTimes._fields_})
+
+
+def write_grid(fname: str, gy: np.ndarray) -> None:
+    """grid.h5:/grids/y (create_grid.m's file) from the Ny + 1 values G(2..Ny+2)."""
+    g = np.ascontiguousarray(gy, np.float64)
+    _check(load().dgpu_h5_write_grid(fname.encode(), _dp(g), len(g)))
+
+
+def read_grid(fname: str) -> np.ndarray:
+    """The values of grid.h5:/grids/y."""
+    n = ctypes.c_int32()
+    _check(load().dgpu_h5_read_grid(fname.encode(), None, ctypes.byref(n)))
+    g = np.zeros(n.value, np.float64)
+    _check(load().dgpu_h5_read_grid(fname.encode(), _dp(g), ctypes.byref(n)))
+    return g
 
 
 def physical_shape(nx: int, ny: int, nz: int):

@@ -78,6 +78,14 @@ int dgpu_h5_append_plane(const char *fname, const char *gname, const double *var
 /* WriteHDF5_real (for/phdf5.f90:7-128): the same series form for one value. */
 int dgpu_h5_append_real(const char *fname, const char *gname, double value);
 
+/* The grid file as proc/init/create_grid.m:84-87 writes it and ReadGridHDF5
+ * (for/domain.f90:420-500) reads it: group /grids, dataset y of n = Ny + 1
+ * doubles G(2..Ny+2) (diablo3_amd.grid.create_grid computes them).  The file is
+ * created or truncated. */
+int dgpu_h5_write_grid(const char *fname, const double *gy, int32_t n);
+/* /grids/y of a grid file: *n receives its length, gy (may be NULL) the values. */
+int dgpu_h5_read_grid(const char *fname, double *gy, int32_t *n);
+
 /* ---- with the device state ---- */
 
 /* WriteHDF5 (for/phdf5.f90:918-1197) of the handle's state: the fields are

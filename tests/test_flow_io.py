@@ -227,3 +227,32 @@ def test_plane_series(tmp_path):
         assert h.dataset("/chi_xz/0001")[0][0, 0] == stats_io.XZ_PLANES.index("chi_xz")
     finally:
         h.close()
+
+
+def test_grid_file_layout_and_metrics(tmp_path):
+    """grid.h5 as create_grid.m writes it (/grids/y, Ny + 1 doubles G(2..Ny+2)) and
+    ReadGridHDF5 reads it: read back by the HDF5 library independently of the
+    writer, and the metrics built from the file's values are the problem's."""
+    from diablo3_amd import configs
+    from diablo3_amd.grid import create_grid, metrics
+    from diablo3_amd.problem import Problem
+    fio = io()
+    cfg = configs.dns_64()
+    gy = create_grid(cfg.ny, cfg.ly, cfg.grid_cs, cfg.grid_type)
+    path = str(tmp_path / "grid.h5")
+    fio.write_grid(path, gy)
+    h = h5raw.File(path)
+    try:
+        data, chunk, f64 = h.dataset("/grids/y")
+    finally:
+        h.close()
+    assert data.shape == (cfg.ny + 1,) and f64 and chunk is None
+    assert np.array_equal(data, gy)
+    assert np.array_equal(fio.read_grid(path), gy)
+    # the run's metrics are built from the file's values (for/domain.f90:276-300, 420-500)
+    p = Problem(cfg)
+    met = metrics(fio.read_grid(path))
+    for k in ("gy", "gyf", "dy", "dyf"):
+        assert np.array_equal(met[k], p.met[k]), k
+    with pytest.raises(RuntimeError):
+        fio.read_grid(str(tmp_path / "missing.h5"))
