@@ -64,45 +64,32 @@ __device__ __forceinline__ float closure_features(float a1, float a2, float a3, 
   const float s2 = a1 * a1 + a2 * a2 + a3 * a3 + 2.f * (a4 * a4 + a5 * a5 + a6 * a6);
   const float w2 = 2.f * (o4 * o4 + o5 * o5 + o6 * o6);
   const float e = s2 + w2;
-  // S = [[a1 a4 a5][a4 a2 a6][a5 a6 a3]], O = [[0 o4 o5][-o4 0 o6][-o5 -o6 0]], P = O O^T
-  const float S[3][3] = {{a1, a4, a5}, {a4, a2, a6}, {a5, a6, a3}};
-  const float O[3][3] = {{0.f, o4, o5}, {-o4, 0.f, o6}, {-o5, -o6, 0.f}};
-  float S2[3][3], P[3][3];
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      float s = 0.f, p = 0.f;
-#pragma unroll
-      for (int m = 0; m < 3; ++m) {
-        s += S[i][m] * S[m][k];
-        p += O[i][m] * O[k][m];
-      }
-      S2[i][k] = s;
-      P[i][k] = p;
-    }
-  float trS3 = 0.f, trSP = 0.f, trS2P = 0.f;
-  const float b[3] = {bx, by, bz};
-  const float b2 = bx * bx + by * by + bz * bz;
-  float bSb = 0.f, bS2b = 0.f, bPb = 0.f;
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      trS3 += S2[i][k] * S[k][i];
-      trSP += S[i][k] * P[k][i];
-      trS2P += S2[i][k] * P[k][i];
-      bSb += b[i] * S[i][k] * b[k];
-      bS2b += b[i] * S2[i][k] * b[k];
-      bPb += b[i] * P[i][k] * b[k];
-    }
+  // S = [[a1 a4 a5][a4 a2 a6][a5 a6 a3]], O = [[0 o4 o5][-o4 0 o6][-o5 -o6 0]]:
+  // S^2 and P = O O^T are symmetric, so every trace and quadratic form below is
+  // its three diagonal terms plus twice its three upper ones
+  const float q11 = a1 * a1 + a4 * a4 + a5 * a5, q22 = a4 * a4 + a2 * a2 + a6 * a6, q33 = a5 * a5 + a6 * a6 + a3 * a3;
+  const float q12 = a1 * a4 + a4 * a2 + a5 * a6, q13 = a1 * a5 + a4 * a6 + a5 * a3, q23 = a4 * a5 + a2 * a6 + a6 * a3;
+  const float p11 = o4 * o4 + o5 * o5, p22 = o4 * o4 + o6 * o6, p33 = o5 * o5 + o6 * o6;
+  const float p12 = o5 * o6, p13 = -(o4 * o6), p23 = o4 * o5;
+  auto sym = [](float x11, float x22, float x33, float x12, float x13, float x23,
+                float y11, float y22, float y33, float y12, float y13, float y23) {
+    return x11 * y11 + x22 * y22 + x33 * y33 + 2.f * (x12 * y12 + x13 * y13 + x23 * y23);
+  };
+  const float trS3 = sym(q11, q22, q33, q12, q13, q23, a1, a2, a3, a4, a5, a6);
+  const float trSP = sym(a1, a2, a3, a4, a5, a6, p11, p22, p33, p12, p13, p23);
+  const float trS2P = sym(q11, q22, q33, q12, q13, q23, p11, p22, p33, p12, p13, p23);
+  const float bxx = bx * bx, byy = by * by, bzz = bz * bz, bxy = bx * by, bxz = bx * bz, byz = by * bz;
+  const float b2 = bxx + byy + bzz;
+  const float bSb = sym(a1, a2, a3, a4, a5, a6, bxx, byy, bzz, bxy, bxz, byz);
+  const float bS2b = sym(q11, q22, q33, q12, q13, q23, bxx, byy, bzz, bxy, bxz, byz);
+  const float bPb = sym(p11, p22, p33, p12, p13, p23, bxx, byy, bzz, bxy, bxz, byz);
   const float rs = sqrtf(s2);
   f[0] = fq(w2 - s2, e);
   f[1] = fq(trS3, s2 * rs);
   f[2] = fq(trSP, rs * w2);
   f[3] = fq(-(trS3 + trSP), rs * e);
   f[4] = fq(trS2P, s2 * w2);
-  f[5] = fq(by * by, b2);
+  f[5] = fq(byy, b2);
   f[6] = fq(bSb, b2 * rs);
   f[7] = fq(bS2b, b2 * s2);
   f[8] = fq(bPb, b2 * w2);
