@@ -8,8 +8,8 @@ One "step" is the full RKW3 step (three rk_chan_1 substeps) through the C ABI.
 Printed JSON line: value = Nx*Ny*Nz / t_step in Mpoint-updates/s; roofline for the
 dominant kernel (algorithmic bytes per launch / its mean HIP-event duration over
 the timed region); cpu_baseline = the CPU oracle timed on this host's cores on a
-bounded sample (the median substep of three steps on all cores, and of one step
-on one core); learned_vs_amd_les = the learned-closure mixed LES step against the
+bounded sample (the median substep of three steps on all cores, and on one
+core); learned_vs_amd_les = the learned-closure mixed LES step against the
 FP64 and mixed AMD LES steps at 512x257x512 (the metric's third part).
 """
 import argparse
