@@ -271,6 +271,38 @@ def test_ten_steps_dns_256_fp64_and_mixed():
     assert np.abs(dm[1:-1]).max() < 1e-13 * np.abs(so.cu1).max() * p.kx.max()
 
 
+def test_dns_256_configuration_length_mixed_vs_fp64():
+    """256x129x256 front DNS over its configuration's 200 steps (dt 5e-4), FP64 and
+    mixed from one projected state on the device: the mixed state within 1e-7 of the
+    FP64 state on u and theta and 1e-6 on cp (relative L2; measured 2.4e-9 and
+    3.2e-8), the mean buoyancy profile within 1e-12, and the divergence at
+    round-off in both.  (200 oracle steps at this size would take minutes; the
+    10-step test above holds the FP64 path to the oracle.)"""
+    cfg = configs.dns_256()
+    p = Problem(cfg)
+    g0 = Solver(p)
+    g0.upload_state(p.initial_state(seed=0, theta_noise=1e-3))
+    g0.rem_div()
+    s0 = g0.download_state()
+    del g0
+    p64, g64 = _gpu_run(cfg, s0, 200, FP64)
+    s64 = g64.download_state()
+    del g64
+    pm, gm = _gpu_run(cfg, s0, 200, MIXED)
+    sm = gm.download_state()
+    del gm
+    errs = {k: rel(a, b) for (k, a), b in zip(fields(sm).items(), fields(s64).values())}
+    mt = rel(sm.cth[0][1:cfg.ny + 1, 0, 0].real, s64.cth[0][1:cfg.ny + 1, 0, 0].real)
+    print("dns_256 x200 mixed vs FP64", {k: f"{v:.2e}" for k, v in errs.items()}, f"mean theta {mt:.2e}",
+          flush=True)
+    for k, e in errs.items():
+        assert e <= (1e-6 if k == "cp" else 1e-7), (k, e)
+    assert mt <= 1e-12
+    for pp, ss in ((p64, s64), (pm, sm)):
+        d = divergence(pp, ss)
+        assert np.abs(d[1:-1]).max() < 1e-13 * np.abs(ss.cu1).max() * pp.kx.max()
+
+
 @pytest.mark.parametrize("ny", [257, 385])
 def test_tall_columns(ny):
     """128 x ny x 128 front DNS, 2 steps, FP64: the column kernels take the shapes
